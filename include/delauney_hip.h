@@ -1,0 +1,134 @@
+/*
+ * delauney_hip.h -- C ABI of libdelauney_hip.so, the MI355X (gfx950) backend of
+ * the `delauney` integer Voronoi / Delaunay pipeline.
+ *
+ * The entry points replace the pybind11 module `_delauney_cuda` of the
+ * reference (src/cuda/bindings.cpp) and compute the semantics of its NumPy
+ * reference path (src/delauney/reference/), bit for bit.  Each declaration
+ * cites the reference interface it stands in for.
+ *
+ * Conventions
+ *   - Every function returns a status: DL_OK, DL_EVALUE (invalid input; the
+ *     Python layer raises ValueError) or DL_ERUNTIME (device or allocation
+ *     failure; RuntimeError).  dl_last_error() returns the message of the last
+ *     failing call on the calling thread.
+ *   - Seeds are n (x, y) int32 pairs in any order.  Seed ids are the
+ *     (x asc, y asc) order (reference voronoi.py:64-66).
+ *   - Host pointers are plain caller-owned C arrays; *_device entry points take
+ *     device pointers and a hipStream_t passed as void* (NULL: the context's
+ *     own stream).
+ *   - border_padding: -1 selects DL_DEFAULT_BORDER_PADDING (the reference's
+ *     border_padding=None / AUTO_BORDER_PADDING, binding_helpers.hpp:18-20);
+ *     other negative values clamp to 0 (triangulation.py:75-78).
+ *   - Canvas sides (image side + 2 * padding) are limited to DL_MAX_CANVAS.
+ *   - Calls on one context are serialised by the caller (one context per
+ *     thread); they block until their results are ready and may be made with
+ *     the Python GIL released.
+ */
+#ifndef DELAUNEY_HIP_H
+#define DELAUNEY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DL_OK 0
+#define DL_EVALUE 1
+#define DL_ERUNTIME 2
+
+#define DL_DEFAULT_BORDER_PADDING 16 /* src/delauney/__init__.py:16 */
+#define DL_MAX_CANVAS 32000
+
+typedef struct dl_context dl_context;
+
+/* Library version string. */
+const char* dl_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* dl_last_error(void);
+
+/* A context owns a HIP stream on `device` and grow-only device workspaces.
+ * (The reference has no context; its calls allocate per call, e.g.
+ * src/cuda/triangulation.cu:143-365.) */
+int dl_context_create(int device, dl_context** out);
+void dl_context_destroy(dl_context* ctx);
+
+/* Voronoi().compute(width, height, seeds) -> (H, W, 2) int32 (seed_id, d^2).
+ * Replaces PyVoronoi::compute, src/cuda/bindings/py_voronoi.hpp:11-24;
+ * semantics of src/delauney/reference/voronoi.py:34-163. */
+int dl_voronoi(dl_context* ctx, int width, int height, const int32_t* seeds_xy, int64_t n,
+               int32_t* out_hw2);
+
+/* GridTriangulation().compute(voronoi_grid, seeds, border_padding).
+ * voronoi_grid: (H, W, 2) int32 host array; only its id channel is read, and
+ * only when the resolved padding is 0 (triangulation.py:79-84).  Writes the
+ * (H, W) triangle-id raster to out_tid_hw (channel 2 of the reference's grid;
+ * channels 0/1 are the caller's grid, triangulation.py:206-209) and the
+ * triangle count to *n_tri.  The triangles are kept in the context for
+ * dl_copy_triangles.  Replaces PyGridTriangulation::compute,
+ * src/cuda/bindings/py_grid_triangulation.hpp:10-24. */
+int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int width,
+                          int height, const int32_t* seeds_xy, int64_t n, int border_padding,
+                          int32_t* out_tid_hw, int64_t* n_tri);
+
+/* triangulate(width, height, seeds, border_padding) -> (H, W, 3) int32 grid,
+ * equal to Voronoi().compute followed by GridTriangulation().compute.
+ * Replaces triangulate(), src/cuda/bindings/triangulate.hpp:15-43. */
+int dl_triangulate(dl_context* ctx, int width, int height, const int32_t* seeds_xy, int64_t n,
+                   int border_padding, int32_t* out_hw3, int64_t* n_tri);
+
+/* Rows (x, y, id_a, id_b, id_c) in tid order of the last triangulation on this
+ * context: the reference's triangle_map values (triangulation.py:157;
+ * binding_helpers.hpp:61-87).  out_rows holds n_tri * 5 int32. */
+int dl_copy_triangles(dl_context* ctx, int32_t* out_rows);
+
+/* The padded Voronoi canvas of the last dl_grid_triangulation /
+ * dl_triangulate call at padding > 0, as (H+2P, W+2P, 2) int32 (seed_id, d^2).
+ * Replaces the padded_voronoi_out of compute_debug,
+ * src/cuda/bindings/py_grid_triangulation.hpp:32-56.  *out_w / *out_h receive
+ * the canvas size; pass out == NULL to query it. */
+int dl_copy_padded_voronoi(dl_context* ctx, int32_t* out, int* out_w, int* out_h);
+
+/* Stage timings of the last call in ms: [0] seeds, [1] flood fill, [2] detect,
+ * [3] dedup/rank, [4] assign, [5] output, [6] total, [7] flood-fill launches,
+ * [8] flood-fill tile visits.  Mirrors the *_timed dictionaries
+ * (py_grid_triangulation.hpp:39-46, triangulation.cuh:29-34). */
+int dl_get_timings(dl_context* ctx, double* out9);
+
+/* Enables event timing of stages for subsequent calls (adds synchronisation). */
+int dl_set_timing(dl_context* ctx, int enabled);
+
+/* Flood-fill tile configuration: waves per tile, rows per thread, iterations
+ * per launch.  (0, 0, 0) restores automatic selection.  No reference
+ * counterpart (the reference uses 16x16 blocks, voronoi.cu:95-113). */
+int dl_set_bfs_config(dl_context* ctx, int waves, int rows, int iters);
+
+/* Device-resident triangulate: seeds, output and triangles stay on the device.
+ * d_seeds_xy: n (x, y) int32 pairs; d_out_hw3: (H, W, 3) int32.  The triangle
+ * rows stay in the context (dl_device_triangles).  No reference counterpart;
+ * it is the device-resident form of triangulate() for callers whose data are
+ * already in HBM. */
+int dl_triangulate_device(dl_context* ctx, int width, int height, const int32_t* d_seeds_xy,
+                          int64_t n, int border_padding, int32_t* d_out_hw3, void* stream,
+                          int64_t* n_tri);
+
+/* Device pointer to the (n_tri, 5) rows of the last triangulation. */
+const int32_t* dl_device_triangles(dl_context* ctx);
+
+/* Voronoi flood fill through the one-pass-per-launch kernel instead of the
+ * tiled one (cross-checking only).  Same output as dl_voronoi. */
+int dl_voronoi_simple(dl_context* ctx, int width, int height, const int32_t* seeds_xy,
+                      int64_t n, int32_t* out_hw2, int64_t* iters);
+
+/* Forces the second (unpadded) flood fill in dl_triangulate even when the
+ * padded fill shows that the image interior never received an update from the
+ * padding (cross-checking only). */
+int dl_set_force_unpadded(dl_context* ctx, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DELAUNEY_HIP_H */

@@ -1,0 +1,124 @@
+"""ctypes binding of libdelauney_hip.so (declarations: include/delauney_hip.h).
+
+This is the only place that touches the C ABI.  It loads the in-tree library
+built by ``__graft_entry__.build()`` (or ``make -C integer_delauney_amd/csrc``),
+maps status codes to the exceptions the reference raises (ValueError for bad
+input, RuntimeError for device failures -- src/cuda/cuda_check.cuh:12-44,
+binding_helpers.hpp:23-46) and keeps one library context per thread.
+
+There is no fallback: if the library or a GPU is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdelauney_hip.so")
+
+DL_OK, DL_EVALUE, DL_ERUNTIME = 0, 1, 2
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_VP = ctypes.c_void_p
+
+_lib = None
+_lib_lock = threading.Lock()
+_tls = threading.local()
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "dl_version": (ctypes.c_char_p, []),
+    "dl_last_error": (ctypes.c_char_p, []),
+    "dl_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_VP)]),
+    "dl_context_destroy": (None, [_VP]),
+    "dl_voronoi": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64, _I32P]),
+    "dl_grid_triangulation": (ctypes.c_int, [_VP, _I32P, ctypes.c_int, ctypes.c_int, _I32P,
+                                             ctypes.c_int64, ctypes.c_int, _I32P, _I64P]),
+    "dl_triangulate": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
+                                      ctypes.c_int, _I32P, _I64P]),
+    "dl_copy_triangles": (ctypes.c_int, [_VP, _I32P]),
+    "dl_copy_padded_voronoi": (ctypes.c_int, [_VP, _I32P, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_int)]),
+    "dl_get_timings": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
+    "dl_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_set_bfs_config": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "dl_triangulate_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP,
+                                             ctypes.c_int64, ctypes.c_int, _VP, _VP, _I64P]),
+    "dl_device_triangles": (_VP, [_VP]),
+    "dl_voronoi_simple": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
+                                         _I32P, _I64P]),
+    "dl_set_force_unpadded": (ctypes.c_int, [_VP, ctypes.c_int]),
+}
+
+
+def exported_symbols():
+    """Names the C ABI declares (include/delauney_hip.h)."""
+    return sorted(_SIGNATURES)
+
+
+def load():
+    """Load libdelauney_hip.so (no GPU needed to load it)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`"
+                " or `make -C integer_delauney_amd/csrc`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc == DL_OK:
+        return
+    msg = load().dl_last_error().decode("utf-8", "replace")
+    if rc == DL_EVALUE:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+class Context:
+    """One library context (HIP stream + device workspaces)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = _VP()
+        check(self._lib.dl_context_create(int(device), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.dl_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def context() -> Context:
+    """The calling thread's default context on device 0."""
+    ctx = getattr(_tls, "ctx", None)
+    if ctx is None:
+        ctx = Context(int(os.environ.get("DELAUNEY_HIP_DEVICE", "0")))
+        _tls.ctx = ctx
+    return ctx
+
+
+def i32p(a: np.ndarray):
+    return a.ctypes.data_as(_I32P)

@@ -1,0 +1,575 @@
+// Voronoi flood fill on gfx950: an exact replay of the reference schedule.
+//
+// Reference: src/delauney/reference/voronoi.py:110-163.  Every iteration runs
+// four passes with sources x-1, x+1, y-1, y+1 in that order; inside a pass each
+// cell reads its source's pre-pass value (Jacobi), and the update is in place so
+// later passes see it.  A cell takes the source's owner if it is undefined, or
+// the owner is strictly closer, or equally close with a higher id.  The fill
+// stops after the first iteration with no change.
+//
+// Why the tiled replay below is exact:
+//  * One iteration reads a Chebyshev-radius-1 neighbourhood, so k iterations of
+//    a tile with a halo of k cells reproduce the interior exactly.
+//  * Every update strictly improves the cell in the order (defined, lower d,
+//    higher id), so the state never revisits a value: a tile whose interior did
+//    not change over a launch is a fixed point of its neighbourhood, and running
+//    iterations past the reference's stop iteration changes nothing.
+//  * A tile is recomputed only if a tile within one tile of it changed in the
+//    previous launch; otherwise both label planes already hold its state.
+//
+// Layout: a tile is 128 columns wide (two columns per lane of a 64-wide wave)
+// and waves*rows tall, with `iters` halo cells on each side.  Passes along x use
+// DPP whole-wave lane shifts (wave_shr / wave_shl); passes along y run down a
+// thread's registers and exchange one boundary row per wave through LDS.  The
+// state per cell is its owner label plus the cached squared distance.
+#include "dl_internal.h"
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+namespace dl {
+namespace {
+
+__device__ __forceinline__ int dist2_lab(int cx, int cy, uint32_t lab)
+{
+    const int dx = cx - static_cast<int>(lab >> 16);
+    const int dy = cy - static_cast<int>(lab & 0xFFFFu);
+    return dx * dx + dy * dy;
+}
+
+// voronoi.py:174-196 for the one-pass kernel: take the source's owner if this
+// cell is undefined, or the owner is strictly closer, or equally close with a
+// higher id (= higher label).
+__device__ __forceinline__ bool relax(uint32_t& lab, int& d, uint32_t nl, int cx, int cy)
+{
+    const int c = dist2_lab(cx, cy, nl);
+    const bool take = (nl != kInvalid) & ((c < d) | ((c == d) & (nl > lab)));
+    lab = take ? nl : lab;
+    d = take ? c : d;
+    return take;
+}
+
+// The tiled kernel keeps each cell as a 64-bit key (d, ~label): the update
+// rule above is then "take the candidate iff its key is smaller", one 64-bit
+// compare.  An undefined cell is the key (0xFFFFFFFF, 0), larger than any
+// candidate; a cell outside the canvas is (0, 0), smaller than any candidate
+// (its d would have to be 0, i.e. a seed at the cell), and both carry ~label 0,
+// which marks an invalid source.  cxm, cym are the cell coordinates minus
+// 0xFFFF, so that dx = (inv >> 16) + cxm with inv = ~label.
+__device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t ninv, int cxm,
+                                          int cym)
+{
+    const int dx = static_cast<int>(ninv >> 16) + cxm;
+    const int dy = static_cast<int>(ninv & 0xFFFFu) + cym;
+    uint32_t c = static_cast<uint32_t>(__mul24(dx, dx) + __mul24(dy, dy));
+    c = ninv == 0u ? 0xFFFFFFFFu : c;
+    const uint64_t ck = (static_cast<uint64_t>(c) << 32) | ninv;
+    const uint64_t ok = (static_cast<uint64_t>(d) << 32) | inv;
+    const bool take = ck < ok;
+    inv = take ? ninv : inv;
+    d = take ? c : d;
+    return take;
+}
+
+__device__ __forceinline__ uint32_t from_left(uint32_t v)   // lane l <- lane l-1; lane 0 <- 0
+{
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x138, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ uint32_t from_right(uint32_t v)  // lane l <- lane l+1; lane 63 <- 0
+{
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xF, 0xF, false));
+}
+
+// Escape bookkeeping for canvases whose image interior must not receive
+// updates from the padding (see BfsCanvas::P).  Lane masks are per thread;
+// rows are local row indices.
+using GU32 = __attribute__((address_space(1))) uint32_t;
+
+struct EscGeom {
+    bool col0_left, col1_left, col0_right, col1_right, col0_in, col1_in;
+    int r_top, r_bot;    // local row index of the interior's first / last row
+    uint32_t rows_in;    // bit r set if row r lies inside the interior rows
+};
+
+template <int NW, int R, int K, bool ESC>
+__device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], uint32_t (&D0)[R],
+                                        uint32_t (&D1)[R], int cxm0, int cym0, const EscGeom& e,
+                                        uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
+                                        int (*lds_any)[NW], int wave, int lane, int& esc)
+{
+    const int cxm1 = cxm0 + 1;
+    bool ch_prev = true;
+    for (int k = 0; k < K; ++k) {
+        bool ch = false;
+        // pass 1: source x-1
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t left = from_left(I1[r]);
+            const uint32_t own0 = I0[r];
+            const bool t1 = relax_key(I1[r], D1[r], own0, cxm1, cym0 + r);
+            const bool t0 = relax_key(I0[r], D0[r], left, cxm0, cym0 + r);
+            ch |= t0 | t1;
+            if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_left) | (t1 & e.col1_left);
+        }
+        // pass 2: source x+1
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t right = from_right(I0[r]);
+            const uint32_t own1 = I1[r];
+            const bool t0 = relax_key(I0[r], D0[r], own1, cxm0, cym0 + r);
+            const bool t1 = relax_key(I1[r], D1[r], right, cxm1, cym0 + r);
+            ch |= t0 | t1;
+            if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_right) | (t1 & e.col1_right);
+        }
+        // pass 3: source y-1.  Publish this wave's last row, read the previous
+        // wave's.  The same barrier publishes whether this wave changed anything
+        // in the previous iteration: an iteration without change anywhere in the
+        // tile is a fixed point of it, so the remaining iterations can be skipped.
+        ldsA[wave][lane][0] = I0[R - 1];
+        ldsA[wave][lane][1] = I1[R - 1];
+        if (lane == 0) lds_any[k & 1][wave] = ch_prev ? 1 : 0;
+        __syncthreads();
+        int any_prev = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) any_prev |= lds_any[k & 1][w];
+        if (!any_prev) break;
+        const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : 0u;
+        const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : 0u;
+#pragma unroll
+        for (int r = R - 1; r >= 1; --r) {
+            const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], cxm0, cym0 + r);
+            const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], cxm1, cym0 + r);
+            ch |= t0 | t1;
+            if (ESC && r == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+        }
+        {
+            const bool t0 = relax_key(I0[0], D0[0], top0, cxm0, cym0);
+            const bool t1 = relax_key(I1[0], D1[0], top1, cxm1, cym0);
+            ch |= t0 | t1;
+            if (ESC && 0 == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+        }
+        // pass 4: source y+1
+        ldsB[wave][lane][0] = I0[0];
+        ldsB[wave][lane][1] = I1[0];
+        __syncthreads();
+        const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : 0u;
+        const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : 0u;
+#pragma unroll
+        for (int r = 0; r < R - 1; ++r) {
+            const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], cxm0, cym0 + r);
+            const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], cxm1, cym0 + r);
+            ch |= t0 | t1;
+            if (ESC && r == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+        }
+        {
+            const bool t0 = relax_key(I0[R - 1], D0[R - 1], bot0, cxm0, cym0 + R - 1);
+            const bool t1 = relax_key(I1[R - 1], D1[R - 1], bot1, cxm1, cym0 + R - 1);
+            ch |= t0 | t1;
+            if (ESC && (R - 1) == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+        }
+        ch_prev = __ballot(ch) != 0ull;
+    }
+}
+
+template <int NW, int R, int K>
+__global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
+    const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
+    const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
+    int* __restrict__ counters)
+{
+    constexpr int IW = 128 - 2 * K;
+    constexpr int IH = NW * R - 2 * K;
+    __shared__ uint32_t ldsA[NW][64][2];
+    __shared__ uint32_t ldsB[NW][64][2];
+    __shared__ int lds_any[2][NW];
+
+    // Contiguous tile ranges per XCD: consecutive dispatches go round-robin over
+    // the 8 XCDs, so give each XCD a contiguous slice of tiles (neighbouring
+    // tiles share halo lines in the same L2).  The grid is launched as a
+    // multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
+    const int per_xcd = gridDim.x / 8;
+    const int b = blockIdx.x;
+    const int tile = (b % 8) * per_xcd + b / 8;
+    if (tile >= total_tiles) return;
+
+    int ci = 0;
+    if (n_canvas > 1) {
+        int lo = 0, hi = n_canvas - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cvs[mid].tile_base <= tile) lo = mid; else hi = mid - 1;
+        }
+        ci = lo;
+    }
+    const BfsCanvas& cv = cvs[ci];
+    const int W = cv.W, H = cv.H;
+    const int tiles_x = cv.tiles_x, tiles_y = cv.tiles_y, tile_base = cv.tile_base;
+    const int local = tile - tile_base;
+    const int tx = local % tiles_x, ty = local / tiles_x;
+    const int tid = threadIdx.x;
+    // The wave index is uniform; readfirstlane lets the compiler keep every
+    // row-dependent quantity in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+
+    // Activity: this tile or one of its 8 neighbours changed last launch.
+    // Every wave reads the nine flags itself, so no barrier is needed.
+    if (!first_launch) {
+        int f = 0;
+        if (lane < 9) {
+            const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
+            if (nx >= 0 && ny >= 0 && nx < tiles_x && ny < tiles_y)
+                f = flags_prev[tile_base + ny * tiles_x + nx];
+        }
+        if (__ballot(f != 0) == 0ull) {
+            if (tid == 0) flags_cur[tile] = 0;
+            return;
+        }
+    }
+
+    const int ox = tx * IW, oy = ty * IH;   // interior origin
+    const int c0 = ox - K + 2 * lane;       // canvas column of this thread's first column
+    const int y0 = oy - K + wave * R;       // canvas row of this thread's first row
+    const GU32* src = (const GU32*)(src_plane ? cv.lab1 : cv.lab0);
+    GU32* dst = (GU32*)(src_plane ? cv.lab0 : cv.lab1);
+
+    // Per-cell state: ~label and squared distance (see relax_key).  Loads use
+    // clamped addresses and selects, so no per-row lane masks stay live.
+    uint32_t I0[R], I1[R], D0[R], D1[R];
+    const bool in0 = c0 >= 0 && c0 < W;
+    const bool in1 = c0 + 1 >= 0 && c0 + 1 < W;
+    const int cc0 = min(max(c0, 0), W - 1);
+    const int cc1 = min(max(c0 + 1, 0), W - 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = y0 + r;
+        if (y >= 0 && y < H) {   // wave-uniform
+            const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
+            const uint32_t a = src[rowoff + cc0];
+            const uint32_t c = src[rowoff + cc1];
+            I0[r] = in0 ? ~a : 0u;
+            I1[r] = in1 ? ~c : 0u;
+            const uint32_t da = a == kInvalid ? 0xFFFFFFFFu : static_cast<uint32_t>(dist2_lab(c0, y, a));
+            const uint32_t dc = c == kInvalid ? 0xFFFFFFFFu : static_cast<uint32_t>(dist2_lab(c0 + 1, y, c));
+            D0[r] = in0 ? da : 0u;
+            D1[r] = in1 ? dc : 0u;
+        } else {
+            I0[r] = 0u;
+            I1[r] = 0u;
+            D0[r] = 0u;
+            D1[r] = 0u;
+        }
+    }
+
+    int esc = 0;
+    const int P = cv.P;
+    const int cxm0 = c0 - 0xFFFF, cym0 = y0 - 0xFFFF;
+    // Does this tile's region touch the interior boundary of the image?
+    const bool esc_tile =
+        P > 0 &&
+        ((ox - K <= P && P < ox + IW + K) || (ox - K <= W - P - 1 && W - P - 1 < ox + IW + K) ||
+         (oy - K <= P && P < oy + IH + K) || (oy - K <= H - P - 1 && H - P - 1 < oy + IH + K));
+    if (esc_tile) {
+        EscGeom e;
+        const int c1 = c0 + 1;
+        e.col0_in = c0 >= P && c0 < W - P;
+        e.col1_in = c1 >= P && c1 < W - P;
+        e.col0_left = c0 == P;
+        e.col1_left = c1 == P;
+        e.col0_right = c0 == W - P - 1;
+        e.col1_right = c1 == W - P - 1;
+        e.r_top = P - y0;
+        e.r_bot = H - P - 1 - y0;
+        e.rows_in = 0;
+        for (int r = 0; r < R; ++r) {
+            const int y = y0 + r;
+            if (y >= P && y < H - P) e.rows_in |= 1u << r;
+        }
+        iterate<NW, R, K, true>(I0, I1, D0, D1, cxm0, cym0, e, ldsA, ldsB, lds_any, wave, lane,
+                                esc);
+    } else {
+        EscGeom e{};
+        iterate<NW, R, K, false>(I0, I1, D0, D1, cxm0, cym0, e, ldsA, ldsB, lds_any, wave, lane,
+                                 esc);
+    }
+
+    // Write the interior back and note whether it changed.  The row origin is
+    // passed through an empty asm statement so that the compiler recomputes
+    // the row addresses here instead of keeping the load-time ones live in
+    // scalar registers across the whole fill loop.
+    int yb = y0;
+    int Wb = W;
+    asm volatile("" : "+s"(yb), "+s"(Wb));
+    bool changed = false;
+    const int lx0 = 2 * lane - K;   // interior-relative column of column 0
+    const bool w0 = lx0 >= 0 && lx0 < IW && c0 < Wb;
+    const bool w1 = lx0 + 1 >= 0 && lx0 + 1 < IW && c0 + 1 < Wb;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ly = wave * R + r - K;
+        const int y = yb + r;
+        if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
+        const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
+        if (w0) {
+            const size_t i = rowoff + c0;
+            const uint32_t v = ~I0[r];
+            changed |= src[i] != v;
+            dst[i] = v;
+        }
+        if (w1) {
+            const size_t i = rowoff + c0 + 1;
+            const uint32_t v = ~I1[r];
+            changed |= src[i] != v;
+            dst[i] = v;
+        }
+    }
+    if (esc && cv.escape) atomicOr(cv.escape, 1);
+    __syncthreads();   // every wave is past its last read of lds_any
+    if (lane == 0) lds_any[0][wave] = __ballot(changed) != 0ull ? 1 : 0;
+    __syncthreads();
+    if (tid == 0) {
+        int any = 0;
+        for (int w = 0; w < NW; ++w) any |= lds_any[0][w];
+        flags_cur[tile] = any;
+        atomicAdd(&counters[1], 1);
+        if (any) atomicAdd(&counters[0], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Simple version: one thread per cell, one pass per launch, ping-pong planes.
+// ---------------------------------------------------------------------------
+__global__ void bfs_pass_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                int W, int H, int dir, int* __restrict__ changed)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const size_t c = static_cast<size_t>(y) * W + x;
+    uint32_t lab = src[c];
+    int d = lab == kInvalid ? INT_MAX : dist2_lab(x, y, lab);
+    int sx = x, sy = y;
+    if (dir == 0) sx = x - 1;
+    else if (dir == 1) sx = x + 1;
+    else if (dir == 2) sy = y - 1;
+    else sy = y + 1;
+    if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+        const uint32_t nl = src[static_cast<size_t>(sy) * W + sx];
+        if (relax(lab, d, nl, x, y)) *changed = 1;
+    }
+    dst[c] = lab;
+}
+
+__global__ void bfs_seed_kernel(uint32_t* __restrict__ a, uint32_t* __restrict__ b, int W,
+                                const uint32_t* __restrict__ labels, int64_t n)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t l = labels[i];
+    const size_t c = static_cast<size_t>(l & 0xFFFFu) * W + (l >> 16);
+    a[c] = l;
+    b[c] = l;
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n)
+{
+    size_t i = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x * 4;
+    for (; i < n; i += stride) {
+        if (i + 4 <= n && (reinterpret_cast<uintptr_t>(p + i) & 15) == 0) {
+            *reinterpret_cast<uint4*>(p + i) = make_uint4(v, v, v, v);
+        } else {
+            for (size_t j = i; j < n && j < i + 4; ++j) p[j] = v;
+        }
+    }
+}
+
+__global__ void zero_ints_kernel(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*);
+
+struct Variant {
+    int waves, rows, iters;
+    TileKernel fn;
+};
+
+// Instantiated configurations.  The tile is 128 wide; height waves*rows.
+const Variant kVariants[] = {
+    {16, 8, 8, bfs_tile_kernel<16, 8, 8>},   {16, 8, 4, bfs_tile_kernel<16, 8, 4>},
+    {16, 8, 12, bfs_tile_kernel<16, 8, 12>}, {16, 8, 16, bfs_tile_kernel<16, 8, 16>},
+    {8, 8, 4, bfs_tile_kernel<8, 8, 4>},     {8, 8, 8, bfs_tile_kernel<8, 8, 8>},
+    {8, 16, 8, bfs_tile_kernel<8, 16, 8>},   {16, 4, 4, bfs_tile_kernel<16, 4, 4>},
+    {16, 4, 8, bfs_tile_kernel<16, 4, 8>},
+};
+
+const Variant* find_variant(const BfsConfig& c)
+{
+    for (const auto& v : kVariants)
+        if (v.waves == c.waves && v.rows == c.rows && v.iters == c.iters) return &v;
+    return nullptr;
+}
+
+}  // namespace
+
+bool bfs_config_supported(const BfsConfig& c) { return find_variant(c) != nullptr; }
+
+void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    const size_t threads = (n + 3) / 4;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((threads + 255) / 256, 65536));
+    fill_u32_kernel<<<blocks, 256, 0, s>>>(p, v, n);
+    DL_LAUNCH_CHECK();
+}
+
+void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n, hipStream_t s)
+{
+    const size_t cells = static_cast<size_t>(cv.W) * static_cast<size_t>(cv.H);
+    fill_u32(cv.lab0, kInvalid, cells, s);
+    fill_u32(cv.lab1, kInvalid, cells, s);
+    if (n > 0) {
+        bfs_seed_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(cv.lab0, cv.lab1,
+                                                                              cv.W, d_labels, n);
+        DL_LAUNCH_CHECK();
+    }
+}
+
+int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+            hipStream_t s, BfsStats* stats)
+{
+    const Variant* v = find_variant(cfg);
+    if (!v) throw runtime_error("unsupported flood-fill tile configuration");
+    const int iw = bfs_tile_iw(cfg), ih = bfs_tile_ih(cfg);
+    int total = 0;
+    int64_t max_iters = 64;
+    for (auto& c : canvases) {
+        c.tiles_x = (c.W + iw - 1) / iw;
+        c.tiles_y = (c.H + ih - 1) / ih;
+        c.tile_base = total;
+        total += c.tiles_x * c.tiles_y;
+        max_iters = std::max<int64_t>(max_iters, 8LL * (c.W + c.H) + 64);
+    }
+    const int n_canvas = static_cast<int>(canvases.size());
+    if (total == 0) return 0;
+
+    work.canvases.reserve(sizeof(BfsCanvas) * n_canvas);
+    DL_HIP(hipMemcpyAsync(work.canvases.ptr, canvases.data(), sizeof(BfsCanvas) * n_canvas,
+                          hipMemcpyHostToDevice, s));
+    work.flags.reserve(sizeof(int) * 2 * static_cast<size_t>(total));
+    int* flags = work.flags.as<int>();
+
+    // Launches are issued in chunks; the host reads the changed-tile counter of
+    // a chunk while the next one runs.  Launches after convergence find no
+    // active tile and return at once.
+    constexpr int kChunk = 4;
+    constexpr int kSlots = 2 * kChunk;  // counters for two chunks in flight
+    work.counters.reserve(sizeof(int) * 2 * kSlots);
+    work.host_counters.reserve(sizeof(int) * 2 * kSlots);
+    int* dcnt = work.counters.as<int>();
+    int* hcnt = work.host_counters.as<int>();
+    zero_ints_kernel<<<1, 64, 0, s>>>(dcnt, 2 * kSlots);
+    DL_LAUNCH_CHECK();
+
+    hipEvent_t ev[2];
+    DL_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    DL_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+
+    const int max_launches = static_cast<int>((max_iters + cfg.iters - 1) / cfg.iters) + 2;
+    int launch = 0;
+    int chunk = 0;
+    int final_plane = 0;
+    bool done = false;
+    int pending_chunk = 0;
+    BfsStats st;
+
+    const int grid_blocks = (total + 7) / 8 * 8;
+    auto issue_chunk = [&](int ch) {
+        const int slot0 = (ch % 2) * kChunk;
+        zero_ints_kernel<<<1, 64, 0, s>>>(dcnt + 2 * slot0, 2 * kChunk);
+        for (int j = 0; j < kChunk; ++j) {
+            const int src_plane = launch & 1;
+            int* fprev = flags + ((launch + 1) & 1) * total;
+            int* fcur = flags + (launch & 1) * total;
+            v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
+                reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
+                src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j));
+            DL_LAUNCH_CHECK();
+            ++launch;
+        }
+        DL_HIP(hipMemcpyAsync(hcnt + 2 * slot0, dcnt + 2 * slot0, sizeof(int) * 2 * kChunk,
+                              hipMemcpyDeviceToHost, s));
+        DL_HIP(hipEventRecord(ev[ch % 2], s));
+    };
+
+    issue_chunk(chunk++);
+    pending_chunk = 0;
+    while (!done) {
+        if (launch < max_launches) issue_chunk(chunk++);  // keep one chunk in flight
+        DL_HIP(hipEventSynchronize(ev[pending_chunk % 2]));
+        const int slot0 = (pending_chunk % 2) * kChunk;
+        for (int j = 0; j < kChunk; ++j) {
+            const int changed = hcnt[2 * (slot0 + j)];
+            const int active = hcnt[2 * (slot0 + j) + 1];
+            st.tile_visits += active;
+            const int launch_idx = pending_chunk * kChunk + j;
+            if (changed == 0) {
+                // Launch launch_idx wrote its (unchanged) result into plane
+                // (launch_idx + 1) & 1; both planes agree from here on.
+                final_plane = (launch_idx + 1) & 1;
+                done = true;
+                break;
+            }
+        }
+        if (done) break;
+        if (pending_chunk + 1 >= chunk) {
+            throw runtime_error("flood fill did not converge within the iteration bound");
+        }
+        pending_chunk = pending_chunk + 1;
+    }
+    // Drain whatever is still queued behind the converged launch.
+    DL_HIP(hipStreamSynchronize(s));
+    DL_HIP(hipEventDestroy(ev[0]));
+    DL_HIP(hipEventDestroy(ev[1]));
+    st.launches = launch;
+    if (stats) *stats = st;
+    return final_plane;
+}
+
+int bfs_run_simple(const BfsCanvas& cv, BfsWork& work, hipStream_t s, int64_t* iters_out)
+{
+    work.counters.reserve(sizeof(int) * 2);
+    work.host_counters.reserve(sizeof(int) * 2);
+    int* dc = work.counters.as<int>();
+    int* hc = work.host_counters.as<int>();
+    dim3 grid((cv.W + 255) / 256, cv.H);
+    int plane = 0;
+    int64_t iters = 0;
+    const int64_t bound = 8LL * (cv.W + cv.H) + 64;
+    for (;;) {
+        ++iters;
+        zero_ints_kernel<<<1, 64, 0, s>>>(dc, 1);
+        for (int dir = 0; dir < 4; ++dir) {
+            const uint32_t* src = plane ? cv.lab1 : cv.lab0;
+            uint32_t* dst = plane ? cv.lab0 : cv.lab1;
+            bfs_pass_kernel<<<grid, 256, 0, s>>>(src, dst, cv.W, cv.H, dir, dc);
+            DL_LAUNCH_CHECK();
+            plane ^= 1;
+        }
+        DL_HIP(hipMemcpyAsync(hc, dc, sizeof(int), hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        if (hc[0] == 0) break;
+        if (iters > bound) throw runtime_error("flood fill did not converge");
+    }
+    if (iters_out) *iters_out = iters;
+    return plane;
+}
+
+}  // namespace dl

@@ -1,0 +1,778 @@
+// Seed normalisation, triangle extraction, pixel assignment and output
+// assembly on gfx950.
+//
+// Reference semantics (src/delauney/reference/triangulation.py):
+//  * seed ids are the (x asc, y asc) order (voronoi.py:64-72);
+//  * a 2x2 block with four distinct regions blocks two triplets globally, chosen
+//    by the cyclic order about the corner centroid (triangulation.py:109-144);
+//  * four L-shape scans, each in row-major order, register pairwise-distinct,
+//    unblocked, unseen triplets; the tid is the order of first registration and
+//    the first registration fixes (x, y) and the vertex order (:146-183);
+//  * a pixel takes the largest tid over triangles containing its integer point
+//    in the closed sense, else -1 (:193-204, 226-253).
+//
+// Design: registration order is reproduced without sorting.  Every candidate
+// (scan s, block (rx, ry)) has a unique key s*NB + ry*(Wp-1) + rx, which is its
+// position in the reference's scan order.  A hash table keeps the minimum key
+// of each distinct triplet; a candidate whose key is its triplet's minimum (and
+// whose triplet is not blocked) sets bit `key` of a bitmap, and the tid of that
+// registration is the number of set bits below it -- a popcount prefix scan.
+#include "dl_internal.h"
+
+#include <algorithm>
+#include <climits>
+
+namespace dl {
+
+// ===========================================================================
+// Popcount prefix scan
+// ===========================================================================
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(
+    const uint32_t* __restrict__ w, int64_t n, uint32_t* __restrict__ bsum)
+{
+    __shared__ uint32_t red[kScanThreads / 64];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanBlock;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t i = base + j * kScanThreads + threadIdx.x;
+        if (i < n) s += __popc(w[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < kScanThreads / 64; ++k) t += red[k];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of bsum[0..nb) in place with one 1024-thread block; writes the
+// grand total to *total.
+__global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* __restrict__ bsum, int64_t nb,
+                                                           uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < nb; base += 1024) {
+        const int64_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        const uint32_t incl = part[threadIdx.x];
+        if (i < nb) bsum[i] = carry + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
+    const uint32_t* __restrict__ w, int64_t n, const uint32_t* __restrict__ bsum,
+    uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t part[kScanThreads];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanBlock +
+                         static_cast<int64_t>(threadIdx.x) * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t i = base + j;
+        c[j] = i < n ? static_cast<uint32_t>(__popc(w[i])) : 0u;
+        s += c[j];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = bsum[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t i = base + j;
+        if (i < n) out[i] = run;
+        run += c[j];
+    }
+}
+
+}  // namespace
+
+void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
+{
+    const int64_t nb = std::max<int64_t>(1, (n + kScanBlock - 1) / kScanBlock);
+    tmp.reserve(sizeof(uint32_t) * static_cast<size_t>(nb));
+    uint32_t* bsum = tmp.as<uint32_t>();
+    if (n > 0) {
+        scan_reduce_kernel<<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum);
+        DL_LAUNCH_CHECK();
+    } else {
+        fill_u32(bsum, 0u, 1, s);
+    }
+    scan_blocks_kernel<<<1, 1024, 0, s>>>(bsum, nb, out + n);
+    DL_LAUNCH_CHECK();
+    if (n > 0) {
+        scan_down_kernel<<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum, out);
+        DL_LAUNCH_CHECK();
+    }
+}
+
+// ===========================================================================
+// Seeds
+// ===========================================================================
+namespace {
+
+// Status flag bits shared with the host.
+constexpr uint32_t kFlagXRange = 1u;
+constexpr uint32_t kFlagYRange = 2u;
+constexpr uint32_t kFlagDuplicate = 4u;
+constexpr uint32_t kFlagCandOverflow = 8u;
+constexpr uint32_t kFlagTableOverflow = 16u;
+constexpr uint32_t kFlagCollision = 32u;
+constexpr uint32_t kFlagBadId = 64u;
+
+// Presence bit of (x, y) is x*H + y: bit order is the id order.
+__global__ void seeds_mark_kernel(const int32_t* __restrict__ xy, int64_t n, int W, int H,
+                                  uint32_t* __restrict__ bm, uint32_t* __restrict__ flags)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = xy[2 * i], y = xy[2 * i + 1];
+    if (x < 0 || x >= W) { atomicOr(flags, kFlagXRange); return; }
+    if (y < 0 || y >= H) { atomicOr(flags, kFlagYRange); return; }
+    const uint64_t bit = static_cast<uint64_t>(x) * H + y;
+    const uint32_t m = 1u << (bit & 31);
+    const uint32_t old = atomicOr(&bm[bit >> 5], m);
+    if (old & m) atomicOr(flags, kFlagDuplicate);
+}
+
+__global__ void seeds_rank_kernel(const int32_t* __restrict__ xy, int64_t n, int W, int H,
+                                  const uint32_t* __restrict__ bm, const uint32_t* __restrict__ pre,
+                                  uint32_t* __restrict__ sorted, int32_t* __restrict__ rank_at)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = xy[2 * i], y = xy[2 * i + 1];
+    const uint64_t bit = static_cast<uint64_t>(x) * H + y;
+    const uint32_t word = bm[bit >> 5];
+    const uint32_t below = word & ((1u << (bit & 31)) - 1u);
+    const uint32_t rank = pre[bit >> 5] + static_cast<uint32_t>(__popc(below));
+    sorted[rank] = pack_label(x, y);
+    rank_at[static_cast<size_t>(y) * W + x] = static_cast<int32_t>(rank);
+}
+
+}  // namespace
+
+void normalise_seeds(int W, int H, const int32_t* d_xy, int64_t n, uint32_t* d_sorted,
+                     int32_t* d_rank_at, TriWork& work, hipStream_t s)
+{
+    const int64_t bits = static_cast<int64_t>(W) * H;
+    const int64_t nw = (bits + 31) / 32;
+    work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nw));
+    work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nw + 1));
+    work.flags.reserve(sizeof(uint32_t) * 8);
+    work.host_flags.reserve(sizeof(uint32_t) * 8);
+    uint32_t* bm = work.bitmap.as<uint32_t>();
+    uint32_t* flags = work.flags.as<uint32_t>();
+    fill_u32(bm, 0u, static_cast<size_t>(nw), s);
+    fill_u32(flags, 0u, 8, s);
+    const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+    seeds_mark_kernel<<<blocks, 256, 0, s>>>(d_xy, n, W, H, bm, flags);
+    DL_LAUNCH_CHECK();
+    DL_HIP(hipMemcpyAsync(work.host_flags.ptr, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    const uint32_t f = work.host_flags.as<uint32_t>()[0];
+    if (f & kFlagXRange) throw value_error("seed x coordinate out of bounds");
+    if (f & kFlagYRange) throw value_error("seed y coordinate out of bounds");
+    if (f & kFlagDuplicate) throw value_error("duplicate seed positions are not allowed");
+    popcount_scan(bm, nw, work.scan.as<uint32_t>(), work.scan_tmp, s);
+    seeds_rank_kernel<<<blocks, 256, 0, s>>>(d_xy, n, W, H, bm, work.scan.as<uint32_t>(), d_sorted,
+                                             d_rank_at);
+    DL_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// Detection
+// ===========================================================================
+namespace {
+
+struct Cand {
+    uint32_t key;
+    uint32_t slot;
+    int32_t a, b, c;
+};
+
+constexpr uint64_t kEmptyTag = ~0ull;
+
+__device__ __forceinline__ void sort3(int32_t& a, int32_t& b, int32_t& c)
+{
+    int32_t t;
+    if (a > b) { t = a; a = b; b = t; }
+    if (b > c) { t = b; b = c; c = t; }
+    if (a > b) { t = a; a = b; b = t; }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct Table {
+    unsigned long long* tag;
+    uint32_t* minkey;
+    int32_t* trip;    // 3 per slot
+    uint32_t* blk;
+    uint32_t* used;   // entry count
+    uint64_t mask;
+    uint32_t limit;   // entries allowed before the table counts as full
+    int exact;        // tags are exact packings of the sorted triplet
+    uint64_t salt;
+};
+
+__device__ __forceinline__ uint64_t triplet_tag(const Table& t, int32_t a, int32_t b, int32_t c)
+{
+    // a < b < c
+    if (t.exact)
+        return static_cast<uint64_t>(a) | (static_cast<uint64_t>(b) << 21) |
+               (static_cast<uint64_t>(c) << 42);
+    uint64_t h = mix64(static_cast<uint64_t>(static_cast<uint32_t>(a)) ^ t.salt);
+    h = mix64(h ^ (static_cast<uint64_t>(static_cast<uint32_t>(b)) << 1));
+    h = mix64(h ^ (static_cast<uint64_t>(static_cast<uint32_t>(c)) << 2));
+    return h & 0x7FFFFFFFFFFFFFFFull;
+}
+
+// Returns the slot of the sorted triplet, inserting it if absent; -1 when the
+// table is full (flag raised).
+__device__ int64_t table_insert(const Table& t, int32_t a, int32_t b, int32_t c,
+                                uint32_t* flags)
+{
+    const uint64_t tg = triplet_tag(t, a, b, c);
+    uint64_t h = mix64(tg ^ 0x9E3779B97F4A7C15ull) & t.mask;
+    for (uint64_t probe = 0; probe <= t.mask; ++probe) {
+        const unsigned long long cur = __atomic_load_n(&t.tag[h], __ATOMIC_RELAXED);
+        if (cur == tg) return static_cast<int64_t>(h);
+        if (cur == kEmptyTag) {
+            const unsigned long long prev =
+                atomicCAS(&t.tag[h], static_cast<unsigned long long>(kEmptyTag),
+                          static_cast<unsigned long long>(tg));
+            if (prev == kEmptyTag) {
+                t.trip[3 * h] = a;
+                t.trip[3 * h + 1] = b;
+                t.trip[3 * h + 2] = c;
+                const uint32_t u = atomicAdd(t.used, 1u);
+                if (u >= t.limit) atomicOr(flags, kFlagTableOverflow);
+                return static_cast<int64_t>(h);
+            }
+            if (prev == tg) return static_cast<int64_t>(h);
+        }
+        h = (h + 1) & t.mask;
+    }
+    atomicOr(flags, kFlagTableOverflow);
+    return -1;
+}
+
+// Orientation class of a direction for the reference's arctan2 ordering:
+// y < 0 -> (-pi, 0); y == 0, x >= 0 -> 0; y > 0 -> (0, pi); y == 0, x < 0 -> pi.
+__device__ __forceinline__ int angle_class(int64_t x, int64_t y)
+{
+    if (y < 0) return 0;
+    if (y == 0) return x >= 0 ? 1 : 3;
+    return 2;
+}
+
+// a sorts strictly before b in np.arctan2 order (exact for integer inputs).
+__device__ __forceinline__ bool angle_less(int64_t ax, int64_t ay, int64_t bx, int64_t by)
+{
+    const int ca = angle_class(ax, ay), cb = angle_class(bx, by);
+    if (ca != cb) return ca < cb;
+    if (ca == 1 || ca == 3) return false;
+    return ax * by - ay * bx > 0;
+}
+
+// triangulation.py:115-144 for one quad block.  ids: TL, TR, BL, BR;
+// (px, py): their seed positions.  Writes the two blocked triplets.
+__device__ void quad_rule(const int32_t ids[4], const int64_t px[4], const int64_t py[4],
+                          int32_t out[2][3])
+{
+    const int64_t sx = px[0] + px[1] + px[2] + px[3];
+    const int64_t sy = py[0] + py[1] + py[2] + py[3];
+    int64_t vx[4], vy[4];
+    int order[4];
+    for (int k = 0; k < 4; ++k) {
+        vx[k] = 4 * px[k] - sx;   // 4 * (p - centroid), exact
+        vy[k] = 4 * py[k] - sy;
+        order[k] = k;
+    }
+    for (int i = 1; i < 4; ++i) {   // stable insertion sort
+        const int v = order[i];
+        int j = i - 1;
+        while (j >= 0 && angle_less(vx[v], vy[v], vx[order[j]], vy[order[j]])) {
+            order[j + 1] = order[j];
+            --j;
+        }
+        order[j + 1] = v;
+    }
+    const int d0a = order[0], d0b = order[2], d1a = order[1], d1b = order[3];
+    const int64_t l0 = (px[d0a] - px[d0b]) * (px[d0a] - px[d0b]) + (py[d0a] - py[d0b]) * (py[d0a] - py[d0b]);
+    const int64_t l1 = (px[d1a] - px[d1b]) * (px[d1a] - px[d1b]) + (py[d1a] - py[d1b]) * (py[d1a] - py[d1b]);
+    bool choose0;
+    if (l0 < l1) choose0 = true;
+    else if (l1 < l0) choose0 = false;
+    else {
+        int lowest = 0;
+        for (int k = 1; k < 4; ++k)
+            if (ids[k] < ids[lowest]) lowest = k;
+        choose0 = !(lowest == d0a || lowest == d0b);
+    }
+    const int ca = choose0 ? d0a : d1a, cb = choose0 ? d0b : d1b;
+    const int ra = choose0 ? d1a : d0a, rb = choose0 ? d1b : d0b;
+    out[0][0] = ids[ra]; out[0][1] = ids[rb]; out[0][2] = ids[ca];
+    out[1][0] = ids[ra]; out[1][1] = ids[rb]; out[1][2] = ids[cb];
+}
+
+struct DetectArgs {
+    const uint32_t* grid;
+    int Wp, Hp, P;
+    int labels;              // grid holds labels (else ids)
+    const int32_t* rank_at;  // image-space id map (labels mode)
+    int W_img;
+    const uint32_t* sorted;  // id -> packed image position
+    int64_t n_seeds;
+    Cand* cand;
+    uint32_t cand_cap;
+    uint32_t* cand_count;
+    uint32_t* quad_count;
+    uint32_t* flags;
+    Table table;
+};
+
+__device__ __forceinline__ int32_t to_id(const DetectArgs& A, uint32_t v, uint32_t* flags)
+{
+    if (A.labels) {
+        if (v == kInvalid) {   // an undefined cell: impossible after a converged fill
+            atomicOr(flags, kFlagBadId);
+            return -1;
+        }
+        const int lx = static_cast<int>(v >> 16) - A.P, ly = static_cast<int>(v & 0xFFFFu) - A.P;
+        return A.rank_at[static_cast<size_t>(ly) * A.W_img + lx];
+    }
+    const int32_t id = static_cast<int32_t>(v);
+    if (id < 0 || id >= A.n_seeds) {
+        atomicOr(flags, kFlagBadId);
+        return -1;
+    }
+    return id;
+}
+
+__device__ __forceinline__ void seed_pos(const DetectArgs& A, int32_t id, int64_t& x, int64_t& y)
+{
+    const uint32_t l = A.sorted[id];
+    x = static_cast<int64_t>(l >> 16);
+    y = static_cast<int64_t>(l & 0xFFFFu);
+}
+
+__global__ __launch_bounds__(256) void detect_kernel(DetectArgs A)
+{
+    const int bw = A.Wp - 1;
+    const int64_t nb = static_cast<int64_t>(bw) * (A.Hp - 1);
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int32_t tri[4][3];
+    bool valid[4] = {false, false, false, false};
+    uint32_t slot[4] = {0, 0, 0, 0};
+    int rx = 0, ry = 0;
+    if (i < nb) {
+        ry = static_cast<int>(i / bw);
+        rx = static_cast<int>(i - static_cast<int64_t>(ry) * bw);
+        const size_t o = static_cast<size_t>(ry) * A.Wp + rx;
+        const uint32_t g_tl = A.grid[o], g_tr = A.grid[o + 1];
+        const uint32_t g_bl = A.grid[o + A.Wp], g_br = A.grid[o + A.Wp + 1];
+        // At least three distinct values are needed for any triangle.
+        const int distinct = 1 + (g_tr != g_tl) + (g_bl != g_tl && g_bl != g_tr) +
+                             (g_br != g_tl && g_br != g_tr && g_br != g_bl);
+        if (distinct >= 3) {
+            const int32_t tl = to_id(A, g_tl, A.flags), tr = to_id(A, g_tr, A.flags);
+            const int32_t bl = to_id(A, g_bl, A.flags), br = to_id(A, g_br, A.flags);
+            if (tl >= 0 && tr >= 0 && bl >= 0 && br >= 0) {
+                // scan order triangulation.py:174,177,180,183
+                const int32_t s[4][3] = {{tl, tr, br}, {tl, tr, bl}, {bl, br, tl}, {bl, br, tr}};
+                for (int k = 0; k < 4; ++k) {
+                    tri[k][0] = s[k][0];
+                    tri[k][1] = s[k][1];
+                    tri[k][2] = s[k][2];
+                    valid[k] = s[k][0] != s[k][1] && s[k][0] != s[k][2] && s[k][1] != s[k][2];
+                }
+                if (distinct == 4) {
+                    const int32_t ids[4] = {tl, tr, bl, br};
+                    int64_t px[4], py[4];
+                    for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
+                    int32_t blk[2][3];
+                    quad_rule(ids, px, py, blk);
+                    atomicAdd(A.quad_count, 1u);
+                    for (int j = 0; j < 2; ++j) {
+                        int32_t a = blk[j][0], b = blk[j][1], c = blk[j][2];
+                        sort3(a, b, c);
+                        const int64_t sl = table_insert(A.table, a, b, c, A.flags);
+                        if (sl >= 0) A.table.blk[sl] = 1u;
+                    }
+                }
+                for (int k = 0; k < 4; ++k) {
+                    if (!valid[k]) continue;
+                    int32_t a = tri[k][0], b = tri[k][1], c = tri[k][2];
+                    sort3(a, b, c);
+                    const int64_t sl = table_insert(A.table, a, b, c, A.flags);
+                    if (sl < 0) {
+                        valid[k] = false;
+                        continue;
+                    }
+                    slot[k] = static_cast<uint32_t>(sl);
+                    const uint32_t key = static_cast<uint32_t>(k) * static_cast<uint32_t>(nb) +
+                                         static_cast<uint32_t>(i);
+                    atomicMin(&A.table.minkey[sl], key);
+                }
+            }
+        }
+    }
+    // Wave-aggregated append of this thread's candidates.
+    const int lane = threadIdx.x & 63;
+    uint32_t cnt[4];
+    uint64_t bal[4];
+    uint32_t total = 0;
+    for (int k = 0; k < 4; ++k) {
+        bal[k] = __ballot(valid[k]);
+        cnt[k] = static_cast<uint32_t>(__popcll(bal[k]));
+        total += cnt[k];
+    }
+    if (total == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(A.cand_count, total);
+    base = __shfl(base, 0, 64);
+    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t off = base;
+    for (int k = 0; k < 4; ++k) {
+        if (valid[k]) {
+            const uint32_t pos = off + static_cast<uint32_t>(__popcll(bal[k] & lower));
+            if (pos < A.cand_cap) {
+                Cand cd;
+                cd.key = static_cast<uint32_t>(k) * static_cast<uint32_t>(nb) + static_cast<uint32_t>(i);
+                cd.slot = slot[k];
+                cd.a = tri[k][0];
+                cd.b = tri[k][1];
+                cd.c = tri[k][2];
+                A.cand[pos] = cd;
+            } else {
+                atomicOr(A.flags, kFlagCandOverflow);
+            }
+        }
+        off += cnt[k];
+    }
+}
+
+__device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
+{
+    return t.minkey[cd.slot] == cd.key && t.blk[cd.slot] == 0u;
+}
+
+__global__ void mark_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ count,
+                            uint32_t cap, Table t, uint32_t* __restrict__ bm,
+                            uint32_t* __restrict__ flags)
+{
+    const uint32_t n = min(*count, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Cand cd = cand[i];
+    int32_t a = cd.a, b = cd.b, c = cd.c;
+    sort3(a, b, c);
+    if (t.trip[3 * cd.slot] != a || t.trip[3 * cd.slot + 1] != b || t.trip[3 * cd.slot + 2] != c) {
+        atomicOr(flags, kFlagCollision);
+        return;
+    }
+    if (is_first(cd, t)) atomicOr(&bm[cd.key >> 5], 1u << (cd.key & 31));
+}
+
+__global__ void emit_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ count,
+                            uint32_t cap, Table t, const uint32_t* __restrict__ bm,
+                            const uint32_t* __restrict__ pre, int Wp, uint32_t nb, int P,
+                            int32_t* __restrict__ rows)
+{
+    const uint32_t n = min(*count, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Cand cd = cand[i];
+    if (!is_first(cd, t)) return;
+    const uint32_t w = cd.key >> 5;
+    const uint32_t tid = pre[w] + static_cast<uint32_t>(__popc(bm[w] & ((1u << (cd.key & 31)) - 1u)));
+    const uint32_t scan = cd.key / nb;
+    const uint32_t rem = cd.key - scan * nb;
+    const int bw = Wp - 1;
+    const int ry = static_cast<int>(rem / static_cast<uint32_t>(bw));
+    const int rx = static_cast<int>(rem - static_cast<uint32_t>(ry) * bw);
+    // detection pixel offsets of the four scans (triangulation.py:174-183)
+    const int gx = rx + ((scan == 0 || scan == 3) ? 1 : 0);
+    const int gy = ry + ((scan >= 2) ? 1 : 0);
+    int32_t* r = rows + 5 * static_cast<size_t>(tid);
+    r[0] = gx - P;
+    r[1] = gy - P;
+    r[2] = cd.a;
+    r[3] = cd.b;
+    r[4] = cd.c;
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+uint64_t next_pow2(uint64_t v)
+{
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp, int P,
+                              const int32_t* rank_at, int W_img, const uint32_t* sorted,
+                              int64_t n_seeds, TriWork& work, hipStream_t s)
+{
+    DetectResult res;
+    const int64_t nb = (Wp >= 2 && Hp >= 2) ? static_cast<int64_t>(Wp - 1) * (Hp - 1) : 0;
+    work.flags.reserve(sizeof(uint32_t) * 8);
+    work.host_flags.reserve(sizeof(uint32_t) * 8);
+    uint32_t* dflags = work.flags.as<uint32_t>();
+    uint32_t* hflags = work.host_flags.as<uint32_t>();
+    if (nb == 0) {
+        work.rows.reserve(sizeof(int32_t) * 5);
+        return res;
+    }
+    // flags layout: [0] status bits, [1] candidate count, [2] quad count,
+    // [3] table entries.
+    uint64_t cand_cap = std::max<uint64_t>(16 * static_cast<uint64_t>(n_seeds) + 65536, 1 << 16);
+    cand_cap = std::min<uint64_t>(cand_cap, 4 * static_cast<uint64_t>(nb));
+    cand_cap = std::min<uint64_t>(cand_cap, 0xFFFFFFF0ull);
+    uint64_t salt = 0x243F6A8885A308D3ull;
+    const int exact = n_seeds <= (1 << 21) ? 1 : 0;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        const uint64_t table_cap = next_pow2(2 * cand_cap + 1024);
+        work.cand.reserve(sizeof(Cand) * cand_cap);
+        work.table_tag.reserve(sizeof(uint64_t) * table_cap);
+        work.table_key.reserve(sizeof(uint32_t) * table_cap);
+        work.table_trip.reserve(sizeof(int32_t) * 3 * table_cap);
+        work.table_blk.reserve(sizeof(uint32_t) * table_cap);
+        Table t;
+        t.tag = work.table_tag.as<unsigned long long>();
+        t.minkey = work.table_key.as<uint32_t>();
+        t.trip = work.table_trip.as<int32_t>();
+        t.blk = work.table_blk.as<uint32_t>();
+        t.used = dflags + 3;
+        t.mask = table_cap - 1;
+        t.limit = static_cast<uint32_t>(std::min<uint64_t>(table_cap * 3 / 4, 0xFFFFFFFFull));
+        t.exact = exact;
+        t.salt = salt;
+        fill_u32(reinterpret_cast<uint32_t*>(t.tag), 0xFFFFFFFFu, 2 * table_cap, s);
+        fill_u32(t.minkey, 0xFFFFFFFFu, table_cap, s);
+        fill_u32(t.blk, 0u, table_cap, s);
+        zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
+        DL_LAUNCH_CHECK();
+
+        DetectArgs A;
+        A.grid = grid;
+        A.Wp = Wp;
+        A.Hp = Hp;
+        A.P = P;
+        A.labels = labels ? 1 : 0;
+        A.rank_at = rank_at;
+        A.W_img = W_img;
+        A.sorted = sorted;
+        A.n_seeds = n_seeds;
+        A.cand = work.cand.as<Cand>();
+        A.cand_cap = static_cast<uint32_t>(cand_cap);
+        A.cand_count = dflags + 1;
+        A.quad_count = dflags + 2;
+        A.flags = dflags;
+        A.table = t;
+        detect_kernel<<<static_cast<unsigned>((nb + 255) / 256), 256, 0, s>>>(A);
+        DL_LAUNCH_CHECK();
+
+        const int64_t nbits = 4 * nb;
+        const int64_t nwords = (nbits + 31) / 32;
+        work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords));
+        work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords + 1));
+        uint32_t* bm = work.bitmap.as<uint32_t>();
+        uint32_t* pre = work.scan.as<uint32_t>();
+        fill_u32(bm, 0u, static_cast<size_t>(nwords), s);
+        DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        uint32_t f = hflags[0];
+        const uint32_t ncand = hflags[1];
+        if (f & kFlagBadId) {
+            if (labels) throw runtime_error("flood fill left an undefined cell");
+            throw value_error("voronoi_grid holds a seed id outside [0, n_seeds)");
+        }
+        if (f & (kFlagCandOverflow | kFlagTableOverflow)) {
+            cand_cap = std::min<uint64_t>(std::max<uint64_t>(cand_cap * 2, ncand + 1024), 0xFFFFFFF0ull);
+            continue;
+        }
+        const unsigned cblocks = static_cast<unsigned>(std::max<uint64_t>(1, (ncand + 255) / 256));
+        mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, dflags);
+        DL_LAUNCH_CHECK();
+        popcount_scan(bm, nwords, pre, work.scan_tmp, s);
+        DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        DL_HIP(hipMemcpyAsync(hflags + 4, pre + nwords, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        f = hflags[0];
+        if (f & kFlagCollision) {
+            salt = salt * 6364136223846793005ull + 1442695040888963407ull;
+            continue;
+        }
+        const uint32_t T = hflags[4];
+        work.rows.reserve(sizeof(int32_t) * 5 * std::max<uint64_t>(T, 1));
+        emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, pre, Wp,
+                                            static_cast<uint32_t>(nb), P, work.rows.as<int32_t>());
+        DL_LAUNCH_CHECK();
+        res.n_tri = T;
+        res.n_quads = hflags[2];
+        res.n_cand = ncand;
+        return res;
+    }
+    throw runtime_error("triangle registration table could not be built");
+}
+
+// ===========================================================================
+// Pixel assignment
+// ===========================================================================
+namespace {
+
+__device__ __forceinline__ int64_t floordiv64(int64_t a, int64_t b)  // b > 0
+{
+    int64_t q = a / b;
+    if ((a % b) != 0 && a < 0) --q;
+    return q;
+}
+
+__global__ void assign_kernel(int W, int H, const int32_t* __restrict__ rows, int64_t T,
+                              const uint32_t* __restrict__ sorted, int32_t* __restrict__ tid)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int32_t* r = rows + 5 * t;
+    int64_t vx[3], vy[3];
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t l = sorted[r[2 + k]];
+        vx[k] = static_cast<int64_t>(l >> 16);
+        vy[k] = static_cast<int64_t>(l & 0xFFFFu);
+    }
+    const int32_t id = static_cast<int32_t>(t);
+    // doubled signed area = sum of the three edge functions
+    const int64_t S = (vx[1] - vx[0]) * (vy[2] - vy[0]) - (vy[1] - vy[0]) * (vx[2] - vx[0]);
+    if (S == 0) {
+        // Distinct collinear vertices contain exactly the lattice points of
+        // their line (triangulation.py:247-253 with all three products zero).
+        const int64_t dx = vx[1] - vx[0], dy = vy[1] - vy[0];
+        if (dy == 0) {
+            if (vy[0] >= 0 && vy[0] < H)
+                for (int x = 0; x < W; ++x) atomicMax(&tid[static_cast<size_t>(vy[0]) * W + x], id);
+            return;
+        }
+        for (int y = 0; y < H; ++y) {
+            const int64_t num = (y - vy[0]) * dx;
+            if (num % dy != 0) continue;
+            const int64_t x = vx[0] + num / dy;
+            if (x >= 0 && x < W) atomicMax(&tid[static_cast<size_t>(y) * W + x], id);
+        }
+        return;
+    }
+    const int64_t sg = S > 0 ? 1 : -1;
+    int64_t ymin = min(vy[0], min(vy[1], vy[2])), ymax = max(vy[0], max(vy[1], vy[2]));
+    ymin = max<int64_t>(ymin, 0);
+    ymax = min<int64_t>(ymax, H - 1);
+    for (int64_t y = ymin; y <= ymax; ++y) {
+        int64_t lo = 0, hi = W - 1;
+        for (int e = 0; e < 3; ++e) {
+            const int n = e == 2 ? 0 : e + 1;
+            // edge function of (A, B) at (px, y): px*(Ay-By) + Ax*(By-y) - Bx*(Ay-y)
+            const int64_t al = sg * (vy[e] - vy[n]);
+            const int64_t be = sg * (vx[e] * (vy[n] - y) - vx[n] * (vy[e] - y));
+            if (al > 0) lo = max(lo, -floordiv64(be, al));
+            else if (al < 0) hi = min(hi, floordiv64(be, -al));
+            else if (be < 0) { lo = 1; hi = 0; }
+        }
+        int32_t* row = tid + static_cast<size_t>(y) * W;
+        for (int64_t x = lo; x <= hi; ++x) atomicMax(&row[x], id);
+    }
+}
+
+// channels: (rank(owner), d^2[, tid]).  Output pixel (x, y) reads canvas cell
+// (x + off, y + off); the owner's image position is its canvas position - shift.
+template <int C>
+__global__ void output_kernel(int W, int H, const uint32_t* __restrict__ lab, int Wc, int off,
+                              int shift, const int32_t* __restrict__ rank_at, int W_img,
+                              const int32_t* __restrict__ tid, int32_t* __restrict__ out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const uint32_t l = lab[static_cast<size_t>(y + off) * Wc + (x + off)];
+    const int lx = static_cast<int>(l >> 16), ly = static_cast<int>(l & 0xFFFFu);
+    const int dx = x + off - lx, dy = y + off - ly;
+    const size_t p = static_cast<size_t>(y) * W + x;
+    int32_t* o = out + C * p;
+    if (l == kInvalid) {   // undefined cell (cannot occur after a converged fill)
+        o[0] = -1;
+        o[1] = -1;
+    } else {
+        o[0] = rank_at[static_cast<size_t>(ly - shift) * W_img + (lx - shift)];
+        o[1] = dx * dx + dy * dy;
+    }
+    if (C == 3) o[2] = tid[p];
+}
+
+}  // namespace
+
+void assign_pixels(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
+                   int32_t* tidgrid, hipStream_t s)
+{
+    fill_u32(reinterpret_cast<uint32_t*>(tidgrid), 0xFFFFFFFFu, static_cast<size_t>(W) * H, s);
+    if (n_tri == 0) return;
+    assign_kernel<<<static_cast<unsigned>((n_tri + 127) / 128), 128, 0, s>>>(W, H, rows, n_tri,
+                                                                             sorted, tidgrid);
+    DL_LAUNCH_CHECK();
+}
+
+void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, int shift,
+                  const int32_t* rank_at, int W_img, const int32_t* tidgrid, int32_t* out, int C,
+                  hipStream_t s)
+{
+    dim3 grid((W + 255) / 256, H);
+    if (C == 3)
+        output_kernel<3><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, rank_at, W_img,
+                                              tidgrid, out);
+    else
+        output_kernel<2><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, rank_at, W_img,
+                                              tidgrid, out);
+    DL_LAUNCH_CHECK();
+}
+
+}  // namespace dl

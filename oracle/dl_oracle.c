@@ -1,0 +1,502 @@
+/*
+ * dl_oracle.c -- CPU restatement of the `delauney.reference` NumPy pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (integer_delauney_amd/)
+ * links, loads or calls this file; only tests/, the smoke check in
+ * __graft_entry__.py and the cpu_baseline leg of bench.py use it, and only as
+ * the checker.
+ *
+ * Each function restates one step of the reference and cites it:
+ *
+ *   orc_sort_seeds        src/delauney/reference/voronoi.py:54-72
+ *                         (validation, np.lexsort((y, x)) id order, duplicates)
+ *   orc_voronoi           src/delauney/reference/voronoi.py:76-163
+ *                         (init to -1 / seed cells (id,0), then iterations of
+ *                          four in-place directional Jacobi passes with sources
+ *                          x-1, x+1, y-1, y+1, stop after the first iteration
+ *                          in which no pass changed a cell)
+ *   orc_detect            src/delauney/reference/triangulation.py:109-183
+ *                         (global cocircular-quad blocking with the float64
+ *                          arctan2 cyclic order, four L-shape scans in
+ *                          row-major order, first-registration dedup)
+ *   orc_assign_brute      src/delauney/reference/triangulation.py:193-204,
+ *                         226-253 (every pixel against every triangle, closed
+ *                          containment, max tid wins)
+ *   orc_assign_raster     the same function computed per triangle by exact
+ *                         scanline intervals; it is checked against
+ *                         orc_assign_brute and exists only so that the oracle
+ *                         can answer at sizes where O(H*W*T) is infeasible.
+ *
+ * Pinning (how each function is checked, see tests/test_oracle.py):
+ *   orc_sort_seeds     reference KATs test_voronoi_ref.py:56-67,106-108,153-159
+ *   orc_voronoi        golden vectors written by importing the reference in the
+ *                      build container (tests/golden/make_golden.py), closed
+ *                      forms (single seed), the reference's tie KATs, and
+ *                      brute-force nearest distance on random grids
+ *   orc_detect         golden vectors (tid order, (x,y), quads), reference KATs
+ *                      test_triangulation_ref.py:32-102,215-262
+ *   orc_assign_*       brute-force definition test_pixel_assignment_correctness
+ *                      .py:16-32 and golden vectors
+ *
+ * Integer arithmetic: squared distances are int32 exactly as NumPy computes
+ * them (the caller keeps (W-1)^2+(H-1)^2 < 2^31).  The containment predicate
+ * uses int64; the reference evaluates it in float64 on integer coordinates
+ * below 2^26, where float64 is exact, so the two agree bit for bit.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Seeds                                                                     */
+/* ------------------------------------------------------------------------ */
+
+static int cmp_xy(const void* pa, const void* pb)
+{
+    const int32_t* a = (const int32_t*)pa;
+    const int32_t* b = (const int32_t*)pb;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
+    return 0;
+}
+
+/* Returns 0 on success, 1 empty, 2 x out of bounds, 3 y out of bounds,
+ * 4 duplicate.  out_sorted receives (x,y) pairs in id order.
+ * voronoi.py:54-72 (the x check precedes the y check there too). */
+int orc_sort_seeds(int W, int H, const int32_t* xy, int64_t n, int32_t* out_sorted)
+{
+    if (n <= 0) return 1;
+    for (int64_t i = 0; i < n; ++i)
+        if (xy[2 * i] < 0 || xy[2 * i] >= W) return 2;
+    for (int64_t i = 0; i < n; ++i)
+        if (xy[2 * i + 1] < 0 || xy[2 * i + 1] >= H) return 3;
+    memcpy(out_sorted, xy, (size_t)n * 2 * sizeof(int32_t));
+    qsort(out_sorted, (size_t)n, 2 * sizeof(int32_t), cmp_xy);
+    for (int64_t i = 1; i < n; ++i)
+        if (out_sorted[2 * i] == out_sorted[2 * i - 2] &&
+            out_sorted[2 * i + 1] == out_sorted[2 * i - 1])
+            return 4;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Voronoi flood fill                                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t *id, *d, *ox, *oy;
+} vstate;
+
+/* One cell update of voronoi.py:174-196: cell c receives the neighbour nb's
+ * owner if the cell is undefined, or the candidate is strictly closer, or it
+ * ties and the neighbour's id is higher. */
+static inline int relax(vstate* s, size_t c, size_t nb, int32_t px, int32_t py)
+{
+    const int32_t nid = s->id[nb];
+    if (nid < 0) return 0;
+    const int32_t dx = px - s->ox[nb];
+    const int32_t dy = py - s->oy[nb];
+    const int32_t cand = dx * dx + dy * dy;
+    const int32_t cid = s->id[c];
+    if (cid == -1 || cand < s->d[c] || (cand == s->d[c] && nid > cid)) {
+        s->id[c] = nid;
+        s->d[c] = cand;
+        s->ox[c] = s->ox[nb];
+        s->oy[c] = s->oy[nb];
+        return 1;
+    }
+    return 0;
+}
+
+/* sorted_xy: n (x,y) pairs already in id order.  out_hw2: (H,W,2) int32.
+ * *iters_out: number of iterations run, including the final unchanged one.
+ * The Jacobi snapshot of each pass is realised by visiting cells against the
+ * propagation direction, so a cell's source has not yet been updated in the
+ * current pass when the cell reads it. */
+int orc_voronoi(int W, int H, const int32_t* sorted_xy, int64_t n,
+                int32_t* out_hw2, int64_t* iters_out)
+{
+    const size_t N = (size_t)W * (size_t)H;
+    vstate s;
+    s.id = (int32_t*)malloc(N * sizeof(int32_t));
+    s.d = (int32_t*)malloc(N * sizeof(int32_t));
+    s.ox = (int32_t*)calloc(N, sizeof(int32_t));
+    s.oy = (int32_t*)calloc(N, sizeof(int32_t));
+    if (!s.id || !s.d || !s.ox || !s.oy) {
+        free(s.id); free(s.d); free(s.ox); free(s.oy);
+        return -1;
+    }
+    for (size_t i = 0; i < N; ++i) { s.id[i] = -1; s.d[i] = -1; }
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t x = sorted_xy[2 * i], y = sorted_xy[2 * i + 1];
+        const size_t c = (size_t)y * W + x;
+        s.id[c] = (int32_t)i;
+        s.d[c] = 0;
+        s.ox[c] = x;
+        s.oy[c] = y;
+    }
+    int64_t iters = 0;
+    for (;;) {
+        int changed = 0;
+        ++iters;
+        /* source x-1 (voronoi.py:157-160) */
+        for (int y = 0; y < H; ++y)
+            for (int x = W - 1; x >= 1; --x) {
+                const size_t c = (size_t)y * W + x;
+                changed |= relax(&s, c, c - 1, x, y);
+            }
+        /* source x+1 (voronoi.py:161-164) */
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x + 1 < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                changed |= relax(&s, c, c + 1, x, y);
+            }
+        /* source y-1 (voronoi.py:165-168) */
+        for (int y = H - 1; y >= 1; --y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                changed |= relax(&s, c, c - (size_t)W, x, y);
+            }
+        /* source y+1 (voronoi.py:169-172) */
+        for (int y = 0; y + 1 < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                changed |= relax(&s, c, c + (size_t)W, x, y);
+            }
+        if (!changed) break;
+    }
+    for (size_t i = 0; i < N; ++i) {
+        out_hw2[2 * i] = s.id[i];
+        out_hw2[2 * i + 1] = s.d[i];
+    }
+    if (iters_out) *iters_out = iters;
+    free(s.id); free(s.d); free(s.ox); free(s.oy);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Triplet set (open addressing on sorted triples)                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t* keys; /* 3 per slot, slot empty when keys[3*i] == -1 */
+    size_t cap;
+} tset;
+
+static int tset_init(tset* t, size_t expect)
+{
+    size_t cap = 64;
+    while (cap < expect * 2 + 16) cap <<= 1;
+    t->cap = cap;
+    t->keys = (int32_t*)malloc(cap * 3 * sizeof(int32_t));
+    if (!t->keys) return -1;
+    for (size_t i = 0; i < cap * 3; ++i) t->keys[i] = -1;
+    return 0;
+}
+
+static void sort3(int32_t* a, int32_t* b, int32_t* c)
+{
+    int32_t t;
+    if (*a > *b) { t = *a; *a = *b; *b = t; }
+    if (*b > *c) { t = *b; *b = *c; *c = t; }
+    if (*a > *b) { t = *a; *a = *b; *b = t; }
+}
+
+static size_t thash(int32_t a, int32_t b, int32_t c)
+{
+    uint64_t h = (uint64_t)(uint32_t)a * 0x9E3779B97F4A7C15ull;
+    h ^= (uint64_t)(uint32_t)b * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+    h ^= (uint64_t)(uint32_t)c * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+    h ^= h >> 29;
+    return (size_t)h;
+}
+
+static int tset_grow(tset* t);
+
+/* returns 1 if inserted, 0 if present, -1 on allocation failure */
+static int tset_insert(tset* t, int32_t a, int32_t b, int32_t c, size_t* count)
+{
+    sort3(&a, &b, &c);
+    if ((*count + 1) * 2 > t->cap) {
+        if (tset_grow(t)) return -1;
+    }
+    size_t m = t->cap - 1, i = thash(a, b, c) & m;
+    for (;;) {
+        int32_t* k = t->keys + 3 * i;
+        if (k[0] == -1) {
+            k[0] = a; k[1] = b; k[2] = c;
+            ++*count;
+            return 1;
+        }
+        if (k[0] == a && k[1] == b && k[2] == c) return 0;
+        i = (i + 1) & m;
+    }
+}
+
+static int tset_has(const tset* t, int32_t a, int32_t b, int32_t c)
+{
+    sort3(&a, &b, &c);
+    size_t m = t->cap - 1, i = thash(a, b, c) & m;
+    for (;;) {
+        const int32_t* k = t->keys + 3 * i;
+        if (k[0] == -1) return 0;
+        if (k[0] == a && k[1] == b && k[2] == c) return 1;
+        i = (i + 1) & m;
+    }
+}
+
+static int tset_grow(tset* t)
+{
+    tset n;
+    size_t cnt = 0;
+    if (tset_init(&n, t->cap)) return -1;
+    for (size_t i = 0; i < t->cap; ++i) {
+        int32_t* k = t->keys + 3 * i;
+        if (k[0] != -1) tset_insert(&n, k[0], k[1], k[2], &cnt);
+    }
+    free(t->keys);
+    *t = n;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Detection                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Cocircular-quad rule, triangulation.py:115-144.  ids = {TL, TR, BL, BR};
+ * seeds are the sorted seed coordinates in any common frame (a uniform shift
+ * changes neither angles nor lengths).  Writes the two blocked triplets. */
+static void quad_block(const int32_t ids[4], const int32_t* seeds_xy, int32_t out[2][3])
+{
+    double px[4], py[4], key[4];
+    int order[4];
+    double cx = 0.0, cy = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        px[k] = (double)seeds_xy[2 * ids[k]];
+        py[k] = (double)seeds_xy[2 * ids[k] + 1];
+    }
+    cx = (px[0] + px[1] + px[2] + px[3]) / 4.0;
+    cy = (py[0] + py[1] + py[2] + py[3]) / 4.0;
+    for (int k = 0; k < 4; ++k) {
+        key[k] = atan2(py[k] - cy, px[k] - cx);
+        order[k] = k;
+    }
+    /* stable insertion sort by key (Python's sorted is stable) */
+    for (int i = 1; i < 4; ++i) {
+        int v = order[i], j = i - 1;
+        while (j >= 0 && key[order[j]] > key[v]) { order[j + 1] = order[j]; --j; }
+        order[j + 1] = v;
+    }
+    const int d0[2] = {order[0], order[2]};
+    const int d1[2] = {order[1], order[3]};
+    double l0 = (px[d0[0]] - px[d0[1]]) * (px[d0[0]] - px[d0[1]]) +
+                (py[d0[0]] - py[d0[1]]) * (py[d0[0]] - py[d0[1]]);
+    double l1 = (px[d1[0]] - px[d1[1]]) * (px[d1[0]] - px[d1[1]]) +
+                (py[d1[0]] - py[d1[1]]) * (py[d1[0]] - py[d1[1]]);
+    const int* chosen;
+    const int* rejected;
+    if (l0 < l1) { chosen = d0; rejected = d1; }
+    else if (l1 < l0) { chosen = d1; rejected = d0; }
+    else {
+        int lowest = 0;
+        for (int k = 1; k < 4; ++k)
+            if (ids[k] < ids[lowest]) lowest = k;
+        if (lowest == d0[0] || lowest == d0[1]) { chosen = d1; rejected = d0; }
+        else { chosen = d0; rejected = d1; }
+    }
+    for (int j = 0; j < 2; ++j) {
+        out[j][0] = ids[rejected[0]];
+        out[j][1] = ids[rejected[1]];
+        out[j][2] = ids[chosen[j]];
+    }
+}
+
+/* Triangle extraction, triangulation.py:109-183.
+ * grid: (Hp, Wp) int32 seed-id grid.  seeds_xy: sorted seed coordinates.
+ * P: the padding subtracted from the detection pixel.
+ * Output: *out_tris receives a malloc'd (T, 5) array (x, y, a, b, c) in tid
+ * order, *out_count its length.  Returns 0 on success. */
+int orc_detect(int Wp, int Hp, const int32_t* grid, const int32_t* seeds_xy,
+               int P, int32_t** out_tris, int64_t* out_count,
+               int64_t* out_quads)
+{
+    tset blocked, seen;
+    size_t nblocked = 0, nseen = 0;
+    int64_t cap = 1024, cnt = 0, quads = 0;
+    int32_t* tris = (int32_t*)malloc((size_t)cap * 5 * sizeof(int32_t));
+    if (!tris || tset_init(&blocked, 16) || tset_init(&seen, 1024)) return -1;
+    const int bw = Wp - 1, bh = Hp - 1;
+
+#define G(x, y) grid[(size_t)(y) * Wp + (x)]
+    for (int ry = 0; ry < bh; ++ry)
+        for (int rx = 0; rx < bw; ++rx) {
+            const int32_t tl = G(rx, ry), tr = G(rx + 1, ry);
+            const int32_t bl = G(rx, ry + 1), br = G(rx + 1, ry + 1);
+            if (tl != tr && tl != bl && tl != br && tr != bl && tr != br && bl != br) {
+                int32_t ids[4] = {tl, tr, bl, br}, b[2][3];
+                quad_block(ids, seeds_xy, b);
+                ++quads;
+                for (int j = 0; j < 2; ++j)
+                    if (tset_insert(&blocked, b[j][0], b[j][1], b[j][2], &nblocked) < 0)
+                        return -1;
+            }
+        }
+
+    for (int scan = 0; scan < 4; ++scan)
+        for (int ry = 0; ry < bh; ++ry)
+            for (int rx = 0; rx < bw; ++rx) {
+                const int32_t tl = G(rx, ry), tr = G(rx + 1, ry);
+                const int32_t bl = G(rx, ry + 1), br = G(rx + 1, ry + 1);
+                int32_t a, b, c;
+                int gx, gy;
+                switch (scan) {
+                case 0: a = tl; b = tr; c = br; gx = rx + 1; gy = ry; break;     /* :174 */
+                case 1: a = tl; b = tr; c = bl; gx = rx; gy = ry; break;         /* :177 */
+                case 2: a = bl; b = br; c = tl; gx = rx; gy = ry + 1; break;     /* :180 */
+                default: a = bl; b = br; c = tr; gx = rx + 1; gy = ry + 1; break; /* :183 */
+                }
+                if (a == b || a == c || b == c) continue;
+                if (tset_has(&blocked, a, b, c)) continue;
+                int r = tset_insert(&seen, a, b, c, &nseen);
+                if (r < 0) return -1;
+                if (r == 0) continue;
+                if (cnt == cap) {
+                    cap *= 2;
+                    int32_t* nt = (int32_t*)realloc(tris, (size_t)cap * 5 * sizeof(int32_t));
+                    if (!nt) return -1;
+                    tris = nt;
+                }
+                int32_t* o = tris + 5 * cnt++;
+                o[0] = gx - P; o[1] = gy - P; o[2] = a; o[3] = b; o[4] = c;
+            }
+#undef G
+    free(blocked.keys);
+    free(seen.keys);
+    *out_tris = tris;
+    *out_count = cnt;
+    if (out_quads) *out_quads = quads;
+    return 0;
+}
+
+void orc_free(void* p) { free(p); }
+
+/* ------------------------------------------------------------------------ */
+/* Pixel assignment                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* _cross2d(px,py, a, b), triangulation.py:226-228 */
+static inline int64_t cross_p(int64_t px, int64_t py, int64_t ax, int64_t ay,
+                              int64_t bx, int64_t by)
+{
+    return (ax - px) * (by - py) - (ay - py) * (bx - px);
+}
+
+/* triangulation.py:231-253 */
+static inline int contains(int64_t px, int64_t py, const int32_t* A, const int32_t* B,
+                           const int32_t* C)
+{
+    int64_t d1 = cross_p(px, py, A[0], A[1], B[0], B[1]);
+    int64_t d2 = cross_p(px, py, B[0], B[1], C[0], C[1]);
+    int64_t d3 = cross_p(px, py, C[0], C[1], A[0], A[1]);
+    int neg = (d1 < 0) || (d2 < 0) || (d3 < 0);
+    int pos = (d1 > 0) || (d2 > 0) || (d3 > 0);
+    return !(neg && pos);
+}
+
+/* triangulation.py:193-204: tri rows are (x, y, a, b, c). */
+void orc_assign_brute(int W, int H, const int32_t* seeds_xy, const int32_t* tris,
+                      int64_t T, int32_t* out_tid)
+{
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            int32_t best = -1;
+            for (int64_t t = 0; t < T; ++t) {
+                const int32_t* r = tris + 5 * t;
+                if (contains(x, y, seeds_xy + 2 * r[2], seeds_xy + 2 * r[3], seeds_xy + 2 * r[4]))
+                    if (best == -1 || t > best) best = (int32_t)t;
+            }
+            out_tid[(size_t)y * W + x] = best;
+        }
+}
+
+static inline int64_t floordiv(int64_t a, int64_t b) /* b > 0 */
+{
+    int64_t q = a / b;
+    if ((a % b) != 0 && a < 0) --q;
+    return q;
+}
+
+static inline int64_t ceildiv(int64_t a, int64_t b) /* b > 0 */
+{
+    return -floordiv(-a, b);
+}
+
+/* The same function as orc_assign_brute, evaluated per triangle.  For a
+ * triangle with signed doubled area S != 0 the three edge functions sum to S,
+ * so containment is sign(S)*d_i >= 0 for all i, each a half-line in x on a row.
+ * For S == 0 the three distinct collinear vertices contain exactly the lattice
+ * points of their line. */
+void orc_assign_raster(int W, int H, const int32_t* seeds_xy, const int32_t* tris,
+                       int64_t T, int32_t* out_tid)
+{
+    for (size_t i = 0; i < (size_t)W * H; ++i) out_tid[i] = -1;
+    for (int64_t t = 0; t < T; ++t) {
+        const int32_t* r = tris + 5 * t;
+        const int32_t* V[3] = {seeds_xy + 2 * r[2], seeds_xy + 2 * r[3], seeds_xy + 2 * r[4]};
+        const int64_t S = cross_p(V[0][0], V[0][1], V[1][0], V[1][1], V[2][0], V[2][1]);
+        if (S == 0) {
+            const int64_t ax = V[0][0], ay = V[0][1];
+            const int64_t dx = V[1][0] - ax, dy = V[1][1] - ay;
+            for (int y = 0; y < H; ++y) {
+                int64_t x;
+                if (dy == 0) {
+                    if (y != ay) continue;
+                    for (x = 0; x < W; ++x) {
+                        int32_t* o = out_tid + (size_t)y * W + x;
+                        if (*o < t) *o = (int32_t)t;
+                    }
+                    continue;
+                }
+                int64_t num = (int64_t)(y - ay) * dx;
+                if (num % dy != 0) continue;
+                x = ax + num / dy;
+                if (x < 0 || x >= W) continue;
+                int32_t* o = out_tid + (size_t)y * W + x;
+                if (*o < t) *o = (int32_t)t;
+            }
+            continue;
+        }
+        const int64_t sg = S > 0 ? 1 : -1;
+        int64_t ymin = V[0][1], ymax = V[0][1];
+        for (int k = 1; k < 3; ++k) {
+            if (V[k][1] < ymin) ymin = V[k][1];
+            if (V[k][1] > ymax) ymax = V[k][1];
+        }
+        if (ymin < 0) ymin = 0;
+        if (ymax > H - 1) ymax = H - 1;
+        for (int64_t y = ymin; y <= ymax; ++y) {
+            int64_t lo = 0, hi = W - 1;
+            for (int e = 0; e < 3; ++e) {
+                const int32_t* A = V[e];
+                const int32_t* B = V[(e + 1) % 3];
+                /* d(px) = px*(Ay-By) + Ax*(By-y) - Bx*(Ay-y) */
+                int64_t al = sg * ((int64_t)A[1] - B[1]);
+                int64_t be = sg * ((int64_t)A[0] * (B[1] - y) - (int64_t)B[0] * (A[1] - y));
+                if (al > 0) {
+                    int64_t l = ceildiv(-be, al);
+                    if (l > lo) lo = l;
+                } else if (al < 0) {
+                    int64_t h = floordiv(be, -al);
+                    if (h < hi) hi = h;
+                } else if (be < 0) {
+                    lo = 1; hi = 0;
+                }
+            }
+            for (int64_t x = lo; x <= hi; ++x) {
+                int32_t* o = out_tid + (size_t)y * W + x;
+                if (*o < t) *o = (int32_t)t;
+            }
+        }
+    }
+}

@@ -1,0 +1,206 @@
+"""Parity of the HIP path (libdelauney_hip.so through the package API) with the
+reference's golden vectors and the CPU oracle.  Bit-exact throughout: every
+quantity is integer."""
+import numpy as np
+import pytest
+
+import integer_delauney_amd as dl
+from integer_delauney_amd import _lib
+from oracle import oracle as O
+from tests import goldens
+
+pytestmark = pytest.mark.gpu
+
+FULL = goldens.names("full_")
+VOR = goldens.names("vor_")
+DET = goldens.names("det_")
+
+
+def _rows(m):
+    return np.array([m[i] for i in range(len(m))], dtype=np.int32).reshape(-1, 5)
+
+
+def _pad_arg(p):
+    return -1 if p is None else p
+
+
+def test_library_loads_and_runs():
+    g = dl.Voronoi().compute(5, 5, [(2, 2)])
+    assert g.shape == (5, 5, 2) and np.all(g[:, :, 0] == 0)
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_golden_triangulate(name):
+    g = goldens.load(name)
+    m, tg = dl.triangulate(g["W"], g["H"], g["seeds"], _pad_arg(g["P"]))
+    np.testing.assert_array_equal(tg[:, :, :2], g["vgrid"])
+    np.testing.assert_array_equal(_rows(m), g["rows"])
+    np.testing.assert_array_equal(tg[:, :, 2], g["tid"])
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_golden_two_step(name):
+    g = goldens.load(name)
+    vg = dl.Voronoi().compute(g["W"], g["H"], g["seeds"])
+    np.testing.assert_array_equal(vg, g["vgrid"])
+    m, tg = dl.GridTriangulation().compute(vg, g["seeds"], _pad_arg(g["P"]))
+    np.testing.assert_array_equal(_rows(m), g["rows"])
+    np.testing.assert_array_equal(tg[:, :, 2], g["tid"])
+    np.testing.assert_array_equal(tg[:, :, :2], vg)
+    verts, tg2 = dl.GridTriangulation().compute(vg, g["seeds"], _pad_arg(g["P"]), as_arrays=True)
+    np.testing.assert_array_equal(verts, g["rows"][:, 2:5])
+    np.testing.assert_array_equal(tg2, tg)
+
+
+@pytest.mark.parametrize("name", VOR)
+def test_golden_voronoi(name):
+    g = goldens.load(name)
+    np.testing.assert_array_equal(dl.Voronoi().compute(g["W"], g["H"], g["seeds"]), g["vgrid"])
+
+
+@pytest.mark.parametrize("name", DET)
+def test_golden_detection(name):
+    g = goldens.load(name)
+    m, _ = dl.triangulate(g["W"], g["H"], g["seeds"], _pad_arg(g["P"]))
+    np.testing.assert_array_equal(_rows(m), g["rows"])
+
+
+def _random_case(rng):
+    kind = rng.integers(0, 4)
+    W, H = int(rng.integers(2, 200)), int(rng.integers(2, 200))
+    if kind == 0:
+        n = int(rng.integers(1, max(2, W * H // 30)))
+    elif kind == 1:
+        n = int(rng.integers(1, max(2, W * H // 3)))
+    elif kind == 2:
+        n = int(rng.integers(1, 6))
+    else:
+        sx, sy = int(rng.integers(2, 9)), int(rng.integers(2, 9))
+        pts = np.array([(x, y) for x in range(0, W, sx) for y in range(0, H, sy)])
+        return W, H, pts.astype(np.int32)
+    idx = rng.choice(W * H, size=min(n, W * H), replace=False)
+    return W, H, np.stack([idx % W, idx // W], 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_against_oracle(seed):
+    rng = np.random.default_rng(1000 + seed)
+    for _ in range(6):
+        W, H, seeds = _random_case(rng)
+        P = int(rng.choice([-1, 0, 1, 2, 7, 16]))
+        rows, ref = O.triangulate(W, H, seeds, None if P == -1 else P, return_rows=True)
+        m, got = dl.triangulate(W, H, seeds, P)
+        np.testing.assert_array_equal(got, ref, err_msg=f"W={W} H={H} n={len(seeds)} P={P}")
+        np.testing.assert_array_equal(_rows(m), rows)
+
+
+@pytest.mark.parametrize("cfg", [(16, 4, 4), (16, 4, 8), (8, 8, 4), (16, 8, 8), (8, 16, 8),
+                                 (16, 8, 16)])
+def test_every_tile_configuration_is_exact(cfg):
+    ctx = _lib.context()
+    lib = _lib.load()
+    _lib.check(lib.dl_set_bfs_config(ctx.handle, *cfg))
+    try:
+        rng = np.random.default_rng(7)
+        for W, H, n in [(300, 200, 600), (97, 411, 300), (513, 129, 50), (1000, 700, 20000)]:
+            idx = rng.choice(W * H, size=n, replace=False)
+            seeds = np.stack([idx % W, idx // W], 1).astype(np.int32)
+            ref = O.voronoi(W, H, seeds)
+            np.testing.assert_array_equal(dl.Voronoi().compute(W, H, seeds), ref)
+            rows, ref3 = O.triangulate(W, H, seeds, None, return_rows=True)
+            m, got = dl.triangulate(W, H, seeds)
+            np.testing.assert_array_equal(got, ref3)
+            np.testing.assert_array_equal(_rows(m), rows)
+    finally:
+        _lib.check(lib.dl_set_bfs_config(ctx.handle, 0, 0, 0))
+
+
+def test_simple_fill_kernel_agrees_with_tiled():
+    import ctypes
+    ctx = _lib.context()
+    lib = _lib.load()
+    rng = np.random.default_rng(5)
+    for W, H, n in [(64, 64, 40), (300, 200, 3000), (50, 400, 10)]:
+        idx = rng.choice(W * H, size=n, replace=False)
+        s = np.ascontiguousarray(np.stack([idx % W, idx // W], 1).astype(np.int32))
+        out = np.empty((H, W, 2), dtype=np.int32)
+        it = ctypes.c_int64(0)
+        _lib.check(lib.dl_voronoi_simple(ctx.handle, W, H, _lib.i32p(s), n, _lib.i32p(out),
+                                         ctypes.byref(it)))
+        ref, ref_it = O.voronoi_sorted(W, H, O.sort_seeds(W, H, s), return_iters=True)
+        np.testing.assert_array_equal(out, ref)
+        assert it.value == ref_it
+        np.testing.assert_array_equal(dl.Voronoi().compute(W, H, s), ref)
+
+
+def test_forced_second_fill_gives_the_same_grid():
+    ctx = _lib.context()
+    lib = _lib.load()
+    rng = np.random.default_rng(9)
+    W, H = 257, 190
+    idx = rng.choice(W * H, size=900, replace=False)
+    s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    m0, g0 = dl.triangulate(W, H, s)
+    _lib.check(lib.dl_set_force_unpadded(ctx.handle, 1))
+    try:
+        m1, g1 = dl.triangulate(W, H, s)
+    finally:
+        _lib.check(lib.dl_set_force_unpadded(ctx.handle, 0))
+    np.testing.assert_array_equal(g0, g1)
+    assert m0 == m1
+
+
+def test_compute_debug_padded_canvas():
+    rng = np.random.default_rng(2)
+    W, H = 120, 90
+    idx = rng.choice(W * H, size=150, replace=False)
+    s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    vg = dl.Voronoi().compute(W, H, s)
+    m, tg, padded = dl.GridTriangulation().compute_debug(vg, s, 12)
+    ref = O.voronoi(W + 24, H + 24, s + 12)
+    np.testing.assert_array_equal(padded, ref)
+    m2, tg2, t = dl.GridTriangulation().compute_timed(vg, s, 12)
+    assert m2 == m and np.array_equal(tg2, tg)
+    assert set(t) == {"detect_ms", "dedup_ms", "assign_ms"} and all(v >= 0 for v in t.values())
+
+
+@pytest.mark.parametrize("bad", [[], [(2, 2), (2, 2)], [(10, 1)], [(1, 10)], [(-1, 0)],
+                                 [(1, 2, 3)]])
+def test_value_errors(bad):
+    with pytest.raises(ValueError):
+        dl.Voronoi().compute(5, 5, bad)
+    with pytest.raises(ValueError):
+        dl.triangulate(5, 5, bad)
+
+
+@pytest.mark.parametrize("W,H", [(0, 10), (10, 0), (-4, 10)])
+def test_non_positive_dimensions(W, H):
+    with pytest.raises(ValueError):
+        dl.triangulate(W, H, [(0, 0), (1, 1), (2, 0)])
+
+
+def test_oversized_canvas_rejected():
+    with pytest.raises(ValueError):
+        dl.triangulate(32000, 10, [(0, 0), (5, 5), (9, 1)])
+
+
+def test_reusing_the_context_across_sizes():
+    rng = np.random.default_rng(4)
+    for W, H, n in [(400, 300, 2000), (20, 20, 5), (640, 480, 5000), (33, 700, 100)]:
+        idx = rng.choice(W * H, size=n, replace=False)
+        s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+        rows, ref = O.triangulate(W, H, s, None, return_rows=True)
+        m, got = dl.triangulate(W, H, s)
+        np.testing.assert_array_equal(got, ref)
+        np.testing.assert_array_equal(_rows(m), rows)
+
+
+def test_run_twice_identical():
+    rng = np.random.default_rng(8)
+    W, H = 700, 500
+    idx = rng.choice(W * H, size=8000, replace=False)
+    s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    a = dl.triangulate(W, H, s, as_arrays=True)
+    b = dl.triangulate(W, H, s, as_arrays=True)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
