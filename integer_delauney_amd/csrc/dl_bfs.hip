@@ -329,7 +329,8 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     }
     if (esc && cv.escape) atomicOr(cv.escape, 1);
     __syncthreads();   // every wave is past its last read of lds_any
-    if (lane == 0) lds_any[0][wave] = __ballot(changed) != 0ull ? 1 : 0;
+    const bool wave_changed = __ballot(changed) != 0ull;   // all lanes take part
+    if (lane == 0) lds_any[0][wave] = wave_changed ? 1 : 0;
     __syncthreads();
     if (tid == 0) {
         int any = 0;
