@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark of the batch pipeline on one MI355X.
+
+Workload: 4096 x 4096 image, 200,000 distinct uniform seeds (default_rng(2)),
+default border padding 16, full pipeline per step -- seed ordering, the padded
+Voronoi flood fill, triangle extraction with first-registration numbering,
+per-pixel triangle assignment and the (H, W, 3) int32 output -- with the seeds
+already resident in HBM and the output left in HBM
+(dl_triangulate_device).  Every step's output is bit-exact with the reference
+NumPy path (tests/test_gpu_parity.py and tests/test_gpu_large.py).
+
+Prints one JSON line (rank 0, N = 1): value = Mpixel/s; a `roofline` object
+for the dominant kernel from HIP-event kernel timings taken over the timed
+region; a `cpu_baseline` object timing the CPU oracle (the C restatement of the
+reference path) on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KINDS = ["fill", "seeds", "detect", "register", "assign", "output", "init"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_stats(lib, ctx):
+    counts = (ctypes.c_int64 * 7)()
+    ms = (ctypes.c_double * 7)()
+    lib.dl_get_kernel_stats(ctx.handle, counts, ms)
+    return {KINDS[i]: (int(counts[i]), float(ms[i])) for i in range(7)}
+
+
+def cpu_baseline(side: int):
+    """The C oracle (1 thread) on a side x side image at the same seed density."""
+    import numpy as np
+
+    from integer_delauney_amd.workloads import uniform_seeds
+    from oracle import oracle as O
+
+    n = int(round(200_000 * side * side / (4096 * 4096)))
+    seeds = uniform_seeds(side, side, n, 2)
+    t0 = time.perf_counter()
+    O.triangulate(side, side, seeds, None, assign_method="raster")
+    dt = time.perf_counter() - t0
+    return {"value": side * side / dt / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
+            "sample": f"{side}x{side} image, {n} uniform seeds (same density as 4096x4096/200k), "
+                      f"full pipeline in the C oracle: unpadded + padded flood fill, detection, "
+                      f"per-triangle exact assignment; {dt:.2f} s",
+            "seconds": dt, "host_cpu_count": os.cpu_count()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--bfs-config", type=str, default="", help="waves,rows,iters")
+    ap.add_argument("--cpu-sample-side", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--e2e-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the pipeline "
+              f"shards by image or by row strip; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from integer_delauney_amd import _lib
+    from integer_delauney_amd.workloads import uniform_seeds
+
+    W = H = 4096
+    N = 200_000
+    P = 16
+    log("generating seeds")
+    seeds = uniform_seeds(W, H, N, 2)
+    lib = _lib.load()
+    ctx = _lib.context()
+    if args.bfs_config:
+        w, r, k = (int(v) for v in args.bfs_config.split(","))
+        _lib.check(lib.dl_set_bfs_config(ctx.handle, w, r, k))
+    d_seeds = torch.from_numpy(seeds).to("cuda")
+    d_out = torch.empty((H, W, 3), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    n_tri = ctypes.c_int64(0)
+
+    def step():
+        _lib.check(lib.dl_triangulate_device(ctx.handle, W, H, ctypes.c_void_p(d_seeds.data_ptr()),
+                                             N, P, ctypes.c_void_p(d_out.data_ptr()), None,
+                                             ctypes.byref(n_tri)))
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    _lib.check(lib.dl_reset_kernel_stats(ctx.handle))
+    _lib.check(lib.dl_set_kernel_profiling(ctx.handle, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.check(lib.dl_set_kernel_profiling(ctx.handle, 0))
+    log(f"timed {args.steps} steps")
+    ms_per_step = (t1 - t0) / args.steps * 1e3
+    value = W * H / (ms_per_step / 1e3) / 1e6
+    stats = kernel_stats(lib, ctx)
+    timings = np.zeros(9)
+    lib.dl_get_timings(ctx.handle, timings.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    fill_launches, fill_visits = float(timings[7]), float(timings[8])
+
+    # Algorithmic bytes per launch of each kernel kind.
+    Wp, Hp = W + 2 * P, H + 2 * P
+    T = int(n_tri.value)
+    cfg_w, cfg_r, cfg_k = (16, 4, 4)
+    if args.bfs_config:
+        cfg_w, cfg_r, cfg_k = (int(v) for v in args.bfs_config.split(","))
+    tile_cells = (128 - 2 * cfg_k) * (cfg_w * cfg_r - 2 * cfg_k)
+    alg = {
+        # 4 B read + 4 B written per cell of every visited tile
+        "fill": 8.0 * tile_cells * fill_visits / max(fill_launches, 1.0),
+        # labels read once (4 B/cell) + candidate records written/read (20 B each)
+        "detect": 4.0 * Wp * Hp,
+        # (H, W, 3) written + label and tid read
+        "output": (12.0 + 4.0 + 4.0) * W * H,
+        # tid raster written once (4 B/px) + triangle rows and vertices read
+        "assign": 4.0 * W * H + 32.0 * T,
+    }
+    per_kernel = {}
+    for k, (c, t) in stats.items():
+        if c:
+            per_kernel[k] = {"launches_per_step": c / args.steps, "ms_per_step": t / args.steps,
+                             "avg_launch_us": t / c * 1e3}
+    dominant = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_step"])
+    dom = per_kernel[dominant]
+    achieved = alg.get(dominant, 0.0) / (dom["avg_launch_us"] * 1e-6) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dominant)
+    b_alg_image = 12.0 * W * H + 8.0 * Wp * Hp + 8.0 * N + 32.0 * T
+
+    # End to end through the host API: H2D of the seeds, D2H of the grid.
+    e2e = None
+    if args.e2e_steps > 0:
+        from integer_delauney_amd import triangulate
+        triangulate(W, H, seeds, P, as_arrays=True)
+        t2 = time.perf_counter()
+        for _ in range(args.e2e_steps):
+            triangulate(W, H, seeds, P, as_arrays=True)
+        t3 = time.perf_counter()
+        e2e = W * H / ((t3 - t2) / args.e2e_steps) / 1e6
+        log("end-to-end steps done")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU oracle sample")
+        cpu = cpu_baseline(args.cpu_sample_side)
+
+    line = {
+        "metric": "Mpixel/s full pipeline (bit-exact) + % HBM roofline; p50/p99 latency served",
+        "value": value,
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "single image 4096x4096, 200000 distinct uniform seeds "
+                               "(default_rng(2)), border padding 16, full pipeline "
+                               "(Voronoi + Delaunay extraction + pixel-to-triangle assignment), "
+                               "inputs and outputs resident in HBM",
+                   "width": W, "height": H, "seeds": N, "border_padding": P,
+                   "triangles": T, "parallelism": "single GPU"},
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "alg_bytes_per_launch": alg.get(dominant),
+                     "avg_launch_us": dom["avg_launch_us"]},
+        "pipeline_alg_bytes_per_image": b_alg_image,
+        "pipeline_hbm_frac": b_alg_image / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "kernels": per_kernel,
+        "fill": {"launches": fill_launches, "tile_visits": fill_visits,
+                 "config": [cfg_w, cfg_r, cfg_k]},
+        "end_to_end_mpixel_per_s": e2e,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

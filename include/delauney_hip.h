@@ -122,6 +122,18 @@ const int32_t* dl_device_triangles(dl_context* ctx);
 int dl_voronoi_simple(dl_context* ctx, int width, int height, const int32_t* seeds_xy,
                       int64_t n, int32_t* out_hw2, int64_t* iters);
 
+/* Per-kernel timing with HIP events on the stream each kernel runs on.
+ * While enabled, every call brackets its kernels with events and accumulates
+ * counts and milliseconds per kind: [0] flood-fill tile kernel, [1] seed
+ * ordering, [2] triangle detection, [3] registration (first-registration
+ * marking, prefix scan, row emission), [4] pixel assignment, [5] output
+ * assembly, [6] initialisation fills.  counts and ms hold 7 entries each.
+ * No reference counterpart (the reference's PhaseTimer, phase_timer.cuh:17-62,
+ * times phases, not kernels). */
+int dl_set_kernel_profiling(dl_context* ctx, int enabled);
+int dl_get_kernel_stats(dl_context* ctx, int64_t* counts, double* ms);
+int dl_reset_kernel_stats(dl_context* ctx);
+
 /* Forces the second (unpadded) flood fill in dl_triangulate even when the
  * padded fill shows that the image interior never received an update from the
  * padding (cross-checking only). */

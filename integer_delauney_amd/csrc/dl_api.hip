@@ -70,6 +70,71 @@ void PinnedBuf::reserve(size_t n)
 }
 
 namespace {
+thread_local KernelProfiler* g_profiler = nullptr;
+}
+
+KernelProfiler* current_profiler() { return g_profiler; }
+void set_current_profiler(KernelProfiler* p) { g_profiler = p; }
+
+hipEvent_t KernelProfiler::take()
+{
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    DL_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void KernelProfiler::collect()
+{
+    for (const auto& r : pending) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            count[r.kind] += 1;
+            ms[r.kind] += t;
+        }
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    pending.clear();
+}
+
+void KernelProfiler::reset()
+{
+    for (int i = 0; i < kKinds; ++i) {
+        count[i] = 0;
+        ms[i] = 0.0;
+    }
+}
+
+KernelProfiler::~KernelProfiler()
+{
+    for (auto& r : pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+ProfScope::ProfScope(int kind, hipStream_t stream) : p(g_profiler), s(stream)
+{
+    if (!p) return;
+    rec.kind = kind;
+    rec.a = p->take();
+    rec.b = p->take();
+    DL_HIP(hipEventRecord(rec.a, s));
+}
+
+ProfScope::~ProfScope()
+{
+    if (!p) return;
+    if (hipEventRecord(rec.b, s) == hipSuccess) p->pending.push_back(rec);
+}
+
+namespace {
 
 __global__ void shift_labels_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                     int64_t n, uint32_t add)
@@ -112,11 +177,30 @@ struct dl_context {
     bool cfg_auto = true;
     bool force_unpadded = false;
     std::vector<hipEvent_t> events;
+    bool profiling = false;
+    KernelProfiler prof;
 };
 
 namespace {
 
 thread_local std::string g_last_error;
+
+// Installs the context's kernel profiler for the duration of one call.
+struct ProfInstall {
+    dl_context* ctx;
+    explicit ProfInstall(dl_context* c) : ctx(c)
+    {
+        if (ctx && ctx->profiling) set_current_profiler(&ctx->prof);
+    }
+    ~ProfInstall()
+    {
+        if (ctx && ctx->profiling) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->prof.collect();
+        }
+        set_current_profiler(nullptr);
+    }
+};
 
 template <class F>
 int guarded(F&& f)
@@ -400,6 +484,33 @@ int dl_set_bfs_config(dl_context* ctx, int waves, int rows, int iters)
     });
 }
 
+int dl_set_kernel_profiling(dl_context* ctx, int enabled)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->profiling = enabled != 0;
+    });
+}
+
+int dl_get_kernel_stats(dl_context* ctx, int64_t* counts, double* ms)
+{
+    return guarded([&] {
+        if (!ctx || !counts || !ms) throw value_error("null argument");
+        for (int i = 0; i < kKinds; ++i) {
+            counts[i] = ctx->prof.count[i];
+            ms[i] = ctx->prof.ms[i];
+        }
+    });
+}
+
+int dl_reset_kernel_stats(dl_context* ctx)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->prof.reset();
+    });
+}
+
 int dl_set_force_unpadded(dl_context* ctx, int enabled)
 {
     return guarded([&] {
@@ -416,6 +527,7 @@ int dl_voronoi(dl_context* ctx, int width, int height, const int32_t* seeds_xy, 
         check_dims(width, height, 0);
         check_seeds(seeds_xy, n);
         DL_HIP(hipSetDevice(ctx->device));
+        ProfInstall pi(ctx);
         hipStream_t s = ctx->stream;
         StageClock clk(ctx);
         upload_seeds(ctx, seeds_xy, n);
@@ -473,6 +585,7 @@ int dl_triangulate(dl_context* ctx, int width, int height, const int32_t* seeds_
         check_dims(width, height, P);
         check_seeds(seeds_xy, n);
         DL_HIP(hipSetDevice(ctx->device));
+        ProfInstall pi(ctx);
         hipStream_t s = ctx->stream;
         StageClock clk(ctx);
         upload_seeds(ctx, seeds_xy, n);
@@ -498,6 +611,19 @@ int dl_triangulate_device(dl_context* ctx, int width, int height, const int32_t*
         check_seeds(d_seeds_xy, n);
         DL_HIP(hipSetDevice(ctx->device));
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        if (ctx->profiling) set_current_profiler(&ctx->prof);
+        struct Drop {
+            dl_context* c;
+            hipStream_t st;
+            ~Drop()
+            {
+                if (c->profiling) {
+                    (void)hipStreamSynchronize(st);
+                    c->prof.collect();
+                }
+                set_current_profiler(nullptr);
+            }
+        } drop{ctx, s};
         StageClock clk(ctx);
         run_triangulate(ctx, width, height, d_seeds_xy, n, P, d_out_hw3, s, clk);
         clk.finish(s);
@@ -520,6 +646,7 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
         check_dims(width, height, P);
         check_seeds(seeds_xy, n);
         DL_HIP(hipSetDevice(ctx->device));
+        ProfInstall pi(ctx);
         hipStream_t s = ctx->stream;
         StageClock clk(ctx);
         upload_seeds(ctx, seeds_xy, n);

@@ -433,6 +433,7 @@ void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
 
 void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n, hipStream_t s)
 {
+    ProfScope ps(kKInit, s);
     const size_t cells = static_cast<size_t>(cv.W) * static_cast<size_t>(cv.H);
     fill_u32(cv.lab0, kInvalid, cells, s);
     fill_u32(cv.lab1, kInvalid, cells, s);
@@ -499,10 +500,13 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
             const int src_plane = launch & 1;
             int* fprev = flags + ((launch + 1) & 1) * total;
             int* fcur = flags + (launch & 1) * total;
-            v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
-                reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
-                src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j));
-            DL_LAUNCH_CHECK();
+            {
+                ProfScope ps(kKFill, s);
+                v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
+                    reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
+                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j));
+                DL_LAUNCH_CHECK();
+            }
             ++launch;
         }
         DL_HIP(hipMemcpyAsync(hcnt + 2 * slot0, dcnt + 2 * slot0, sizeof(int) * 2 * kChunk,

@@ -190,4 +190,47 @@ void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, 
 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Per-kernel timing.  When a profiler is installed for the calling thread,
+// launch sites bracket their kernels with events; collect() turns the pairs
+// into per-kind counts and milliseconds after the stream has been synchronised.
+// ---------------------------------------------------------------------------
+enum KernelKind {
+    kKFill = 0,      // flood-fill tile kernel
+    kKSeeds = 1,     // seed marking / ranking / scan
+    kKDetect = 2,    // triangle candidate detection
+    kKRegister = 3,  // first-registration marking, scan, row emission
+    kKAssign = 4,    // pixel assignment
+    kKOutput = 5,    // (H, W, C) output assembly
+    kKInit = 6,      // canvas initialisation and fills
+    kKinds = 7
+};
+
+struct KernelProfiler {
+    int64_t count[kKinds] = {0};
+    double ms[kKinds] = {0};
+    std::vector<hipEvent_t> pool;
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    hipEvent_t take();
+    void collect();   // the stream of every pending event must be synchronised
+    void reset();
+    ~KernelProfiler();
+};
+
+KernelProfiler* current_profiler();
+void set_current_profiler(KernelProfiler* p);
+
+// Brackets one kernel launch: construct before the launch, destroy after it.
+struct ProfScope {
+    KernelProfiler* p;
+    hipStream_t s;
+    KernelProfiler::Pending rec;
+    ProfScope(int kind, hipStream_t stream);
+    ~ProfScope();
+};
+
 }  // namespace dl

@@ -193,17 +193,21 @@ void normalise_seeds(int W, int H, const int32_t* d_xy, int64_t n, uint32_t* d_s
     work.host_flags.reserve(sizeof(uint32_t) * 8);
     uint32_t* bm = work.bitmap.as<uint32_t>();
     uint32_t* flags = work.flags.as<uint32_t>();
-    fill_u32(bm, 0u, static_cast<size_t>(nw), s);
-    fill_u32(flags, 0u, 8, s);
     const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
-    seeds_mark_kernel<<<blocks, 256, 0, s>>>(d_xy, n, W, H, bm, flags);
-    DL_LAUNCH_CHECK();
+    {
+        ProfScope ps(kKSeeds, s);
+        fill_u32(bm, 0u, static_cast<size_t>(nw), s);
+        fill_u32(flags, 0u, 8, s);
+        seeds_mark_kernel<<<blocks, 256, 0, s>>>(d_xy, n, W, H, bm, flags);
+        DL_LAUNCH_CHECK();
+    }
     DL_HIP(hipMemcpyAsync(work.host_flags.ptr, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
     const uint32_t f = work.host_flags.as<uint32_t>()[0];
     if (f & kFlagXRange) throw value_error("seed x coordinate out of bounds");
     if (f & kFlagYRange) throw value_error("seed y coordinate out of bounds");
     if (f & kFlagDuplicate) throw value_error("duplicate seed positions are not allowed");
+    ProfScope ps(kKSeeds, s);
     popcount_scan(bm, nw, work.scan.as<uint32_t>(), work.scan_tmp, s);
     seeds_rank_kernel<<<blocks, 256, 0, s>>>(d_xy, n, W, H, bm, work.scan.as<uint32_t>(), d_sorted,
                                              d_rank_at);
@@ -591,11 +595,14 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         t.limit = static_cast<uint32_t>(std::min<uint64_t>(table_cap * 3 / 4, 0xFFFFFFFFull));
         t.exact = exact;
         t.salt = salt;
-        fill_u32(reinterpret_cast<uint32_t*>(t.tag), 0xFFFFFFFFu, 2 * table_cap, s);
-        fill_u32(t.minkey, 0xFFFFFFFFu, table_cap, s);
-        fill_u32(t.blk, 0u, table_cap, s);
-        zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
-        DL_LAUNCH_CHECK();
+        {
+            ProfScope ps(kKInit, s);
+            fill_u32(reinterpret_cast<uint32_t*>(t.tag), 0xFFFFFFFFu, 2 * table_cap, s);
+            fill_u32(t.minkey, 0xFFFFFFFFu, table_cap, s);
+            fill_u32(t.blk, 0u, table_cap, s);
+            zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
+            DL_LAUNCH_CHECK();
+        }
 
         DetectArgs A;
         A.grid = grid;
@@ -613,8 +620,11 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         A.quad_count = dflags + 2;
         A.flags = dflags;
         A.table = t;
-        detect_kernel<<<static_cast<unsigned>((nb + 255) / 256), 256, 0, s>>>(A);
-        DL_LAUNCH_CHECK();
+        {
+            ProfScope ps(kKDetect, s);
+            detect_kernel<<<static_cast<unsigned>((nb + 255) / 256), 256, 0, s>>>(A);
+            DL_LAUNCH_CHECK();
+        }
 
         const int64_t nbits = 4 * nb;
         const int64_t nwords = (nbits + 31) / 32;
@@ -622,7 +632,10 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords + 1));
         uint32_t* bm = work.bitmap.as<uint32_t>();
         uint32_t* pre = work.scan.as<uint32_t>();
-        fill_u32(bm, 0u, static_cast<size_t>(nwords), s);
+        {
+            ProfScope ps(kKInit, s);
+            fill_u32(bm, 0u, static_cast<size_t>(nwords), s);
+        }
         DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
         uint32_t f = hflags[0];
@@ -636,9 +649,12 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
             continue;
         }
         const unsigned cblocks = static_cast<unsigned>(std::max<uint64_t>(1, (ncand + 255) / 256));
-        mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, dflags);
-        DL_LAUNCH_CHECK();
-        popcount_scan(bm, nwords, pre, work.scan_tmp, s);
+        {
+            ProfScope ps(kKRegister, s);
+            mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, dflags);
+            DL_LAUNCH_CHECK();
+            popcount_scan(bm, nwords, pre, work.scan_tmp, s);
+        }
         DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         DL_HIP(hipMemcpyAsync(hflags + 4, pre + nwords, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
@@ -649,9 +665,12 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         }
         const uint32_t T = hflags[4];
         work.rows.reserve(sizeof(int32_t) * 5 * std::max<uint64_t>(T, 1));
-        emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, pre, Wp,
-                                            static_cast<uint32_t>(nb), P, work.rows.as<int32_t>());
-        DL_LAUNCH_CHECK();
+        {
+            ProfScope ps(kKRegister, s);
+            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, pre, Wp,
+                                                static_cast<uint32_t>(nb), P, work.rows.as<int32_t>());
+            DL_LAUNCH_CHECK();
+        }
         res.n_tri = T;
         res.n_quads = hflags[2];
         res.n_cand = ncand;
@@ -754,8 +773,12 @@ __global__ void output_kernel(int W, int H, const uint32_t* __restrict__ lab, in
 void assign_pixels(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
                    int32_t* tidgrid, hipStream_t s)
 {
-    fill_u32(reinterpret_cast<uint32_t*>(tidgrid), 0xFFFFFFFFu, static_cast<size_t>(W) * H, s);
+    {
+        ProfScope ps(kKInit, s);
+        fill_u32(reinterpret_cast<uint32_t*>(tidgrid), 0xFFFFFFFFu, static_cast<size_t>(W) * H, s);
+    }
     if (n_tri == 0) return;
+    ProfScope ps(kKAssign, s);
     assign_kernel<<<static_cast<unsigned>((n_tri + 127) / 128), 128, 0, s>>>(W, H, rows, n_tri,
                                                                              sorted, tidgrid);
     DL_LAUNCH_CHECK();
@@ -765,6 +788,7 @@ void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, 
                   const int32_t* rank_at, int W_img, const int32_t* tidgrid, int32_t* out, int C,
                   hipStream_t s)
 {
+    ProfScope ps(kKOutput, s);
     dim3 grid((W + 255) / 256, H);
     if (C == 3)
         output_kernel<3><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, rank_at, W_img,

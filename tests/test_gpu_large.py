@@ -1,0 +1,84 @@
+"""Parity at the benchmark's full size (4096 x 4096, 200,000 uniform seeds,
+padding 16) and at intermediate sizes.
+
+At full size every stage is checked against the oracle on the same data: the
+padded flood fill and the unpadded one (C oracle, ~10 s each), triangle
+extraction on the GPU's padded canvas, and the pixel raster for the GPU's
+triangle rows.  Bit-exact throughout."""
+import time
+
+import numpy as np
+import pytest
+
+import integer_delauney_amd as dl
+from integer_delauney_amd.workloads import clustered_seeds, uniform_seeds
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(m):
+    return np.array([m[i] for i in range(len(m))], dtype=np.int32).reshape(-1, 5)
+
+
+def test_full_size_4096_200k():
+    W = H = 4096
+    P = 16
+    seeds = uniform_seeds(W, H, 200_000, 2)
+    t0 = time.perf_counter()
+    m, grid = dl.triangulate(W, H, seeds, P)
+    rows = _rows(m)
+    print(f"gpu triangulate {time.perf_counter() - t0:.2f} s, {len(rows)} triangles", flush=True)
+    s = O.sort_seeds(W, H, seeds)
+    # unpadded flood fill (channels 0/1)
+    t0 = time.perf_counter()
+    vg = O.voronoi_sorted(W, H, s)
+    print(f"oracle voronoi {time.perf_counter() - t0:.1f} s", flush=True)
+    np.testing.assert_array_equal(grid[:, :, :2], vg)
+    # padded flood fill + extraction
+    vgrid = np.ascontiguousarray(grid[:, :, :2])
+    m2, tg2, padded = dl.GridTriangulation().compute_debug(vgrid, seeds, P)
+    t0 = time.perf_counter()
+    pad_ref = O.voronoi_sorted(W + 2 * P, H + 2 * P, s + P)
+    print(f"oracle padded voronoi {time.perf_counter() - t0:.1f} s", flush=True)
+    np.testing.assert_array_equal(padded, pad_ref)
+    ref_rows, quads = O.detect(pad_ref[:, :, 0], s, P)
+    np.testing.assert_array_equal(rows, ref_rows)
+    np.testing.assert_array_equal(_rows(m2), ref_rows)
+    # assignment for these rows
+    tid = O.assign(W, H, s, ref_rows, "raster")
+    np.testing.assert_array_equal(grid[:, :, 2], tid)
+    np.testing.assert_array_equal(tg2, grid)
+
+
+@pytest.mark.parametrize("side,n,kind", [(1024, 12_500, "uniform"), (1536, 3000, "clustered"),
+                                         (777, 40_000, "uniform")])
+def test_intermediate_sizes_against_oracle(side, n, kind):
+    if kind == "uniform":
+        seeds = uniform_seeds(side, side, n, 17)
+    else:
+        seeds = clustered_seeds(side, side, n, 5, components=6, sigma=60.0)
+    ref_rows, ref = O.triangulate(side, side, seeds, None, return_rows=True)
+    m, got = dl.triangulate(side, side, seeds)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(_rows(m), ref_rows)
+
+
+def test_non_square_and_extreme_aspect():
+    for W, H, n in [(4000, 37, 900), (29, 3000, 400), (2, 2000, 30), (1500, 1, 40)]:
+        rng = np.random.default_rng(W + H)
+        idx = rng.choice(W * H, size=n, replace=False)
+        seeds = np.stack([idx % W, idx // W], 1).astype(np.int32)
+        for P in (-1, 0, 5):
+            ref_rows, ref = O.triangulate(W, H, seeds, None if P == -1 else P, return_rows=True)
+            m, got = dl.triangulate(W, H, seeds, P)
+            np.testing.assert_array_equal(got, ref)
+            np.testing.assert_array_equal(_rows(m), ref_rows)
+
+
+def test_single_seed_and_two_seeds():
+    for seeds in ([(3, 4)], [(0, 0), (9, 9)], [(5, 5), (6, 5)]):
+        ref_rows, ref = O.triangulate(10, 10, seeds, None, return_rows=True)
+        m, got = dl.triangulate(10, 10, seeds)
+        np.testing.assert_array_equal(got, ref)
+        assert len(m) == len(ref_rows)
