@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--bfs-config", type=str, default="", help="waves,rows,iters")
-    ap.add_argument("--cpu-sample-side", type=int, default=2048)
+    ap.add_argument("--cpu-sample-side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--e2e-steps", type=int, default=3)
     args = ap.parse_args()

@@ -394,14 +394,9 @@ void run_triangulate(dl_context* ctx, int W, int H, const int32_t* d_xy, int64_t
     clk.mark(2);
     clk.mark(3);
     ctx->n_tri = dr.n_tri;
-    ctx->tri.tidgrid.reserve(sizeof(int32_t) * static_cast<size_t>(W) * H);
-    assign_pixels(W, H, ctx->tri.rows.as<int32_t>(), dr.n_tri, ctx->sorted.as<uint32_t>(),
-                  ctx->tri.tidgrid.as<int32_t>(), s);
     clk.mark(4);
-    if (d_out) {
-        write_output(W, H, out_labels, out_Wc, out_off, out_off, ctx->rank_at.as<int32_t>(), W,
-                     ctx->tri.tidgrid.as<int32_t>(), d_out, 3, s);
-    }
+    assign_tiled(W, H, ctx->tri.rows.as<int32_t>(), dr.n_tri, ctx->sorted.as<uint32_t>(), out_labels,
+                 out_Wc, out_off, out_off, ctx->rank_at.as<int32_t>(), W, d_out, 3, ctx->tri, s);
     clk.mark(5);
     ctx->last_W = W;
     ctx->last_H = H;
@@ -683,8 +678,9 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
         clk.mark(3);
         ctx->n_tri = dr.n_tri;
         ctx->tri.tidgrid.reserve(sizeof(int32_t) * px);
-        assign_pixels(width, height, ctx->tri.rows.as<int32_t>(), dr.n_tri,
-                      ctx->sorted.as<uint32_t>(), ctx->tri.tidgrid.as<int32_t>(), s);
+        assign_tiled(width, height, ctx->tri.rows.as<int32_t>(), dr.n_tri,
+                     ctx->sorted.as<uint32_t>(), nullptr, 0, 0, 0, nullptr, width,
+                     ctx->tri.tidgrid.as<int32_t>(), 1, ctx->tri, s);
         clk.mark(4);
         DL_HIP(hipMemcpyAsync(out_tid_hw, ctx->tri.tidgrid.ptr, sizeof(int32_t) * px,
                               hipMemcpyDeviceToHost, s));

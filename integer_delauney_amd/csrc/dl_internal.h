@@ -147,9 +147,11 @@ struct TriWork {
     PinnedBuf host_flags;
     DevBuf table_tag, table_key, table_trip, table_blk;
     DevBuf cand;        // raw candidate records
-    DevBuf counts;
+    DevBuf counts;      // per-thread / per-block candidate counts
+    DevBuf offsets;     // their prefix sums
     DevBuf rows;        // (T, 5) triangle rows
     DevBuf tidgrid;     // W x H tid raster
+    DevBuf bin_counts, bin_offsets, bin_list;   // triangle -> tile bins
 };
 
 // Validates (bounds, duplicates) and orders seeds on the device.
@@ -162,6 +164,9 @@ void normalise_seeds(int W, int H, const int32_t* d_xy, int64_t n, uint32_t* d_s
 // host (total is out[n] written by the kernel).  out has n+1 entries.
 void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp,
                    hipStream_t s);
+
+// Exclusive prefix sum of in[0..n) into out[0..n); out[n] receives the total.
+void exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s);
 
 struct DetectResult {
     int64_t n_tri = 0;
@@ -189,6 +194,13 @@ void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, 
                   hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+// Pixel assignment binned by 64x64 tiles and fused with the output write.
+// C == 3: out is the (H, W, 3) grid, channels 0/1 from the canvas labels as in
+// write_output; C == 1: out is the (H, W) tid raster.
+void assign_tiled(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
+                  const uint32_t* labels, int Wc, int off, int shift, const int32_t* rank_at,
+                  int W_img, int32_t* out, int C, struct TriWork& work, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Per-kernel timing.  When a profiler is installed for the calling thread,

@@ -33,6 +33,13 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanBlock = kScanThreads * kScanItems;
 
+template <bool POPC>
+__device__ __forceinline__ uint32_t scan_item(uint32_t v)
+{
+    return POPC ? static_cast<uint32_t>(__popc(v)) : v;
+}
+
+template <bool POPC>
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(
     const uint32_t* __restrict__ w, int64_t n, uint32_t* __restrict__ bsum)
 {
@@ -42,7 +49,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const int64_t i = base + j * kScanThreads + threadIdx.x;
-        if (i < n) s += __popc(w[i]);
+        if (i < n) s += scan_item<POPC>(w[i]);
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -83,6 +90,7 @@ __global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* __restrict_
     if (threadIdx.x == 0) *total = carry;
 }
 
+template <bool POPC>
 __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
     const uint32_t* __restrict__ w, int64_t n, const uint32_t* __restrict__ bsum,
     uint32_t* __restrict__ out)
@@ -95,7 +103,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const int64_t i = base + j;
-        c[j] = i < n ? static_cast<uint32_t>(__popc(w[i])) : 0u;
+        c[j] = i < n ? scan_item<POPC>(w[i]) : 0u;
         s += c[j];
     }
     part[threadIdx.x] = s;
@@ -117,13 +125,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
 
 }  // namespace
 
-void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
+template <bool POPC>
+static void scan_impl(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
 {
     const int64_t nb = std::max<int64_t>(1, (n + kScanBlock - 1) / kScanBlock);
     tmp.reserve(sizeof(uint32_t) * static_cast<size_t>(nb));
     uint32_t* bsum = tmp.as<uint32_t>();
     if (n > 0) {
-        scan_reduce_kernel<<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum);
+        scan_reduce_kernel<POPC><<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum);
         DL_LAUNCH_CHECK();
     } else {
         fill_u32(bsum, 0u, 1, s);
@@ -131,9 +140,19 @@ void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp,
     scan_blocks_kernel<<<1, 1024, 0, s>>>(bsum, nb, out + n);
     DL_LAUNCH_CHECK();
     if (n > 0) {
-        scan_down_kernel<<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum, out);
+        scan_down_kernel<POPC><<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum, out);
         DL_LAUNCH_CHECK();
     }
+}
+
+void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
+{
+    scan_impl<true>(words, n, out, tmp, s);
+}
+
+void exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
+{
+    scan_impl<false>(in, n, out, tmp, s);
 }
 
 // ===========================================================================
@@ -145,7 +164,6 @@ namespace {
 constexpr uint32_t kFlagXRange = 1u;
 constexpr uint32_t kFlagYRange = 2u;
 constexpr uint32_t kFlagDuplicate = 4u;
-constexpr uint32_t kFlagCandOverflow = 8u;
 constexpr uint32_t kFlagTableOverflow = 16u;
 constexpr uint32_t kFlagCollision = 32u;
 constexpr uint32_t kFlagBadId = 64u;
@@ -247,9 +265,7 @@ struct Table {
     uint32_t* minkey;
     int32_t* trip;    // 3 per slot
     uint32_t* blk;
-    uint32_t* used;   // entry count
     uint64_t mask;
-    uint32_t limit;   // entries allowed before the table counts as full
     int exact;        // tags are exact packings of the sorted triplet
     uint64_t salt;
 };
@@ -284,8 +300,6 @@ __device__ int64_t table_insert(const Table& t, int32_t a, int32_t b, int32_t c,
                 t.trip[3 * h] = a;
                 t.trip[3 * h + 1] = b;
                 t.trip[3 * h + 2] = c;
-                const uint32_t u = atomicAdd(t.used, 1u);
-                if (u >= t.limit) atomicOr(flags, kFlagTableOverflow);
                 return static_cast<int64_t>(h);
             }
             if (prev == tg) return static_cast<int64_t>(h);
@@ -363,13 +377,20 @@ struct DetectArgs {
     int W_img;
     const uint32_t* sorted;  // id -> packed image position
     int64_t n_seeds;
+    int64_t nb;              // number of 2x2 blocks, (Wp-1)*(Hp-1)
+    uint32_t* counts;        // per-thread candidate counts (count pass)
+    const uint32_t* offsets; // their exclusive prefix sums (insert pass)
     Cand* cand;
-    uint32_t cand_cap;
-    uint32_t* cand_count;
     uint32_t* quad_count;
     uint32_t* flags;
     Table table;
 };
+
+// Each thread owns kDetItems block positions, strided by the block size so that
+// a wave reads consecutive cells.
+constexpr int kDetThreads = 256;
+constexpr int kDetItems = 16;
+constexpr int kDetChunk = kDetThreads * kDetItems;
 
 __device__ __forceinline__ int32_t to_id(const DetectArgs& A, uint32_t v, uint32_t* flags)
 {
@@ -396,99 +417,96 @@ __device__ __forceinline__ void seed_pos(const DetectArgs& A, int32_t id, int64_
     y = static_cast<int64_t>(l & 0xFFFFu);
 }
 
-__global__ __launch_bounds__(256) void detect_kernel(DetectArgs A)
+// The four corners of block i (TL, TR, BL, BR) and which of the four L-scans
+// (triangulation.py:174,177,180,183) see three distinct regions there.
+struct BlockCorners {
+    uint32_t tl, tr, bl, br;
+    bool v[4];
+    int distinct;
+};
+
+__device__ __forceinline__ BlockCorners block_corners(const DetectArgs& A, int64_t i)
 {
+    BlockCorners c;
     const int bw = A.Wp - 1;
-    const int64_t nb = static_cast<int64_t>(bw) * (A.Hp - 1);
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    int32_t tri[4][3];
-    bool valid[4] = {false, false, false, false};
-    uint32_t slot[4] = {0, 0, 0, 0};
-    int rx = 0, ry = 0;
-    if (i < nb) {
-        ry = static_cast<int>(i / bw);
-        rx = static_cast<int>(i - static_cast<int64_t>(ry) * bw);
-        const size_t o = static_cast<size_t>(ry) * A.Wp + rx;
-        const uint32_t g_tl = A.grid[o], g_tr = A.grid[o + 1];
-        const uint32_t g_bl = A.grid[o + A.Wp], g_br = A.grid[o + A.Wp + 1];
-        // At least three distinct values are needed for any triangle.
-        const int distinct = 1 + (g_tr != g_tl) + (g_bl != g_tl && g_bl != g_tr) +
-                             (g_br != g_tl && g_br != g_tr && g_br != g_bl);
-        if (distinct >= 3) {
-            const int32_t tl = to_id(A, g_tl, A.flags), tr = to_id(A, g_tr, A.flags);
-            const int32_t bl = to_id(A, g_bl, A.flags), br = to_id(A, g_br, A.flags);
-            if (tl >= 0 && tr >= 0 && bl >= 0 && br >= 0) {
-                // scan order triangulation.py:174,177,180,183
-                const int32_t s[4][3] = {{tl, tr, br}, {tl, tr, bl}, {bl, br, tl}, {bl, br, tr}};
-                for (int k = 0; k < 4; ++k) {
-                    tri[k][0] = s[k][0];
-                    tri[k][1] = s[k][1];
-                    tri[k][2] = s[k][2];
-                    valid[k] = s[k][0] != s[k][1] && s[k][0] != s[k][2] && s[k][1] != s[k][2];
-                }
-                if (distinct == 4) {
-                    const int32_t ids[4] = {tl, tr, bl, br};
-                    int64_t px[4], py[4];
-                    for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
-                    int32_t blk[2][3];
-                    quad_rule(ids, px, py, blk);
-                    atomicAdd(A.quad_count, 1u);
-                    for (int j = 0; j < 2; ++j) {
-                        int32_t a = blk[j][0], b = blk[j][1], c = blk[j][2];
-                        sort3(a, b, c);
-                        const int64_t sl = table_insert(A.table, a, b, c, A.flags);
-                        if (sl >= 0) A.table.blk[sl] = 1u;
-                    }
-                }
-                for (int k = 0; k < 4; ++k) {
-                    if (!valid[k]) continue;
-                    int32_t a = tri[k][0], b = tri[k][1], c = tri[k][2];
-                    sort3(a, b, c);
-                    const int64_t sl = table_insert(A.table, a, b, c, A.flags);
-                    if (sl < 0) {
-                        valid[k] = false;
-                        continue;
-                    }
-                    slot[k] = static_cast<uint32_t>(sl);
-                    const uint32_t key = static_cast<uint32_t>(k) * static_cast<uint32_t>(nb) +
-                                         static_cast<uint32_t>(i);
-                    atomicMin(&A.table.minkey[sl], key);
-                }
+    const int ry = static_cast<int>(i / bw);
+    const int rx = static_cast<int>(i - static_cast<int64_t>(ry) * bw);
+    const size_t o = static_cast<size_t>(ry) * A.Wp + rx;
+    c.tl = A.grid[o];
+    c.tr = A.grid[o + 1];
+    c.bl = A.grid[o + A.Wp];
+    c.br = A.grid[o + A.Wp + 1];
+    const bool e_tl_tr = c.tl == c.tr, e_tl_bl = c.tl == c.bl, e_tl_br = c.tl == c.br;
+    const bool e_tr_bl = c.tr == c.bl, e_tr_br = c.tr == c.br, e_bl_br = c.bl == c.br;
+    c.v[0] = !e_tl_tr && !e_tl_br && !e_tr_br;   // (TL, TR, BR)
+    c.v[1] = !e_tl_tr && !e_tl_bl && !e_tr_bl;   // (TL, TR, BL)
+    c.v[2] = !e_bl_br && !e_tl_bl && !e_tl_br;   // (BL, BR, TL)
+    c.v[3] = !e_bl_br && !e_tr_bl && !e_tr_br;   // (BL, BR, TR)
+    c.distinct = (c.v[0] && c.v[1] && c.v[2] && c.v[3]) ? 4 : 0;
+    return c;
+}
+
+__global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
+{
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kDetChunk + threadIdx.x;
+    uint32_t cnt = 0, quads = 0;
+#pragma unroll 4
+    for (int j = 0; j < kDetItems; ++j) {
+        const int64_t i = base + static_cast<int64_t>(j) * kDetThreads;
+        if (i >= A.nb) break;
+        const BlockCorners c = block_corners(A, i);
+        cnt += c.v[0] + c.v[1] + c.v[2] + c.v[3];
+        quads += c.distinct == 4;
+    }
+    A.counts[static_cast<size_t>(blockIdx.x) * kDetThreads + threadIdx.x] = cnt;
+    for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
+    if ((threadIdx.x & 63) == 0 && quads) atomicAdd(A.quad_count, quads);
+}
+
+__global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A)
+{
+    const size_t gt = static_cast<size_t>(blockIdx.x) * kDetThreads + threadIdx.x;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kDetChunk + threadIdx.x;
+    uint32_t out = A.offsets[gt];
+    for (int j = 0; j < kDetItems; ++j) {
+        const int64_t i = base + static_cast<int64_t>(j) * kDetThreads;
+        if (i >= A.nb) break;
+        const BlockCorners c = block_corners(A, i);
+        if (!(c.v[0] | c.v[1] | c.v[2] | c.v[3])) continue;
+        const int32_t tl = to_id(A, c.tl, A.flags), tr = to_id(A, c.tr, A.flags);
+        const int32_t bl = to_id(A, c.bl, A.flags), br = to_id(A, c.br, A.flags);
+        if (tl < 0 || tr < 0 || bl < 0 || br < 0) return;
+        if (c.distinct == 4) {
+            const int32_t ids[4] = {tl, tr, bl, br};
+            int64_t px[4], py[4];
+            for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
+            int32_t blk[2][3];
+            quad_rule(ids, px, py, blk);
+            for (int q = 0; q < 2; ++q) {
+                int32_t a = blk[q][0], b = blk[q][1], cc = blk[q][2];
+                sort3(a, b, cc);
+                const int64_t sl = table_insert(A.table, a, b, cc, A.flags);
+                if (sl >= 0) A.table.blk[sl] = 1u;
             }
         }
-    }
-    // Wave-aggregated append of this thread's candidates.
-    const int lane = threadIdx.x & 63;
-    uint32_t cnt[4];
-    uint64_t bal[4];
-    uint32_t total = 0;
-    for (int k = 0; k < 4; ++k) {
-        bal[k] = __ballot(valid[k]);
-        cnt[k] = static_cast<uint32_t>(__popcll(bal[k]));
-        total += cnt[k];
-    }
-    if (total == 0) return;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(A.cand_count, total);
-    base = __shfl(base, 0, 64);
-    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t off = base;
-    for (int k = 0; k < 4; ++k) {
-        if (valid[k]) {
-            const uint32_t pos = off + static_cast<uint32_t>(__popcll(bal[k] & lower));
-            if (pos < A.cand_cap) {
-                Cand cd;
-                cd.key = static_cast<uint32_t>(k) * static_cast<uint32_t>(nb) + static_cast<uint32_t>(i);
-                cd.slot = slot[k];
-                cd.a = tri[k][0];
-                cd.b = tri[k][1];
-                cd.c = tri[k][2];
-                A.cand[pos] = cd;
-            } else {
-                atomicOr(A.flags, kFlagCandOverflow);
-            }
+        const int32_t tri[4][3] = {{tl, tr, br}, {tl, tr, bl}, {bl, br, tl}, {bl, br, tr}};
+        for (int k = 0; k < 4; ++k) {
+            if (!c.v[k]) continue;
+            int32_t a = tri[k][0], b = tri[k][1], cc = tri[k][2];
+            sort3(a, b, cc);
+            const int64_t sl = table_insert(A.table, a, b, cc, A.flags);
+            if (sl < 0) return;
+            const uint32_t key = static_cast<uint32_t>(k) * static_cast<uint32_t>(A.nb) +
+                                 static_cast<uint32_t>(i);
+            atomicMin(&A.table.minkey[sl], key);
+            Cand cd;
+            cd.key = key;
+            cd.slot = static_cast<uint32_t>(sl);
+            cd.a = tri[k][0];
+            cd.b = tri[k][1];
+            cd.c = tri[k][2];
+            A.cand[out++] = cd;
         }
-        off += cnt[k];
     }
 }
 
@@ -497,11 +515,9 @@ __device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
     return t.minkey[cd.slot] == cd.key && t.blk[cd.slot] == 0u;
 }
 
-__global__ void mark_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ count,
-                            uint32_t cap, Table t, uint32_t* __restrict__ bm,
-                            uint32_t* __restrict__ flags)
+__global__ void mark_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
+                            uint32_t* __restrict__ bm, uint32_t* __restrict__ flags)
 {
-    const uint32_t n = min(*count, cap);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Cand cd = cand[i];
@@ -514,12 +530,10 @@ __global__ void mark_kernel(const Cand* __restrict__ cand, const uint32_t* __res
     if (is_first(cd, t)) atomicOr(&bm[cd.key >> 5], 1u << (cd.key & 31));
 }
 
-__global__ void emit_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ count,
-                            uint32_t cap, Table t, const uint32_t* __restrict__ bm,
-                            const uint32_t* __restrict__ pre, int Wp, uint32_t nb, int P,
-                            int32_t* __restrict__ rows)
+__global__ void emit_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
+                            const uint32_t* __restrict__ bm, const uint32_t* __restrict__ pre, int Wp,
+                            uint32_t nb, int P, int32_t* __restrict__ rows)
 {
-    const uint32_t n = min(*count, cap);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Cand cd = cand[i];
@@ -571,94 +585,93 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         work.rows.reserve(sizeof(int32_t) * 5);
         return res;
     }
-    // flags layout: [0] status bits, [1] candidate count, [2] quad count,
-    // [3] table entries.
-    uint64_t cand_cap = std::max<uint64_t>(16 * static_cast<uint64_t>(n_seeds) + 65536, 1 << 16);
-    cand_cap = std::min<uint64_t>(cand_cap, 4 * static_cast<uint64_t>(nb));
-    cand_cap = std::min<uint64_t>(cand_cap, 0xFFFFFFF0ull);
+    // flags layout: [0] status bits, [2] quad count.
+    const int64_t nblk = (nb + kDetChunk - 1) / kDetChunk;
+    const int64_t nthr = nblk * kDetThreads;
+    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr));
+    work.offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 1));
+    DetectArgs A{};
+    A.grid = grid;
+    A.Wp = Wp;
+    A.Hp = Hp;
+    A.P = P;
+    A.labels = labels ? 1 : 0;
+    A.rank_at = rank_at;
+    A.W_img = W_img;
+    A.sorted = sorted;
+    A.n_seeds = n_seeds;
+    A.nb = nb;
+    A.counts = work.counts.as<uint32_t>();
+    A.offsets = work.offsets.as<uint32_t>();
+    A.quad_count = dflags + 2;
+    A.flags = dflags;
+    {
+        ProfScope ps(kKDetect, s);
+        zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
+        detect_count_kernel<<<static_cast<unsigned>(nblk), kDetThreads, 0, s>>>(A);
+        DL_LAUNCH_CHECK();
+        exclusive_scan_u32(A.counts, nthr, work.offsets.as<uint32_t>(), work.scan_tmp, s);
+    }
+    DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(hflags + 4, work.offsets.as<uint32_t>() + nthr, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    const uint64_t ncand = hflags[4];
+    const uint64_t nquads = hflags[2];
+    // Distinct triplets <= candidates + 2 per quad: at most half full.
+    const uint64_t table_cap = next_pow2(2 * (ncand + 2 * nquads) + 1024);
+    work.cand.reserve(sizeof(Cand) * std::max<uint64_t>(ncand, 1));
+    work.table_tag.reserve(sizeof(uint64_t) * table_cap);
+    work.table_key.reserve(sizeof(uint32_t) * table_cap);
+    work.table_trip.reserve(sizeof(int32_t) * 3 * table_cap);
+    work.table_blk.reserve(sizeof(uint32_t) * table_cap);
+    const int64_t nbits = 4 * nb;
+    const int64_t nwords = (nbits + 31) / 32;
+    work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords));
+    work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords + 1));
+    uint32_t* bm = work.bitmap.as<uint32_t>();
+    uint32_t* pre = work.scan.as<uint32_t>();
     uint64_t salt = 0x243F6A8885A308D3ull;
     const int exact = n_seeds <= (1 << 21) ? 1 : 0;
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        const uint64_t table_cap = next_pow2(2 * cand_cap + 1024);
-        work.cand.reserve(sizeof(Cand) * cand_cap);
-        work.table_tag.reserve(sizeof(uint64_t) * table_cap);
-        work.table_key.reserve(sizeof(uint32_t) * table_cap);
-        work.table_trip.reserve(sizeof(int32_t) * 3 * table_cap);
-        work.table_blk.reserve(sizeof(uint32_t) * table_cap);
+    for (int attempt = 0; attempt < 4; ++attempt) {
         Table t;
         t.tag = work.table_tag.as<unsigned long long>();
         t.minkey = work.table_key.as<uint32_t>();
         t.trip = work.table_trip.as<int32_t>();
         t.blk = work.table_blk.as<uint32_t>();
-        t.used = dflags + 3;
         t.mask = table_cap - 1;
-        t.limit = static_cast<uint32_t>(std::min<uint64_t>(table_cap * 3 / 4, 0xFFFFFFFFull));
         t.exact = exact;
         t.salt = salt;
+        A.cand = work.cand.as<Cand>();
+        A.table = t;
         {
             ProfScope ps(kKInit, s);
             fill_u32(reinterpret_cast<uint32_t*>(t.tag), 0xFFFFFFFFu, 2 * table_cap, s);
             fill_u32(t.minkey, 0xFFFFFFFFu, table_cap, s);
             fill_u32(t.blk, 0u, table_cap, s);
-            zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
-            DL_LAUNCH_CHECK();
-        }
-
-        DetectArgs A;
-        A.grid = grid;
-        A.Wp = Wp;
-        A.Hp = Hp;
-        A.P = P;
-        A.labels = labels ? 1 : 0;
-        A.rank_at = rank_at;
-        A.W_img = W_img;
-        A.sorted = sorted;
-        A.n_seeds = n_seeds;
-        A.cand = work.cand.as<Cand>();
-        A.cand_cap = static_cast<uint32_t>(cand_cap);
-        A.cand_count = dflags + 1;
-        A.quad_count = dflags + 2;
-        A.flags = dflags;
-        A.table = t;
-        {
-            ProfScope ps(kKDetect, s);
-            detect_kernel<<<static_cast<unsigned>((nb + 255) / 256), 256, 0, s>>>(A);
-            DL_LAUNCH_CHECK();
-        }
-
-        const int64_t nbits = 4 * nb;
-        const int64_t nwords = (nbits + 31) / 32;
-        work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords));
-        work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords + 1));
-        uint32_t* bm = work.bitmap.as<uint32_t>();
-        uint32_t* pre = work.scan.as<uint32_t>();
-        {
-            ProfScope ps(kKInit, s);
             fill_u32(bm, 0u, static_cast<size_t>(nwords), s);
         }
-        DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, s));
-        DL_HIP(hipStreamSynchronize(s));
-        uint32_t f = hflags[0];
-        const uint32_t ncand = hflags[1];
-        if (f & kFlagBadId) {
-            if (labels) throw runtime_error("flood fill left an undefined cell");
-            throw value_error("voronoi_grid holds a seed id outside [0, n_seeds)");
-        }
-        if (f & (kFlagCandOverflow | kFlagTableOverflow)) {
-            cand_cap = std::min<uint64_t>(std::max<uint64_t>(cand_cap * 2, ncand + 1024), 0xFFFFFFF0ull);
-            continue;
+        {
+            ProfScope ps(kKDetect, s);
+            detect_insert_kernel<<<static_cast<unsigned>(nblk), kDetThreads, 0, s>>>(A);
+            DL_LAUNCH_CHECK();
         }
         const unsigned cblocks = static_cast<unsigned>(std::max<uint64_t>(1, (ncand + 255) / 256));
         {
             ProfScope ps(kKRegister, s);
-            mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, dflags);
+            mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, bm, dflags);
             DL_LAUNCH_CHECK();
             popcount_scan(bm, nwords, pre, work.scan_tmp, s);
         }
         DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         DL_HIP(hipMemcpyAsync(hflags + 4, pre + nwords, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
-        f = hflags[0];
+        const uint32_t f = hflags[0];
+        if (f & kFlagBadId) {
+            if (labels) throw runtime_error("flood fill left an undefined cell");
+            throw value_error("voronoi_grid holds a seed id outside [0, n_seeds)");
+        }
+        if (f & kFlagTableOverflow) throw runtime_error("triangle registration table overflow");
         if (f & kFlagCollision) {
             salt = salt * 6364136223846793005ull + 1442695040888963407ull;
             continue;
@@ -667,16 +680,17 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         work.rows.reserve(sizeof(int32_t) * 5 * std::max<uint64_t>(T, 1));
         {
             ProfScope ps(kKRegister, s);
-            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.cand_count, A.cand_cap, t, bm, pre, Wp,
-                                                static_cast<uint32_t>(nb), P, work.rows.as<int32_t>());
+            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, bm, pre,
+                                                Wp, static_cast<uint32_t>(nb), P,
+                                                work.rows.as<int32_t>());
             DL_LAUNCH_CHECK();
         }
         res.n_tri = T;
-        res.n_quads = hflags[2];
-        res.n_cand = ncand;
+        res.n_quads = static_cast<int64_t>(nquads);
+        res.n_cand = static_cast<int64_t>(ncand);
         return res;
     }
-    throw runtime_error("triangle registration table could not be built");
+    throw runtime_error("triangle registration hash collided repeatedly");
 }
 
 // ===========================================================================
@@ -796,6 +810,252 @@ void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, 
     else
         output_kernel<2><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, rank_at, W_img,
                                               tidgrid, out);
+    DL_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// Tiled assignment fused with output assembly
+//
+// Triangles are binned to 64x64 pixel tiles by their clipped bounding box; one
+// workgroup per tile rasterises its triangles into an LDS tile with LDS max
+// (the reduction is order independent) and then writes the tile's output:
+// channel 2 of the (H, W, 3) grid together with channels 0/1 read from the
+// flood-fill labels, or the bare tid raster.
+// ===========================================================================
+namespace {
+
+constexpr int kTS = 64;   // tile side in pixels
+
+struct TriVerts {
+    int64_t x[3], y[3];
+    int64_t S;   // doubled signed area
+};
+
+__device__ __forceinline__ TriVerts load_tri(const int32_t* rows, const uint32_t* sorted, int64_t t)
+{
+    TriVerts v;
+    const int32_t* r = rows + 5 * t;
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t l = sorted[r[2 + k]];
+        v.x[k] = static_cast<int64_t>(l >> 16);
+        v.y[k] = static_cast<int64_t>(l & 0xFFFFu);
+    }
+    v.S = (v.x[1] - v.x[0]) * (v.y[2] - v.y[0]) - (v.y[1] - v.y[0]) * (v.x[2] - v.x[0]);
+    return v;
+}
+
+// Pixel bounding box of the triangle's closed region clipped to the image;
+// false if empty.  A collinear triple contains its whole line, so its box is
+// the line's extent across the image.
+__device__ __forceinline__ bool tri_box(const TriVerts& v, int W, int H, int& x0, int& y0, int& x1,
+                                        int& y1)
+{
+    if (v.S != 0) {
+        int64_t a = min(v.x[0], min(v.x[1], v.x[2])), b = max(v.x[0], max(v.x[1], v.x[2]));
+        int64_t c = min(v.y[0], min(v.y[1], v.y[2])), d = max(v.y[0], max(v.y[1], v.y[2]));
+        a = max<int64_t>(a, 0); b = min<int64_t>(b, W - 1);
+        c = max<int64_t>(c, 0); d = min<int64_t>(d, H - 1);
+        x0 = static_cast<int>(a); x1 = static_cast<int>(b);
+        y0 = static_cast<int>(c); y1 = static_cast<int>(d);
+        return a <= b && c <= d;
+    }
+    const int64_t dx = v.x[1] - v.x[0], dy = v.y[1] - v.y[0];
+    if (dy == 0) {
+        if (v.y[0] < 0 || v.y[0] >= H) return false;
+        x0 = 0; x1 = W - 1;
+        y0 = y1 = static_cast<int>(v.y[0]);
+        return true;
+    }
+    if (dx == 0) {
+        if (v.x[0] < 0 || v.x[0] >= W) return false;
+        y0 = 0; y1 = H - 1;
+        x0 = x1 = static_cast<int>(v.x[0]);
+        return true;
+    }
+    x0 = 0; x1 = W - 1; y0 = 0; y1 = H - 1;
+    return true;
+}
+
+// Closed x-interval of row y inside a non-degenerate triangle (empty: lo > hi).
+// Each edge function is linear in x on a row; its boundary is found in double
+// (exact here: |numerator| < 2^33, |denominator| < 2^16) and then confirmed in
+// integers.
+__device__ __forceinline__ void row_span(const TriVerts& v, int64_t y, int64_t& lo, int64_t& hi)
+{
+    const int64_t sg = v.S > 0 ? 1 : -1;
+    for (int e = 0; e < 3; ++e) {
+        const int n = e == 2 ? 0 : e + 1;
+        // edge function at (px, y): px*(Ay-By) + Ax*(By-y) - Bx*(Ay-y), times sign(S)
+        const int64_t al = sg * (v.y[e] - v.y[n]);
+        const int64_t be = sg * (v.x[e] * (v.y[n] - y) - v.x[n] * (v.y[e] - y));
+        if (al > 0) {   // px >= -be/al
+            int64_t x = static_cast<int64_t>(ceil(static_cast<double>(-be) / static_cast<double>(al)));
+            if (al * x + be < 0) ++x;
+            else if (al * (x - 1) + be >= 0) --x;
+            lo = max(lo, x);
+        } else if (al < 0) {   // px <= be/(-al)
+            int64_t x = static_cast<int64_t>(floor(static_cast<double>(be) / static_cast<double>(-al)));
+            if (al * x + be < 0) --x;
+            else if (al * (x + 1) + be >= 0) ++x;
+            hi = min(hi, x);
+        } else if (be < 0) {
+            lo = 1;
+            hi = 0;
+        }
+    }
+}
+
+__global__ void bin_count_kernel(const int32_t* __restrict__ rows, int64_t T,
+                                 const uint32_t* __restrict__ sorted, int W, int H, int tiles_x,
+                                 uint32_t* __restrict__ counts)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const TriVerts v = load_tri(rows, sorted, t);
+    int x0, y0, x1, y1;
+    if (!tri_box(v, W, H, x0, y0, x1, y1)) return;
+    for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
+        for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) atomicAdd(&counts[ty * tiles_x + tx], 1u);
+}
+
+__global__ void bin_fill_kernel(const int32_t* __restrict__ rows, int64_t T,
+                                const uint32_t* __restrict__ sorted, int W, int H, int tiles_x,
+                                const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                                uint32_t* __restrict__ list)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const TriVerts v = load_tri(rows, sorted, t);
+    int x0, y0, x1, y1;
+    if (!tri_box(v, W, H, x0, y0, x1, y1)) return;
+    for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
+        for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) {
+            const int tile = ty * tiles_x + tx;
+            const uint32_t pos = atomicAdd(&cursor[tile], 1u);
+            list[offsets[tile] + pos] = static_cast<uint32_t>(t);
+        }
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void tile_assign_kernel(
+    int W, int H, int tiles_x, const int32_t* __restrict__ rows, const uint32_t* __restrict__ sorted,
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ lab, int Wc, int off, int shift, const int32_t* __restrict__ rank_at,
+    int W_img, int32_t* __restrict__ out)
+{
+    __shared__ int32_t tid_tile[kTS * kTS];
+    const int tile = blockIdx.x;
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    const int X0 = tx * kTS, Y0 = ty * kTS;
+    const int X1 = min(X0 + kTS, W) - 1, Y1 = min(Y0 + kTS, H) - 1;
+    for (int i = threadIdx.x; i < kTS * kTS; i += blockDim.x) tid_tile[i] = -1;
+    __syncthreads();
+    const uint32_t b = offsets[tile], e = offsets[tile + 1];
+    for (uint32_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+        const uint32_t t = list[k];
+        const TriVerts v = load_tri(rows, sorted, t);
+        const int32_t id = static_cast<int32_t>(t);
+        if (v.S == 0) {
+            const int64_t dx = v.x[1] - v.x[0], dy = v.y[1] - v.y[0];
+            for (int y = Y0; y <= Y1; ++y) {
+                if (dy == 0) {
+                    if (y != v.y[0]) continue;
+                    for (int x = X0; x <= X1; ++x) atomicMax(&tid_tile[(y - Y0) * kTS + (x - X0)], id);
+                    continue;
+                }
+                const int64_t num = (y - v.y[0]) * dx;
+                if (num % dy != 0) continue;
+                const int64_t x = v.x[0] + num / dy;
+                if (x >= X0 && x <= X1) atomicMax(&tid_tile[(y - Y0) * kTS + (x - X0)], id);
+            }
+            continue;
+        }
+        const int64_t ya = max<int64_t>(Y0, min(v.y[0], min(v.y[1], v.y[2])));
+        const int64_t yb = min<int64_t>(Y1, max(v.y[0], max(v.y[1], v.y[2])));
+        for (int64_t y = ya; y <= yb; ++y) {
+            int64_t lo = X0, hi = X1;
+            row_span(v, y, lo, hi);
+            const int base = static_cast<int>(y - Y0) * kTS - X0;
+            for (int64_t x = lo; x <= hi; ++x) atomicMax(&tid_tile[base + static_cast<int>(x)], id);
+        }
+    }
+    __syncthreads();
+    // Output: each wave writes whole tile rows (lane = column).
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x = X0 + lane;
+    for (int r = wave; r < kTS; r += blockDim.x / 64) {
+        const int y = Y0 + r;
+        if (y > Y1 || x > X1) continue;
+        const size_t p = static_cast<size_t>(y) * W + x;
+        const int32_t t = tid_tile[r * kTS + lane];
+        if (C == 1) {
+            out[p] = t;
+        } else {
+            const uint32_t l = lab[static_cast<size_t>(y + off) * Wc + (x + off)];
+            int32_t id = -1, d = -1;
+            if (l != kInvalid) {
+                const int lx = static_cast<int>(l >> 16), ly = static_cast<int>(l & 0xFFFFu);
+                const int dx = x + off - lx, dy = y + off - ly;
+                id = rank_at[static_cast<size_t>(ly - shift) * W_img + (lx - shift)];
+                d = dx * dx + dy * dy;
+            }
+            int32_t* o = out + 3 * p;
+            o[0] = id;
+            o[1] = d;
+            o[2] = t;
+        }
+    }
+}
+
+}  // namespace
+
+void assign_tiled(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
+                  const uint32_t* labels, int Wc, int off, int shift, const int32_t* rank_at,
+                  int W_img, int32_t* out, int C, TriWork& work, hipStream_t s)
+{
+    const int tiles_x = (W + kTS - 1) / kTS, tiles_y = (H + kTS - 1) / kTS;
+    const int64_t ntiles = static_cast<int64_t>(tiles_x) * tiles_y;
+    work.bin_counts.reserve(sizeof(uint32_t) * static_cast<size_t>(2 * ntiles + 1));
+    work.bin_offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(ntiles + 1));
+    work.host_flags.reserve(sizeof(uint32_t) * 8);
+    uint32_t* counts = work.bin_counts.as<uint32_t>();
+    uint32_t* cursor = counts + ntiles;
+    uint32_t* offs = work.bin_offsets.as<uint32_t>();
+    uint32_t total = 0;
+    {
+        ProfScope ps(kKAssign, s);
+        fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
+        if (n_tri > 0) {
+            bin_count_kernel<<<static_cast<unsigned>((n_tri + 255) / 256), 256, 0, s>>>(
+                rows, n_tri, sorted, W, H, tiles_x, counts);
+            DL_LAUNCH_CHECK();
+        }
+        exclusive_scan_u32(counts, ntiles, offs, work.scan_tmp, s);
+    }
+    if (n_tri > 0) {
+        DL_HIP(hipMemcpyAsync(work.host_flags.ptr, offs + ntiles, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        total = work.host_flags.as<uint32_t>()[0];
+    }
+    work.bin_list.reserve(sizeof(uint32_t) * std::max<uint32_t>(total, 1));
+    {
+        ProfScope ps(kKAssign, s);
+        if (total > 0) {
+            bin_fill_kernel<<<static_cast<unsigned>((n_tri + 255) / 256), 256, 0, s>>>(
+                rows, n_tri, sorted, W, H, tiles_x, offs, cursor, work.bin_list.as<uint32_t>());
+            DL_LAUNCH_CHECK();
+        }
+    }
+    ProfScope ps(kKOutput, s);
+    if (C == 3)
+        tile_assign_kernel<3><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
+            W, H, tiles_x, rows, sorted, offs, work.bin_list.as<uint32_t>(), labels, Wc, off, shift,
+            rank_at, W_img, out);
+    else
+        tile_assign_kernel<1><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
+            W, H, tiles_x, rows, sorted, offs, work.bin_list.as<uint32_t>(), labels, Wc, off, shift,
+            rank_at, W_img, out);
     DL_LAUNCH_CHECK();
 }
 
