@@ -95,58 +95,75 @@ struct EscGeom {
     uint32_t rows_in;    // bit r set if row r lies inside the interior rows
 };
 
+// K iterations of the four passes on the tile held in registers.
+//
+// Work skipping, per wave (a band of R rows across the tile), exact by
+// monotonicity:
+//  * passes 1-2 (along x) read only the wave's own rows, so they cannot change
+//    anything if the wave's state has not changed since they last ran without
+//    change (chx and chy of the previous iteration both false);
+//  * passes 3-4 (along y) read the wave's rows and one boundary row from each
+//    neighbouring wave; they cannot change anything if, for this wave and both
+//    neighbours, nothing changed since this wave's last passes 3-4 ran without
+//    change (q = chy of the previous iteration or chx of this one, all false).
+// A tile where no wave has q set is a fixed point: the remaining iterations
+// are skipped.
 template <int NW, int R, int K, bool ESC>
 __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], uint32_t (&D0)[R],
                                         uint32_t (&D1)[R], int cxm0, int cym0, const EscGeom& e,
                                         uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
-                                        int (*lds_any)[NW], int wave, int lane, int& esc)
+                                        int (*lds_q)[NW], int wave, int lane, int& esc)
 {
     const int cxm1 = cxm0 + 1;
-    bool ch_prev = true;
+    bool chx = true, chy = true;   // changes of the previous iteration (forced on at entry)
     for (int k = 0; k < K; ++k) {
-        bool ch = false;
-        // pass 1: source x-1
+        bool cx = false;
+        if (chx || chy) {   // wave-uniform
+            bool ch = false;
+            // pass 1: source x-1
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t left = from_left(I1[r]);
-            const uint32_t own0 = I0[r];
-            const bool t1 = relax_key(I1[r], D1[r], own0, cxm1, cym0 + r);
-            const bool t0 = relax_key(I0[r], D0[r], left, cxm0, cym0 + r);
-            ch |= t0 | t1;
-            if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_left) | (t1 & e.col1_left);
-        }
-        // pass 2: source x+1
+            for (int r = 0; r < R; ++r) {
+                const uint32_t left = from_left(I1[r]);
+                const uint32_t own0 = I0[r];
+                const bool t1 = relax_key(I1[r], D1[r], own0, cxm1, cym0 + r);
+                const bool t0 = relax_key(I0[r], D0[r], left, cxm0, cym0 + r);
+                ch |= t0 | t1;
+                if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_left) | (t1 & e.col1_left);
+            }
+            // pass 2: source x+1
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t right = from_right(I0[r]);
-            const uint32_t own1 = I1[r];
-            const bool t0 = relax_key(I0[r], D0[r], own1, cxm0, cym0 + r);
-            const bool t1 = relax_key(I1[r], D1[r], right, cxm1, cym0 + r);
-            ch |= t0 | t1;
-            if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_right) | (t1 & e.col1_right);
+            for (int r = 0; r < R; ++r) {
+                const uint32_t right = from_right(I0[r]);
+                const uint32_t own1 = I1[r];
+                const bool t0 = relax_key(I0[r], D0[r], own1, cxm0, cym0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], right, cxm1, cym0 + r);
+                ch |= t0 | t1;
+                if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_right) | (t1 & e.col1_right);
+            }
+            cx = __ballot(ch) != 0ull;
         }
-        // pass 3: source y-1.  Publish this wave's last row, read the previous
-        // wave's.  The same barrier publishes whether this wave changed anything
-        // in the previous iteration: an iteration without change anywhere in the
-        // tile is a fixed point of it, so the remaining iterations can be skipped.
+        // pass 3: source y-1.  Publish this wave's last row and its q flag.
         ldsA[wave][lane][0] = I0[R - 1];
         ldsA[wave][lane][1] = I1[R - 1];
-        if (lane == 0) lds_any[k & 1][wave] = ch_prev ? 1 : 0;
+        if (lane == 0) lds_q[k & 1][wave] = (chy || cx) ? 1 : 0;
         __syncthreads();
-        int any_prev = 0;
+        int any = 0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) any_prev |= lds_any[k & 1][w];
-        if (!any_prev) break;
-        const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : 0u;
-        const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : 0u;
+        for (int w = 0; w < NW; ++w) any |= lds_q[k & 1][w];
+        if (!any) break;   // uniform across the block
+        const bool run_y = lds_q[k & 1][wave] || (wave > 0 && lds_q[k & 1][wave - 1]) ||
+                           (wave < NW - 1 && lds_q[k & 1][wave + 1]);
+        bool ch = false;
+        if (run_y) {
+            const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : 0u;
+            const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : 0u;
 #pragma unroll
-        for (int r = R - 1; r >= 1; --r) {
-            const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], cxm0, cym0 + r);
-            const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], cxm1, cym0 + r);
-            ch |= t0 | t1;
-            if (ESC && r == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
-        }
-        {
+            for (int r = R - 1; r >= 1; --r) {
+                const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], cxm0, cym0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], cxm1, cym0 + r);
+                ch |= t0 | t1;
+                if (ESC && r == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+            }
             const bool t0 = relax_key(I0[0], D0[0], top0, cxm0, cym0);
             const bool t1 = relax_key(I1[0], D1[0], top1, cxm1, cym0);
             ch |= t0 | t1;
@@ -156,22 +173,23 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
         ldsB[wave][lane][0] = I0[0];
         ldsB[wave][lane][1] = I1[0];
         __syncthreads();
-        const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : 0u;
-        const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : 0u;
+        if (run_y) {
+            const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : 0u;
+            const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : 0u;
 #pragma unroll
-        for (int r = 0; r < R - 1; ++r) {
-            const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], cxm0, cym0 + r);
-            const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], cxm1, cym0 + r);
-            ch |= t0 | t1;
-            if (ESC && r == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
-        }
-        {
+            for (int r = 0; r < R - 1; ++r) {
+                const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], cxm0, cym0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], cxm1, cym0 + r);
+                ch |= t0 | t1;
+                if (ESC && r == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+            }
             const bool t0 = relax_key(I0[R - 1], D0[R - 1], bot0, cxm0, cym0 + R - 1);
             const bool t1 = relax_key(I1[R - 1], D1[R - 1], bot1, cxm1, cym0 + R - 1);
             ch |= t0 | t1;
             if (ESC && (R - 1) == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
         }
-        ch_prev = __ballot(ch) != 0ull;
+        chx = cx;
+        chy = __ballot(ch) != 0ull;
     }
 }
 
@@ -185,7 +203,7 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     constexpr int IH = NW * R - 2 * K;
     __shared__ uint32_t ldsA[NW][64][2];
     __shared__ uint32_t ldsB[NW][64][2];
-    __shared__ int lds_any[2][NW];
+    __shared__ int lds_any[2][NW];   // per-wave flags (see iterate)
 
     // Contiguous tile ranges per XCD: consecutive dispatches go round-robin over
     // the 8 XCDs, so give each XCD a contiguous slice of tiles (neighbouring

@@ -145,7 +145,7 @@ struct TriWork {
     DevBuf scan_tmp;
     DevBuf flags;       // device status flags
     PinnedBuf host_flags;
-    DevBuf table_tag, table_key, table_trip, table_blk;
+    DevBuf table_tag;   // registration hash table slots
     DevBuf cand;        // raw candidate records
     DevBuf counts;      // per-thread / per-block candidate counts
     DevBuf offsets;     // their prefix sums

@@ -260,11 +260,17 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
+// One 32-byte slot per distinct triplet: an insert touches one cache line.
+struct alignas(32) Slot {
+    unsigned long long tag;
+    uint32_t minkey;   // smallest registration key seen for the triplet
+    uint32_t blk;      // 1: blocked by a cocircular quad
+    int32_t a, b, c;   // the sorted triplet (collision check)
+    uint32_t pad;
+};
+
 struct Table {
-    unsigned long long* tag;
-    uint32_t* minkey;
-    int32_t* trip;    // 3 per slot
-    uint32_t* blk;
+    Slot* slot;
     uint64_t mask;
     int exact;        // tags are exact packings of the sorted triplet
     uint64_t salt;
@@ -282,24 +288,25 @@ __device__ __forceinline__ uint64_t triplet_tag(const Table& t, int32_t a, int32
     return h & 0x7FFFFFFFFFFFFFFFull;
 }
 
-// Returns the slot of the sorted triplet, inserting it if absent; -1 when the
-// table is full (flag raised).
+// Returns the slot of the sorted triplet, inserting it if absent; -1 (flag
+// raised) if the table is full, which its sizing rules out.
 __device__ int64_t table_insert(const Table& t, int32_t a, int32_t b, int32_t c,
                                 uint32_t* flags)
 {
     const uint64_t tg = triplet_tag(t, a, b, c);
     uint64_t h = mix64(tg ^ 0x9E3779B97F4A7C15ull) & t.mask;
     for (uint64_t probe = 0; probe <= t.mask; ++probe) {
-        const unsigned long long cur = __atomic_load_n(&t.tag[h], __ATOMIC_RELAXED);
+        Slot& sl = t.slot[h];
+        const unsigned long long cur = __atomic_load_n(&sl.tag, __ATOMIC_RELAXED);
         if (cur == tg) return static_cast<int64_t>(h);
         if (cur == kEmptyTag) {
             const unsigned long long prev =
-                atomicCAS(&t.tag[h], static_cast<unsigned long long>(kEmptyTag),
+                atomicCAS(&sl.tag, static_cast<unsigned long long>(kEmptyTag),
                           static_cast<unsigned long long>(tg));
             if (prev == kEmptyTag) {
-                t.trip[3 * h] = a;
-                t.trip[3 * h + 1] = b;
-                t.trip[3 * h + 2] = c;
+                sl.a = a;
+                sl.b = b;
+                sl.c = c;
                 return static_cast<int64_t>(h);
             }
             if (prev == tg) return static_cast<int64_t>(h);
@@ -308,6 +315,14 @@ __device__ int64_t table_insert(const Table& t, int32_t a, int32_t b, int32_t c,
     }
     atomicOr(flags, kFlagTableOverflow);
     return -1;
+}
+
+__global__ void table_init_kernel(Slot* __restrict__ slots, uint64_t n)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint4* p = reinterpret_cast<uint4*>(slots + i);
+    p[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);   // tag, minkey, blk
 }
 
 // Orientation class of a direction for the reference's arctan2 ordering:
@@ -377,7 +392,8 @@ struct DetectArgs {
     int W_img;
     const uint32_t* sorted;  // id -> packed image position
     int64_t n_seeds;
-    int64_t nb;              // number of 2x2 blocks, (Wp-1)*(Hp-1)
+    uint32_t nb;             // number of 2x2 blocks, (Wp-1)*(Hp-1)
+    int bx_per_row;          // thread blocks per block row
     uint32_t* counts;        // per-thread candidate counts (count pass)
     const uint32_t* offsets; // their exclusive prefix sums (insert pass)
     Cand* cand;
@@ -386,10 +402,12 @@ struct DetectArgs {
     Table table;
 };
 
-// Each thread owns kDetItems block positions, strided by the block size so that
-// a wave reads consecutive cells.
+// A thread block covers kDetItems*kDetThreads consecutive 2x2 blocks of one
+// block row (grid: x = chunk of the row, y = row).  Each thread owns
+// kDetItems of them, strided by the thread count so that a wave reads
+// consecutive cells.
 constexpr int kDetThreads = 256;
-constexpr int kDetItems = 16;
+constexpr int kDetItems = 4;
 constexpr int kDetChunk = kDetThreads * kDetItems;
 
 __device__ __forceinline__ int32_t to_id(const DetectArgs& A, uint32_t v, uint32_t* flags)
@@ -417,20 +435,17 @@ __device__ __forceinline__ void seed_pos(const DetectArgs& A, int32_t id, int64_
     y = static_cast<int64_t>(l & 0xFFFFu);
 }
 
-// The four corners of block i (TL, TR, BL, BR) and which of the four L-scans
-// (triangulation.py:174,177,180,183) see three distinct regions there.
+// The four corners of a 2x2 block (TL, TR, BL, BR) and which of the four
+// L-scans (triangulation.py:174,177,180,183) see three distinct regions there.
 struct BlockCorners {
     uint32_t tl, tr, bl, br;
     bool v[4];
-    int distinct;
+    bool quad;
 };
 
-__device__ __forceinline__ BlockCorners block_corners(const DetectArgs& A, int64_t i)
+__device__ __forceinline__ BlockCorners block_corners(const DetectArgs& A, int rx, int ry)
 {
     BlockCorners c;
-    const int bw = A.Wp - 1;
-    const int ry = static_cast<int>(i / bw);
-    const int rx = static_cast<int>(i - static_cast<int64_t>(ry) * bw);
     const size_t o = static_cast<size_t>(ry) * A.Wp + rx;
     c.tl = A.grid[o];
     c.tr = A.grid[o + 1];
@@ -442,77 +457,108 @@ __device__ __forceinline__ BlockCorners block_corners(const DetectArgs& A, int64
     c.v[1] = !e_tl_tr && !e_tl_bl && !e_tr_bl;   // (TL, TR, BL)
     c.v[2] = !e_bl_br && !e_tl_bl && !e_tl_br;   // (BL, BR, TL)
     c.v[3] = !e_bl_br && !e_tr_bl && !e_tr_br;   // (BL, BR, TR)
-    c.distinct = (c.v[0] && c.v[1] && c.v[2] && c.v[3]) ? 4 : 0;
+    c.quad = c.v[0] && c.v[1] && c.v[2] && c.v[3];
     return c;
+}
+
+// Candidates a block contributes.  With three distinct regions every valid
+// scan names the same triplet, and only the lowest scan's key can be that
+// triplet's first registration: one candidate.  A four-region block blocks
+// two of its four triplets globally; they can never register, so only the
+// two triangles of the chosen diagonal are candidates.
+__device__ __forceinline__ uint32_t block_candidates(const BlockCorners& c)
+{
+    if (c.quad) return 2;
+    return (c.v[0] | c.v[1] | c.v[2] | c.v[3]) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
 {
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kDetChunk + threadIdx.x;
+    const int ry = blockIdx.y;
+    const int rx0 = blockIdx.x * kDetChunk + threadIdx.x;
+    const int bw = A.Wp - 1;
     uint32_t cnt = 0, quads = 0;
-#pragma unroll 4
+#pragma unroll
     for (int j = 0; j < kDetItems; ++j) {
-        const int64_t i = base + static_cast<int64_t>(j) * kDetThreads;
-        if (i >= A.nb) break;
-        const BlockCorners c = block_corners(A, i);
-        cnt += c.v[0] + c.v[1] + c.v[2] + c.v[3];
-        quads += c.distinct == 4;
+        const int rx = rx0 + j * kDetThreads;
+        if (rx >= bw) break;
+        const BlockCorners c = block_corners(A, rx, ry);
+        cnt += block_candidates(c);
+        quads += c.quad;
     }
-    A.counts[static_cast<size_t>(blockIdx.x) * kDetThreads + threadIdx.x] = cnt;
+    A.counts[(static_cast<size_t>(ry) * A.bx_per_row + blockIdx.x) * kDetThreads + threadIdx.x] = cnt;
     for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
     if ((threadIdx.x & 63) == 0 && quads) atomicAdd(A.quad_count, quads);
 }
 
+__device__ __forceinline__ void add_candidate(const DetectArgs& A, uint32_t& out, int scan,
+                                              uint32_t pos, int32_t a, int32_t b, int32_t c)
+{
+    int32_t sa = a, sb = b, sc = c;
+    sort3(sa, sb, sc);
+    const int64_t sl = table_insert(A.table, sa, sb, sc, A.flags);
+    if (sl < 0) return;
+    const uint32_t key = static_cast<uint32_t>(scan) * A.nb + pos;
+    atomicMin(&A.table.slot[sl].minkey, key);
+    Cand cd;
+    cd.key = key;
+    cd.slot = static_cast<uint32_t>(sl);
+    cd.a = a;
+    cd.b = b;
+    cd.c = c;
+    A.cand[out++] = cd;
+}
+
 __global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A)
 {
-    const size_t gt = static_cast<size_t>(blockIdx.x) * kDetThreads + threadIdx.x;
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kDetChunk + threadIdx.x;
-    uint32_t out = A.offsets[gt];
+    const int ry = blockIdx.y;
+    const int rx0 = blockIdx.x * kDetChunk + threadIdx.x;
+    const int bw = A.Wp - 1;
+    uint32_t out = A.offsets[(static_cast<size_t>(ry) * A.bx_per_row + blockIdx.x) * kDetThreads +
+                             threadIdx.x];
     for (int j = 0; j < kDetItems; ++j) {
-        const int64_t i = base + static_cast<int64_t>(j) * kDetThreads;
-        if (i >= A.nb) break;
-        const BlockCorners c = block_corners(A, i);
+        const int rx = rx0 + j * kDetThreads;
+        if (rx >= bw) break;
+        const BlockCorners c = block_corners(A, rx, ry);
         if (!(c.v[0] | c.v[1] | c.v[2] | c.v[3])) continue;
         const int32_t tl = to_id(A, c.tl, A.flags), tr = to_id(A, c.tr, A.flags);
         const int32_t bl = to_id(A, c.bl, A.flags), br = to_id(A, c.br, A.flags);
         if (tl < 0 || tr < 0 || bl < 0 || br < 0) return;
-        if (c.distinct == 4) {
-            const int32_t ids[4] = {tl, tr, bl, br};
-            int64_t px[4], py[4];
-            for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
-            int32_t blk[2][3];
-            quad_rule(ids, px, py, blk);
-            for (int q = 0; q < 2; ++q) {
-                int32_t a = blk[q][0], b = blk[q][1], cc = blk[q][2];
-                sort3(a, b, cc);
-                const int64_t sl = table_insert(A.table, a, b, cc, A.flags);
-                if (sl >= 0) A.table.blk[sl] = 1u;
-            }
-        }
+        const uint32_t pos = static_cast<uint32_t>(ry) * static_cast<uint32_t>(bw) +
+                             static_cast<uint32_t>(rx);
         const int32_t tri[4][3] = {{tl, tr, br}, {tl, tr, bl}, {bl, br, tl}, {bl, br, tr}};
-        for (int k = 0; k < 4; ++k) {
-            if (!c.v[k]) continue;
-            int32_t a = tri[k][0], b = tri[k][1], cc = tri[k][2];
+        if (!c.quad) {
+            int k = 0;
+            while (!c.v[k]) ++k;
+            add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
+            continue;
+        }
+        const int32_t ids[4] = {tl, tr, bl, br};
+        int64_t px[4], py[4];
+        for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
+        int32_t blk[2][3];
+        quad_rule(ids, px, py, blk);
+        bool blocked[4] = {false, false, false, false};
+        for (int q = 0; q < 2; ++q) {
+            int32_t a = blk[q][0], b = blk[q][1], cc = blk[q][2];
             sort3(a, b, cc);
             const int64_t sl = table_insert(A.table, a, b, cc, A.flags);
-            if (sl < 0) return;
-            const uint32_t key = static_cast<uint32_t>(k) * static_cast<uint32_t>(A.nb) +
-                                 static_cast<uint32_t>(i);
-            atomicMin(&A.table.minkey[sl], key);
-            Cand cd;
-            cd.key = key;
-            cd.slot = static_cast<uint32_t>(sl);
-            cd.a = tri[k][0];
-            cd.b = tri[k][1];
-            cd.c = tri[k][2];
-            A.cand[out++] = cd;
+            if (sl >= 0) A.table.slot[sl].blk = 1u;
+            for (int k = 0; k < 4; ++k) {
+                int32_t x = tri[k][0], y = tri[k][1], z = tri[k][2];
+                sort3(x, y, z);
+                if (x == a && y == b && z == cc) blocked[k] = true;
+            }
         }
+        for (int k = 0; k < 4; ++k)
+            if (!blocked[k]) add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
     }
 }
 
 __device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
 {
-    return t.minkey[cd.slot] == cd.key && t.blk[cd.slot] == 0u;
+    const Slot& sl = t.slot[cd.slot];
+    return sl.minkey == cd.key && sl.blk == 0u;
 }
 
 __global__ void mark_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
@@ -523,7 +569,8 @@ __global__ void mark_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
     const Cand cd = cand[i];
     int32_t a = cd.a, b = cd.b, c = cd.c;
     sort3(a, b, c);
-    if (t.trip[3 * cd.slot] != a || t.trip[3 * cd.slot + 1] != b || t.trip[3 * cd.slot + 2] != c) {
+    const Slot& sl = t.slot[cd.slot];
+    if (sl.a != a || sl.b != b || sl.c != c) {
         atomicOr(flags, kFlagCollision);
         return;
     }
@@ -586,8 +633,9 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         return res;
     }
     // flags layout: [0] status bits, [2] quad count.
-    const int64_t nblk = (nb + kDetChunk - 1) / kDetChunk;
-    const int64_t nthr = nblk * kDetThreads;
+    const int bw = Wp - 1, bh = Hp - 1;
+    const int bx_per_row = (bw + kDetChunk - 1) / kDetChunk;
+    const int64_t nthr = static_cast<int64_t>(bx_per_row) * bh * kDetThreads;
     work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr));
     work.offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 1));
     DetectArgs A{};
@@ -600,15 +648,17 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     A.W_img = W_img;
     A.sorted = sorted;
     A.n_seeds = n_seeds;
-    A.nb = nb;
+    A.nb = static_cast<uint32_t>(nb);
+    A.bx_per_row = bx_per_row;
     A.counts = work.counts.as<uint32_t>();
     A.offsets = work.offsets.as<uint32_t>();
     A.quad_count = dflags + 2;
     A.flags = dflags;
+    const dim3 dgrid(static_cast<unsigned>(bx_per_row), static_cast<unsigned>(bh));
     {
         ProfScope ps(kKDetect, s);
         zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
-        detect_count_kernel<<<static_cast<unsigned>(nblk), kDetThreads, 0, s>>>(A);
+        detect_count_kernel<<<dgrid, kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
         exclusive_scan_u32(A.counts, nthr, work.offsets.as<uint32_t>(), work.scan_tmp, s);
     }
@@ -621,10 +671,7 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     // Distinct triplets <= candidates + 2 per quad: at most half full.
     const uint64_t table_cap = next_pow2(2 * (ncand + 2 * nquads) + 1024);
     work.cand.reserve(sizeof(Cand) * std::max<uint64_t>(ncand, 1));
-    work.table_tag.reserve(sizeof(uint64_t) * table_cap);
-    work.table_key.reserve(sizeof(uint32_t) * table_cap);
-    work.table_trip.reserve(sizeof(int32_t) * 3 * table_cap);
-    work.table_blk.reserve(sizeof(uint32_t) * table_cap);
+    work.table_tag.reserve(sizeof(Slot) * table_cap);
     const int64_t nbits = 4 * nb;
     const int64_t nwords = (nbits + 31) / 32;
     work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords));
@@ -635,10 +682,7 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     const int exact = n_seeds <= (1 << 21) ? 1 : 0;
     for (int attempt = 0; attempt < 4; ++attempt) {
         Table t;
-        t.tag = work.table_tag.as<unsigned long long>();
-        t.minkey = work.table_key.as<uint32_t>();
-        t.trip = work.table_trip.as<int32_t>();
-        t.blk = work.table_blk.as<uint32_t>();
+        t.slot = work.table_tag.as<Slot>();
         t.mask = table_cap - 1;
         t.exact = exact;
         t.salt = salt;
@@ -646,14 +690,14 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         A.table = t;
         {
             ProfScope ps(kKInit, s);
-            fill_u32(reinterpret_cast<uint32_t*>(t.tag), 0xFFFFFFFFu, 2 * table_cap, s);
-            fill_u32(t.minkey, 0xFFFFFFFFu, table_cap, s);
-            fill_u32(t.blk, 0u, table_cap, s);
+            table_init_kernel<<<static_cast<unsigned>((table_cap + 255) / 256), 256, 0, s>>>(
+                t.slot, table_cap);
+            DL_LAUNCH_CHECK();
             fill_u32(bm, 0u, static_cast<size_t>(nwords), s);
         }
         {
             ProfScope ps(kKDetect, s);
-            detect_insert_kernel<<<static_cast<unsigned>(nblk), kDetThreads, 0, s>>>(A);
+            detect_insert_kernel<<<dgrid, kDetThreads, 0, s>>>(A);
             DL_LAUNCH_CHECK();
         }
         const unsigned cblocks = static_cast<unsigned>(std::max<uint64_t>(1, (ncand + 255) / 256));
@@ -980,29 +1024,60 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
         }
     }
     __syncthreads();
-    // Output: each wave writes whole tile rows (lane = column).
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x = X0 + lane;
-    for (int r = wave; r < kTS; r += blockDim.x / 64) {
+    // Output.  Each thread writes four consecutive pixels of a tile row: for
+    // C == 3 that is 48 bytes, three 16-byte stores when W is a multiple of 4.
+    const int q = threadIdx.x & 15;            // quad index within the row
+    const int x = X0 + 4 * q;
+    for (int r = threadIdx.x >> 4; r < kTS; r += blockDim.x / 16) {
         const int y = Y0 + r;
         if (y > Y1 || x > X1) continue;
         const size_t p = static_cast<size_t>(y) * W + x;
-        const int32_t t = tid_tile[r * kTS + lane];
+        const int4 t4 = *reinterpret_cast<const int4*>(&tid_tile[r * kTS + 4 * q]);
+        const int32_t tv[4] = {t4.x, t4.y, t4.z, t4.w};
+        const int n = min(4, X1 - x + 1);
         if (C == 1) {
-            out[p] = t;
-        } else {
-            const uint32_t l = lab[static_cast<size_t>(y + off) * Wc + (x + off)];
-            int32_t id = -1, d = -1;
-            if (l != kInvalid) {
-                const int lx = static_cast<int>(l >> 16), ly = static_cast<int>(l & 0xFFFFu);
-                const int dx = x + off - lx, dy = y + off - ly;
-                id = rank_at[static_cast<size_t>(ly - shift) * W_img + (lx - shift)];
-                d = dx * dx + dy * dy;
+            if (n == 4 && (W & 3) == 0) {
+                *reinterpret_cast<int4*>(out + p) = t4;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k < n) out[p + k] = tv[k];
             }
-            int32_t* o = out + 3 * p;
-            o[0] = id;
-            o[1] = d;
-            o[2] = t;
+            continue;
+        }
+        int32_t o[12];
+        const size_t lrow = static_cast<size_t>(y + off) * Wc + off;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int32_t id = -1, d = -1;
+            if (k < n) {
+                const uint32_t l = lab[lrow + x + k];
+                if (l != kInvalid) {
+                    const int lx = static_cast<int>(l >> 16), ly = static_cast<int>(l & 0xFFFFu);
+                    const int dx = x + k + off - lx, dy = y + off - ly;
+                    id = rank_at[static_cast<size_t>(ly - shift) * W_img + (lx - shift)];
+                    d = dx * dx + dy * dy;
+                }
+            }
+            o[3 * k] = id;
+            o[3 * k + 1] = d;
+            o[3 * k + 2] = tv[k];
+        }
+        int32_t* dst = out + 3 * p;
+        if (n == 4 && (W & 3) == 0) {
+            int4* d4 = reinterpret_cast<int4*>(dst);
+            d4[0] = make_int4(o[0], o[1], o[2], o[3]);
+            d4[1] = make_int4(o[4], o[5], o[6], o[7]);
+            d4[2] = make_int4(o[8], o[9], o[10], o[11]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k < n) {
+                    dst[3 * k] = o[3 * k];
+                    dst[3 * k + 1] = o[3 * k + 1];
+                    dst[3 * k + 2] = o[3 * k + 2];
+                }
+            }
         }
     }
 }
