@@ -395,9 +395,9 @@ struct DetectArgs {
     uint32_t nb;             // number of 2x2 blocks, (Wp-1)*(Hp-1)
     int bx_per_row;          // thread blocks per block row
     uint32_t* counts;        // per-thread candidate counts (count pass)
+    uint32_t* block_quads;   // per-thread-block quad counts (count pass)
     const uint32_t* offsets; // their exclusive prefix sums (insert pass)
     Cand* cand;
-    uint32_t* quad_count;
     uint32_t* flags;
     Table table;
 };
@@ -486,9 +486,18 @@ __global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
         cnt += block_candidates(c);
         quads += c.quad;
     }
-    A.counts[(static_cast<size_t>(ry) * A.bx_per_row + blockIdx.x) * kDetThreads + threadIdx.x] = cnt;
+    const size_t blk = static_cast<size_t>(ry) * A.bx_per_row + blockIdx.x;
+    A.counts[blk * kDetThreads + threadIdx.x] = cnt;
+    // Quads per thread block (a single counter would serialise ~10^4 atomics).
+    __shared__ uint32_t red[kDetThreads / 64];
     for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
-    if ((threadIdx.x & 63) == 0 && quads) atomicAdd(A.quad_count, quads);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = quads;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kDetThreads / 64; ++w) t += red[w];
+        A.block_quads[blk] = t;
+    }
 }
 
 __device__ __forceinline__ void add_candidate(const DetectArgs& A, uint32_t& out, int scan,
@@ -636,7 +645,8 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     const int bw = Wp - 1, bh = Hp - 1;
     const int bx_per_row = (bw + kDetChunk - 1) / kDetChunk;
     const int64_t nthr = static_cast<int64_t>(bx_per_row) * bh * kDetThreads;
-    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr));
+    const int64_t nblocks = static_cast<int64_t>(bx_per_row) * bh;
+    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 2 * (nblocks + 1)));
     work.offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 1));
     DetectArgs A{};
     A.grid = grid;
@@ -652,7 +662,8 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     A.bx_per_row = bx_per_row;
     A.counts = work.counts.as<uint32_t>();
     A.offsets = work.offsets.as<uint32_t>();
-    A.quad_count = dflags + 2;
+    A.block_quads = work.counts.as<uint32_t>() + nthr;
+    uint32_t* quad_scan = A.block_quads + nblocks + 1;
     A.flags = dflags;
     const dim3 dgrid(static_cast<unsigned>(bx_per_row), static_cast<unsigned>(bh));
     {
@@ -661,8 +672,9 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         detect_count_kernel<<<dgrid, kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
         exclusive_scan_u32(A.counts, nthr, work.offsets.as<uint32_t>(), work.scan_tmp, s);
+        exclusive_scan_u32(A.block_quads, nblocks, quad_scan, work.scan_tmp, s);
     }
-    DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(hflags + 2, quad_scan + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     DL_HIP(hipMemcpyAsync(hflags + 4, work.offsets.as<uint32_t>() + nthr, sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
