@@ -149,6 +149,7 @@ struct TriWork {
     DevBuf cand;        // raw candidate records
     DevBuf counts;      // per-thread / per-block candidate counts
     DevBuf offsets;     // their prefix sums
+    DevBuf positions;   // compact list of block positions holding candidates
     DevBuf rows;        // (T, 5) triangle rows
     DevBuf tidgrid;     // W x H tid raster
     DevBuf bin_counts, bin_offsets, bin_list;   // triangle -> tile bins

@@ -396,6 +396,8 @@ struct DetectArgs {
     int bx_per_row;          // thread blocks per block row
     uint32_t* counts;        // per-thread candidate counts (count pass)
     uint32_t* block_quads;   // per-thread-block quad counts (count pass)
+    uint8_t* masks;          // per-thread masks of items holding candidates
+    uint32_t* positions;     // compact list of those block positions
     const uint32_t* offsets; // their exclusive prefix sums (insert pass)
     Cand* cand;
     uint32_t* flags;
@@ -477,17 +479,18 @@ __global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
     const int ry = blockIdx.y;
     const int rx0 = blockIdx.x * kDetChunk + threadIdx.x;
     const int bw = A.Wp - 1;
-    uint32_t cnt = 0, quads = 0;
+    uint32_t mask = 0, quads = 0;
 #pragma unroll
     for (int j = 0; j < kDetItems; ++j) {
         const int rx = rx0 + j * kDetThreads;
         if (rx >= bw) break;
         const BlockCorners c = block_corners(A, rx, ry);
-        cnt += block_candidates(c);
+        if (block_candidates(c)) mask |= 1u << j;
         quads += c.quad;
     }
     const size_t blk = static_cast<size_t>(ry) * A.bx_per_row + blockIdx.x;
-    A.counts[blk * kDetThreads + threadIdx.x] = cnt;
+    A.counts[blk * kDetThreads + threadIdx.x] = static_cast<uint32_t>(__popc(mask));
+    A.masks[blk * kDetThreads + threadIdx.x] = static_cast<uint8_t>(mask);
     // Quads per thread block (a single counter would serialise ~10^4 atomics).
     __shared__ uint32_t red[kDetThreads / 64];
     for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
@@ -506,7 +509,10 @@ __device__ __forceinline__ void add_candidate(const DetectArgs& A, uint32_t& out
     int32_t sa = a, sb = b, sc = c;
     sort3(sa, sb, sc);
     const int64_t sl = table_insert(A.table, sa, sb, sc, A.flags);
-    if (sl < 0) return;
+    if (sl < 0) {
+        A.cand[out++].key = 0xFFFFFFFFu;
+        return;
+    }
     const uint32_t key = static_cast<uint32_t>(scan) * A.nb + pos;
     atomicMin(&A.table.slot[sl].minkey, key);
     Cand cd;
@@ -518,50 +524,73 @@ __device__ __forceinline__ void add_candidate(const DetectArgs& A, uint32_t& out
     A.cand[out++] = cd;
 }
 
-__global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A)
+// Compact list of the block positions that hold candidates, from the count
+// pass's per-thread masks: the insert pass then runs one thread per position.
+__global__ __launch_bounds__(kDetThreads) void detect_collect_kernel(DetectArgs A, int64_t nthr)
 {
-    const int ry = blockIdx.y;
-    const int rx0 = blockIdx.x * kDetChunk + threadIdx.x;
-    const int bw = A.Wp - 1;
-    uint32_t out = A.offsets[(static_cast<size_t>(ry) * A.bx_per_row + blockIdx.x) * kDetThreads +
-                             threadIdx.x];
-    for (int j = 0; j < kDetItems; ++j) {
-        const int rx = rx0 + j * kDetThreads;
-        if (rx >= bw) break;
-        const BlockCorners c = block_corners(A, rx, ry);
-        if (!(c.v[0] | c.v[1] | c.v[2] | c.v[3])) continue;
-        const int32_t tl = to_id(A, c.tl, A.flags), tr = to_id(A, c.tr, A.flags);
-        const int32_t bl = to_id(A, c.bl, A.flags), br = to_id(A, c.br, A.flags);
-        if (tl < 0 || tr < 0 || bl < 0 || br < 0) return;
-        const uint32_t pos = static_cast<uint32_t>(ry) * static_cast<uint32_t>(bw) +
-                             static_cast<uint32_t>(rx);
-        const int32_t tri[4][3] = {{tl, tr, br}, {tl, tr, bl}, {bl, br, tl}, {bl, br, tr}};
-        if (!c.quad) {
-            int k = 0;
-            while (!c.v[k]) ++k;
-            add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
-            continue;
-        }
-        const int32_t ids[4] = {tl, tr, bl, br};
-        int64_t px[4], py[4];
-        for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
-        int32_t blk[2][3];
-        quad_rule(ids, px, py, blk);
-        bool blocked[4] = {false, false, false, false};
-        for (int q = 0; q < 2; ++q) {
-            int32_t a = blk[q][0], b = blk[q][1], cc = blk[q][2];
-            sort3(a, b, cc);
-            const int64_t sl = table_insert(A.table, a, b, cc, A.flags);
-            if (sl >= 0) A.table.slot[sl].blk = 1u;
-            for (int k = 0; k < 4; ++k) {
-                int32_t x = tri[k][0], y = tri[k][1], z = tri[k][2];
-                sort3(x, y, z);
-                if (x == a && y == b && z == cc) blocked[k] = true;
-            }
-        }
-        for (int k = 0; k < 4; ++k)
-            if (!blocked[k]) add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nthr) return;
+    uint32_t mask = A.masks[t];
+    if (!mask) return;
+    uint32_t out = A.offsets[t];
+    const int64_t blk = t / kDetThreads;
+    const int tid = static_cast<int>(t - blk * kDetThreads);
+    const int ry = static_cast<int>(blk / A.bx_per_row);
+    const int bx = static_cast<int>(blk - static_cast<int64_t>(ry) * A.bx_per_row);
+    const uint32_t bw = static_cast<uint32_t>(A.Wp - 1);
+    while (mask) {
+        const int j = __ffs(mask) - 1;
+        mask &= mask - 1;
+        const uint32_t rx = static_cast<uint32_t>(bx * kDetChunk + j * kDetThreads + tid);
+        A.positions[out++] = static_cast<uint32_t>(ry) * bw + rx;
     }
+}
+
+// One thread per listed position.  A position's candidates go to slots 2i and
+// 2i+1 of the candidate array; an unused slot keeps key 0xFFFFFFFF.
+__global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A, uint32_t npos)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npos) return;
+    const uint32_t pos = A.positions[i];
+    const uint32_t bw = static_cast<uint32_t>(A.Wp - 1);
+    const int ry = static_cast<int>(pos / bw);
+    const int rx = static_cast<int>(pos - static_cast<uint32_t>(ry) * bw);
+    const BlockCorners c = block_corners(A, rx, ry);
+    uint32_t out = 2 * i;
+    A.cand[out + 1].key = 0xFFFFFFFFu;
+    const int32_t tl = to_id(A, c.tl, A.flags), tr = to_id(A, c.tr, A.flags);
+    const int32_t bl = to_id(A, c.bl, A.flags), br = to_id(A, c.br, A.flags);
+    if (tl < 0 || tr < 0 || bl < 0 || br < 0) {
+        A.cand[out].key = 0xFFFFFFFFu;
+        return;
+    }
+    const int32_t tri[4][3] = {{tl, tr, br}, {tl, tr, bl}, {bl, br, tl}, {bl, br, tr}};
+    if (!c.quad) {
+        int k = 0;
+        while (!c.v[k]) ++k;
+        add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
+        return;
+    }
+    const int32_t ids[4] = {tl, tr, bl, br};
+    int64_t px[4], py[4];
+    for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
+    int32_t blk[2][3];
+    quad_rule(ids, px, py, blk);
+    bool blocked[4] = {false, false, false, false};
+    for (int q = 0; q < 2; ++q) {
+        int32_t a = blk[q][0], b = blk[q][1], cc = blk[q][2];
+        sort3(a, b, cc);
+        const int64_t sl = table_insert(A.table, a, b, cc, A.flags);
+        if (sl >= 0) A.table.slot[sl].blk = 1u;
+        for (int k = 0; k < 4; ++k) {
+            int32_t x = tri[k][0], y = tri[k][1], z = tri[k][2];
+            sort3(x, y, z);
+            if (x == a && y == b && z == cc) blocked[k] = true;
+        }
+    }
+    for (int k = 0; k < 4; ++k)
+        if (!blocked[k]) add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
 }
 
 __device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
@@ -576,6 +605,7 @@ __global__ void mark_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Cand cd = cand[i];
+    if (cd.key == 0xFFFFFFFFu) return;
     int32_t a = cd.a, b = cd.b, c = cd.c;
     sort3(a, b, c);
     const Slot& sl = t.slot[cd.slot];
@@ -593,7 +623,7 @@ __global__ void emit_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Cand cd = cand[i];
-    if (!is_first(cd, t)) return;
+    if (cd.key == 0xFFFFFFFFu || !is_first(cd, t)) return;
     const uint32_t w = cd.key >> 5;
     const uint32_t tid = pre[w] + static_cast<uint32_t>(__popc(bm[w] & ((1u << (cd.key & 31)) - 1u)));
     const uint32_t scan = cd.key / nb;
@@ -646,7 +676,7 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     const int bx_per_row = (bw + kDetChunk - 1) / kDetChunk;
     const int64_t nthr = static_cast<int64_t>(bx_per_row) * bh * kDetThreads;
     const int64_t nblocks = static_cast<int64_t>(bx_per_row) * bh;
-    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 2 * (nblocks + 1)));
+    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 2 * (nblocks + 1)) + static_cast<size_t>(nthr));
     work.offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 1));
     DetectArgs A{};
     A.grid = grid;
@@ -664,6 +694,7 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     A.offsets = work.offsets.as<uint32_t>();
     A.block_quads = work.counts.as<uint32_t>() + nthr;
     uint32_t* quad_scan = A.block_quads + nblocks + 1;
+    A.masks = reinterpret_cast<uint8_t*>(quad_scan + nblocks + 1);
     A.flags = dflags;
     const dim3 dgrid(static_cast<unsigned>(bx_per_row), static_cast<unsigned>(bh));
     {
@@ -678,12 +709,21 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     DL_HIP(hipMemcpyAsync(hflags + 4, work.offsets.as<uint32_t>() + nthr, sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
-    const uint64_t ncand = hflags[4];
+    const uint64_t npos = hflags[4];
     const uint64_t nquads = hflags[2];
-    // Distinct triplets <= candidates + 2 per quad: at most half full.
-    const uint64_t table_cap = next_pow2(2 * (ncand + 2 * nquads) + 1024);
+    const uint64_t ncand = 2 * npos;   // two slots per position
+    // Distinct triplets <= 2 per position (a quad adds two blocked ones): at
+    // most half full.
+    const uint64_t table_cap = next_pow2(2 * (npos + 2 * nquads) + 1024);
     work.cand.reserve(sizeof(Cand) * std::max<uint64_t>(ncand, 1));
     work.table_tag.reserve(sizeof(Slot) * table_cap);
+    work.positions.reserve(sizeof(uint32_t) * std::max<uint64_t>(npos, 1));
+    A.positions = work.positions.as<uint32_t>();
+    if (npos > 0) {
+        ProfScope ps(kKDetect, s);
+        detect_collect_kernel<<<static_cast<unsigned>((nthr + 255) / 256), 256, 0, s>>>(A, nthr);
+        DL_LAUNCH_CHECK();
+    }
     const int64_t nbits = 4 * nb;
     const int64_t nwords = (nbits + 31) / 32;
     work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords));
@@ -709,8 +749,11 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         }
         {
             ProfScope ps(kKDetect, s);
-            detect_insert_kernel<<<dgrid, kDetThreads, 0, s>>>(A);
-            DL_LAUNCH_CHECK();
+            if (npos > 0) {
+                detect_insert_kernel<<<static_cast<unsigned>((npos + kDetThreads - 1) / kDetThreads),
+                                       kDetThreads, 0, s>>>(A, static_cast<uint32_t>(npos));
+                DL_LAUNCH_CHECK();
+            }
         }
         const unsigned cblocks = static_cast<unsigned>(std::max<uint64_t>(1, (ncand + 255) / 256));
         {
