@@ -55,14 +55,24 @@ __device__ __forceinline__ bool relax(uint32_t& lab, int& d, uint32_t nl, int cx
 // compare.  An undefined cell is the key (0xFFFFFFFF, 0), larger than any
 // candidate; a cell outside the canvas is (0, 0), smaller than any candidate
 // (its d would have to be 0, i.e. a seed at the cell), and both carry ~label 0,
-// which marks an invalid source.  cxm, cym are the cell coordinates minus
-// 0xFFFF, so that dx = (inv >> 16) + cxm with inv = ~label.
-__device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t ninv, int cxm,
-                                          int cym)
+// which marks an invalid source.
+//
+// The squared distance comes from packed 16-bit arithmetic: a label is the
+// pair (x, y) in its two halves, so with inv = ~label and
+// cxy1 = ((cx + 1) << 16) | (cy + 1), the half-wise sum inv + cxy1 is
+// (cx - x, cy - y) modulo 2^16, exact as int16 because canvas sides are below
+// 32768, and one dot2 gives dx^2 + dy^2 (below 2^31).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t cell_xy1(int cx, int cy)
 {
-    const int dx = static_cast<int>(ninv >> 16) + cxm;
-    const int dy = static_cast<int>(ninv & 0xFFFFu) + cym;
-    uint32_t c = static_cast<uint32_t>(__mul24(dx, dx) + __mul24(dy, dy));
+    return (static_cast<uint32_t>(cx + 1) << 16) + (static_cast<uint32_t>(cy + 1) & 0xFFFFu);
+}
+
+__device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t ninv, uint32_t cxy1)
+{
+    const s16x2 dd = __builtin_bit_cast(s16x2, ninv) + __builtin_bit_cast(s16x2, cxy1);
+    uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_sdot2(dd, dd, 0, false));
     c = ninv == 0u ? 0xFFFFFFFFu : c;
     const uint64_t ck = (static_cast<uint64_t>(c) << 32) | ninv;
     const uint64_t ok = (static_cast<uint64_t>(d) << 32) | inv;
@@ -110,11 +120,12 @@ struct EscGeom {
 // are skipped.
 template <int NW, int R, int K, bool ESC>
 __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], uint32_t (&D0)[R],
-                                        uint32_t (&D1)[R], int cxm0, int cym0, const EscGeom& e,
+                                        uint32_t (&D1)[R], int c0, int y0, const EscGeom& e,
                                         uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
                                         int (*lds_q)[NW], int wave, int lane, int& esc)
 {
-    const int cxm1 = cxm0 + 1;
+    // packed (cx + 1, cy + 1) of column 0 and column 1 in row 0; row r adds r
+    const uint32_t p0 = cell_xy1(c0, y0), p1 = cell_xy1(c0 + 1, y0);
     bool chx = true, chy = true;   // changes of the previous iteration (forced on at entry)
     for (int k = 0; k < K; ++k) {
         bool cx = false;
@@ -125,8 +136,8 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             for (int r = 0; r < R; ++r) {
                 const uint32_t left = from_left(I1[r]);
                 const uint32_t own0 = I0[r];
-                const bool t1 = relax_key(I1[r], D1[r], own0, cxm1, cym0 + r);
-                const bool t0 = relax_key(I0[r], D0[r], left, cxm0, cym0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], own0, p1 + r);
+                const bool t0 = relax_key(I0[r], D0[r], left, p0 + r);
                 ch |= t0 | t1;
                 if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_left) | (t1 & e.col1_left);
             }
@@ -135,8 +146,8 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             for (int r = 0; r < R; ++r) {
                 const uint32_t right = from_right(I0[r]);
                 const uint32_t own1 = I1[r];
-                const bool t0 = relax_key(I0[r], D0[r], own1, cxm0, cym0 + r);
-                const bool t1 = relax_key(I1[r], D1[r], right, cxm1, cym0 + r);
+                const bool t0 = relax_key(I0[r], D0[r], own1, p0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], right, p1 + r);
                 ch |= t0 | t1;
                 if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_right) | (t1 & e.col1_right);
             }
@@ -159,13 +170,13 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : 0u;
 #pragma unroll
             for (int r = R - 1; r >= 1; --r) {
-                const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], cxm0, cym0 + r);
-                const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], cxm1, cym0 + r);
+                const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], p0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], p1 + r);
                 ch |= t0 | t1;
                 if (ESC && r == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
             }
-            const bool t0 = relax_key(I0[0], D0[0], top0, cxm0, cym0);
-            const bool t1 = relax_key(I1[0], D1[0], top1, cxm1, cym0);
+            const bool t0 = relax_key(I0[0], D0[0], top0, p0);
+            const bool t1 = relax_key(I1[0], D1[0], top1, p1);
             ch |= t0 | t1;
             if (ESC && 0 == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
         }
@@ -178,13 +189,13 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : 0u;
 #pragma unroll
             for (int r = 0; r < R - 1; ++r) {
-                const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], cxm0, cym0 + r);
-                const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], cxm1, cym0 + r);
+                const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], p0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], p1 + r);
                 ch |= t0 | t1;
                 if (ESC && r == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
             }
-            const bool t0 = relax_key(I0[R - 1], D0[R - 1], bot0, cxm0, cym0 + R - 1);
-            const bool t1 = relax_key(I1[R - 1], D1[R - 1], bot1, cxm1, cym0 + R - 1);
+            const bool t0 = relax_key(I0[R - 1], D0[R - 1], bot0, p0 + R - 1);
+            const bool t1 = relax_key(I1[R - 1], D1[R - 1], bot1, p1 + R - 1);
             ch |= t0 | t1;
             if (ESC && (R - 1) == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
         }
@@ -193,11 +204,25 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
     }
 }
 
-template <int NW, int R, int K>
+// Does tile (tx, ty) of a watched canvas touch the image interior's boundary
+// (within its halo)?  Such tiles run the variant that records updates reaching
+// the interior from the padding.
+__host__ __device__ inline bool tile_watched(int P, int W, int H, int tx, int ty, int IW, int IH,
+                                             int K)
+{
+    if (P <= 0) return false;
+    const int ox = tx * IW, oy = ty * IH;
+    return (ox - K <= P && P < ox + IW + K) || (ox - K <= W - P - 1 && W - P - 1 < ox + IW + K) ||
+           (oy - K <= P && P < oy + IH + K) || (oy - K <= H - P - 1 && H - P - 1 < oy + IH + K);
+}
+
+// ESC = false: every tile not watched (grid over all tiles, XCD-sliced);
+// ESC = true: the watched tiles listed in esc_list (one block each).
+template <int NW, int R, int K, bool ESC>
 __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
-    int* __restrict__ counters)
+    int* __restrict__ counters, const int* __restrict__ esc_list)
 {
     constexpr int IW = 128 - 2 * K;
     constexpr int IH = NW * R - 2 * K;
@@ -209,10 +234,15 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     // the 8 XCDs, so give each XCD a contiguous slice of tiles (neighbouring
     // tiles share halo lines in the same L2).  The grid is launched as a
     // multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
-    const int per_xcd = gridDim.x / 8;
-    const int b = blockIdx.x;
-    const int tile = (b % 8) * per_xcd + b / 8;
-    if (tile >= total_tiles) return;
+    int tile;
+    if (ESC) {
+        tile = esc_list[blockIdx.x];
+    } else {
+        const int per_xcd = gridDim.x / 8;
+        const int b = blockIdx.x;
+        tile = (b % 8) * per_xcd + b / 8;
+        if (tile >= total_tiles) return;
+    }
 
     int ci = 0;
     if (n_canvas > 1) {
@@ -229,6 +259,7 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     const int local = tile - tile_base;
     const int tx = local % tiles_x, ty = local / tiles_x;
     const int tid = threadIdx.x;
+    if (!ESC && tile_watched(cv.P, cv.W, cv.H, tx, ty, IW, IH, K)) return;
     // The wave index is uniform; readfirstlane lets the compiler keep every
     // row-dependent quantity in scalar registers.
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -285,13 +316,7 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
 
     int esc = 0;
     const int P = cv.P;
-    const int cxm0 = c0 - 0xFFFF, cym0 = y0 - 0xFFFF;
-    // Does this tile's region touch the interior boundary of the image?
-    const bool esc_tile =
-        P > 0 &&
-        ((ox - K <= P && P < ox + IW + K) || (ox - K <= W - P - 1 && W - P - 1 < ox + IW + K) ||
-         (oy - K <= P && P < oy + IH + K) || (oy - K <= H - P - 1 && H - P - 1 < oy + IH + K));
-    if (esc_tile) {
+    if (ESC) {
         EscGeom e;
         const int c1 = c0 + 1;
         e.col0_in = c0 >= P && c0 < W - P;
@@ -307,12 +332,10 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
             const int y = y0 + r;
             if (y >= P && y < H - P) e.rows_in |= 1u << r;
         }
-        iterate<NW, R, K, true>(I0, I1, D0, D1, cxm0, cym0, e, ldsA, ldsB, lds_any, wave, lane,
-                                esc);
+        iterate<NW, R, K, true>(I0, I1, D0, D1, c0, y0, e, ldsA, ldsB, lds_any, wave, lane, esc);
     } else {
         EscGeom e{};
-        iterate<NW, R, K, false>(I0, I1, D0, D1, cxm0, cym0, e, ldsA, ldsB, lds_any, wave, lane,
-                                 esc);
+        iterate<NW, R, K, false>(I0, I1, D0, D1, c0, y0, e, ldsA, ldsB, lds_any, wave, lane, esc);
     }
 
     // Write the interior back and note whether it changed.  The row origin is
@@ -413,20 +436,23 @@ __global__ void zero_ints_kernel(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
-using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*);
+using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*,
+                           const int*);
 
 struct Variant {
     int waves, rows, iters;
-    TileKernel fn;
+    TileKernel fn;      // unwatched tiles
+    TileKernel fn_esc;  // watched tiles
 };
+
+#define DL_VARIANT(W_, R_, K_) \
+    {W_, R_, K_, bfs_tile_kernel<W_, R_, K_, false>, bfs_tile_kernel<W_, R_, K_, true>}
 
 // Instantiated configurations.  The tile is 128 wide; height waves*rows.
 const Variant kVariants[] = {
-    {16, 8, 8, bfs_tile_kernel<16, 8, 8>},   {16, 8, 4, bfs_tile_kernel<16, 8, 4>},
-    {16, 8, 12, bfs_tile_kernel<16, 8, 12>}, {16, 8, 16, bfs_tile_kernel<16, 8, 16>},
-    {8, 8, 4, bfs_tile_kernel<8, 8, 4>},     {8, 8, 8, bfs_tile_kernel<8, 8, 8>},
-    {8, 16, 8, bfs_tile_kernel<8, 16, 8>},   {16, 4, 4, bfs_tile_kernel<16, 4, 4>},
-    {16, 4, 8, bfs_tile_kernel<16, 4, 8>},
+    DL_VARIANT(16, 8, 8), DL_VARIANT(16, 8, 4),  DL_VARIANT(16, 8, 12), DL_VARIANT(16, 8, 16),
+    DL_VARIANT(8, 8, 4),  DL_VARIANT(8, 8, 8),   DL_VARIANT(8, 16, 8),  DL_VARIANT(16, 4, 4),
+    DL_VARIANT(16, 4, 8),
 };
 
 const Variant* find_variant(const BfsConfig& c)
@@ -479,6 +505,21 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     }
     const int n_canvas = static_cast<int>(canvases.size());
     if (total == 0) return 0;
+    // Watched tiles (see tile_watched) run a separate variant.
+    std::vector<int> watched;
+    for (const auto& c : canvases) {
+        if (c.P <= 0) continue;
+        for (int ty = 0; ty < c.tiles_y; ++ty)
+            for (int tx = 0; tx < c.tiles_x; ++tx)
+                if (tile_watched(c.P, c.W, c.H, tx, ty, iw, ih, cfg.iters))
+                    watched.push_back(c.tile_base + ty * c.tiles_x + tx);
+    }
+    const int n_watched = static_cast<int>(watched.size());
+    if (n_watched) {
+        work.tile_map.reserve(sizeof(int) * watched.size());
+        DL_HIP(hipMemcpyAsync(work.tile_map.ptr, watched.data(), sizeof(int) * watched.size(),
+                              hipMemcpyHostToDevice, s));
+    }
 
     work.canvases.reserve(sizeof(BfsCanvas) * n_canvas);
     DL_HIP(hipMemcpyAsync(work.canvases.ptr, canvases.data(), sizeof(BfsCanvas) * n_canvas,
@@ -522,7 +563,15 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
                 ProfScope ps(kKFill, s);
                 v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
                     reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
-                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j));
+                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j), nullptr);
+                DL_LAUNCH_CHECK();
+            }
+            if (n_watched) {
+                ProfScope ps(kKFill, s);
+                v->fn_esc<<<n_watched, cfg.waves * 64, 0, s>>>(
+                    reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
+                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j),
+                    work.tile_map.as<int>());
                 DL_LAUNCH_CHECK();
             }
             ++launch;

@@ -183,10 +183,6 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
                               const int32_t* rank_at, int W_img, const uint32_t* sorted,
                               int64_t n_seeds, TriWork& work, hipStream_t s);
 
-// Per-pixel max tid over closed containing triangles, into work.tidgrid (W x H).
-void assign_pixels(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
-                   int32_t* tidgrid, hipStream_t s);
-
 // (H, W, C) int32 output.  Output pixel (x, y) reads canvas cell
 // (x + off, y + off); the owner's image position is its canvas position minus
 // shift.  channels 0/1 = (rank(owner), d^2), channel 2 (C == 3) = tidgrid.

@@ -793,68 +793,9 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
 }
 
 // ===========================================================================
-// Pixel assignment
+// Voronoi output
 // ===========================================================================
 namespace {
-
-__device__ __forceinline__ int64_t floordiv64(int64_t a, int64_t b)  // b > 0
-{
-    int64_t q = a / b;
-    if ((a % b) != 0 && a < 0) --q;
-    return q;
-}
-
-__global__ void assign_kernel(int W, int H, const int32_t* __restrict__ rows, int64_t T,
-                              const uint32_t* __restrict__ sorted, int32_t* __restrict__ tid)
-{
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const int32_t* r = rows + 5 * t;
-    int64_t vx[3], vy[3];
-    for (int k = 0; k < 3; ++k) {
-        const uint32_t l = sorted[r[2 + k]];
-        vx[k] = static_cast<int64_t>(l >> 16);
-        vy[k] = static_cast<int64_t>(l & 0xFFFFu);
-    }
-    const int32_t id = static_cast<int32_t>(t);
-    // doubled signed area = sum of the three edge functions
-    const int64_t S = (vx[1] - vx[0]) * (vy[2] - vy[0]) - (vy[1] - vy[0]) * (vx[2] - vx[0]);
-    if (S == 0) {
-        // Distinct collinear vertices contain exactly the lattice points of
-        // their line (triangulation.py:247-253 with all three products zero).
-        const int64_t dx = vx[1] - vx[0], dy = vy[1] - vy[0];
-        if (dy == 0) {
-            if (vy[0] >= 0 && vy[0] < H)
-                for (int x = 0; x < W; ++x) atomicMax(&tid[static_cast<size_t>(vy[0]) * W + x], id);
-            return;
-        }
-        for (int y = 0; y < H; ++y) {
-            const int64_t num = (y - vy[0]) * dx;
-            if (num % dy != 0) continue;
-            const int64_t x = vx[0] + num / dy;
-            if (x >= 0 && x < W) atomicMax(&tid[static_cast<size_t>(y) * W + x], id);
-        }
-        return;
-    }
-    const int64_t sg = S > 0 ? 1 : -1;
-    int64_t ymin = min(vy[0], min(vy[1], vy[2])), ymax = max(vy[0], max(vy[1], vy[2]));
-    ymin = max<int64_t>(ymin, 0);
-    ymax = min<int64_t>(ymax, H - 1);
-    for (int64_t y = ymin; y <= ymax; ++y) {
-        int64_t lo = 0, hi = W - 1;
-        for (int e = 0; e < 3; ++e) {
-            const int n = e == 2 ? 0 : e + 1;
-            // edge function of (A, B) at (px, y): px*(Ay-By) + Ax*(By-y) - Bx*(Ay-y)
-            const int64_t al = sg * (vy[e] - vy[n]);
-            const int64_t be = sg * (vx[e] * (vy[n] - y) - vx[n] * (vy[e] - y));
-            if (al > 0) lo = max(lo, -floordiv64(be, al));
-            else if (al < 0) hi = min(hi, floordiv64(be, -al));
-            else if (be < 0) { lo = 1; hi = 0; }
-        }
-        int32_t* row = tid + static_cast<size_t>(y) * W;
-        for (int64_t x = lo; x <= hi; ++x) atomicMax(&row[x], id);
-    }
-}
 
 // channels: (rank(owner), d^2[, tid]).  Output pixel (x, y) reads canvas cell
 // (x + off, y + off); the owner's image position is its canvas position - shift.
@@ -882,20 +823,6 @@ __global__ void output_kernel(int W, int H, const uint32_t* __restrict__ lab, in
 }
 
 }  // namespace
-
-void assign_pixels(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
-                   int32_t* tidgrid, hipStream_t s)
-{
-    {
-        ProfScope ps(kKInit, s);
-        fill_u32(reinterpret_cast<uint32_t*>(tidgrid), 0xFFFFFFFFu, static_cast<size_t>(W) * H, s);
-    }
-    if (n_tri == 0) return;
-    ProfScope ps(kKAssign, s);
-    assign_kernel<<<static_cast<unsigned>((n_tri + 127) / 128), 128, 0, s>>>(W, H, rows, n_tri,
-                                                                             sorted, tidgrid);
-    DL_LAUNCH_CHECK();
-}
 
 void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, int shift,
                   const int32_t* rank_at, int W_img, const int32_t* tidgrid, int32_t* out, int C,
@@ -975,31 +902,51 @@ __device__ __forceinline__ bool tri_box(const TriVerts& v, int W, int H, int& x0
     return true;
 }
 
-// Closed x-interval of row y inside a non-degenerate triangle (empty: lo > hi).
-// Each edge function is linear in x on a row; its boundary is found in double
-// (exact here: |numerator| < 2^33, |denominator| < 2^16) and then confirmed in
-// integers.
-__device__ __forceinline__ void row_span(const TriVerts& v, int64_t y, int64_t& lo, int64_t& hi)
+// Row spans of a non-degenerate triangle.  Edge e's function at (px, y),
+// multiplied by sign(S), is al*px + k0 + k1*y; the pixel is inside iff all
+// three are >= 0 (the three sum to |S| > 0, so "all <= 0" cannot hold).  All
+// quantities are integers below 2^34 and exact in double; the boundary from
+// the reciprocal can be off by one next to an integer, which the exact test
+// of the candidate and its neighbour corrects.
+struct EdgeRows {
+    double al[3], k0[3], k1[3], inv[3];
+};
+
+__device__ __forceinline__ EdgeRows edge_rows(const TriVerts& v)
 {
-    const int64_t sg = v.S > 0 ? 1 : -1;
+    EdgeRows E;
+    const double sg = v.S > 0 ? 1.0 : -1.0;
     for (int e = 0; e < 3; ++e) {
         const int n = e == 2 ? 0 : e + 1;
-        // edge function at (px, y): px*(Ay-By) + Ax*(By-y) - Bx*(Ay-y), times sign(S)
-        const int64_t al = sg * (v.y[e] - v.y[n]);
-        const int64_t be = sg * (v.x[e] * (v.y[n] - y) - v.x[n] * (v.y[e] - y));
-        if (al > 0) {   // px >= -be/al
-            int64_t x = static_cast<int64_t>(ceil(static_cast<double>(-be) / static_cast<double>(al)));
-            if (al * x + be < 0) ++x;
-            else if (al * (x - 1) + be >= 0) --x;
-            lo = max(lo, x);
-        } else if (al < 0) {   // px <= be/(-al)
-            int64_t x = static_cast<int64_t>(floor(static_cast<double>(be) / static_cast<double>(-al)));
-            if (al * x + be < 0) --x;
-            else if (al * (x + 1) + be >= 0) ++x;
-            hi = min(hi, x);
-        } else if (be < 0) {
-            lo = 1;
-            hi = 0;
+        const double ax = static_cast<double>(v.x[e]), ay = static_cast<double>(v.y[e]);
+        const double bx = static_cast<double>(v.x[n]), by = static_cast<double>(v.y[n]);
+        E.al[e] = sg * (ay - by);
+        E.k0[e] = sg * (ax * by - bx * ay);
+        E.k1[e] = sg * (bx - ax);
+        E.inv[e] = E.al[e] != 0.0 ? 1.0 / E.al[e] : 0.0;
+    }
+    return E;
+}
+
+__device__ __forceinline__ void row_span(const EdgeRows& E, double y, double& lo, double& hi)
+{
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double al = E.al[e];
+        const double be = fma(E.k1[e], y, E.k0[e]);
+        if (al > 0.0) {   // px >= -be/al
+            double x = ceil(-be * E.inv[e]);
+            if (fma(al, x, be) < 0.0) x += 1.0;
+            else if (fma(al, x - 1.0, be) >= 0.0) x -= 1.0;
+            lo = fmax(lo, x);
+        } else if (al < 0.0) {   // px <= -be/al
+            double x = floor(-be * E.inv[e]);
+            if (fma(al, x, be) < 0.0) x -= 1.0;
+            else if (fma(al, x + 1.0, be) >= 0.0) x += 1.0;
+            hi = fmin(hi, x);
+        } else if (be < 0.0) {
+            lo = 1.0;
+            hi = 0.0;
         }
     }
 }
@@ -1069,13 +1016,15 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
             }
             continue;
         }
-        const int64_t ya = max<int64_t>(Y0, min(v.y[0], min(v.y[1], v.y[2])));
-        const int64_t yb = min<int64_t>(Y1, max(v.y[0], max(v.y[1], v.y[2])));
-        for (int64_t y = ya; y <= yb; ++y) {
-            int64_t lo = X0, hi = X1;
-            row_span(v, y, lo, hi);
-            const int base = static_cast<int>(y - Y0) * kTS - X0;
-            for (int64_t x = lo; x <= hi; ++x) atomicMax(&tid_tile[base + static_cast<int>(x)], id);
+        const int ya = static_cast<int>(max<int64_t>(Y0, min(v.y[0], min(v.y[1], v.y[2]))));
+        const int yb = static_cast<int>(min<int64_t>(Y1, max(v.y[0], max(v.y[1], v.y[2]))));
+        const EdgeRows E = edge_rows(v);
+        for (int y = ya; y <= yb; ++y) {
+            double lo = X0, hi = X1;
+            row_span(E, static_cast<double>(y), lo, hi);
+            const int base = (y - Y0) * kTS - X0;
+            const int l = static_cast<int>(lo), h = static_cast<int>(hi);
+            for (int x = l; x <= h; ++x) atomicMax(&tid_tile[base + x], id);
         }
     }
     __syncthreads();
