@@ -64,11 +64,6 @@ __device__ __forceinline__ bool relax(uint32_t& lab, int& d, uint32_t nl, int cx
 // 32768, and one dot2 gives dx^2 + dy^2 (below 2^31).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ uint32_t cell_xy1(int cx, int cy)
-{
-    return (static_cast<uint32_t>(cx + 1) << 16) + (static_cast<uint32_t>(cy + 1) & 0xFFFFu);
-}
-
 __device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t ninv, uint32_t cxy1)
 {
     const s16x2 dd = __builtin_bit_cast(s16x2, ninv) + __builtin_bit_cast(s16x2, cxy1);
@@ -124,8 +119,13 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
                                         uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
                                         int (*lds_q)[NW], int wave, int lane, int& esc)
 {
-    // packed (cx + 1, cy + 1) of column 0 and column 1 in row 0; row r adds r
-    const uint32_t p0 = cell_xy1(c0, y0), p1 = cell_xy1(c0 + 1, y0);
+    // packed (cx + 1, cy + 1) is (px | ((y0 + r + 1) & 0xFFFF)) for row r: the
+    // halves are formed separately so that a negative halo row cannot carry
+    // into the x half.
+    const uint32_t px0 = static_cast<uint32_t>(c0 + 1) << 16;
+    const uint32_t px1 = static_cast<uint32_t>(c0 + 2) << 16;
+#define DL_P0(r) (px0 | (static_cast<uint32_t>(y0 + (r) + 1) & 0xFFFFu))
+#define DL_P1(r) (px1 | (static_cast<uint32_t>(y0 + (r) + 1) & 0xFFFFu))
     bool chx = true, chy = true;   // changes of the previous iteration (forced on at entry)
     for (int k = 0; k < K; ++k) {
         bool cx = false;
@@ -136,8 +136,8 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             for (int r = 0; r < R; ++r) {
                 const uint32_t left = from_left(I1[r]);
                 const uint32_t own0 = I0[r];
-                const bool t1 = relax_key(I1[r], D1[r], own0, p1 + r);
-                const bool t0 = relax_key(I0[r], D0[r], left, p0 + r);
+                const bool t1 = relax_key(I1[r], D1[r], own0, DL_P1(r));
+                const bool t0 = relax_key(I0[r], D0[r], left, DL_P0(r));
                 ch |= t0 | t1;
                 if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_left) | (t1 & e.col1_left);
             }
@@ -146,8 +146,8 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             for (int r = 0; r < R; ++r) {
                 const uint32_t right = from_right(I0[r]);
                 const uint32_t own1 = I1[r];
-                const bool t0 = relax_key(I0[r], D0[r], own1, p0 + r);
-                const bool t1 = relax_key(I1[r], D1[r], right, p1 + r);
+                const bool t0 = relax_key(I0[r], D0[r], own1, DL_P0(r));
+                const bool t1 = relax_key(I1[r], D1[r], right, DL_P1(r));
                 ch |= t0 | t1;
                 if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_right) | (t1 & e.col1_right);
             }
@@ -170,13 +170,13 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : 0u;
 #pragma unroll
             for (int r = R - 1; r >= 1; --r) {
-                const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], p0 + r);
-                const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], p1 + r);
+                const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], DL_P0(r));
+                const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], DL_P1(r));
                 ch |= t0 | t1;
                 if (ESC && r == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
             }
-            const bool t0 = relax_key(I0[0], D0[0], top0, p0);
-            const bool t1 = relax_key(I1[0], D1[0], top1, p1);
+            const bool t0 = relax_key(I0[0], D0[0], top0, DL_P0(0));
+            const bool t1 = relax_key(I1[0], D1[0], top1, DL_P1(0));
             ch |= t0 | t1;
             if (ESC && 0 == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
         }
@@ -189,19 +189,21 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : 0u;
 #pragma unroll
             for (int r = 0; r < R - 1; ++r) {
-                const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], p0 + r);
-                const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], p1 + r);
+                const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], DL_P0(r));
+                const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], DL_P1(r));
                 ch |= t0 | t1;
                 if (ESC && r == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
             }
-            const bool t0 = relax_key(I0[R - 1], D0[R - 1], bot0, p0 + R - 1);
-            const bool t1 = relax_key(I1[R - 1], D1[R - 1], bot1, p1 + R - 1);
+            const bool t0 = relax_key(I0[R - 1], D0[R - 1], bot0, DL_P0(R - 1));
+            const bool t1 = relax_key(I1[R - 1], D1[R - 1], bot1, DL_P1(R - 1));
             ch |= t0 | t1;
             if (ESC && (R - 1) == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
         }
         chx = cx;
         chy = __ballot(ch) != 0ull;
     }
+#undef DL_P0
+#undef DL_P1
 }
 
 // Does tile (tx, ty) of a watched canvas touch the image interior's boundary

@@ -94,15 +94,16 @@ def test_random_against_oracle(seed):
         np.testing.assert_array_equal(_rows(m), rows)
 
 
-@pytest.mark.parametrize("cfg", [(16, 4, 4), (16, 4, 8), (8, 8, 4), (16, 8, 8), (8, 16, 8),
-                                 (16, 8, 16)])
+@pytest.mark.parametrize("cfg", [(16, 4, 4), (16, 4, 8), (8, 8, 4), (8, 8, 8), (16, 8, 4),
+                                 (16, 8, 8), (16, 8, 12), (16, 8, 16), (8, 16, 8)])
 def test_every_tile_configuration_is_exact(cfg):
     ctx = _lib.context()
     lib = _lib.load()
     _lib.check(lib.dl_set_bfs_config(ctx.handle, *cfg))
     try:
         rng = np.random.default_rng(7)
-        for W, H, n in [(300, 200, 600), (97, 411, 300), (513, 129, 50), (1000, 700, 20000)]:
+        for W, H, n in [(300, 200, 600), (97, 411, 300), (513, 129, 50), (1000, 700, 20000),
+                        (260, 3, 20), (5, 333, 30)]:
             idx = rng.choice(W * H, size=n, replace=False)
             seeds = np.stack([idx % W, idx // W], 1).astype(np.int32)
             ref = O.voronoi(W, H, seeds)
