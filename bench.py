@@ -124,7 +124,7 @@ def main():
     # Algorithmic bytes per launch of each kernel kind.
     Wp, Hp = W + 2 * P, H + 2 * P
     T = int(n_tri.value)
-    cfg_w, cfg_r, cfg_k = (16, 4, 4)
+    cfg_w, cfg_r, cfg_k = (16, 4, 8)
     if args.bfs_config:
         cfg_w, cfg_r, cfg_k = (int(v) for v in args.bfs_config.split(","))
     tile_cells = (128 - 2 * cfg_k) * (cfg_w * cfg_r - 2 * cfg_k)

@@ -173,7 +173,7 @@ struct dl_context {
 
     bool timing = false;
     double timings[9] = {0};
-    BfsConfig cfg{16, 4, 4};
+    BfsConfig cfg{16, 4, 8};
     bool cfg_auto = true;
     bool force_unpadded = false;
     std::vector<hipEvent_t> events;
@@ -250,7 +250,7 @@ BfsConfig auto_config(const dl_context* ctx, int Wc, int Hc)
     if (!ctx->cfg_auto) return ctx->cfg;
     (void)Wc;
     (void)Hc;
-    return BfsConfig{16, 4, 4};
+    return BfsConfig{16, 4, 8};
 }
 
 struct StageClock {
