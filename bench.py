@@ -69,7 +69,19 @@ def main():
     ap.add_argument("--cpu-sample-side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--e2e-steps", type=int, default=3)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mb:
+        import torch
+        n = args.copy_probe_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "dispatches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the pipeline "
               f"shards by image or by row strip; see DESIGN.md)", file=sys.stderr)
@@ -124,9 +136,9 @@ def main():
     # Algorithmic bytes per launch of each kernel kind.
     Wp, Hp = W + 2 * P, H + 2 * P
     T = int(n_tri.value)
-    cfg_w, cfg_r, cfg_k = (16, 4, 8)
-    if args.bfs_config:
-        cfg_w, cfg_r, cfg_k = (int(v) for v in args.bfs_config.split(","))
+    cw, cr, ck = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _lib.check(lib.dl_get_bfs_config(ctx.handle, ctypes.byref(cw), ctypes.byref(cr), ctypes.byref(ck)))
+    cfg_w, cfg_r, cfg_k = cw.value, cr.value, ck.value
     tile_cells = (128 - 2 * cfg_k) * (cfg_w * cfg_r - 2 * cfg_k)
     alg = {
         # 4 B read + 4 B written per cell of every visited tile

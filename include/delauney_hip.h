@@ -105,6 +105,9 @@ int dl_set_timing(dl_context* ctx, int enabled);
  * counterpart (the reference uses 16x16 blocks, voronoi.cu:95-113). */
 int dl_set_bfs_config(dl_context* ctx, int waves, int rows, int iters);
 
+/* The configuration the last flood fill on this context ran with. */
+int dl_get_bfs_config(dl_context* ctx, int* waves, int* rows, int* iters);
+
 /* Device-resident triangulate: seeds, output and triangles stay on the device.
  * d_seeds_xy: n (x, y) int32 pairs; d_out_hw3: (H, W, 3) int32.  The triangle
  * rows stay in the context (dl_device_triangles).  No reference counterpart;

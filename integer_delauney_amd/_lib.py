@@ -46,6 +46,8 @@ _SIGNATURES = {
     "dl_get_timings": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "dl_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_set_bfs_config": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "dl_get_bfs_config": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
     "dl_triangulate_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP,
                                              ctypes.c_int64, ctypes.c_int, _VP, _VP, _I64P]),
     "dl_device_triangles": (_VP, [_VP]),

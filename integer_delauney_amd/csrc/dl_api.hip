@@ -175,6 +175,7 @@ struct dl_context {
     double timings[9] = {0};
     BfsConfig cfg{16, 4, 8};
     bool cfg_auto = true;
+    BfsConfig last_cfg{0, 0, 0};
     bool force_unpadded = false;
     std::vector<hipEvent_t> events;
     bool profiling = false;
@@ -330,7 +331,9 @@ BfsCanvas fill_canvas(dl_context* ctx, int W, int H, int P, int64_t n, bool watc
     bfs_init_canvas(cv, ctx->canvas_labels.as<uint32_t>(), n, s);
     std::vector<BfsCanvas> cvs{cv};
     BfsStats st;
-    const int plane = bfs_run(cvs, auto_config(ctx, Wc, Hc), ctx->bfs, s, &st);
+    const BfsConfig cfg = auto_config(ctx, Wc, Hc);
+    ctx->last_cfg = cfg;
+    const int plane = bfs_run(cvs, cfg, ctx->bfs, s, &st);
     ctx->timings[7] += st.launches;
     ctx->timings[8] += st.tile_visits;
     *plane_out = plane;
@@ -503,6 +506,16 @@ int dl_reset_kernel_stats(dl_context* ctx)
     return guarded([&] {
         if (!ctx) throw value_error("null context");
         ctx->prof.reset();
+    });
+}
+
+int dl_get_bfs_config(dl_context* ctx, int* waves, int* rows, int* iters)
+{
+    return guarded([&] {
+        if (!ctx || !waves || !rows || !iters) throw value_error("null argument");
+        *waves = ctx->last_cfg.waves;
+        *rows = ctx->last_cfg.rows;
+        *iters = ctx->last_cfg.iters;
     });
 }
 
