@@ -142,6 +142,17 @@ int dl_reset_kernel_stats(dl_context* ctx);
  * padding (cross-checking only). */
 int dl_set_force_unpadded(dl_context* ctx, int enabled);
 
+/* Flood fill of the width x height canvas (seeds as given, no padding) that
+ * also reports in *escaped whether any cell of the box
+ * [margin, width-margin) x [margin, height-margin) ever received an update
+ * from a cell outside the box.  The pipeline runs the same watch on its padded
+ * canvas with the box = the image; this entry point exposes it on arbitrary
+ * boxes (cross-checking only; no reference counterpart).  The flag may be set
+ * without a crossing (tiles recompute halo cells) but is never clear when one
+ * happened.  out_hw2 may be null. */
+int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* seeds_xy,
+                        int64_t n, int margin, int32_t* out_hw2, int* escaped);
+
 #ifdef __cplusplus
 }
 #endif

@@ -173,7 +173,7 @@ struct dl_context {
 
     bool timing = false;
     double timings[9] = {0};
-    BfsConfig cfg{16, 4, 8};
+    BfsConfig cfg{8, 8, 8};
     bool cfg_auto = true;
     BfsConfig last_cfg{0, 0, 0};
     bool force_unpadded = false;
@@ -251,7 +251,7 @@ BfsConfig auto_config(const dl_context* ctx, int Wc, int Hc)
     if (!ctx->cfg_auto) return ctx->cfg;
     (void)Wc;
     (void)Hc;
-    return BfsConfig{16, 4, 8};
+    return BfsConfig{8, 8, 8};
 }
 
 struct StageClock {
@@ -303,10 +303,13 @@ void prepare_seeds(dl_context* ctx, int W, int H, const int32_t* d_xy, int64_t n
 }
 
 // Flood fill of the canvas (W + 2P) x (H + 2P) with the seeds shifted by P.
-// Returns the canvas descriptor (with the final plane in lab0's position).
-BfsCanvas fill_canvas(dl_context* ctx, int W, int H, int P, int64_t n, bool watch, DevBuf& A,
-                      DevBuf& B, hipStream_t s, int* plane_out)
+// With watch_margin >= 0 the fill records in ctx->esc whether a cell inside
+// [m, Wc-m) x [m, Hc-m) ever received an update from a cell outside it.
+// Returns the canvas descriptor; *plane_out names the plane holding the result.
+BfsCanvas fill_canvas(dl_context* ctx, int W, int H, int P, int64_t n, int watch_margin,
+                      DevBuf& A, DevBuf& B, hipStream_t s, int* plane_out)
 {
+    const bool watch = watch_margin >= 0;
     const int Wc = W + 2 * P, Hc = H + 2 * P;
     const size_t cells = static_cast<size_t>(Wc) * Hc;
     A.reserve(sizeof(uint32_t) * cells);
@@ -321,7 +324,7 @@ BfsCanvas fill_canvas(dl_context* ctx, int W, int H, int P, int64_t n, bool watc
     cv.lab1 = B.as<uint32_t>();
     cv.W = Wc;
     cv.H = Hc;
-    cv.P = watch ? P : 0;
+    cv.P = watch ? watch_margin : 0;
     cv.escape = nullptr;
     if (watch) {
         ctx->esc.reserve(sizeof(int) * 4);
@@ -356,7 +359,7 @@ void run_triangulate(dl_context* ctx, int W, int H, const int32_t* d_xy, int64_t
     ctx->have_padded = false;
     if (P > 0) {
         int plane = 0;
-        BfsCanvas cv = fill_canvas(ctx, W, H, P, n, true, ctx->padA, ctx->padB, s, &plane);
+        BfsCanvas cv = fill_canvas(ctx, W, H, P, n, P, ctx->padA, ctx->padB, s, &plane);
         det_grid = plane_ptr(cv, plane);
         det_W = cv.W;
         det_H = cv.H;
@@ -371,7 +374,7 @@ void run_triangulate(dl_context* ctx, int W, int H, const int32_t* d_xy, int64_t
         escaped = ctx->host_small.as<int>()[0];
         if (escaped || ctx->force_unpadded) {
             int uplane = 0;
-            BfsCanvas ucv = fill_canvas(ctx, W, H, 0, n, false, ctx->unA, ctx->unB, s, &uplane);
+            BfsCanvas ucv = fill_canvas(ctx, W, H, 0, n, -1, ctx->unA, ctx->unB, s, &uplane);
             out_labels = plane_ptr(ucv, uplane);
             out_Wc = W;
             out_off = 0;
@@ -382,7 +385,7 @@ void run_triangulate(dl_context* ctx, int W, int H, const int32_t* d_xy, int64_t
         }
     } else {
         int plane = 0;
-        BfsCanvas cv = fill_canvas(ctx, W, H, 0, n, false, ctx->unA, ctx->unB, s, &plane);
+        BfsCanvas cv = fill_canvas(ctx, W, H, 0, n, -1, ctx->unA, ctx->unB, s, &plane);
         det_grid = plane_ptr(cv, plane);
         det_W = W;
         det_H = H;
@@ -542,7 +545,7 @@ int dl_voronoi(dl_context* ctx, int width, int height, const int32_t* seeds_xy, 
         prepare_seeds(ctx, width, height, ctx->xy.as<int32_t>(), n, s);
         clk.mark(0);
         int plane = 0;
-        BfsCanvas cv = fill_canvas(ctx, width, height, 0, n, false, ctx->unA, ctx->unB, s, &plane);
+        BfsCanvas cv = fill_canvas(ctx, width, height, 0, n, -1, ctx->unA, ctx->unB, s, &plane);
         clk.mark(1);
         const size_t px = static_cast<size_t>(width) * height;
         ctx->out.reserve(sizeof(int32_t) * 2 * px);
@@ -551,6 +554,36 @@ int dl_voronoi(dl_context* ctx, int width, int height, const int32_t* seeds_xy, 
         DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * px,
                               hipMemcpyDeviceToHost, s));
         clk.finish(s);
+    });
+}
+
+int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* seeds_xy,
+                        int64_t n, int margin, int32_t* out_hw2, int* escaped)
+{
+    return guarded([&] {
+        if (!ctx || !escaped) throw value_error("null argument");
+        check_dims(width, height, 0);
+        check_seeds(seeds_xy, n);
+        if (margin < 0 || 2 * margin >= width || 2 * margin >= height)
+            throw value_error("watch margin must leave a non-empty box");
+        DL_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        upload_seeds(ctx, seeds_xy, n);
+        prepare_seeds(ctx, width, height, ctx->xy.as<int32_t>(), n, s);
+        int plane = 0;
+        BfsCanvas cv = fill_canvas(ctx, width, height, 0, n, margin, ctx->unA, ctx->unB, s, &plane);
+        DL_HIP(hipMemcpyAsync(ctx->host_small.ptr, ctx->esc.ptr, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        if (out_hw2) {
+            const size_t px = static_cast<size_t>(width) * height;
+            ctx->out.reserve(sizeof(int32_t) * 2 * px);
+            write_output(width, height, plane_ptr(cv, plane), width, 0, 0,
+                         ctx->rank_at.as<int32_t>(), width, nullptr, ctx->out.as<int32_t>(), 2, s);
+            DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * px,
+                                  hipMemcpyDeviceToHost, s));
+        }
+        DL_HIP(hipStreamSynchronize(s));
+        *escaped = ctx->host_small.as<int>()[0] != 0;
     });
 }
 
@@ -665,7 +698,7 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
         ctx->have_padded = false;
         if (P > 0) {
             int plane = 0;
-            BfsCanvas cv = fill_canvas(ctx, width, height, P, n, false, ctx->padA, ctx->padB, s,
+            BfsCanvas cv = fill_canvas(ctx, width, height, P, n, -1, ctx->padA, ctx->padB, s,
                                        &plane);
             ctx->have_padded = true;
             ctx->pad_plane = plane;

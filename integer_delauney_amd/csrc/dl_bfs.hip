@@ -52,23 +52,35 @@ __device__ __forceinline__ bool relax(uint32_t& lab, int& d, uint32_t nl, int cx
 
 // The tiled kernel keeps each cell as a 64-bit key (d, ~label): the update
 // rule above is then "take the candidate iff its key is smaller", one 64-bit
-// compare.  An undefined cell is the key (0xFFFFFFFF, 0), larger than any
-// candidate; a cell outside the canvas is (0, 0), smaller than any candidate
-// (its d would have to be 0, i.e. a seed at the cell), and both carry ~label 0,
-// which marks an invalid source.
+// compare.
 //
 // The squared distance comes from packed 16-bit arithmetic: a label is the
 // pair (x, y) in its two halves, so with inv = ~label and
 // cxy1 = ((cx + 1) << 16) | (cy + 1), the half-wise sum inv + cxy1 is
-// (cx - x, cy - y) modulo 2^16, exact as int16 because canvas sides are below
-// 32768, and one dot2 gives dx^2 + dy^2 (below 2^31).
+// (cx - x, cy - y), exact as int16 because canvas sides (with a tile halo) stay
+// below 32767, and one dot2 gives dx^2 + dy^2 (below 2^31).
+//
+// Undefined and outside cells carry inv = kFarInv = 0x7FFF7FFF, which no real
+// label has (it would be x = 32768).  The half-wise sum saturates, so at any
+// cell with cx, cy >= 0 that label is at (32767, 32767) relative offset:
+// distance kFarD = 2 * 32767^2, above every real distance.  An undefined cell
+// is the key (kFarD, kFarInv): a real candidate is smaller, and the far label
+// itself ties, so it never updates anything.  A cell outside the canvas is
+// (0, kFarInv): no candidate is smaller (d = 0 would need a seed there), so it
+// never changes and only ever offers the far label.  No per-update validity
+// test is needed.
 typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+constexpr uint32_t kFarInv = 0x7FFF7FFFu;
+constexpr uint32_t kFarD = 2u * 32767u * 32767u;
 
 __device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t ninv, uint32_t cxy1)
 {
-    const s16x2 dd = __builtin_bit_cast(s16x2, ninv) + __builtin_bit_cast(s16x2, cxy1);
-    uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_sdot2(dd, dd, 0, false));
-    c = ninv == 0u ? 0xFFFFFFFFu : c;
+    const s16x2 dd = __builtin_elementwise_add_sat(__builtin_bit_cast(s16x2, ninv),
+                                                   __builtin_bit_cast(s16x2, cxy1));
+    // Clamping cannot change the sum (it stays below 2^31); it selects the
+    // three-operand dot2 with an inline zero accumulator.
+    const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_sdot2(dd, dd, 0, true));
     const uint64_t ck = (static_cast<uint64_t>(c) << 32) | ninv;
     const uint64_t ok = (static_cast<uint64_t>(d) << 32) | inv;
     const bool take = ck < ok;
@@ -77,27 +89,35 @@ __device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t n
     return take;
 }
 
-__device__ __forceinline__ uint32_t from_left(uint32_t v)   // lane l <- lane l-1; lane 0 <- 0
+__device__ __forceinline__ uint32_t from_left(uint32_t v)   // lane l <- lane l-1; lane 0 <- far
 {
-    return static_cast<uint32_t>(
-        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x138, 0xF, 0xF, false));
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+        static_cast<int>(kFarInv), static_cast<int>(v), 0x138, 0xF, 0xF, false));
 }
 
-__device__ __forceinline__ uint32_t from_right(uint32_t v)  // lane l <- lane l+1; lane 63 <- 0
+__device__ __forceinline__ uint32_t from_right(uint32_t v)  // lane l <- lane l+1; lane 63 <- far
 {
-    return static_cast<uint32_t>(
-        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xF, 0xF, false));
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+        static_cast<int>(kFarInv), static_cast<int>(v), 0x130, 0xF, 0xF, false));
 }
 
-// Escape bookkeeping for canvases whose image interior must not receive
-// updates from the padding (see BfsCanvas::P).  Lane masks are per thread;
-// rows are local row indices.
+// Escape watch for canvases whose image interior must not receive updates
+// from the padding (see BfsCanvas::P).  In pass 1 the only source of a cell is
+// its left neighbour, so a cell of the interior's first column changes in
+// pass 1 exactly when a padding cell updates it; likewise the last column in
+// pass 2, the first row in pass 3 and the last row in pass 4.  A watched tile
+// tests the update flags of those cells in those passes.  The flags are lane
+// masks already (one bit per lane), so the test is scalar work: the ballot of
+// a pass's update flag ANDed with a mask of the lanes holding watched cells.
+// Every mask is zero outside watched tiles.
 using GU32 = __attribute__((address_space(1))) uint32_t;
 
-struct EscGeom {
-    bool col0_left, col1_left, col0_right, col1_right, col0_in, col1_in;
-    int r_top, r_bot;    // local row index of the interior's first / last row
-    uint32_t rows_in;    // bit r set if row r lies inside the interior rows
+struct Watch {
+    uint64_t l0, l1;     // lanes whose column 0 / 1 is the interior's first column
+    uint64_t r0, r1;     // ... last column
+    uint64_t i0, i1;     // lanes whose column 0 / 1 lies inside the interior
+    uint32_t rows_in;    // bit r: row r of the wave lies inside the interior
+    int r_top, r_bot;    // wave row of the interior's first / last row (-1: none)
 };
 
 // K iterations of the four passes on the tile held in registers.
@@ -113,9 +133,9 @@ struct EscGeom {
 //    change (q = chy of the previous iteration or chx of this one, all false).
 // A tile where no wave has q set is a fixed point: the remaining iterations
 // are skipped.
-template <int NW, int R, int K, bool ESC>
+template <int NW, int R, int K>
 __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], uint32_t (&D0)[R],
-                                        uint32_t (&D1)[R], int c0, int y0, const EscGeom& e,
+                                        uint32_t (&D1)[R], int c0, int y0, const Watch& wt,
                                         uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
                                         int (*lds_q)[NW], int wave, int lane, int& esc)
 {
@@ -131,6 +151,7 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
         bool cx = false;
         if (chx || chy) {   // wave-uniform
             bool ch = false;
+            uint64_t hit = 0ull;
             // pass 1: source x-1
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -139,7 +160,8 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
                 const bool t1 = relax_key(I1[r], D1[r], own0, DL_P1(r));
                 const bool t0 = relax_key(I0[r], D0[r], left, DL_P0(r));
                 ch |= t0 | t1;
-                if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_left) | (t1 & e.col1_left);
+                const uint64_t rb = ((wt.rows_in >> r) & 1u) ? ~0ull : 0ull;
+                hit |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) & rb;
             }
             // pass 2: source x+1
 #pragma unroll
@@ -149,8 +171,10 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
                 const bool t0 = relax_key(I0[r], D0[r], own1, DL_P0(r));
                 const bool t1 = relax_key(I1[r], D1[r], right, DL_P1(r));
                 ch |= t0 | t1;
-                if (ESC && ((e.rows_in >> r) & 1u)) esc |= (t0 & e.col0_right) | (t1 & e.col1_right);
+                const uint64_t rb = ((wt.rows_in >> r) & 1u) ? ~0ull : 0ull;
+                hit |= ((__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1)) & rb;
             }
+            esc |= hit != 0ull;
             cx = __ballot(ch) != 0ull;
         }
         // pass 3: source y-1.  Publish this wave's last row and its q flag.
@@ -165,40 +189,40 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
         const bool run_y = lds_q[k & 1][wave] || (wave > 0 && lds_q[k & 1][wave - 1]) ||
                            (wave < NW - 1 && lds_q[k & 1][wave + 1]);
         bool ch = false;
+        uint64_t hit = 0ull;
         if (run_y) {
-            const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : 0u;
-            const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : 0u;
+            const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : kFarInv;
+            const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : kFarInv;
 #pragma unroll
-            for (int r = R - 1; r >= 1; --r) {
-                const bool t0 = relax_key(I0[r], D0[r], I0[r - 1], DL_P0(r));
-                const bool t1 = relax_key(I1[r], D1[r], I1[r - 1], DL_P1(r));
+            for (int r = R - 1; r >= 0; --r) {
+                const uint32_t a0 = r > 0 ? I0[r > 0 ? r - 1 : 0] : top0;
+                const uint32_t a1 = r > 0 ? I1[r > 0 ? r - 1 : 0] : top1;
+                const bool t0 = relax_key(I0[r], D0[r], a0, DL_P0(r));
+                const bool t1 = relax_key(I1[r], D1[r], a1, DL_P1(r));
                 ch |= t0 | t1;
-                if (ESC && r == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+                const uint64_t rb = r == wt.r_top ? ~0ull : 0ull;
+                hit |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) & rb;
             }
-            const bool t0 = relax_key(I0[0], D0[0], top0, DL_P0(0));
-            const bool t1 = relax_key(I1[0], D1[0], top1, DL_P1(0));
-            ch |= t0 | t1;
-            if (ESC && 0 == e.r_top) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
         }
         // pass 4: source y+1
         ldsB[wave][lane][0] = I0[0];
         ldsB[wave][lane][1] = I1[0];
         __syncthreads();
         if (run_y) {
-            const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : 0u;
-            const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : 0u;
+            const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : kFarInv;
+            const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : kFarInv;
 #pragma unroll
-            for (int r = 0; r < R - 1; ++r) {
-                const bool t0 = relax_key(I0[r], D0[r], I0[r + 1], DL_P0(r));
-                const bool t1 = relax_key(I1[r], D1[r], I1[r + 1], DL_P1(r));
+            for (int r = 0; r < R; ++r) {
+                const uint32_t b0 = r < R - 1 ? I0[r < R - 1 ? r + 1 : r] : bot0;
+                const uint32_t b1 = r < R - 1 ? I1[r < R - 1 ? r + 1 : r] : bot1;
+                const bool t0 = relax_key(I0[r], D0[r], b0, DL_P0(r));
+                const bool t1 = relax_key(I1[r], D1[r], b1, DL_P1(r));
                 ch |= t0 | t1;
-                if (ESC && r == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
+                const uint64_t rb = r == wt.r_bot ? ~0ull : 0ull;
+                hit |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) & rb;
             }
-            const bool t0 = relax_key(I0[R - 1], D0[R - 1], bot0, DL_P0(R - 1));
-            const bool t1 = relax_key(I1[R - 1], D1[R - 1], bot1, DL_P1(R - 1));
-            ch |= t0 | t1;
-            if (ESC && (R - 1) == e.r_bot) esc |= (t0 & e.col0_in) | (t1 & e.col1_in);
         }
+        esc |= hit != 0ull;
         chx = cx;
         chy = __ballot(ch) != 0ull;
     }
@@ -218,13 +242,11 @@ __host__ __device__ inline bool tile_watched(int P, int W, int H, int tx, int ty
            (oy - K <= P && P < oy + IH + K) || (oy - K <= H - P - 1 && H - P - 1 < oy + IH + K);
 }
 
-// ESC = false: every tile not watched (grid over all tiles, XCD-sliced);
-// ESC = true: the watched tiles listed in esc_list (one block each).
-template <int NW, int R, int K, bool ESC>
+template <int NW, int R, int K>
 __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
-    int* __restrict__ counters, const int* __restrict__ esc_list)
+    int* __restrict__ counters)
 {
     constexpr int IW = 128 - 2 * K;
     constexpr int IH = NW * R - 2 * K;
@@ -236,15 +258,10 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     // the 8 XCDs, so give each XCD a contiguous slice of tiles (neighbouring
     // tiles share halo lines in the same L2).  The grid is launched as a
     // multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
-    int tile;
-    if (ESC) {
-        tile = esc_list[blockIdx.x];
-    } else {
-        const int per_xcd = gridDim.x / 8;
-        const int b = blockIdx.x;
-        tile = (b % 8) * per_xcd + b / 8;
-        if (tile >= total_tiles) return;
-    }
+    const int per_xcd = gridDim.x / 8;
+    const int b = blockIdx.x;
+    const int tile = (b % 8) * per_xcd + b / 8;
+    if (tile >= total_tiles) return;
 
     int ci = 0;
     if (n_canvas > 1) {
@@ -261,7 +278,6 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     const int local = tile - tile_base;
     const int tx = local % tiles_x, ty = local / tiles_x;
     const int tid = threadIdx.x;
-    if (!ESC && tile_watched(cv.P, cv.W, cv.H, tx, ty, IW, IH, K)) return;
     // The wave index is uniform; readfirstlane lets the compiler keep every
     // row-dependent quantity in scalar registers.
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -302,15 +318,15 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
             const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
             const uint32_t a = src[rowoff + cc0];
             const uint32_t c = src[rowoff + cc1];
-            I0[r] = in0 ? ~a : 0u;
-            I1[r] = in1 ? ~c : 0u;
-            const uint32_t da = a == kInvalid ? 0xFFFFFFFFu : static_cast<uint32_t>(dist2_lab(c0, y, a));
-            const uint32_t dc = c == kInvalid ? 0xFFFFFFFFu : static_cast<uint32_t>(dist2_lab(c0 + 1, y, c));
+            I0[r] = (in0 && a != kInvalid) ? ~a : kFarInv;
+            I1[r] = (in1 && c != kInvalid) ? ~c : kFarInv;
+            const uint32_t da = a == kInvalid ? kFarD : static_cast<uint32_t>(dist2_lab(c0, y, a));
+            const uint32_t dc = c == kInvalid ? kFarD : static_cast<uint32_t>(dist2_lab(c0 + 1, y, c));
             D0[r] = in0 ? da : 0u;
             D1[r] = in1 ? dc : 0u;
         } else {
-            I0[r] = 0u;
-            I1[r] = 0u;
+            I0[r] = kFarInv;
+            I1[r] = kFarInv;
             D0[r] = 0u;
             D1[r] = 0u;
         }
@@ -318,27 +334,22 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
 
     int esc = 0;
     const int P = cv.P;
-    if (ESC) {
-        EscGeom e;
-        const int c1 = c0 + 1;
-        e.col0_in = c0 >= P && c0 < W - P;
-        e.col1_in = c1 >= P && c1 < W - P;
-        e.col0_left = c0 == P;
-        e.col1_left = c1 == P;
-        e.col0_right = c0 == W - P - 1;
-        e.col1_right = c1 == W - P - 1;
-        e.r_top = P - y0;
-        e.r_bot = H - P - 1 - y0;
-        e.rows_in = 0;
-        for (int r = 0; r < R; ++r) {
-            const int y = y0 + r;
-            if (y >= P && y < H - P) e.rows_in |= 1u << r;
-        }
-        iterate<NW, R, K, true>(I0, I1, D0, D1, c0, y0, e, ldsA, ldsB, lds_any, wave, lane, esc);
-    } else {
-        EscGeom e{};
-        iterate<NW, R, K, false>(I0, I1, D0, D1, c0, y0, e, ldsA, ldsB, lds_any, wave, lane, esc);
-    }
+    Watch wt;
+    const bool on = tile_watched(P, W, H, tx, ty, IW, IH, K);   // block-uniform
+    const uint64_t onm = on ? ~0ull : 0ull;
+    wt.l0 = __ballot(c0 == P) & onm;
+    wt.l1 = __ballot(c0 + 1 == P) & onm;
+    wt.r0 = __ballot(c0 == W - P - 1) & onm;
+    wt.r1 = __ballot(c0 + 1 == W - P - 1) & onm;
+    wt.i0 = __ballot(c0 >= P && c0 < W - P) & onm;
+    wt.i1 = __ballot(c0 + 1 >= P && c0 + 1 < W - P) & onm;
+    wt.r_top = on ? P - y0 : -1;
+    wt.r_bot = on ? H - P - 1 - y0 : -1;
+    wt.rows_in = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (on && y0 + r >= P && y0 + r < H - P) wt.rows_in |= 1u << r;
+    iterate<NW, R, K>(I0, I1, D0, D1, c0, y0, wt, ldsA, ldsB, lds_any, wave, lane, esc);
 
     // Write the interior back and note whether it changed.  The row origin is
     // passed through an empty asm statement so that the compiler recomputes
@@ -359,18 +370,18 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
         const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
         if (w0) {
             const size_t i = rowoff + c0;
-            const uint32_t v = ~I0[r];
+            const uint32_t v = I0[r] == kFarInv ? kInvalid : ~I0[r];
             changed |= src[i] != v;
             dst[i] = v;
         }
         if (w1) {
             const size_t i = rowoff + c0 + 1;
-            const uint32_t v = ~I1[r];
+            const uint32_t v = I1[r] == kFarInv ? kInvalid : ~I1[r];
             changed |= src[i] != v;
             dst[i] = v;
         }
     }
-    if (esc && cv.escape) atomicOr(cv.escape, 1);
+    if (esc && lane == 0 && cv.escape) atomicOr(cv.escape, 1);   // esc is wave-uniform
     __syncthreads();   // every wave is past its last read of lds_any
     const bool wave_changed = __ballot(changed) != 0ull;   // all lanes take part
     if (lane == 0) lds_any[0][wave] = wave_changed ? 1 : 0;
@@ -438,17 +449,14 @@ __global__ void zero_ints_kernel(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
-using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*,
-                           const int*);
+using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*);
 
 struct Variant {
     int waves, rows, iters;
-    TileKernel fn;      // unwatched tiles
-    TileKernel fn_esc;  // watched tiles
+    TileKernel fn;
 };
 
-#define DL_VARIANT(W_, R_, K_) \
-    {W_, R_, K_, bfs_tile_kernel<W_, R_, K_, false>, bfs_tile_kernel<W_, R_, K_, true>}
+#define DL_VARIANT(W_, R_, K_) {W_, R_, K_, bfs_tile_kernel<W_, R_, K_>}
 
 // Instantiated configurations.  The tile is 128 wide; height waves*rows.
 const Variant kVariants[] = {
@@ -507,22 +515,6 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     }
     const int n_canvas = static_cast<int>(canvases.size());
     if (total == 0) return 0;
-    // Watched tiles (see tile_watched) run a separate variant.
-    std::vector<int> watched;
-    for (const auto& c : canvases) {
-        if (c.P <= 0) continue;
-        for (int ty = 0; ty < c.tiles_y; ++ty)
-            for (int tx = 0; tx < c.tiles_x; ++tx)
-                if (tile_watched(c.P, c.W, c.H, tx, ty, iw, ih, cfg.iters))
-                    watched.push_back(c.tile_base + ty * c.tiles_x + tx);
-    }
-    const int n_watched = static_cast<int>(watched.size());
-    if (n_watched) {
-        work.tile_map.reserve(sizeof(int) * watched.size());
-        DL_HIP(hipMemcpyAsync(work.tile_map.ptr, watched.data(), sizeof(int) * watched.size(),
-                              hipMemcpyHostToDevice, s));
-    }
-
     work.canvases.reserve(sizeof(BfsCanvas) * n_canvas);
     DL_HIP(hipMemcpyAsync(work.canvases.ptr, canvases.data(), sizeof(BfsCanvas) * n_canvas,
                           hipMemcpyHostToDevice, s));
@@ -565,15 +557,7 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
                 ProfScope ps(kKFill, s);
                 v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
                     reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
-                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j), nullptr);
-                DL_LAUNCH_CHECK();
-            }
-            if (n_watched) {
-                ProfScope ps(kKFill, s);
-                v->fn_esc<<<n_watched, cfg.waves * 64, 0, s>>>(
-                    reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
-                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j),
-                    work.tile_map.as<int>());
+                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j));
                 DL_LAUNCH_CHECK();
             }
             ++launch;

@@ -22,6 +22,9 @@
  *   orc_assign_brute      src/delauney/reference/triangulation.py:193-204,
  *                         226-253 (every pixel against every triangle, closed
  *                          containment, max tid wins)
+ *   orc_voronoi_watch     orc_voronoi plus a count of the updates that carry a
+ *                         value into a box from outside it (checks the GPU
+ *                         fill's padding watch; not a reference function)
  *   orc_assign_raster     the same function computed per triangle by exact
  *                         scanline intervals; it is checked against
  *                         orc_assign_brute and exists only so that the oracle
@@ -88,9 +91,20 @@ typedef struct {
     int32_t *id, *d, *ox, *oy;
 } vstate;
 
+/* Box watched by orc_voronoi_watch: cells inside [P, W-P) x [P, H-P). */
+static int g_watch_P = -1, g_watch_W = 0, g_watch_H = 0;
+static int64_t g_watch_cross = 0;
+
+static inline int in_box(int x, int y)
+{
+    return x >= g_watch_P && x < g_watch_W - g_watch_P && y >= g_watch_P && y < g_watch_H - g_watch_P;
+}
+
 /* One cell update of voronoi.py:174-196: cell c receives the neighbour nb's
  * owner if the cell is undefined, or the candidate is strictly closer, or it
  * ties and the neighbour's id is higher. */
+static int W_cur = 0;
+
 static inline int relax(vstate* s, size_t c, size_t nb, int32_t px, int32_t py)
 {
     const int32_t nid = s->id[nb];
@@ -104,6 +118,9 @@ static inline int relax(vstate* s, size_t c, size_t nb, int32_t px, int32_t py)
         s->d[c] = cand;
         s->ox[c] = s->ox[nb];
         s->oy[c] = s->oy[nb];
+        if (g_watch_P >= 0 && in_box(px, py) &&
+            !in_box((int)(nb % (size_t)W_cur), (int)(nb / (size_t)W_cur)))
+            ++g_watch_cross;
         return 1;
     }
     return 0;
@@ -137,6 +154,7 @@ int orc_voronoi(int W, int H, const int32_t* sorted_xy, int64_t n,
         s.oy[c] = y;
     }
     int64_t iters = 0;
+    W_cur = W;
     for (;;) {
         int changed = 0;
         ++iters;
@@ -173,6 +191,22 @@ int orc_voronoi(int W, int H, const int32_t* sorted_xy, int64_t n,
     if (iters_out) *iters_out = iters;
     free(s.id); free(s.d); free(s.ox); free(s.oy);
     return 0;
+}
+
+/* orc_voronoi, also counting the updates that carry a value from outside the
+ * box [P, W-P) x [P, H-P) into it (cross-check of the GPU fill's escape
+ * watch; P is the canvas padding in the pipeline). */
+int orc_voronoi_watch(int W, int H, const int32_t* sorted_xy, int64_t n, int P,
+                      int32_t* out_hw2, int64_t* crossings)
+{
+    g_watch_P = P;
+    g_watch_W = W;
+    g_watch_H = H;
+    g_watch_cross = 0;
+    int rc = orc_voronoi(W, H, sorted_xy, n, out_hw2, NULL);
+    *crossings = g_watch_cross;
+    g_watch_P = -1;
+    return rc;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -56,6 +56,9 @@ def _load():
     lib.orc_detect.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, ctypes.c_int,
                                ctypes.POINTER(_I32P), _I64P, _I64P]
     lib.orc_detect.restype = ctypes.c_int
+    lib.orc_voronoi_watch.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
+                                      ctypes.c_int, _I32P, _I64P]
+    lib.orc_voronoi_watch.restype = ctypes.c_int
     lib.orc_free.argtypes = [ctypes.c_void_p]
     lib.orc_free.restype = None
     for name in ("orc_assign_brute", "orc_assign_raster"):
@@ -108,6 +111,20 @@ def voronoi_sorted(width: int, height: int, sorted_xy: np.ndarray, return_iters=
     if rc != 0:
         raise MemoryError("oracle voronoi allocation failed")
     return (out, int(iters.value)) if return_iters else out
+
+
+def voronoi_crossings(width: int, height: int, sorted_xy: np.ndarray, box_margin: int) -> int:
+    """Number of flood-fill updates that carry a value into the box
+    [m, W-m) x [m, H-m) from a cell outside it (0 means the box's cells evolve
+    exactly as if the canvas ended at the box)."""
+    sorted_xy = np.ascontiguousarray(sorted_xy, dtype=np.int32)
+    out = np.empty((height, width, 2), dtype=np.int32)
+    cross = ctypes.c_int64(0)
+    rc = _load().orc_voronoi_watch(int(width), int(height), _p(sorted_xy), sorted_xy.shape[0],
+                                   int(box_margin), _p(out), ctypes.byref(cross))
+    if rc != 0:
+        raise MemoryError("oracle voronoi allocation failed")
+    return int(cross.value)
 
 
 def voronoi(width: int, height: int, seeds) -> np.ndarray:

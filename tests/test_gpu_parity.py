@@ -205,3 +205,65 @@ def test_run_twice_identical():
     b = dl.triangulate(W, H, s, as_arrays=True)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+def _fill_watch(W, H, seeds, margin):
+    import ctypes
+    ctx = _lib.context()
+    lib = _lib.load()
+    s = np.ascontiguousarray(seeds, dtype=np.int32)
+    out = np.empty((H, W, 2), dtype=np.int32)
+    esc = ctypes.c_int(0)
+    _lib.check(lib.dl_debug_fill_watch(ctx.handle, W, H, _lib.i32p(s), s.shape[0], margin,
+                                       _lib.i32p(out), ctypes.byref(esc)))
+    return out, bool(esc.value)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fill_watch_never_misses_a_crossing(seed):
+    """The watch that lets the pipeline skip the second (unpadded) fill: on
+    boxes with seeds on both sides, every update that the oracle sees carried
+    into the box from outside must raise the GPU flag."""
+    rng = np.random.default_rng(300 + seed)
+    seen_cross = seen_clear = 0
+    for _ in range(10):
+        W, H = int(rng.integers(40, 700)), int(rng.integers(40, 500))
+        margin = int(rng.choice([1, 2, 5, 13, min(W, H) // 3]))
+        n = int(rng.integers(2, max(3, W * H // int(rng.choice([20, 300, 5000])))))
+        idx = rng.choice(W * H, size=n, replace=False)
+        s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+        cross = O.voronoi_crossings(W, H, O.sort_seeds(W, H, s), margin)
+        got, esc = _fill_watch(W, H, s, margin)
+        np.testing.assert_array_equal(got, O.voronoi(W, H, s))
+        assert esc or cross == 0, f"W={W} H={H} n={n} margin={margin} crossings={cross}"
+        seen_cross += cross > 0
+        seen_clear += not esc
+    assert seen_cross > 0
+
+
+@pytest.mark.parametrize("cfg", [(16, 4, 8), (8, 8, 4), (16, 8, 16)])
+def test_fill_watch_on_padded_canvases(cfg):
+    """On the pipeline's own padded canvas (seeds only inside the image) the
+    oracle sees no crossing in these cases; the GPU flag must stay clear for
+    most of them, or the pipeline would run its second fill needlessly."""
+    ctx = _lib.context()
+    lib = _lib.load()
+    _lib.check(lib.dl_set_bfs_config(ctx.handle, *cfg))
+    try:
+        rng = np.random.default_rng(11)
+        flagged = 0
+        cases = 0
+        for W, H, n, P in [(300, 200, 600, 16), (97, 411, 300, 3), (513, 129, 50, 1),
+                           (1000, 700, 20000, 16), (800, 800, 12, 16), (260, 30, 20, 7)]:
+            idx = rng.choice(W * H, size=n, replace=False)
+            s = np.stack([idx % W, idx // W], 1).astype(np.int32) + P
+            Wc, Hc = W + 2 * P, H + 2 * P
+            cross = O.voronoi_crossings(Wc, Hc, O.sort_seeds(Wc, Hc, s), P)
+            got, esc = _fill_watch(Wc, Hc, s, P)
+            np.testing.assert_array_equal(got, O.voronoi(Wc, Hc, s))
+            assert esc or cross == 0
+            flagged += esc
+            cases += 1
+        assert flagged <= cases // 2, f"{flagged} of {cases} padded fills flagged"
+    finally:
+        _lib.check(lib.dl_set_bfs_config(ctx.handle, 0, 0, 0))
