@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KINDS = ["fill", "seeds", "detect", "register", "assign", "output", "init"]
+KINDS = ["fill", "seeds", "detect", "register", "bin", "assign_output", "init"]
 
 
 def log(msg):
@@ -145,10 +145,9 @@ def main():
         "fill": 8.0 * tile_cells * fill_visits / max(fill_launches, 1.0),
         # labels read once (4 B/cell) + candidate records written/read (20 B each)
         "detect": 4.0 * Wp * Hp,
-        # (H, W, 3) written + label and tid read
-        "output": (12.0 + 4.0 + 4.0) * W * H,
-        # tid raster written once (4 B/px) + triangle rows and vertices read
-        "assign": 4.0 * W * H + 32.0 * T,
+        # fused assignment + output: (H, W, 3) int32 written once, the label
+        # plane read once, triangle rows and vertex labels read (32 B each)
+        "assign_output": (12.0 + 4.0) * W * H + 32.0 * T,
     }
     per_kernel = {}
     for k, (c, t) in stats.items():
@@ -158,11 +157,19 @@ def main():
     dominant = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_step"])
     dom = per_kernel[dominant]
     achieved = alg.get(dominant, 0.0) / (dom["avg_launch_us"] * 1e-6) / 1e9
+    # HBM bytes per launch of the dominant kernel, from the FETCH_SIZE and
+    # WRITE_SIZE counter runs summarised under profiles/ (valid for the
+    # flood-fill configuration they were taken with).
     traffic = None
+    traffic_src = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dominant)
+            tj = json.load(f)
+        ent = tj.get(dominant)
+        if ent and (dominant != "fill" or ent.get("config") == [cfg_w, cfg_r, cfg_k]):
+            traffic = ent["hbm_bytes_per_launch"]
+            traffic_src = tj.get("source")
     b_alg_image = 12.0 * W * H + 8.0 * Wp * Hp + 8.0 * N + 32.0 * T
 
     # End to end through the host API: H2D of the seeds, D2H of the grid.
@@ -204,6 +211,7 @@ def main():
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "alg_bytes_per_launch": alg.get(dominant),
                      "avg_launch_us": dom["avg_launch_us"]},
         "pipeline_alg_bytes_per_image": b_alg_image,

@@ -129,7 +129,8 @@ int dl_voronoi_simple(dl_context* ctx, int width, int height, const int32_t* see
  * While enabled, every call brackets its kernels with events and accumulates
  * counts and milliseconds per kind: [0] flood-fill tile kernel, [1] seed
  * ordering, [2] triangle detection, [3] registration (first-registration
- * marking, prefix scan, row emission), [4] pixel assignment, [5] output
+ * marking, prefix scan, row emission), [4] triangle binning, [5] per-tile
+ * pixel assignment fused with the output
  * assembly, [6] initialisation fills.  counts and ms hold 7 entries each.
  * No reference counterpart (the reference's PhaseTimer, phase_timer.cuh:17-62,
  * times phases, not kernels). */

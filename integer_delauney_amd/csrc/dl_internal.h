@@ -209,8 +209,8 @@ enum KernelKind {
     kKSeeds = 1,     // seed marking / ranking / scan
     kKDetect = 2,    // triangle candidate detection
     kKRegister = 3,  // first-registration marking, scan, row emission
-    kKAssign = 4,    // pixel assignment
-    kKOutput = 5,    // (H, W, C) output assembly
+    kKAssign = 4,    // triangle binning for the pixel assignment
+    kKOutput = 5,    // per-tile assignment fused with the (H, W, C) output
     kKInit = 6,      // canvas initialisation and fills
     kKinds = 7
 };

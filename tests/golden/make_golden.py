@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--skip-c1", action="store_true")
     args = ap.parse_args()
     sys.path.insert(0, args.reference_src)
+    sys.dont_write_bytecode = True   # the reference checkout is read-only: no __pycache__
     import delauney.reference as ref  # noqa: E402
     import delauney.reference.triangulation as tri_mod  # noqa: E402
 

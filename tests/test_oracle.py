@@ -201,3 +201,64 @@ def test_planted_padding_offset_is_caught():
     pad = O.voronoi_sorted(g["W"] + 32, g["H"] + 32, s + 16)
     rows, _ = O.detect(pad[:, :, 0], s, 15)   # wrong shift back
     assert not np.array_equal(rows, g["rows"])
+
+
+# --- the padding watch -------------------------------------------------------
+
+def _numpy_crossings(W, H, sorted_xy, m):
+    """voronoi.py:110-163 written with whole-array snapshots per pass (an
+    implementation independent of the C loops), counting updates whose cell
+    lies in [m, W-m) x [m, H-m) and whose source does not."""
+    ids = -np.ones((H, W), np.int64)
+    d = -np.ones((H, W), np.int64)
+    ox = np.zeros((H, W), np.int64)
+    oy = np.zeros((H, W), np.int64)
+    for i, (x, y) in enumerate(sorted_xy):
+        ids[y, x], d[y, x], ox[y, x], oy[y, x] = i, 0, x, y
+    yy, xx = np.mgrid[0:H, 0:W]
+    box = (xx >= m) & (xx < W - m) & (yy >= m) & (yy < H - m)
+    cross = 0
+    shifts = [(0, 1), (0, -1), (1, 0), (-1, 0)]   # (dy, dx): source = cell - (dy, dx)
+    while True:
+        changed = False
+        for dy, dx in shifts:
+            sid, sox, soy, sbox = (np.full((H, W), -1, np.int64), np.zeros((H, W), np.int64),
+                                   np.zeros((H, W), np.int64), np.zeros((H, W), bool))
+            ys, yd = (slice(0, H - dy), slice(dy, H)) if dy >= 0 else (slice(-dy, H), slice(0, H + dy))
+            xs, xd = (slice(0, W - dx), slice(dx, W)) if dx >= 0 else (slice(-dx, W), slice(0, W + dx))
+            sid[yd, xd], sox[yd, xd], soy[yd, xd], sbox[yd, xd] = ids[ys, xs], ox[ys, xs], oy[ys, xs], box[ys, xs]
+            cand = (xx - sox) ** 2 + (yy - soy) ** 2
+            take = (sid >= 0) & ((ids < 0) | (cand < d) | ((cand == d) & (sid > ids)))
+            cross += int((take & box & ~sbox).sum())
+            ids = np.where(take, sid, ids)
+            d = np.where(take, cand, d)
+            ox = np.where(take, sox, ox)
+            oy = np.where(take, soy, oy)
+            changed |= bool(take.any())
+        if not changed:
+            return cross, np.stack([ids, d], -1).astype(np.int32)
+
+
+def test_crossing_count_matches_numpy_restatement():
+    rng = np.random.default_rng(21)
+    nonzero = 0
+    for _ in range(25):
+        W, H = int(rng.integers(4, 40)), int(rng.integers(4, 40))
+        m = int(rng.integers(0, (min(W, H) - 1) // 2 + 1))
+        n = int(rng.integers(1, max(2, W * H // 8)))
+        idx = rng.choice(W * H, size=n, replace=False)
+        s = O.sort_seeds(W, H, np.stack([idx % W, idx // W], 1))
+        want, grid = _numpy_crossings(W, H, s, m)
+        np.testing.assert_array_equal(grid, O.voronoi_sorted(W, H, s))
+        assert O.voronoi_crossings(W, H, s, m) == want, (W, H, m, n)
+        nonzero += want > 0
+    assert nonzero > 5
+
+
+def test_crossing_count_closed_cases():
+    s = O.sort_seeds(20, 20, [(3, 4), (15, 9)])
+    assert O.voronoi_crossings(20, 20, s, 0) == 0      # the box is the canvas
+    # one seed outside a box: every box cell is defined once, and the box's
+    # first row and column receive the label from outside at least once
+    s1 = O.sort_seeds(20, 20, [(0, 0)])
+    assert O.voronoi_crossings(20, 20, s1, 5) > 0
