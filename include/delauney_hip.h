@@ -79,6 +79,29 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
 int dl_triangulate(dl_context* ctx, int width, int height, const int32_t* seeds_xy, int64_t n,
                    int border_padding, int32_t* out_hw3, int64_t* n_tri);
 
+/* Batch of n_images independent triangulations in one pipeline run: every
+ * stage (seed ordering, flood fill, detection, assignment) runs once for the
+ * whole batch.  Image i is widths[i] x heights[i] with seed_counts[i] seeds;
+ * seeds_xy holds the images' seeds one image after another (any order within
+ * an image), paddings[i] its border padding (NULL: all -1, the default).  out
+ * receives the (H_i, W_i, 3) grids one after another, n_tri[i] image i's
+ * triangle count; dl_copy_triangles then returns all rows, image after image,
+ * each image's tids starting at 0.  Every image's result equals dl_triangulate
+ * on that image alone.  Errors name the failing image.  No single reference
+ * counterpart: the reference triangulates one image per call
+ * (triangulate.hpp:15-43). */
+int dl_triangulate_batch(dl_context* ctx, int n_images, const int32_t* widths,
+                         const int32_t* heights, const int64_t* seed_counts,
+                         const int32_t* seeds_xy, const int32_t* paddings, int32_t* out,
+                         int64_t* n_tri);
+
+/* dl_triangulate_batch on device buffers: d_seeds_xy and d_out are device
+ * pointers laid out as above; the size arrays stay on the host. */
+int dl_triangulate_batch_device(dl_context* ctx, int n_images, const int32_t* widths,
+                                const int32_t* heights, const int64_t* seed_counts,
+                                const int32_t* d_seeds_xy, const int32_t* paddings,
+                                int32_t* d_out, void* stream, int64_t* n_tri);
+
 /* Rows (x, y, id_a, id_b, id_c) in tid order of the last triangulation on this
  * context: the reference's triangle_map values (triangulation.py:157;
  * binding_helpers.hpp:61-87).  out_rows holds n_tri * 5 int32. */

@@ -10,9 +10,9 @@ Importing the package needs no GPU; the first computation loads the library
 and creates a HIP context, and raises if either is unavailable.
 """
 from .api import (AUTO_BORDER_PADDING, DEFAULT_BORDER_PADDING, GridTriangulation, Voronoi,
-                  max_boundary_spacing, triangulate)
+                  max_boundary_spacing, triangulate, triangulate_batch)
 from .delaunay import Delaunay
 
-__all__ = ["Voronoi", "GridTriangulation", "Delaunay", "triangulate", "DEFAULT_BORDER_PADDING",
-           "max_boundary_spacing"]
-__version__ = "0.1.0"
+__all__ = ["Voronoi", "GridTriangulation", "Delaunay", "triangulate", "triangulate_batch",
+           "DEFAULT_BORDER_PADDING", "max_boundary_spacing"]
+__version__ = "0.2.0"

@@ -57,6 +57,10 @@ _SIGNATURES = {
     "dl_set_kernel_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_get_kernel_stats": (ctypes.c_int, [_VP, _I64P, ctypes.POINTER(ctypes.c_double)]),
     "dl_reset_kernel_stats": (ctypes.c_int, [_VP]),
+    "dl_triangulate_batch": (ctypes.c_int, [_VP, ctypes.c_int, _I32P, _I32P, _I64P, _I32P, _I32P,
+                                            _I32P, _I64P]),
+    "dl_triangulate_batch_device": (ctypes.c_int, [_VP, ctypes.c_int, _I32P, _I32P, _I64P, _VP,
+                                                   _I32P, _VP, _VP, _I64P]),
     "dl_debug_fill_watch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                            ctypes.c_int, _I32P, ctypes.POINTER(ctypes.c_int)]),
 }

@@ -178,3 +178,58 @@ def triangulate(width: int, height: int, seeds, border_padding=AUTO_BORDER_PADDI
                                           _lib.i32p(out), ctypes.byref(n)))
     rows = _fetch_rows(ctx, n.value)
     return _pack(rows, as_arrays), out
+
+
+def triangulate_batch(images, border_padding=AUTO_BORDER_PADDING, as_arrays: bool = False):
+    """Triangulate many independent images in one pipeline run.
+
+    ``images``: a sequence of ``(width, height, seeds)``.  ``border_padding``:
+    one value for every image or a sequence with one per image.  Returns a list
+    with, per image, what ``triangulate(width, height, seeds, border_padding,
+    as_arrays)`` returns for that image alone -- bit for bit.  Every stage runs
+    once for the whole batch (no reference counterpart: the reference
+    triangulates one image per call, src/cuda/bindings/triangulate.hpp).
+    """
+    images = list(images)
+    if not images:
+        return []
+    k = len(images)
+    widths = np.empty(k, dtype=np.int32)
+    heights = np.empty(k, dtype=np.int32)
+    counts = np.empty(k, dtype=np.int64)
+    parts = []
+    for i, (w, h, seeds) in enumerate(images):
+        if int(w) <= 0 or int(h) <= 0:
+            raise ValueError(f"image {i}: width and height must be positive")
+        try:
+            s = as_seed_array(seeds)
+        except ValueError as e:
+            raise ValueError(f"image {i}: {e}") from None
+        widths[i], heights[i], counts[i] = int(w), int(h), s.shape[0]
+        parts.append(s)
+    if np.ndim(border_padding) == 0:
+        pads = np.full(k, resolve_padding(border_padding), dtype=np.int32)
+    else:
+        pads = np.array([resolve_padding(p) for p in border_padding], dtype=np.int32)
+        if pads.shape[0] != k:
+            raise ValueError("border_padding needs one value per image")
+    xy = np.ascontiguousarray(np.concatenate(parts))
+    px = widths.astype(np.int64) * heights.astype(np.int64)
+    out = np.empty(int(3 * px.sum()), dtype=np.int32)
+    ntri = np.zeros(k, dtype=np.int64)
+    ctx = _lib.context()
+    lib = _lib.load()
+    _lib.check(lib.dl_triangulate_batch(
+        ctx.handle, k, _lib.i32p(widths), _lib.i32p(heights),
+        counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _lib.i32p(xy), _lib.i32p(pads),
+        _lib.i32p(out), ntri.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+    rows = _fetch_rows(ctx, int(ntri.sum()))
+    res = []
+    r0 = p0 = 0
+    for i in range(k):
+        t = int(ntri[i])
+        grid = out[p0:p0 + 3 * int(px[i])].reshape(int(heights[i]), int(widths[i]), 3)
+        res.append((_pack(rows[r0:r0 + t], as_arrays), grid))
+        r0 += t
+        p0 += 3 * int(px[i])
+    return res

@@ -136,13 +136,6 @@ ProfScope::~ProfScope()
 
 namespace {
 
-__global__ void shift_labels_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                    int64_t n, uint32_t add)
-{
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[i] + add;
-}
-
 __global__ void extract_ids_kernel(const int32_t* __restrict__ hw2, uint32_t* __restrict__ ids,
                                    int64_t n)
 {
@@ -160,14 +153,14 @@ struct dl_context {
     hipStream_t stream = nullptr;
     BfsWork bfs;
     TriWork tri;
-    DevBuf xy, sorted, canvas_labels, rank_at;
+    BatchPlan plan;   // images of the last call
+    DevBuf xy, sorted, rank_at;
     DevBuf padA, padB, unA, unB;
     DevBuf out, vgrid, ids, esc;
     PinnedBuf host_small;
 
     // state of the last triangulation
-    int64_t n_tri = 0;
-    int last_W = 0, last_H = 0, last_P = 0;
+    int64_t n_tri = 0;   // all images
     bool have_padded = false;
     int pad_plane = 0;
 
@@ -293,120 +286,153 @@ struct StageClock {
     }
 };
 
-// Seeds on the device (d_xy) -> ctx->sorted (id order) and ctx->rank_at.
-void prepare_seeds(dl_context* ctx, int W, int H, const int32_t* d_xy, int64_t n, hipStream_t s)
+// Plans ctx->plan (images already listed), uploads it and orders the seeds on
+// the device: d_xy -> ctx->sorted (id order per image) and ctx->rank_at.
+void prepare(dl_context* ctx, const int32_t* d_xy, hipStream_t s)
 {
-    ctx->sorted.reserve(sizeof(uint32_t) * static_cast<size_t>(n));
-    ctx->rank_at.reserve(sizeof(int32_t) * static_cast<size_t>(W) * H);
-    normalise_seeds(W, H, d_xy, n, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(),
-                    ctx->tri, s);
+    BatchPlan& plan = ctx->plan;
+    plan_batch(plan);
+    upload_plan(plan, ctx->tri, s);
+    ctx->sorted.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_seeds, 1)));
+    ctx->rank_at.reserve(sizeof(int32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_pix, 1)));
+    normalise_seeds(plan, d_xy, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), ctx->tri,
+                    s);
 }
 
-// Flood fill of the canvas (W + 2P) x (H + 2P) with the seeds shifted by P.
-// With watch_margin >= 0 the fill records in ctx->esc whether a cell inside
-// [m, Wc-m) x [m, Hc-m) ever received an update from a cell outside it.
-// Returns the canvas descriptor; *plane_out names the plane holding the result.
-BfsCanvas fill_canvas(dl_context* ctx, int W, int H, int P, int64_t n, int watch_margin,
-                      DevBuf& A, DevBuf& B, hipStream_t s, int* plane_out)
+// Flood fill of every image's detection canvas ((W + 2P) x (H + 2P), seeds
+// shifted by P).  watch[i] >= 0 records in ctx->esc[i] whether a cell of
+// [m, Wc-m) x [m, Hc-m) ever received an update from outside it.  Returns the
+// plane (0: padA, 1: padB) holding every canvas's result.
+int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s)
 {
-    const bool watch = watch_margin >= 0;
-    const int Wc = W + 2 * P, Hc = H + 2 * P;
-    const size_t cells = static_cast<size_t>(Wc) * Hc;
-    A.reserve(sizeof(uint32_t) * cells);
-    B.reserve(sizeof(uint32_t) * cells);
-    ctx->canvas_labels.reserve(sizeof(uint32_t) * static_cast<size_t>(n));
-    const uint32_t add = (static_cast<uint32_t>(P) << 16) | static_cast<uint32_t>(P);
-    shift_labels_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(
-        ctx->sorted.as<uint32_t>(), ctx->canvas_labels.as<uint32_t>(), n, add);
-    DL_LAUNCH_CHECK();
-    BfsCanvas cv{};
-    cv.lab0 = A.as<uint32_t>();
-    cv.lab1 = B.as<uint32_t>();
-    cv.W = Wc;
-    cv.H = Hc;
-    cv.P = watch ? watch_margin : 0;
-    cv.escape = nullptr;
-    if (watch) {
-        ctx->esc.reserve(sizeof(int) * 4);
-        fill_u32(ctx->esc.as<uint32_t>(), 0u, 4, s);
-        cv.escape = ctx->esc.as<int>();
+    const BatchPlan& plan = ctx->plan;
+    const int n_img = static_cast<int>(plan.img.size());
+    const size_t cells = static_cast<size_t>(std::max<int64_t>(plan.n_cells, 1));
+    ctx->padA.reserve(sizeof(uint32_t) * cells);
+    ctx->padB.reserve(sizeof(uint32_t) * cells);
+    ctx->esc.reserve(sizeof(int) * static_cast<size_t>(n_img));
+    fill_u32(ctx->esc.as<uint32_t>(), 0u, static_cast<size_t>(n_img), s);
+    init_canvases(plan, ctx->sorted.as<uint32_t>(), ctx->padA.as<uint32_t>(),
+                  ctx->padB.as<uint32_t>(), ctx->tri, s);
+    std::vector<BfsCanvas> cvs(n_img);
+    for (int i = 0; i < n_img; ++i) {
+        const ImageDesc& d = plan.img[i];
+        BfsCanvas& cv = cvs[i];
+        cv = BfsCanvas{};
+        cv.lab0 = ctx->padA.as<uint32_t>() + d.cv_off;
+        cv.lab1 = ctx->padB.as<uint32_t>() + d.cv_off;
+        cv.W = d.Wc;
+        cv.H = d.Hc;
+        cv.P = watch[i] >= 0 ? watch[i] : 0;
+        cv.escape = watch[i] >= 0 ? ctx->esc.as<int>() + i : nullptr;
     }
-    bfs_init_canvas(cv, ctx->canvas_labels.as<uint32_t>(), n, s);
-    std::vector<BfsCanvas> cvs{cv};
     BfsStats st;
-    const BfsConfig cfg = auto_config(ctx, Wc, Hc);
+    const BfsConfig cfg = auto_config(ctx, 0, 0);
     ctx->last_cfg = cfg;
     const int plane = bfs_run(cvs, cfg, ctx->bfs, s, &st);
     ctx->timings[7] += st.launches;
     ctx->timings[8] += st.tile_visits;
-    *plane_out = plane;
-    return cvs[0];
+    ctx->have_padded = true;
+    ctx->pad_plane = plane;
+    return plane;
 }
 
-const uint32_t* plane_ptr(const BfsCanvas& cv, int plane) { return plane ? cv.lab1 : cv.lab0; }
-
-// The batch pipeline on device-resident seeds.  Writes the (H, W, 3) grid to
-// d_out when d_out != nullptr, else only the tid raster (ctx->tri.tidgrid).
-void run_triangulate(dl_context* ctx, int W, int H, const int32_t* d_xy, int64_t n, int P,
-                     int32_t* d_out, hipStream_t s, StageClock& clk)
+const uint32_t* pad_plane_ptr(const dl_context* ctx, int plane)
 {
-    prepare_seeds(ctx, W, H, d_xy, n, s);
+    return plane ? ctx->padB.as<uint32_t>() : ctx->padA.as<uint32_t>();
+}
+
+// Unpadded fills (image-sized canvases) for the listed images; sets their
+// output label source.
+void fill_unpadded(dl_context* ctx, const std::vector<int>& which, hipStream_t s)
+{
+    BatchPlan& plan = ctx->plan;
+    std::vector<int64_t> off(which.size());
+    int64_t total = 0;
+    for (size_t k = 0; k < which.size(); ++k) {
+        off[k] = total;
+        const ImageDesc& d = plan.img[which[k]];
+        total += static_cast<int64_t>(d.W) * d.H;
+    }
+    ctx->unA.reserve(sizeof(uint32_t) * static_cast<size_t>(total));
+    ctx->unB.reserve(sizeof(uint32_t) * static_cast<size_t>(total));
+    std::vector<BfsCanvas> cvs(which.size());
+    for (size_t k = 0; k < which.size(); ++k) {
+        const ImageDesc& d = plan.img[which[k]];
+        BfsCanvas& cv = cvs[k];
+        cv = BfsCanvas{};
+        cv.lab0 = ctx->unA.as<uint32_t>() + off[k];
+        cv.lab1 = ctx->unB.as<uint32_t>() + off[k];
+        cv.W = d.W;
+        cv.H = d.H;
+        bfs_init_canvas(cv, ctx->sorted.as<uint32_t>() + d.seed_off, d.n, s);
+    }
+    BfsStats st;
+    const int plane = bfs_run(cvs, ctx->last_cfg, ctx->bfs, s, &st);
+    ctx->timings[7] += st.launches;
+    ctx->timings[8] += st.tile_visits;
+    for (size_t k = 0; k < which.size(); ++k) {
+        ImageDesc& d = plan.img[which[k]];
+        d.out_lab = (plane ? ctx->unB.as<uint32_t>() : ctx->unA.as<uint32_t>()) + off[k];
+        d.out_W = d.W;
+        d.out_off = 0;
+        d.out_shift = 0;
+    }
+}
+
+// The batch pipeline on device-resident seeds (ctx->plan lists the images):
+// seeds -> padded flood fill -> [unpadded fill where the padding reached the
+// image] -> detection -> assignment fused with the (H, W, 3) output, or the
+// bare tid raster when C == 1.
+void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s,
+                  StageClock& clk)
+{
+    prepare(ctx, d_xy, s);
     clk.mark(0);
-    const uint32_t* det_grid;
-    int det_W, det_H;
-    const uint32_t* out_labels;
-    int out_Wc, out_off;
-    ctx->have_padded = false;
-    if (P > 0) {
-        int plane = 0;
-        BfsCanvas cv = fill_canvas(ctx, W, H, P, n, P, ctx->padA, ctx->padB, s, &plane);
-        det_grid = plane_ptr(cv, plane);
-        det_W = cv.W;
-        det_H = cv.H;
-        ctx->have_padded = true;
-        ctx->pad_plane = plane;
-        // Did any padding cell ever update an interior cell?  If not, the
-        // interior's trajectory is that of the unpadded fill, pass by pass.
-        int escaped = 0;
-        DL_HIP(hipMemcpyAsync(ctx->host_small.ptr, ctx->esc.ptr, sizeof(int),
-                              hipMemcpyDeviceToHost, s));
-        DL_HIP(hipStreamSynchronize(s));
-        escaped = ctx->host_small.as<int>()[0];
-        if (escaped || ctx->force_unpadded) {
-            int uplane = 0;
-            BfsCanvas ucv = fill_canvas(ctx, W, H, 0, n, -1, ctx->unA, ctx->unB, s, &uplane);
-            out_labels = plane_ptr(ucv, uplane);
-            out_Wc = W;
-            out_off = 0;
-        } else {
-            out_labels = det_grid;
-            out_Wc = det_W;
-            out_off = P;
+    BatchPlan& plan = ctx->plan;
+    const int n_img = static_cast<int>(plan.img.size());
+    const bool labels_out = C == 3;
+    std::vector<int> watch(n_img, -1);
+    bool any_pad = false;
+    for (int i = 0; i < n_img; ++i)
+        if (labels_out && plan.img[i].P > 0) {
+            watch[i] = plan.img[i].P;
+            any_pad = true;
         }
-    } else {
-        int plane = 0;
-        BfsCanvas cv = fill_canvas(ctx, W, H, 0, n, -1, ctx->unA, ctx->unB, s, &plane);
-        det_grid = plane_ptr(cv, plane);
-        det_W = W;
-        det_H = H;
-        out_labels = det_grid;
-        out_Wc = W;
-        out_off = 0;
+    const int plane = fill_detection(ctx, watch, s);
+    const uint32_t* base = pad_plane_ptr(ctx, plane);
+    for (int i = 0; i < n_img; ++i) {
+        ImageDesc& d = plan.img[i];
+        d.out_lab = base + d.cv_off;
+        d.out_W = d.Wc;
+        d.out_off = d.P;
+        d.out_shift = d.P;
+    }
+    if (labels_out) {
+        // Did a padding cell ever update an image cell?  If not, the image's
+        // part of the padded canvas evolved exactly as the unpadded fill.
+        std::vector<int> which;
+        if (any_pad) {
+            ctx->host_small.reserve(sizeof(int) * static_cast<size_t>(n_img));
+            DL_HIP(hipMemcpyAsync(ctx->host_small.ptr, ctx->esc.ptr, sizeof(int) * n_img,
+                                  hipMemcpyDeviceToHost, s));
+            DL_HIP(hipStreamSynchronize(s));
+            const int* esc = ctx->host_small.as<int>();
+            for (int i = 0; i < n_img; ++i)
+                if (plan.img[i].P > 0 && (esc[i] || ctx->force_unpadded)) which.push_back(i);
+        }
+        if (!which.empty()) fill_unpadded(ctx, which, s);
+        upload_desc(plan, ctx->tri, s);
     }
     clk.mark(1);
-    const DetectResult dr = detect_triangles(det_grid, true, det_W, det_H, P,
-                                             ctx->rank_at.as<int32_t>(), W,
-                                             ctx->sorted.as<uint32_t>(), n, ctx->tri, s);
+    const DetectResult dr = detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(),
+                                             ctx->sorted.as<uint32_t>(), ctx->tri, s);
     clk.mark(2);
     clk.mark(3);
     ctx->n_tri = dr.n_tri;
+    assign_tiled(plan, ctx->tri.rows.as<int32_t>(), dr.n_tri, ctx->sorted.as<uint32_t>(),
+                 ctx->rank_at.as<int32_t>(), d_out, C, ctx->tri, s);
     clk.mark(4);
-    assign_tiled(W, H, ctx->tri.rows.as<int32_t>(), dr.n_tri, ctx->sorted.as<uint32_t>(), out_labels,
-                 out_Wc, out_off, out_off, ctx->rank_at.as<int32_t>(), W, d_out, 3, ctx->tri, s);
-    clk.mark(5);
-    ctx->last_W = W;
-    ctx->last_H = H;
-    ctx->last_P = P;
 }
 
 void upload_seeds(dl_context* ctx, const int32_t* xy, int64_t n)
@@ -416,11 +442,65 @@ void upload_seeds(dl_context* ctx, const int32_t* xy, int64_t n)
                           hipMemcpyHostToDevice, ctx->stream));
 }
 
+void plan_single(dl_context* ctx, int W, int H, int P, int64_t n)
+{
+    ctx->plan = BatchPlan{};
+    ImageDesc d{};
+    d.W = W;
+    d.H = H;
+    d.P = P;
+    d.n = n;
+    ctx->plan.img.push_back(d);
+}
+
+// Batch description from the C arrays; validates every image.
+int64_t plan_many(dl_context* ctx, int n_images, const int32_t* widths, const int32_t* heights,
+                  const int64_t* seed_counts, const int32_t* paddings)
+{
+    if (n_images <= 0) throw value_error("n_images must be positive");
+    if (!widths || !heights || !seed_counts) throw value_error("null argument");
+    ctx->plan = BatchPlan{};
+    int64_t total = 0;
+    for (int i = 0; i < n_images; ++i) {
+        ImageDesc d{};
+        d.W = widths[i];
+        d.H = heights[i];
+        d.P = resolve_padding(paddings ? paddings[i] : -1);
+        d.n = seed_counts[i];
+        try {
+            check_dims(d.W, d.H, d.P);
+            if (d.n <= 0) throw value_error("seeds must not be empty");
+        } catch (const value_error& e) {
+            throw value_error("image " + std::to_string(i) + ": " + e.what());
+        }
+        total += d.n;
+        ctx->plan.img.push_back(d);
+    }
+    return total;
+}
+
+struct ProfDrop {
+    dl_context* c;
+    hipStream_t st;
+    ProfDrop(dl_context* ctx, hipStream_t s) : c(ctx), st(s)
+    {
+        if (c->profiling) set_current_profiler(&c->prof);
+    }
+    ~ProfDrop()
+    {
+        if (c->profiling) {
+            (void)hipStreamSynchronize(st);
+            c->prof.collect();
+        }
+        set_current_profiler(nullptr);
+    }
+};
+
 }  // namespace
 
 extern "C" {
 
-const char* dl_version(void) { return "integer_delauney_amd 0.1.0 (gfx950)"; }
+const char* dl_version(void) { return "integer_delauney_amd 0.2.0 (gfx950)"; }
 
 const char* dl_last_error(void) { return g_last_error.c_str(); }
 
@@ -542,17 +622,18 @@ int dl_voronoi(dl_context* ctx, int width, int height, const int32_t* seeds_xy, 
         hipStream_t s = ctx->stream;
         StageClock clk(ctx);
         upload_seeds(ctx, seeds_xy, n);
-        prepare_seeds(ctx, width, height, ctx->xy.as<int32_t>(), n, s);
+        plan_single(ctx, width, height, 0, n);
+        prepare(ctx, ctx->xy.as<int32_t>(), s);
         clk.mark(0);
-        int plane = 0;
-        BfsCanvas cv = fill_canvas(ctx, width, height, 0, n, -1, ctx->unA, ctx->unB, s, &plane);
+        const int plane = fill_detection(ctx, std::vector<int>{-1}, s);
         clk.mark(1);
         const size_t px = static_cast<size_t>(width) * height;
         ctx->out.reserve(sizeof(int32_t) * 2 * px);
-        write_output(width, height, plane_ptr(cv, plane), width, 0, 0, ctx->rank_at.as<int32_t>(),
-                     width, nullptr, ctx->out.as<int32_t>(), 2, s);
+        write_output(width, height, pad_plane_ptr(ctx, plane), width, 0, 0,
+                     ctx->rank_at.as<int32_t>(), width, nullptr, ctx->out.as<int32_t>(), 2, s);
         DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * px,
                               hipMemcpyDeviceToHost, s));
+        ctx->have_padded = false;
         clk.finish(s);
     });
 }
@@ -569,21 +650,22 @@ int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* s
         DL_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         upload_seeds(ctx, seeds_xy, n);
-        prepare_seeds(ctx, width, height, ctx->xy.as<int32_t>(), n, s);
-        int plane = 0;
-        BfsCanvas cv = fill_canvas(ctx, width, height, 0, n, margin, ctx->unA, ctx->unB, s, &plane);
+        plan_single(ctx, width, height, 0, n);
+        prepare(ctx, ctx->xy.as<int32_t>(), s);
+        const int plane = fill_detection(ctx, std::vector<int>{margin}, s);
         DL_HIP(hipMemcpyAsync(ctx->host_small.ptr, ctx->esc.ptr, sizeof(int),
                               hipMemcpyDeviceToHost, s));
         if (out_hw2) {
             const size_t px = static_cast<size_t>(width) * height;
             ctx->out.reserve(sizeof(int32_t) * 2 * px);
-            write_output(width, height, plane_ptr(cv, plane), width, 0, 0,
+            write_output(width, height, pad_plane_ptr(ctx, plane), width, 0, 0,
                          ctx->rank_at.as<int32_t>(), width, nullptr, ctx->out.as<int32_t>(), 2, s);
             DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * px,
                                   hipMemcpyDeviceToHost, s));
         }
         DL_HIP(hipStreamSynchronize(s));
         *escaped = ctx->host_small.as<int>()[0] != 0;
+        ctx->have_padded = false;
     });
 }
 
@@ -597,7 +679,8 @@ int dl_voronoi_simple(dl_context* ctx, int width, int height, const int32_t* see
         DL_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         upload_seeds(ctx, seeds_xy, n);
-        prepare_seeds(ctx, width, height, ctx->xy.as<int32_t>(), n, s);
+        plan_single(ctx, width, height, 0, n);
+        prepare(ctx, ctx->xy.as<int32_t>(), s);
         const size_t cells = static_cast<size_t>(width) * height;
         ctx->unA.reserve(sizeof(uint32_t) * cells);
         ctx->unB.reserve(sizeof(uint32_t) * cells);
@@ -609,11 +692,12 @@ int dl_voronoi_simple(dl_context* ctx, int width, int height, const int32_t* see
         bfs_init_canvas(cv, ctx->sorted.as<uint32_t>(), n, s);
         const int plane = bfs_run_simple(cv, ctx->bfs, s, iters);
         ctx->out.reserve(sizeof(int32_t) * 2 * cells);
-        write_output(width, height, plane_ptr(cv, plane), width, 0, 0, ctx->rank_at.as<int32_t>(),
-                     width, nullptr, ctx->out.as<int32_t>(), 2, s);
+        write_output(width, height, plane ? cv.lab1 : cv.lab0, width, 0, 0,
+                     ctx->rank_at.as<int32_t>(), width, nullptr, ctx->out.as<int32_t>(), 2, s);
         DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * cells,
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
+        ctx->have_padded = false;
     });
 }
 
@@ -630,12 +714,13 @@ int dl_triangulate(dl_context* ctx, int width, int height, const int32_t* seeds_
         hipStream_t s = ctx->stream;
         StageClock clk(ctx);
         upload_seeds(ctx, seeds_xy, n);
+        plan_single(ctx, width, height, P, n);
         const size_t px = static_cast<size_t>(width) * height;
         ctx->out.reserve(sizeof(int32_t) * 3 * px);
-        run_triangulate(ctx, width, height, ctx->xy.as<int32_t>(), n, P, ctx->out.as<int32_t>(), s,
-                        clk);
+        run_pipeline(ctx, ctx->xy.as<int32_t>(), ctx->out.as<int32_t>(), 3, s, clk);
         DL_HIP(hipMemcpyAsync(out_hw3, ctx->out.ptr, sizeof(int32_t) * 3 * px,
                               hipMemcpyDeviceToHost, s));
+        clk.mark(5);
         clk.finish(s);
         if (n_tri) *n_tri = ctx->n_tri;
     });
@@ -652,23 +737,57 @@ int dl_triangulate_device(dl_context* ctx, int width, int height, const int32_t*
         check_seeds(d_seeds_xy, n);
         DL_HIP(hipSetDevice(ctx->device));
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-        if (ctx->profiling) set_current_profiler(&ctx->prof);
-        struct Drop {
-            dl_context* c;
-            hipStream_t st;
-            ~Drop()
-            {
-                if (c->profiling) {
-                    (void)hipStreamSynchronize(st);
-                    c->prof.collect();
-                }
-                set_current_profiler(nullptr);
-            }
-        } drop{ctx, s};
+        ProfDrop drop(ctx, s);
         StageClock clk(ctx);
-        run_triangulate(ctx, width, height, d_seeds_xy, n, P, d_out_hw3, s, clk);
+        plan_single(ctx, width, height, P, n);
+        run_pipeline(ctx, d_seeds_xy, d_out_hw3, 3, s, clk);
         clk.finish(s);
         if (n_tri) *n_tri = ctx->n_tri;
+    });
+}
+
+int dl_triangulate_batch(dl_context* ctx, int n_images, const int32_t* widths,
+                         const int32_t* heights, const int64_t* seed_counts,
+                         const int32_t* seeds_xy, const int32_t* paddings, int32_t* out,
+                         int64_t* n_tri)
+{
+    return guarded([&] {
+        if (!ctx || !out || !seeds_xy) throw value_error("null argument");
+        const int64_t total = plan_many(ctx, n_images, widths, heights, seed_counts, paddings);
+        DL_HIP(hipSetDevice(ctx->device));
+        ProfInstall pi(ctx);
+        hipStream_t s = ctx->stream;
+        StageClock clk(ctx);
+        upload_seeds(ctx, seeds_xy, total);
+        int64_t px = 0;
+        for (const auto& d : ctx->plan.img) px += static_cast<int64_t>(d.W) * d.H;
+        ctx->out.reserve(sizeof(int32_t) * 3 * static_cast<size_t>(px));
+        run_pipeline(ctx, ctx->xy.as<int32_t>(), ctx->out.as<int32_t>(), 3, s, clk);
+        DL_HIP(hipMemcpyAsync(out, ctx->out.ptr, sizeof(int32_t) * 3 * static_cast<size_t>(px),
+                              hipMemcpyDeviceToHost, s));
+        clk.mark(5);
+        clk.finish(s);
+        if (n_tri)
+            for (int i = 0; i < n_images; ++i) n_tri[i] = ctx->tri.tri_base[i + 1] - ctx->tri.tri_base[i];
+    });
+}
+
+int dl_triangulate_batch_device(dl_context* ctx, int n_images, const int32_t* widths,
+                                const int32_t* heights, const int64_t* seed_counts,
+                                const int32_t* d_seeds_xy, const int32_t* paddings,
+                                int32_t* d_out, void* stream, int64_t* n_tri)
+{
+    return guarded([&] {
+        if (!ctx || !d_out || !d_seeds_xy) throw value_error("null argument");
+        plan_many(ctx, n_images, widths, heights, seed_counts, paddings);
+        DL_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        ProfDrop drop(ctx, s);
+        StageClock clk(ctx);
+        run_pipeline(ctx, d_seeds_xy, d_out, 3, s, clk);
+        clk.finish(s);
+        if (n_tri)
+            for (int i = 0; i < n_images; ++i) n_tri[i] = ctx->tri.tri_base[i + 1] - ctx->tri.tri_base[i];
     });
 }
 
@@ -691,23 +810,16 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
         hipStream_t s = ctx->stream;
         StageClock clk(ctx);
         upload_seeds(ctx, seeds_xy, n);
-        prepare_seeds(ctx, width, height, ctx->xy.as<int32_t>(), n, s);
-        clk.mark(0);
+        plan_single(ctx, width, height, P, n);
         const size_t px = static_cast<size_t>(width) * height;
-        DetectResult dr;
-        ctx->have_padded = false;
+        ctx->tri.tidgrid.reserve(sizeof(int32_t) * px);
         if (P > 0) {
-            int plane = 0;
-            BfsCanvas cv = fill_canvas(ctx, width, height, P, n, -1, ctx->padA, ctx->padB, s,
-                                       &plane);
-            ctx->have_padded = true;
-            ctx->pad_plane = plane;
-            clk.mark(1);
-            dr = detect_triangles(plane_ptr(cv, plane), true, cv.W, cv.H, P,
-                                  ctx->rank_at.as<int32_t>(), width, ctx->sorted.as<uint32_t>(), n,
-                                  ctx->tri, s);
+            run_pipeline(ctx, ctx->xy.as<int32_t>(), ctx->tri.tidgrid.as<int32_t>(), 1, s, clk);
         } else {
             if (!voronoi_grid_hw2) throw value_error("voronoi_grid is required at padding 0");
+            prepare(ctx, ctx->xy.as<int32_t>(), s);
+            clk.mark(0);
+            ctx->have_padded = false;
             ctx->vgrid.reserve(sizeof(int32_t) * 2 * px);
             ctx->ids.reserve(sizeof(uint32_t) * px);
             DL_HIP(hipMemcpyAsync(ctx->vgrid.ptr, voronoi_grid_hw2, sizeof(int32_t) * 2 * px,
@@ -716,25 +828,21 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
                 ctx->vgrid.as<int32_t>(), ctx->ids.as<uint32_t>(), static_cast<int64_t>(px));
             DL_LAUNCH_CHECK();
             clk.mark(1);
-            dr = detect_triangles(ctx->ids.as<uint32_t>(), false, width, height, 0,
-                                  ctx->rank_at.as<int32_t>(), width, ctx->sorted.as<uint32_t>(), n,
-                                  ctx->tri, s);
+            const DetectResult dr =
+                detect_triangles(ctx->plan, ctx->ids.as<uint32_t>(), false,
+                                 ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(), ctx->tri, s);
+            clk.mark(2);
+            clk.mark(3);
+            ctx->n_tri = dr.n_tri;
+            assign_tiled(ctx->plan, ctx->tri.rows.as<int32_t>(), dr.n_tri,
+                         ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(),
+                         ctx->tri.tidgrid.as<int32_t>(), 1, ctx->tri, s);
+            clk.mark(4);
         }
-        clk.mark(2);
-        clk.mark(3);
-        ctx->n_tri = dr.n_tri;
-        ctx->tri.tidgrid.reserve(sizeof(int32_t) * px);
-        assign_tiled(width, height, ctx->tri.rows.as<int32_t>(), dr.n_tri,
-                     ctx->sorted.as<uint32_t>(), nullptr, 0, 0, 0, nullptr, width,
-                     ctx->tri.tidgrid.as<int32_t>(), 1, ctx->tri, s);
-        clk.mark(4);
         DL_HIP(hipMemcpyAsync(out_tid_hw, ctx->tri.tidgrid.ptr, sizeof(int32_t) * px,
                               hipMemcpyDeviceToHost, s));
         clk.mark(5);
         clk.finish(s);
-        ctx->last_W = width;
-        ctx->last_H = height;
-        ctx->last_P = P;
         if (n_tri) *n_tri = ctx->n_tri;
     });
 }
@@ -757,19 +865,21 @@ int dl_copy_padded_voronoi(dl_context* ctx, int32_t* out, int* out_w, int* out_h
 {
     return guarded([&] {
         if (!ctx) throw value_error("null context");
-        const int P = ctx->last_P;
-        const int Wc = ctx->last_W + 2 * P, Hc = ctx->last_H + 2 * P;
+        if (ctx->plan.img.size() != 1) throw value_error("the last call was not a single image");
+        const ImageDesc& d = ctx->plan.img[0];
+        const int P = d.P;
+        const int Wc = d.W + 2 * P, Hc = d.H + 2 * P;
         if (out_w) *out_w = Wc;
         if (out_h) *out_h = Hc;
         if (!out) return;
-        if (!ctx->have_padded) throw value_error("no padded canvas was computed by the last call");
+        if (!ctx->have_padded || P == 0)
+            throw value_error("no padded canvas was computed by the last call");
         DL_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         const size_t cells = static_cast<size_t>(Wc) * Hc;
         ctx->out.reserve(sizeof(int32_t) * 2 * cells);
-        const uint32_t* lab = ctx->pad_plane ? ctx->padB.as<uint32_t>() : ctx->padA.as<uint32_t>();
-        write_output(Wc, Hc, lab, Wc, 0, P, ctx->rank_at.as<int32_t>(), ctx->last_W, nullptr,
-                     ctx->out.as<int32_t>(), 2, s);
+        write_output(Wc, Hc, pad_plane_ptr(ctx, ctx->pad_plane), Wc, 0, P,
+                     ctx->rank_at.as<int32_t>(), d.W, nullptr, ctx->out.as<int32_t>(), 2, s);
         DL_HIP(hipMemcpyAsync(out, ctx->out.ptr, sizeof(int32_t) * 2 * cells,
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));

@@ -131,38 +131,86 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
 int bfs_run_simple(const BfsCanvas& cv, BfsWork& work, hipStream_t s, int64_t* iters);
 
 // ---------------------------------------------------------------------------
-// Seeds, detection, assignment, output.
+// Seeds, detection, assignment, output -- over a batch of images.
+//
+// Every stage runs once per batch: per-image buffers are concatenated, each
+// image's slice starts at the offsets below, and a kernel maps its global work
+// index to the image by binary search over an offset table.  A single image is
+// a batch of one.
 // ---------------------------------------------------------------------------
-struct SeedSet {
-    // Outputs of normalise_seeds, all on the device.
-    uint32_t* sorted;   // n packed image-space labels in id order
-    int32_t* rank_at;   // W x H: id of the seed at (x, y); only seed cells written
+struct ImageDesc {
+    int W, H, P;          // image size and border padding
+    int Wc, Hc;           // detection canvas (W + 2P) x (H + 2P)
+    int64_t n;            // seeds
+    int64_t seed_off;     // first seed: xy pairs, sorted labels
+    int64_t bm_off;       // first word of the seed presence bitmap (W*H bits)
+    int64_t pix_off;      // first pixel: rank_at and output (times channels)
+    int64_t cv_off;       // first cell of the detection canvas planes
+    int64_t nb;           // 2x2 blocks of the canvas, (Wc-1)*(Hc-1)
+    int64_t det_off;      // first detection thread block
+    int bx_per_row;       // detection thread blocks per row of 2x2 blocks
+    int tiles_x;          // 64x64 assignment tiles per row
+    int64_t key_off;      // first word of the registration bitmap (4*nb bits)
+    int64_t tile_off;     // first assignment tile
+    // Source of output channels 0/1: cell (x + out_off, y + out_off) of a
+    // plane out_W wide; an owner's image position is its label's position
+    // minus (out_shift, out_shift).
+    const uint32_t* out_lab;
+    int out_W, out_off, out_shift;
 };
+
+struct BatchPlan {
+    std::vector<ImageDesc> img;
+    // Offset tables (n_img + 1 entries) for the binary searches.
+    std::vector<int64_t> seed_seg, det_seg, tile_seg;
+    int64_t n_seeds = 0, n_words = 0, n_pix = 0, n_cells = 0;
+    int64_t n_det_blocks = 0, n_key_words = 0, n_tiles = 0;
+    int max_seed_bits = 1;   // bits of the largest per-image seed id
+};
+
+// Fills the offsets of plan.img (W, H, P and n set by the caller).
+void plan_batch(BatchPlan& plan);
+
+struct TriWork;
 
 struct TriWork {
     DevBuf bitmap;      // seed presence / registration bitmaps
     DevBuf scan;        // word prefix sums
     DevBuf scan_tmp;
-    DevBuf flags;       // device status flags
+    DevBuf flags;       // device status flags (per image, then batch-wide)
     PinnedBuf host_flags;
     DevBuf table_tag;   // registration hash table slots
     DevBuf cand;        // raw candidate records
     DevBuf counts;      // per-thread / per-block candidate counts
     DevBuf offsets;     // their prefix sums
     DevBuf positions;   // compact list of block positions holding candidates
-    DevBuf rows;        // (T, 5) triangle rows
+    DevBuf rows;        // (T, 5) triangle rows of all images, image after image
     DevBuf tidgrid;     // W x H tid raster
     DevBuf bin_counts, bin_offsets, bin_list;   // triangle -> tile bins
+    DevBuf desc;        // device copy of the plan's ImageDesc array
+    DevBuf segs;        // device copy of the offset tables + tri_base
+    PinnedBuf host_stage;   // staging for the uploads above
+    std::vector<int64_t> tri_base;   // host copy: first row of each image, then the total
 };
 
-// Validates (bounds, duplicates) and orders seeds on the device.
-// d_xy: n (x, y) int32 pairs in caller order.  Throws value_error on
-// duplicates or out-of-bounds coordinates.
-void normalise_seeds(int W, int H, const int32_t* d_xy, int64_t n, uint32_t* d_sorted,
+// Uploads the plan's descriptors and offset tables (the output sources may be
+// re-uploaded later with upload_desc).
+void upload_plan(const BatchPlan& plan, TriWork& work, hipStream_t s);
+void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s);
+
+// Validates (bounds, duplicates) and orders the seeds of every image.
+// d_xy: concatenated (x, y) int32 pairs in caller order.  Throws value_error
+// naming the first failing image when the batch has more than one.
+void normalise_seeds(const BatchPlan& plan, const int32_t* d_xy, uint32_t* d_sorted,
                      int32_t* d_rank_at, TriWork& work, hipStream_t s);
 
-// Exclusive prefix sum of popcount(words[i]) into out[i]; returns nothing on the
-// host (total is out[n] written by the kernel).  out has n+1 entries.
+// Writes kInvalid over both planes of every canvas, then each seed's canvas
+// label (image label + (P, P)) at its cell.  plane_a/plane_b: n_cells each.
+void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* plane_a,
+                   uint32_t* plane_b, struct TriWork& work, hipStream_t s);
+
+// Exclusive prefix sum of popcount(words[i]) into out[i]; out[n] receives the
+// total.  out has n+1 entries.
 void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp,
                    hipStream_t s);
 
@@ -170,34 +218,36 @@ void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp,
 void exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s);
 
 struct DetectResult {
-    int64_t n_tri = 0;
+    int64_t n_tri = 0;     // triangles of the whole batch
     int64_t n_quads = 0;
     int64_t n_cand = 0;
 };
 
-// Triangle extraction on a canvas of ids or labels.  If labels is true, grid
-// holds packed canvas labels and ids are looked up through rank_at (image
-// space, shifted by P); otherwise grid holds seed ids directly.
-// Writes (T, 5) rows (x - P, y - P, a, b, c) into work.rows.
-DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp, int P,
-                              const int32_t* rank_at, int W_img, const uint32_t* sorted,
-                              int64_t n_seeds, TriWork& work, hipStream_t s);
+// Triangle extraction on every image's canvas.  grid_base + cv_off is image
+// i's canvas; it holds packed canvas labels (labels = true; ids through
+// rank_at) or seed ids (labels = false).  Writes rows
+// (x - P, y - P, a, b, c) of all images, image after image, into work.rows and
+// the first row of each image into work.tri_base (host) and the device table.
+DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
+                              const int32_t* rank_at, const uint32_t* sorted, TriWork& work,
+                              hipStream_t s);
 
-// (H, W, C) int32 output.  Output pixel (x, y) reads canvas cell
-// (x + off, y + off); the owner's image position is its canvas position minus
-// shift.  channels 0/1 = (rank(owner), d^2), channel 2 (C == 3) = tidgrid.
+// Pixel assignment binned by 64x64 tiles and fused with the output write.
+// C == 3: out_base + 3 * pix_off is image i's (H, W, 3) grid, channels 0/1
+// from its output label source; C == 1: out_base + pix_off its (H, W) tid
+// raster.
+void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
+                  const uint32_t* sorted, const int32_t* rank_at, int32_t* out_base, int C,
+                  TriWork& work, hipStream_t s);
+
+// (H, W, C) int32 output of one image without triangles (C == 2) or with the
+// tid raster (C == 3).  Output pixel (x, y) reads canvas cell (x + off, y + off);
+// the owner's image position is its canvas position minus shift.
 void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, int shift,
                   const int32_t* rank_at, int W_img, const int32_t* tidgrid, int32_t* out, int C,
                   hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
-
-// Pixel assignment binned by 64x64 tiles and fused with the output write.
-// C == 3: out is the (H, W, 3) grid, channels 0/1 from the canvas labels as in
-// write_output; C == 1: out is the (H, W) tid raster.
-void assign_tiled(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
-                  const uint32_t* labels, int Wc, int off, int shift, const int32_t* rank_at,
-                  int W_img, int32_t* out, int C, struct TriWork& work, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Per-kernel timing.  When a profiler is installed for the calling thread,

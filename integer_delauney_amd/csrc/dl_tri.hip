@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 
 namespace dl {
 
@@ -155,12 +156,126 @@ void exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, DevBuf& tm
     scan_impl<false>(in, n, out, tmp, s);
 }
 
+
+// ===========================================================================
+// Batch plan
+// ===========================================================================
+namespace {
+
+// Detection thread blocks cover kDetChunk consecutive 2x2 blocks of one row of
+// blocks; each thread owns kDetItems of them, strided by the thread count so
+// that a wave reads consecutive cells.
+constexpr int kDetThreads = 256;
+constexpr int kDetItems = 4;
+constexpr int kDetChunk = kDetThreads * kDetItems;
+constexpr int kTileSide = 64;   // assignment tile side in pixels
+
+// Largest i in [0, n) with off[i] <= g (off is non-decreasing, off[0] = 0).
+// Empty segments (off[i] == off[i + 1]) are skipped by construction.
+__device__ __forceinline__ int find_seg(const int64_t* __restrict__ off, int n, int64_t g)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+int bits_for(int64_t v)   // bits needed to hold values in [0, v]
+{
+    int b = 1;
+    while (b < 62 && (v >> b) != 0) ++b;
+    return b;
+}
+
+}  // namespace
+
+void plan_batch(BatchPlan& plan)
+{
+    const size_t n = plan.img.size();
+    plan.seed_seg.assign(n + 1, 0);
+    plan.det_seg.assign(n + 1, 0);
+    plan.tile_seg.assign(n + 1, 0);
+    int64_t seed = 0, words = 0, pix = 0, cells = 0, det = 0, keyw = 0, tiles = 0, maxn = 1;
+    for (size_t i = 0; i < n; ++i) {
+        ImageDesc& d = plan.img[i];
+        d.Wc = d.W + 2 * d.P;
+        d.Hc = d.H + 2 * d.P;
+        d.seed_off = seed;
+        seed += d.n;
+        d.bm_off = words;
+        words += (static_cast<int64_t>(d.W) * d.H + 31) / 32;
+        d.pix_off = pix;
+        pix += static_cast<int64_t>(d.W) * d.H;
+        d.cv_off = cells;
+        cells += static_cast<int64_t>(d.Wc) * d.Hc;
+        const int bw = d.Wc - 1, bh = d.Hc - 1;
+        d.nb = (bw > 0 && bh > 0) ? static_cast<int64_t>(bw) * bh : 0;
+        d.bx_per_row = d.nb ? (bw + kDetChunk - 1) / kDetChunk : 0;
+        d.det_off = det;
+        det += d.nb ? static_cast<int64_t>(d.bx_per_row) * bh : 0;
+        d.key_off = keyw;
+        keyw += (4 * d.nb + 31) / 32;
+        d.tiles_x = (d.W + kTileSide - 1) / kTileSide;
+        d.tile_off = tiles;
+        tiles += static_cast<int64_t>(d.tiles_x) * ((d.H + kTileSide - 1) / kTileSide);
+        maxn = std::max<int64_t>(maxn, d.n);
+        plan.seed_seg[i + 1] = seed;
+        plan.det_seg[i + 1] = det;
+        plan.tile_seg[i + 1] = tiles;
+    }
+    plan.n_seeds = seed;
+    plan.n_words = words;
+    plan.n_pix = pix;
+    plan.n_cells = cells;
+    plan.n_det_blocks = det;
+    plan.n_key_words = keyw;
+    plan.n_tiles = tiles;
+    plan.max_seed_bits = bits_for(maxn - 1);
+}
+
+// Device layout of work.segs: seed_seg, det_seg, tile_seg, tri_base, each
+// n_img + 1 int64.
+namespace {
+inline int64_t* seg_ptr(const TriWork& w, int which, size_t n_img)
+{
+    return w.segs.as<int64_t>() + static_cast<size_t>(which) * (n_img + 1);
+}
+}  // namespace
+
+void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s)
+{
+    // The staging buffer is only rewritten after the stream has been
+    // synchronised (every pipeline stage that re-uploads follows a sync).
+    const size_t n = plan.img.size();
+    const size_t db = sizeof(ImageDesc) * n;
+    work.desc.reserve(db);
+    work.host_stage.reserve(db + sizeof(int64_t) * 4 * (n + 1));
+    std::memcpy(work.host_stage.ptr, plan.img.data(), db);
+    DL_HIP(hipMemcpyAsync(work.desc.ptr, work.host_stage.ptr, db, hipMemcpyHostToDevice, s));
+}
+
+void upload_plan(const BatchPlan& plan, TriWork& work, hipStream_t s)
+{
+    const size_t n = plan.img.size();
+    const size_t db = sizeof(ImageDesc) * n;
+    upload_desc(plan, work, s);
+    work.segs.reserve(sizeof(int64_t) * 4 * (n + 1));
+    int64_t* h = reinterpret_cast<int64_t*>(static_cast<char*>(work.host_stage.ptr) + db);
+    std::memcpy(h, plan.seed_seg.data(), sizeof(int64_t) * (n + 1));
+    std::memcpy(h + (n + 1), plan.det_seg.data(), sizeof(int64_t) * (n + 1));
+    std::memcpy(h + 2 * (n + 1), plan.tile_seg.data(), sizeof(int64_t) * (n + 1));
+    DL_HIP(hipMemcpyAsync(work.segs.ptr, h, sizeof(int64_t) * 3 * (n + 1), hipMemcpyHostToDevice,
+                          s));
+}
+
 // ===========================================================================
 // Seeds
 // ===========================================================================
 namespace {
 
-// Status flag bits shared with the host.
+// Status flag bits shared with the host: per image (seeds) and batch-wide.
 constexpr uint32_t kFlagXRange = 1u;
 constexpr uint32_t kFlagYRange = 2u;
 constexpr uint32_t kFlagDuplicate = 4u;
@@ -168,67 +283,113 @@ constexpr uint32_t kFlagTableOverflow = 16u;
 constexpr uint32_t kFlagCollision = 32u;
 constexpr uint32_t kFlagBadId = 64u;
 
-// Presence bit of (x, y) is x*H + y: bit order is the id order.
-__global__ void seeds_mark_kernel(const int32_t* __restrict__ xy, int64_t n, int W, int H,
-                                  uint32_t* __restrict__ bm, uint32_t* __restrict__ flags)
+// Presence bit of (x, y) in its image's bitmap is x*H + y: bit order is the id
+// order.
+__global__ void seeds_mark_kernel(const int32_t* __restrict__ xy, int64_t n_total,
+                                  const int64_t* __restrict__ seed_seg, int n_img,
+                                  const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
+                                  uint32_t* __restrict__ flags)
 {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int x = xy[2 * i], y = xy[2 * i + 1];
-    if (x < 0 || x >= W) { atomicOr(flags, kFlagXRange); return; }
-    if (y < 0 || y >= H) { atomicOr(flags, kFlagYRange); return; }
-    const uint64_t bit = static_cast<uint64_t>(x) * H + y;
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_total) return;
+    const int img = n_img > 1 ? find_seg(seed_seg, n_img, g) : 0;
+    const ImageDesc& d = D[img];
+    const int x = xy[2 * g], y = xy[2 * g + 1];
+    if (x < 0 || x >= d.W) { atomicOr(&flags[img], kFlagXRange); return; }
+    if (y < 0 || y >= d.H) { atomicOr(&flags[img], kFlagYRange); return; }
+    const uint64_t bit = static_cast<uint64_t>(x) * d.H + y;
     const uint32_t m = 1u << (bit & 31);
-    const uint32_t old = atomicOr(&bm[bit >> 5], m);
-    if (old & m) atomicOr(flags, kFlagDuplicate);
+    const uint32_t old = atomicOr(&bm[d.bm_off + static_cast<int64_t>(bit >> 5)], m);
+    if (old & m) atomicOr(&flags[img], kFlagDuplicate);
 }
 
-__global__ void seeds_rank_kernel(const int32_t* __restrict__ xy, int64_t n, int W, int H,
-                                  const uint32_t* __restrict__ bm, const uint32_t* __restrict__ pre,
-                                  uint32_t* __restrict__ sorted, int32_t* __restrict__ rank_at)
+__global__ void seeds_rank_kernel(const int32_t* __restrict__ xy, int64_t n_total,
+                                  const int64_t* __restrict__ seed_seg, int n_img,
+                                  const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
+                                  const uint32_t* __restrict__ pre, uint32_t* __restrict__ sorted,
+                                  int32_t* __restrict__ rank_at)
 {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int x = xy[2 * i], y = xy[2 * i + 1];
-    const uint64_t bit = static_cast<uint64_t>(x) * H + y;
-    const uint32_t word = bm[bit >> 5];
-    const uint32_t below = word & ((1u << (bit & 31)) - 1u);
-    const uint32_t rank = pre[bit >> 5] + static_cast<uint32_t>(__popc(below));
-    sorted[rank] = pack_label(x, y);
-    rank_at[static_cast<size_t>(y) * W + x] = static_cast<int32_t>(rank);
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_total) return;
+    const int img = n_img > 1 ? find_seg(seed_seg, n_img, g) : 0;
+    const ImageDesc& d = D[img];
+    const int x = xy[2 * g], y = xy[2 * g + 1];
+    const uint64_t bit = static_cast<uint64_t>(x) * d.H + y;
+    const int64_t w = d.bm_off + static_cast<int64_t>(bit >> 5);
+    const uint32_t below = bm[w] & ((1u << (bit & 31)) - 1u);
+    const uint32_t rank = pre[w] - pre[d.bm_off] + static_cast<uint32_t>(__popc(below));
+    sorted[d.seed_off + rank] = pack_label(x, y);
+    rank_at[d.pix_off + static_cast<int64_t>(y) * d.W + x] = static_cast<int32_t>(rank);
+}
+
+// Seed cells of every canvas: the image label shifted by (P, P), in both planes.
+__global__ void canvas_seed_kernel(const uint32_t* __restrict__ sorted, int64_t n_total,
+                                   const int64_t* __restrict__ seed_seg, int n_img,
+                                   const ImageDesc* __restrict__ D, uint32_t* __restrict__ a,
+                                   uint32_t* __restrict__ b)
+{
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_total) return;
+    const int img = n_img > 1 ? find_seg(seed_seg, n_img, g) : 0;
+    const ImageDesc& d = D[img];
+    const uint32_t l = sorted[g] + ((static_cast<uint32_t>(d.P) << 16) | static_cast<uint32_t>(d.P));
+    const int64_t c = d.cv_off + static_cast<int64_t>(l & 0xFFFFu) * d.Wc + (l >> 16);
+    a[c] = l;
+    b[c] = l;
 }
 
 }  // namespace
 
-void normalise_seeds(int W, int H, const int32_t* d_xy, int64_t n, uint32_t* d_sorted,
+void normalise_seeds(const BatchPlan& plan, const int32_t* d_xy, uint32_t* d_sorted,
                      int32_t* d_rank_at, TriWork& work, hipStream_t s)
 {
-    const int64_t bits = static_cast<int64_t>(W) * H;
-    const int64_t nw = (bits + 31) / 32;
-    work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nw));
+    const int n_img = static_cast<int>(plan.img.size());
+    const int64_t nw = plan.n_words;
+    work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(nw, 1)));
     work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nw + 1));
-    work.flags.reserve(sizeof(uint32_t) * 8);
-    work.host_flags.reserve(sizeof(uint32_t) * 8);
+    work.flags.reserve(sizeof(uint32_t) * (n_img + 8));
+    work.host_flags.reserve(sizeof(uint32_t) * (n_img + 8));
     uint32_t* bm = work.bitmap.as<uint32_t>();
     uint32_t* flags = work.flags.as<uint32_t>();
-    const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+    const ImageDesc* D = work.desc.as<ImageDesc>();
+    const int64_t* sseg = seg_ptr(work, 0, n_img);
+    const unsigned blocks = static_cast<unsigned>((plan.n_seeds + 255) / 256);
     {
         ProfScope ps(kKSeeds, s);
         fill_u32(bm, 0u, static_cast<size_t>(nw), s);
-        fill_u32(flags, 0u, 8, s);
-        seeds_mark_kernel<<<blocks, 256, 0, s>>>(d_xy, n, W, H, bm, flags);
+        fill_u32(flags, 0u, static_cast<size_t>(n_img + 8), s);
+        seeds_mark_kernel<<<blocks, 256, 0, s>>>(d_xy, plan.n_seeds, sseg, n_img, D, bm, flags);
         DL_LAUNCH_CHECK();
     }
-    DL_HIP(hipMemcpyAsync(work.host_flags.ptr, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(work.host_flags.ptr, flags, sizeof(uint32_t) * n_img,
+                          hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
-    const uint32_t f = work.host_flags.as<uint32_t>()[0];
-    if (f & kFlagXRange) throw value_error("seed x coordinate out of bounds");
-    if (f & kFlagYRange) throw value_error("seed y coordinate out of bounds");
-    if (f & kFlagDuplicate) throw value_error("duplicate seed positions are not allowed");
+    const uint32_t* hf = work.host_flags.as<uint32_t>();
+    for (int i = 0; i < n_img; ++i) {
+        const uint32_t f = hf[i];
+        if (!f) continue;
+        const std::string who = n_img > 1 ? "image " + std::to_string(i) + ": " : "";
+        if (f & kFlagXRange) throw value_error(who + "seed x coordinate out of bounds");
+        if (f & kFlagYRange) throw value_error(who + "seed y coordinate out of bounds");
+        if (f & kFlagDuplicate) throw value_error(who + "duplicate seed positions are not allowed");
+    }
     ProfScope ps(kKSeeds, s);
     popcount_scan(bm, nw, work.scan.as<uint32_t>(), work.scan_tmp, s);
-    seeds_rank_kernel<<<blocks, 256, 0, s>>>(d_xy, n, W, H, bm, work.scan.as<uint32_t>(), d_sorted,
-                                             d_rank_at);
+    seeds_rank_kernel<<<blocks, 256, 0, s>>>(d_xy, plan.n_seeds, sseg, n_img, D, bm,
+                                             work.scan.as<uint32_t>(), d_sorted, d_rank_at);
+    DL_LAUNCH_CHECK();
+}
+
+void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* plane_a,
+                   uint32_t* plane_b, TriWork& work, hipStream_t s)
+{
+    const int n_img = static_cast<int>(plan.img.size());
+    ProfScope ps(kKInit, s);
+    fill_u32(plane_a, kInvalid, static_cast<size_t>(plan.n_cells), s);
+    fill_u32(plane_b, kInvalid, static_cast<size_t>(plan.n_cells), s);
+    canvas_seed_kernel<<<static_cast<unsigned>((plan.n_seeds + 255) / 256), 256, 0, s>>>(
+        d_sorted, plan.n_seeds, seg_ptr(work, 0, n_img), n_img, work.desc.as<ImageDesc>(),
+        plane_a, plane_b);
     DL_LAUNCH_CHECK();
 }
 
@@ -238,9 +399,10 @@ void normalise_seeds(int W, int H, const int32_t* d_xy, int64_t n, uint32_t* d_s
 namespace {
 
 struct Cand {
-    uint32_t key;
+    uint32_t key;    // registration key scan * nb + block, 0xFFFFFFFF = none
     uint32_t slot;
     int32_t a, b, c;
+    uint32_t img;
 };
 
 constexpr uint64_t kEmptyTag = ~0ull;
@@ -260,40 +422,44 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// One 32-byte slot per distinct triplet: an insert touches one cache line.
+// One 32-byte slot per distinct (image, triplet): an insert touches one line.
 struct alignas(32) Slot {
     unsigned long long tag;
     uint32_t minkey;   // smallest registration key seen for the triplet
     uint32_t blk;      // 1: blocked by a cocircular quad
     int32_t a, b, c;   // the sorted triplet (collision check)
-    uint32_t pad;
+    uint32_t img;
 };
 
 struct Table {
     Slot* slot;
     uint64_t mask;
-    int exact;        // tags are exact packings of the sorted triplet
+    int exact;        // tags are exact packings of (image, sorted triplet)
+    int bits;         // bits per seed id in an exact tag
     uint64_t salt;
 };
 
-__device__ __forceinline__ uint64_t triplet_tag(const Table& t, int32_t a, int32_t b, int32_t c)
+__device__ __forceinline__ uint64_t triplet_tag(const Table& t, uint32_t img, int32_t a, int32_t b,
+                                                int32_t c)
 {
     // a < b < c
     if (t.exact)
-        return static_cast<uint64_t>(a) | (static_cast<uint64_t>(b) << 21) |
-               (static_cast<uint64_t>(c) << 42);
+        return static_cast<uint64_t>(a) | (static_cast<uint64_t>(b) << t.bits) |
+               (static_cast<uint64_t>(c) << (2 * t.bits)) |
+               (static_cast<uint64_t>(img) << (3 * t.bits));
     uint64_t h = mix64(static_cast<uint64_t>(static_cast<uint32_t>(a)) ^ t.salt);
     h = mix64(h ^ (static_cast<uint64_t>(static_cast<uint32_t>(b)) << 1));
     h = mix64(h ^ (static_cast<uint64_t>(static_cast<uint32_t>(c)) << 2));
+    h = mix64(h ^ (static_cast<uint64_t>(img) << 3));
     return h & 0x7FFFFFFFFFFFFFFFull;
 }
 
-// Returns the slot of the sorted triplet, inserting it if absent; -1 (flag
+// Returns the slot of (img, sorted triplet), inserting it if absent; -1 (flag
 // raised) if the table is full, which its sizing rules out.
-__device__ int64_t table_insert(const Table& t, int32_t a, int32_t b, int32_t c,
+__device__ int64_t table_insert(const Table& t, uint32_t img, int32_t a, int32_t b, int32_t c,
                                 uint32_t* flags)
 {
-    const uint64_t tg = triplet_tag(t, a, b, c);
+    const uint64_t tg = triplet_tag(t, img, a, b, c);
     uint64_t h = mix64(tg ^ 0x9E3779B97F4A7C15ull) & t.mask;
     for (uint64_t probe = 0; probe <= t.mask; ++probe) {
         Slot& sl = t.slot[h];
@@ -307,6 +473,7 @@ __device__ int64_t table_insert(const Table& t, int32_t a, int32_t b, int32_t c,
                 sl.a = a;
                 sl.b = b;
                 sl.c = c;
+                sl.img = img;
                 return static_cast<int64_t>(h);
             }
             if (prev == tg) return static_cast<int64_t>(h);
@@ -385,56 +552,39 @@ __device__ void quad_rule(const int32_t ids[4], const int64_t px[4], const int64
 }
 
 struct DetectArgs {
-    const uint32_t* grid;
-    int Wp, Hp, P;
-    int labels;              // grid holds labels (else ids)
-    const int32_t* rank_at;  // image-space id map (labels mode)
-    int W_img;
-    const uint32_t* sorted;  // id -> packed image position
-    int64_t n_seeds;
-    uint32_t nb;             // number of 2x2 blocks, (Wp-1)*(Hp-1)
-    int bx_per_row;          // thread blocks per block row
-    uint32_t* counts;        // per-thread candidate counts (count pass)
-    uint32_t* block_quads;   // per-thread-block quad counts (count pass)
-    uint8_t* masks;          // per-thread masks of items holding candidates
-    uint32_t* positions;     // compact list of those block positions
-    const uint32_t* offsets; // their exclusive prefix sums (insert pass)
+    const uint32_t* grid;         // canvas planes of all images (image i at D[i].cv_off)
+    int labels;                   // grid holds labels (else ids)
+    const ImageDesc* D;
+    const int64_t* det_seg;       // first detection thread block of each image
+    int n_img;
+    const int32_t* rank_at;       // image-space id maps (labels mode)
+    const uint32_t* sorted;       // id -> packed image position, per image
+    uint32_t* counts;             // per-thread candidate counts (count pass)
+    uint32_t* block_quads;        // per-thread-block quad counts (count pass)
+    uint8_t* masks;               // per-thread masks of items holding candidates
+    uint2* positions;             // compact list (image, block) of those blocks
+    const uint32_t* offsets;      // exclusive prefix sums of counts
     Cand* cand;
-    uint32_t* flags;
+    uint32_t* flags;              // batch-wide status word
     Table table;
 };
 
-// A thread block covers kDetItems*kDetThreads consecutive 2x2 blocks of one
-// block row (grid: x = chunk of the row, y = row).  Each thread owns
-// kDetItems of them, strided by the thread count so that a wave reads
-// consecutive cells.
-constexpr int kDetThreads = 256;
-constexpr int kDetItems = 4;
-constexpr int kDetChunk = kDetThreads * kDetItems;
-
-__device__ __forceinline__ int32_t to_id(const DetectArgs& A, uint32_t v, uint32_t* flags)
+__device__ __forceinline__ int32_t to_id(const DetectArgs& A, const ImageDesc& d, uint32_t v)
 {
     if (A.labels) {
         if (v == kInvalid) {   // an undefined cell: impossible after a converged fill
-            atomicOr(flags, kFlagBadId);
+            atomicOr(A.flags, kFlagBadId);
             return -1;
         }
-        const int lx = static_cast<int>(v >> 16) - A.P, ly = static_cast<int>(v & 0xFFFFu) - A.P;
-        return A.rank_at[static_cast<size_t>(ly) * A.W_img + lx];
+        const int lx = static_cast<int>(v >> 16) - d.P, ly = static_cast<int>(v & 0xFFFFu) - d.P;
+        return A.rank_at[d.pix_off + static_cast<int64_t>(ly) * d.W + lx];
     }
     const int32_t id = static_cast<int32_t>(v);
-    if (id < 0 || id >= A.n_seeds) {
-        atomicOr(flags, kFlagBadId);
+    if (id < 0 || id >= d.n) {
+        atomicOr(A.flags, kFlagBadId);
         return -1;
     }
     return id;
-}
-
-__device__ __forceinline__ void seed_pos(const DetectArgs& A, int32_t id, int64_t& x, int64_t& y)
-{
-    const uint32_t l = A.sorted[id];
-    x = static_cast<int64_t>(l >> 16);
-    y = static_cast<int64_t>(l & 0xFFFFu);
 }
 
 // The four corners of a 2x2 block (TL, TR, BL, BR) and which of the four
@@ -445,14 +595,14 @@ struct BlockCorners {
     bool quad;
 };
 
-__device__ __forceinline__ BlockCorners block_corners(const DetectArgs& A, int rx, int ry)
+__device__ __forceinline__ BlockCorners block_corners(const uint32_t* g, int Wc, int rx, int ry)
 {
     BlockCorners c;
-    const size_t o = static_cast<size_t>(ry) * A.Wp + rx;
-    c.tl = A.grid[o];
-    c.tr = A.grid[o + 1];
-    c.bl = A.grid[o + A.Wp];
-    c.br = A.grid[o + A.Wp + 1];
+    const size_t o = static_cast<size_t>(ry) * Wc + rx;
+    c.tl = g[o];
+    c.tr = g[o + 1];
+    c.bl = g[o + Wc];
+    c.br = g[o + Wc + 1];
     const bool e_tl_tr = c.tl == c.tr, e_tl_bl = c.tl == c.bl, e_tl_br = c.tl == c.br;
     const bool e_tr_bl = c.tr == c.bl, e_tr_br = c.tr == c.br, e_bl_br = c.bl == c.br;
     c.v[0] = !e_tl_tr && !e_tl_br && !e_tr_br;   // (TL, TR, BR)
@@ -476,21 +626,26 @@ __device__ __forceinline__ uint32_t block_candidates(const BlockCorners& c)
 
 __global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
 {
-    const int ry = blockIdx.y;
-    const int rx0 = blockIdx.x * kDetChunk + threadIdx.x;
-    const int bw = A.Wp - 1;
+    const int64_t gb = blockIdx.x;
+    const int img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
+    const ImageDesc& d = A.D[img];
+    const int64_t lb = gb - A.det_seg[img];
+    const int ry = static_cast<int>(lb / d.bx_per_row);
+    const int bx = static_cast<int>(lb - static_cast<int64_t>(ry) * d.bx_per_row);
+    const int rx0 = bx * kDetChunk + threadIdx.x;
+    const int bw = d.Wc - 1;
+    const uint32_t* g = A.grid + d.cv_off;
     uint32_t mask = 0, quads = 0;
 #pragma unroll
     for (int j = 0; j < kDetItems; ++j) {
         const int rx = rx0 + j * kDetThreads;
         if (rx >= bw) break;
-        const BlockCorners c = block_corners(A, rx, ry);
+        const BlockCorners c = block_corners(g, d.Wc, rx, ry);
         if (block_candidates(c)) mask |= 1u << j;
         quads += c.quad;
     }
-    const size_t blk = static_cast<size_t>(ry) * A.bx_per_row + blockIdx.x;
-    A.counts[blk * kDetThreads + threadIdx.x] = static_cast<uint32_t>(__popc(mask));
-    A.masks[blk * kDetThreads + threadIdx.x] = static_cast<uint8_t>(mask);
+    A.counts[gb * kDetThreads + threadIdx.x] = static_cast<uint32_t>(__popc(mask));
+    A.masks[gb * kDetThreads + threadIdx.x] = static_cast<uint8_t>(mask);
     // Quads per thread block (a single counter would serialise ~10^4 atomics).
     __shared__ uint32_t red[kDetThreads / 64];
     for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
@@ -499,29 +654,8 @@ __global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
     if (threadIdx.x == 0) {
         uint32_t t = 0;
         for (int w = 0; w < kDetThreads / 64; ++w) t += red[w];
-        A.block_quads[blk] = t;
+        A.block_quads[gb] = t;
     }
-}
-
-__device__ __forceinline__ void add_candidate(const DetectArgs& A, uint32_t& out, int scan,
-                                              uint32_t pos, int32_t a, int32_t b, int32_t c)
-{
-    int32_t sa = a, sb = b, sc = c;
-    sort3(sa, sb, sc);
-    const int64_t sl = table_insert(A.table, sa, sb, sc, A.flags);
-    if (sl < 0) {
-        A.cand[out++].key = 0xFFFFFFFFu;
-        return;
-    }
-    const uint32_t key = static_cast<uint32_t>(scan) * A.nb + pos;
-    atomicMin(&A.table.slot[sl].minkey, key);
-    Cand cd;
-    cd.key = key;
-    cd.slot = static_cast<uint32_t>(sl);
-    cd.a = a;
-    cd.b = b;
-    cd.c = c;
-    A.cand[out++] = cd;
 }
 
 // Compact list of the block positions that hold candidates, from the count
@@ -533,17 +667,43 @@ __global__ __launch_bounds__(kDetThreads) void detect_collect_kernel(DetectArgs 
     uint32_t mask = A.masks[t];
     if (!mask) return;
     uint32_t out = A.offsets[t];
-    const int64_t blk = t / kDetThreads;
-    const int tid = static_cast<int>(t - blk * kDetThreads);
-    const int ry = static_cast<int>(blk / A.bx_per_row);
-    const int bx = static_cast<int>(blk - static_cast<int64_t>(ry) * A.bx_per_row);
-    const uint32_t bw = static_cast<uint32_t>(A.Wp - 1);
+    const int64_t gb = t / kDetThreads;
+    const int tid = static_cast<int>(t - gb * kDetThreads);
+    const int img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
+    const ImageDesc& d = A.D[img];
+    const int64_t lb = gb - A.det_seg[img];
+    const int ry = static_cast<int>(lb / d.bx_per_row);
+    const int bx = static_cast<int>(lb - static_cast<int64_t>(ry) * d.bx_per_row);
+    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
     while (mask) {
         const int j = __ffs(mask) - 1;
         mask &= mask - 1;
         const uint32_t rx = static_cast<uint32_t>(bx * kDetChunk + j * kDetThreads + tid);
-        A.positions[out++] = static_cast<uint32_t>(ry) * bw + rx;
+        A.positions[out++] = make_uint2(static_cast<uint32_t>(img), static_cast<uint32_t>(ry) * bw + rx);
     }
+}
+
+__device__ __forceinline__ void add_candidate(const DetectArgs& A, const ImageDesc& d, uint32_t img,
+                                              uint32_t& out, int scan, uint32_t pos, int32_t a,
+                                              int32_t b, int32_t c)
+{
+    int32_t sa = a, sb = b, sc = c;
+    sort3(sa, sb, sc);
+    const int64_t sl = table_insert(A.table, img, sa, sb, sc, A.flags);
+    if (sl < 0) {
+        A.cand[out++].key = 0xFFFFFFFFu;
+        return;
+    }
+    const uint32_t key = static_cast<uint32_t>(scan) * static_cast<uint32_t>(d.nb) + pos;
+    atomicMin(&A.table.slot[sl].minkey, key);
+    Cand cd;
+    cd.key = key;
+    cd.slot = static_cast<uint32_t>(sl);
+    cd.a = a;
+    cd.b = b;
+    cd.c = c;
+    cd.img = img;
+    A.cand[out++] = cd;
 }
 
 // One thread per listed position.  A position's candidates go to slots 2i and
@@ -552,15 +712,17 @@ __global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npos) return;
-    const uint32_t pos = A.positions[i];
-    const uint32_t bw = static_cast<uint32_t>(A.Wp - 1);
+    const uint2 ip = A.positions[i];
+    const uint32_t img = ip.x, pos = ip.y;
+    const ImageDesc& d = A.D[img];
+    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
     const int ry = static_cast<int>(pos / bw);
     const int rx = static_cast<int>(pos - static_cast<uint32_t>(ry) * bw);
-    const BlockCorners c = block_corners(A, rx, ry);
+    const BlockCorners c = block_corners(A.grid + d.cv_off, d.Wc, rx, ry);
     uint32_t out = 2 * i;
     A.cand[out + 1].key = 0xFFFFFFFFu;
-    const int32_t tl = to_id(A, c.tl, A.flags), tr = to_id(A, c.tr, A.flags);
-    const int32_t bl = to_id(A, c.bl, A.flags), br = to_id(A, c.br, A.flags);
+    const int32_t tl = to_id(A, d, c.tl), tr = to_id(A, d, c.tr);
+    const int32_t bl = to_id(A, d, c.bl), br = to_id(A, d, c.br);
     if (tl < 0 || tr < 0 || bl < 0 || br < 0) {
         A.cand[out].key = 0xFFFFFFFFu;
         return;
@@ -569,19 +731,24 @@ __global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A
     if (!c.quad) {
         int k = 0;
         while (!c.v[k]) ++k;
-        add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
+        add_candidate(A, d, img, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
         return;
     }
     const int32_t ids[4] = {tl, tr, bl, br};
+    const uint32_t* srt = A.sorted + d.seed_off;
     int64_t px[4], py[4];
-    for (int k = 0; k < 4; ++k) seed_pos(A, ids[k], px[k], py[k]);
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t l = srt[ids[k]];
+        px[k] = static_cast<int64_t>(l >> 16);
+        py[k] = static_cast<int64_t>(l & 0xFFFFu);
+    }
     int32_t blk[2][3];
     quad_rule(ids, px, py, blk);
     bool blocked[4] = {false, false, false, false};
     for (int q = 0; q < 2; ++q) {
         int32_t a = blk[q][0], b = blk[q][1], cc = blk[q][2];
         sort3(a, b, cc);
-        const int64_t sl = table_insert(A.table, a, b, cc, A.flags);
+        const int64_t sl = table_insert(A.table, img, a, b, cc, A.flags);
         if (sl >= 0) A.table.slot[sl].blk = 1u;
         for (int k = 0; k < 4; ++k) {
             int32_t x = tri[k][0], y = tri[k][1], z = tri[k][2];
@@ -590,7 +757,7 @@ __global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A
         }
     }
     for (int k = 0; k < 4; ++k)
-        if (!blocked[k]) add_candidate(A, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
+        if (!blocked[k]) add_candidate(A, d, img, out, k, pos, tri[k][0], tri[k][1], tri[k][2]);
 }
 
 __device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
@@ -599,8 +766,10 @@ __device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
     return sl.minkey == cd.key && sl.blk == 0u;
 }
 
+// Bit key_off * 32 + key of the registration bitmap marks a first registration.
 __global__ void mark_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
-                            uint32_t* __restrict__ bm, uint32_t* __restrict__ flags)
+                            const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
+                            uint32_t* __restrict__ flags)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -609,34 +778,50 @@ __global__ void mark_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
     int32_t a = cd.a, b = cd.b, c = cd.c;
     sort3(a, b, c);
     const Slot& sl = t.slot[cd.slot];
-    if (sl.a != a || sl.b != b || sl.c != c) {
+    if (sl.a != a || sl.b != b || sl.c != c || sl.img != cd.img) {
         atomicOr(flags, kFlagCollision);
         return;
     }
-    if (is_first(cd, t)) atomicOr(&bm[cd.key >> 5], 1u << (cd.key & 31));
+    if (is_first(cd, t)) {
+        const int64_t w = D[cd.img].key_off + (cd.key >> 5);
+        atomicOr(&bm[w], 1u << (cd.key & 31));
+    }
+}
+
+// First row of each image = number of first registrations before it.
+__global__ void tri_base_kernel(const ImageDesc* __restrict__ D, int n_img,
+                                const uint32_t* __restrict__ pre, int64_t n_words,
+                                int64_t* __restrict__ tri_base)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n_img) return;
+    tri_base[i] = i < n_img ? pre[D[i].key_off] : pre[n_words];
 }
 
 __global__ void emit_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
-                            const uint32_t* __restrict__ bm, const uint32_t* __restrict__ pre, int Wp,
-                            uint32_t nb, int P, int32_t* __restrict__ rows)
+                            const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
+                            const uint32_t* __restrict__ pre, int32_t* __restrict__ rows)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Cand cd = cand[i];
     if (cd.key == 0xFFFFFFFFu || !is_first(cd, t)) return;
-    const uint32_t w = cd.key >> 5;
-    const uint32_t tid = pre[w] + static_cast<uint32_t>(__popc(bm[w] & ((1u << (cd.key & 31)) - 1u)));
+    const ImageDesc& d = D[cd.img];
+    const int64_t w = d.key_off + (cd.key >> 5);
+    // global row = first registrations of earlier images and earlier keys
+    const uint32_t row = pre[w] + static_cast<uint32_t>(__popc(bm[w] & ((1u << (cd.key & 31)) - 1u)));
+    const uint32_t nb = static_cast<uint32_t>(d.nb);
     const uint32_t scan = cd.key / nb;
     const uint32_t rem = cd.key - scan * nb;
-    const int bw = Wp - 1;
-    const int ry = static_cast<int>(rem / static_cast<uint32_t>(bw));
+    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
+    const int ry = static_cast<int>(rem / bw);
     const int rx = static_cast<int>(rem - static_cast<uint32_t>(ry) * bw);
     // detection pixel offsets of the four scans (triangulation.py:174-183)
     const int gx = rx + ((scan == 0 || scan == 3) ? 1 : 0);
     const int gy = ry + ((scan >= 2) ? 1 : 0);
-    int32_t* r = rows + 5 * static_cast<size_t>(tid);
-    r[0] = gx - P;
-    r[1] = gy - P;
+    int32_t* r = rows + 5 * static_cast<size_t>(row);
+    r[0] = gx - d.P;
+    r[1] = gy - d.P;
     r[2] = cd.a;
     r[3] = cd.b;
     r[4] = cd.c;
@@ -657,50 +842,46 @@ uint64_t next_pow2(uint64_t v)
 
 }  // namespace
 
-DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp, int P,
-                              const int32_t* rank_at, int W_img, const uint32_t* sorted,
-                              int64_t n_seeds, TriWork& work, hipStream_t s)
+DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
+                              const int32_t* rank_at, const uint32_t* sorted, TriWork& work,
+                              hipStream_t s)
 {
     DetectResult res;
-    const int64_t nb = (Wp >= 2 && Hp >= 2) ? static_cast<int64_t>(Wp - 1) * (Hp - 1) : 0;
-    work.flags.reserve(sizeof(uint32_t) * 8);
-    work.host_flags.reserve(sizeof(uint32_t) * 8);
+    const int n_img = static_cast<int>(plan.img.size());
+    work.flags.reserve(sizeof(uint32_t) * (n_img + 8));
+    work.host_flags.reserve(sizeof(uint32_t) * 8 + sizeof(int64_t) * (n_img + 1));
     uint32_t* dflags = work.flags.as<uint32_t>();
     uint32_t* hflags = work.host_flags.as<uint32_t>();
-    if (nb == 0) {
+    int64_t* tri_base = seg_ptr(work, 3, n_img);
+    work.tri_base.assign(n_img + 1, 0);
+    const int64_t nblocks = plan.n_det_blocks;
+    if (nblocks == 0) {
         work.rows.reserve(sizeof(int32_t) * 5);
         return res;
     }
-    // flags layout: [0] status bits, [2] quad count.
-    const int bw = Wp - 1, bh = Hp - 1;
-    const int bx_per_row = (bw + kDetChunk - 1) / kDetChunk;
-    const int64_t nthr = static_cast<int64_t>(bx_per_row) * bh * kDetThreads;
-    const int64_t nblocks = static_cast<int64_t>(bx_per_row) * bh;
-    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 2 * (nblocks + 1)) + static_cast<size_t>(nthr));
+    // flags layout: [0] status bits, [2] quad count, [4] candidate positions.
+    const int64_t nthr = nblocks * kDetThreads;
+    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 2 * (nblocks + 1)) +
+                        static_cast<size_t>(nthr));
     work.offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 1));
     DetectArgs A{};
-    A.grid = grid;
-    A.Wp = Wp;
-    A.Hp = Hp;
-    A.P = P;
+    A.grid = grid_base;
     A.labels = labels ? 1 : 0;
+    A.D = work.desc.as<ImageDesc>();
+    A.det_seg = seg_ptr(work, 1, n_img);
+    A.n_img = n_img;
     A.rank_at = rank_at;
-    A.W_img = W_img;
     A.sorted = sorted;
-    A.n_seeds = n_seeds;
-    A.nb = static_cast<uint32_t>(nb);
-    A.bx_per_row = bx_per_row;
     A.counts = work.counts.as<uint32_t>();
     A.offsets = work.offsets.as<uint32_t>();
     A.block_quads = work.counts.as<uint32_t>() + nthr;
     uint32_t* quad_scan = A.block_quads + nblocks + 1;
     A.masks = reinterpret_cast<uint8_t*>(quad_scan + nblocks + 1);
     A.flags = dflags;
-    const dim3 dgrid(static_cast<unsigned>(bx_per_row), static_cast<unsigned>(bh));
     {
         ProfScope ps(kKDetect, s);
         zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
-        detect_count_kernel<<<dgrid, kDetThreads, 0, s>>>(A);
+        detect_count_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
         exclusive_scan_u32(A.counts, nthr, work.offsets.as<uint32_t>(), work.scan_tmp, s);
         exclusive_scan_u32(A.block_quads, nblocks, quad_scan, work.scan_tmp, s);
@@ -717,26 +898,27 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
     const uint64_t table_cap = next_pow2(2 * (npos + 2 * nquads) + 1024);
     work.cand.reserve(sizeof(Cand) * std::max<uint64_t>(ncand, 1));
     work.table_tag.reserve(sizeof(Slot) * table_cap);
-    work.positions.reserve(sizeof(uint32_t) * std::max<uint64_t>(npos, 1));
-    A.positions = work.positions.as<uint32_t>();
+    work.positions.reserve(sizeof(uint2) * std::max<uint64_t>(npos, 1));
+    A.positions = work.positions.as<uint2>();
     if (npos > 0) {
         ProfScope ps(kKDetect, s);
         detect_collect_kernel<<<static_cast<unsigned>((nthr + 255) / 256), 256, 0, s>>>(A, nthr);
         DL_LAUNCH_CHECK();
     }
-    const int64_t nbits = 4 * nb;
-    const int64_t nwords = (nbits + 31) / 32;
-    work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords));
+    const int64_t nwords = plan.n_key_words;
+    work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(nwords, 1)));
     work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords + 1));
     uint32_t* bm = work.bitmap.as<uint32_t>();
     uint32_t* pre = work.scan.as<uint32_t>();
     uint64_t salt = 0x243F6A8885A308D3ull;
-    const int exact = n_seeds <= (1 << 21) ? 1 : 0;
+    const int img_bits = n_img > 1 ? bits_for(n_img - 1) : 0;
+    const int exact = 3 * plan.max_seed_bits + img_bits <= 63 ? 1 : 0;
     for (int attempt = 0; attempt < 4; ++attempt) {
         Table t;
         t.slot = work.table_tag.as<Slot>();
         t.mask = table_cap - 1;
         t.exact = exact;
+        t.bits = plan.max_seed_bits;
         t.salt = salt;
         A.cand = work.cand.as<Cand>();
         A.table = t;
@@ -758,12 +940,17 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
         const unsigned cblocks = static_cast<unsigned>(std::max<uint64_t>(1, (ncand + 255) / 256));
         {
             ProfScope ps(kKRegister, s);
-            mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, bm, dflags);
+            mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, A.D, bm,
+                                                dflags);
             DL_LAUNCH_CHECK();
             popcount_scan(bm, nwords, pre, work.scan_tmp, s);
+            tri_base_kernel<<<static_cast<unsigned>((n_img + 256) / 256), 256, 0, s>>>(
+                A.D, n_img, pre, nwords, tri_base);
+            DL_LAUNCH_CHECK();
         }
+        int64_t* htb = reinterpret_cast<int64_t*>(hflags + 8);
         DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        DL_HIP(hipMemcpyAsync(hflags + 4, pre + nwords, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        DL_HIP(hipMemcpyAsync(htb, tri_base, sizeof(int64_t) * (n_img + 1), hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
         const uint32_t f = hflags[0];
         if (f & kFlagBadId) {
@@ -775,13 +962,13 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
             salt = salt * 6364136223846793005ull + 1442695040888963407ull;
             continue;
         }
-        const uint32_t T = hflags[4];
-        work.rows.reserve(sizeof(int32_t) * 5 * std::max<uint64_t>(T, 1));
+        work.tri_base.assign(htb, htb + n_img + 1);
+        const int64_t T = htb[n_img];
+        work.rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(std::max<int64_t>(T, 1)));
         {
             ProfScope ps(kKRegister, s);
-            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, bm, pre,
-                                                Wp, static_cast<uint32_t>(nb), P,
-                                                work.rows.as<int32_t>());
+            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, A.D, bm,
+                                                pre, work.rows.as<int32_t>());
             DL_LAUNCH_CHECK();
         }
         res.n_tri = T;
@@ -793,7 +980,7 @@ DetectResult detect_triangles(const uint32_t* grid, bool labels, int Wp, int Hp,
 }
 
 // ===========================================================================
-// Voronoi output
+// Voronoi output (one image, no triangles)
 // ===========================================================================
 namespace {
 
@@ -846,7 +1033,7 @@ void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, 
 // workgroup per tile rasterises its triangles into an LDS tile with LDS max
 // (the reduction is order independent) and then writes the tile's output:
 // channel 2 of the (H, W, 3) grid together with channels 0/1 read from the
-// flood-fill labels, or the bare tid raster.
+// image's output label source, or the bare tid raster.
 // ===========================================================================
 namespace {
 
@@ -952,31 +1139,38 @@ __device__ __forceinline__ void row_span(const EdgeRows& E, double y, double& lo
 }
 
 __global__ void bin_count_kernel(const int32_t* __restrict__ rows, int64_t T,
-                                 const uint32_t* __restrict__ sorted, int W, int H, int tiles_x,
+                                 const int64_t* __restrict__ tri_base, int n_img,
+                                 const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
                                  uint32_t* __restrict__ counts)
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= T) return;
-    const TriVerts v = load_tri(rows, sorted, t);
+    const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
+    const ImageDesc& d = D[img];
+    const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
     int x0, y0, x1, y1;
-    if (!tri_box(v, W, H, x0, y0, x1, y1)) return;
+    if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) return;
+    uint32_t* c = counts + d.tile_off;
     for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
-        for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) atomicAdd(&counts[ty * tiles_x + tx], 1u);
+        for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) atomicAdd(&c[ty * d.tiles_x + tx], 1u);
 }
 
 __global__ void bin_fill_kernel(const int32_t* __restrict__ rows, int64_t T,
-                                const uint32_t* __restrict__ sorted, int W, int H, int tiles_x,
+                                const int64_t* __restrict__ tri_base, int n_img,
+                                const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
                                 const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
                                 uint32_t* __restrict__ list)
 {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= T) return;
-    const TriVerts v = load_tri(rows, sorted, t);
+    const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
+    const ImageDesc& d = D[img];
+    const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
     int x0, y0, x1, y1;
-    if (!tri_box(v, W, H, x0, y0, x1, y1)) return;
+    if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) return;
     for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
         for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) {
-            const int tile = ty * tiles_x + tx;
+            const int64_t tile = d.tile_off + ty * d.tiles_x + tx;
             const uint32_t pos = atomicAdd(&cursor[tile], 1u);
             list[offsets[tile] + pos] = static_cast<uint32_t>(t);
         }
@@ -984,23 +1178,30 @@ __global__ void bin_fill_kernel(const int32_t* __restrict__ rows, int64_t T,
 
 template <int C>
 __global__ __launch_bounds__(256) void tile_assign_kernel(
-    int W, int H, int tiles_x, const int32_t* __restrict__ rows, const uint32_t* __restrict__ sorted,
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ list,
-    const uint32_t* __restrict__ lab, int Wc, int off, int shift, const int32_t* __restrict__ rank_at,
-    int W_img, int32_t* __restrict__ out)
+    const int64_t* __restrict__ tile_seg, const int64_t* __restrict__ tri_base, int n_img,
+    const ImageDesc* __restrict__ D, const int32_t* __restrict__ rows,
+    const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ list, const int32_t* __restrict__ rank_at,
+    int32_t* __restrict__ out_base)
 {
     __shared__ int32_t tid_tile[kTS * kTS];
-    const int tile = blockIdx.x;
-    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    const int64_t gt = blockIdx.x;
+    const int img = n_img > 1 ? find_seg(tile_seg, n_img, gt) : 0;
+    const ImageDesc& d = D[img];
+    const int W = d.W, H = d.H;
+    const int tile = static_cast<int>(gt - d.tile_off);
+    const int tx = tile % d.tiles_x, ty = tile / d.tiles_x;
     const int X0 = tx * kTS, Y0 = ty * kTS;
     const int X1 = min(X0 + kTS, W) - 1, Y1 = min(Y0 + kTS, H) - 1;
+    const uint32_t* srt = sorted + d.seed_off;
+    const int64_t t0 = tri_base[img];
     for (int i = threadIdx.x; i < kTS * kTS; i += blockDim.x) tid_tile[i] = -1;
     __syncthreads();
-    const uint32_t b = offsets[tile], e = offsets[tile + 1];
+    const uint32_t b = offsets[gt], e = offsets[gt + 1];
     for (uint32_t k = b + threadIdx.x; k < e; k += blockDim.x) {
         const uint32_t t = list[k];
-        const TriVerts v = load_tri(rows, sorted, t);
-        const int32_t id = static_cast<int32_t>(t);
+        const TriVerts v = load_tri(rows, srt, t);
+        const int32_t id = static_cast<int32_t>(t - t0);   // the image's own tid
         if (v.S == 0) {
             const int64_t dx = v.x[1] - v.x[0], dy = v.y[1] - v.y[0];
             for (int y = Y0; y <= Y1; ++y) {
@@ -1029,46 +1230,48 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     }
     __syncthreads();
     // Output.  Each thread writes four consecutive pixels of a tile row: for
-    // C == 3 that is 48 bytes, three 16-byte stores when W is a multiple of 4.
+    // C == 3 that is 48 bytes, three 16-byte stores when they are aligned.
     const int q = threadIdx.x & 15;            // quad index within the row
     const int x = X0 + 4 * q;
+    const int off = d.out_off, shift = d.out_shift, Wl = d.out_W;
     for (int r = threadIdx.x >> 4; r < kTS; r += blockDim.x / 16) {
         const int y = Y0 + r;
         if (y > Y1 || x > X1) continue;
-        const size_t p = static_cast<size_t>(y) * W + x;
+        const int64_t p = d.pix_off + static_cast<int64_t>(y) * W + x;
         const int4 t4 = *reinterpret_cast<const int4*>(&tid_tile[r * kTS + 4 * q]);
         const int32_t tv[4] = {t4.x, t4.y, t4.z, t4.w};
         const int n = min(4, X1 - x + 1);
         if (C == 1) {
-            if (n == 4 && (W & 3) == 0) {
-                *reinterpret_cast<int4*>(out + p) = t4;
+            int32_t* dst = out_base + p;
+            if (n == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+                *reinterpret_cast<int4*>(dst) = t4;
             } else {
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    if (k < n) out[p + k] = tv[k];
+                    if (k < n) dst[k] = tv[k];
             }
             continue;
         }
         int32_t o[12];
-        const size_t lrow = static_cast<size_t>(y + off) * Wc + off;
+        const size_t lrow = static_cast<size_t>(y + off) * Wl + off;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            int32_t id = -1, d = -1;
+            int32_t id = -1, dd = -1;
             if (k < n) {
-                const uint32_t l = lab[lrow + x + k];
+                const uint32_t l = d.out_lab[lrow + x + k];
                 if (l != kInvalid) {
                     const int lx = static_cast<int>(l >> 16), ly = static_cast<int>(l & 0xFFFFu);
                     const int dx = x + k + off - lx, dy = y + off - ly;
-                    id = rank_at[static_cast<size_t>(ly - shift) * W_img + (lx - shift)];
-                    d = dx * dx + dy * dy;
+                    id = rank_at[d.pix_off + static_cast<int64_t>(ly - shift) * W + (lx - shift)];
+                    dd = dx * dx + dy * dy;
                 }
             }
             o[3 * k] = id;
-            o[3 * k + 1] = d;
+            o[3 * k + 1] = dd;
             o[3 * k + 2] = tv[k];
         }
-        int32_t* dst = out + 3 * p;
-        if (n == 4 && (W & 3) == 0) {
+        int32_t* dst = out_base + 3 * p;
+        if (n == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
             int4* d4 = reinterpret_cast<int4*>(dst);
             d4[0] = make_int4(o[0], o[1], o[2], o[3]);
             d4[1] = make_int4(o[4], o[5], o[6], o[7]);
@@ -1088,25 +1291,29 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
 
 }  // namespace
 
-void assign_tiled(int W, int H, const int32_t* rows, int64_t n_tri, const uint32_t* sorted,
-                  const uint32_t* labels, int Wc, int off, int shift, const int32_t* rank_at,
-                  int W_img, int32_t* out, int C, TriWork& work, hipStream_t s)
+void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
+                  const uint32_t* sorted, const int32_t* rank_at, int32_t* out_base, int C,
+                  TriWork& work, hipStream_t s)
 {
-    const int tiles_x = (W + kTS - 1) / kTS, tiles_y = (H + kTS - 1) / kTS;
-    const int64_t ntiles = static_cast<int64_t>(tiles_x) * tiles_y;
+    const int n_img = static_cast<int>(plan.img.size());
+    const int64_t ntiles = plan.n_tiles;
+    if (ntiles == 0) return;
     work.bin_counts.reserve(sizeof(uint32_t) * static_cast<size_t>(2 * ntiles + 1));
     work.bin_offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(ntiles + 1));
     work.host_flags.reserve(sizeof(uint32_t) * 8);
     uint32_t* counts = work.bin_counts.as<uint32_t>();
     uint32_t* cursor = counts + ntiles;
     uint32_t* offs = work.bin_offsets.as<uint32_t>();
+    const ImageDesc* D = work.desc.as<ImageDesc>();
+    const int64_t* tri_base = seg_ptr(work, 3, n_img);
+    const int64_t* tile_seg = seg_ptr(work, 2, n_img);
     uint32_t total = 0;
     {
         ProfScope ps(kKAssign, s);
         fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
         if (n_tri > 0) {
             bin_count_kernel<<<static_cast<unsigned>((n_tri + 255) / 256), 256, 0, s>>>(
-                rows, n_tri, sorted, W, H, tiles_x, counts);
+                rows, n_tri, tri_base, n_img, D, sorted, counts);
             DL_LAUNCH_CHECK();
         }
         exclusive_scan_u32(counts, ntiles, offs, work.scan_tmp, s);
@@ -1122,19 +1329,19 @@ void assign_tiled(int W, int H, const int32_t* rows, int64_t n_tri, const uint32
         ProfScope ps(kKAssign, s);
         if (total > 0) {
             bin_fill_kernel<<<static_cast<unsigned>((n_tri + 255) / 256), 256, 0, s>>>(
-                rows, n_tri, sorted, W, H, tiles_x, offs, cursor, work.bin_list.as<uint32_t>());
+                rows, n_tri, tri_base, n_img, D, sorted, offs, cursor, work.bin_list.as<uint32_t>());
             DL_LAUNCH_CHECK();
         }
     }
     ProfScope ps(kKOutput, s);
     if (C == 3)
         tile_assign_kernel<3><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            W, H, tiles_x, rows, sorted, offs, work.bin_list.as<uint32_t>(), labels, Wc, off, shift,
-            rank_at, W_img, out);
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
+            out_base);
     else
         tile_assign_kernel<1><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            W, H, tiles_x, rows, sorted, offs, work.bin_list.as<uint32_t>(), labels, Wc, off, shift,
-            rank_at, W_img, out);
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
+            out_base);
     DL_LAUNCH_CHECK();
 }
 
