@@ -129,3 +129,13 @@ def batch_shapes(count: int, seed: int, side_lo: int = 128, side_hi: int = 2048,
         n = max(1, int(round(dens * w * h)))
         out.append((w, h, n, int(seed) * 100_003 + i))
     return out
+
+
+def batch_images(count: int, seed: int, **kw):
+    """(width, height, seeds) per image of the batched workload: shapes from
+    :func:`batch_shapes`, seeds the first n distinct uniform draws of each
+    image's own generator (vectorised)."""
+    out = []
+    for w, h, n, sd in batch_shapes(count, seed, **kw):
+        out.append((w, h, _uniform_fast(w, h, n, np.random.default_rng(sd))))
+    return out
