@@ -102,6 +102,37 @@ int dl_triangulate_batch_device(dl_context* ctx, int n_images, const int32_t* wi
                                 const int32_t* d_seeds_xy, const int32_t* paddings,
                                 int32_t* d_out, void* stream, int64_t* n_tri);
 
+/* Incremental sessions (the reference's Delaunay class, bindings.cpp:86-227,
+ * delaunay.cuh:23-378), with one change of contract: after any sequence of
+ * inserts and removals, finalise gives bit for bit what dl_triangulate gives
+ * for the live seed set (the reference's incremental path follows insertion
+ * order and drifts from its batch result, test_incremental_deferred.py:333-355).
+ * The live seeds stay on the device between calls (presence bitmap + list in
+ * insertion order), and so do the last result grid and rows.  A call that
+ * fails leaves the session unchanged.  Removal has no reference counterpart.
+ *   create:   border_padding < 0 selects the default (delaunay.cu:72).
+ *   insert:   ValueError on a position out of bounds, already live or repeated
+ *             in the batch, or when max_seeds would be exceeded
+ *             (delaunay.cu:191-211).
+ *   remove:   ValueError on a position that is not live or repeats.
+ *   finalise: runs the pipeline on the live seeds (finalise,
+ *             delaunay.cu:323-345); with none live the grid is all -1.
+ *   copy_result: the last finalise's (H, W, 3) grid and/or rows (either may
+ *             be NULL); ValueError while changes are pending.
+ *   copy_seeds: live seeds in insertion order (seed_count pairs). */
+typedef struct dl_session dl_session;
+int dl_session_create(dl_context* ctx, int width, int height, int64_t max_seeds,
+                      int border_padding, dl_session** out);
+void dl_session_destroy(dl_session* s);
+int dl_session_insert(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, int64_t n);
+int dl_session_remove(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, int64_t n);
+int dl_session_seed_count(dl_session* s, int64_t* n);
+int dl_session_finalise(dl_context* ctx, dl_session* s, int64_t* n_tri);
+int dl_session_copy_result(dl_session* s, int32_t* out_hw3, int32_t* out_rows);
+int dl_session_copy_seeds(dl_session* s, int32_t* out_xy);
+/* Device pointer of the last finalise's (H, W, 3) grid (NULL while pending). */
+const int32_t* dl_session_device_grid(dl_session* s);
+
 /* Rows (x, y, id_a, id_b, id_c) in tid order of the last triangulation on this
  * context: the reference's triangle_map values (triangulation.py:157;
  * binding_helpers.hpp:61-87).  out_rows holds n_tri * 5 int32. */

@@ -61,6 +61,16 @@ _SIGNATURES = {
                                             _I32P, _I64P]),
     "dl_triangulate_batch_device": (ctypes.c_int, [_VP, ctypes.c_int, _I32P, _I32P, _I64P, _VP,
                                                    _I32P, _VP, _VP, _I64P]),
+    "dl_session_create": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                         ctypes.c_int, ctypes.POINTER(_VP)]),
+    "dl_session_destroy": (None, [_VP]),
+    "dl_session_insert": (ctypes.c_int, [_VP, _VP, _I32P, ctypes.c_int64]),
+    "dl_session_remove": (ctypes.c_int, [_VP, _VP, _I32P, ctypes.c_int64]),
+    "dl_session_seed_count": (ctypes.c_int, [_VP, _I64P]),
+    "dl_session_finalise": (ctypes.c_int, [_VP, _VP, _I64P]),
+    "dl_session_copy_result": (ctypes.c_int, [_VP, _I32P, _I32P]),
+    "dl_session_copy_seeds": (ctypes.c_int, [_VP, _I32P]),
+    "dl_session_device_grid": (_VP, [_VP]),
     "dl_debug_fill_watch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                            ctypes.c_int, _I32P, ctypes.POINTER(ctypes.c_int)]),
 }

@@ -175,6 +175,17 @@ struct dl_context {
     KernelProfiler prof;
 };
 
+struct dl_session {
+    int device = 0;
+    int W = 0, H = 0, P = 0;
+    SessionState st;
+    DevBuf stage;        // the call's uploaded (x, y) pairs
+    DevBuf out;          // (H, W, 3) grid of the last finalise
+    DevBuf rows;         // its triangle rows
+    int64_t n_tri = 0;
+    bool done = false;   // out/rows hold the current seed set's result
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -884,6 +895,143 @@ int dl_copy_padded_voronoi(dl_context* ctx, int32_t* out, int* out_w, int* out_h
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
     });
+}
+
+int dl_session_create(dl_context* ctx, int width, int height, int64_t max_seeds,
+                      int border_padding, dl_session** out)
+{
+    return guarded([&] {
+        if (!ctx || !out) throw value_error("null argument");
+        if (max_seeds <= 0) throw value_error("max_seeds must be positive");
+        // any negative padding selects the default (delaunay.cu:72)
+        const int P = border_padding < 0 ? DL_DEFAULT_BORDER_PADDING : border_padding;
+        check_dims(width, height, P);
+        DL_HIP(hipSetDevice(ctx->device));
+        auto* s = new dl_session();
+        try {
+            s->device = ctx->device;
+            s->W = width;
+            s->H = height;
+            s->P = P;
+            session_alloc(s->st, width, height, max_seeds, ctx->stream);
+            DL_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+void dl_session_destroy(dl_session* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    delete s;
+}
+
+int dl_session_insert(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, int64_t n)
+{
+    return guarded([&] {
+        if (!ctx || !s) throw value_error("null argument");
+        if (n <= 0) return;
+        if (!seeds_xy) throw value_error("null seeds");
+        DL_HIP(hipSetDevice(ctx->device));
+        if (s->st.n_live + n > s->st.max_seeds)
+            throw value_error("insert would exceed max_seeds capacity");
+        s->stage.reserve(sizeof(int32_t) * 2 * static_cast<size_t>(n));
+        DL_HIP(hipMemcpyAsync(s->stage.ptr, seeds_xy, sizeof(int32_t) * 2 * static_cast<size_t>(n),
+                              hipMemcpyHostToDevice, ctx->stream));
+        session_insert(s->st, s->stage.as<int32_t>(), n, ctx->stream);
+        DL_HIP(hipStreamSynchronize(ctx->stream));
+        s->done = false;
+    });
+}
+
+int dl_session_remove(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, int64_t n)
+{
+    return guarded([&] {
+        if (!ctx || !s) throw value_error("null argument");
+        if (n <= 0) return;
+        if (!seeds_xy) throw value_error("null seeds");
+        DL_HIP(hipSetDevice(ctx->device));
+        s->stage.reserve(sizeof(int32_t) * 2 * static_cast<size_t>(n));
+        DL_HIP(hipMemcpyAsync(s->stage.ptr, seeds_xy, sizeof(int32_t) * 2 * static_cast<size_t>(n),
+                              hipMemcpyHostToDevice, ctx->stream));
+        session_remove(s->st, s->stage.as<int32_t>(), n, ctx->tri.scan_tmp, ctx->stream);
+        DL_HIP(hipStreamSynchronize(ctx->stream));
+        s->done = false;
+    });
+}
+
+int dl_session_seed_count(dl_session* s, int64_t* n)
+{
+    return guarded([&] {
+        if (!s || !n) throw value_error("null argument");
+        *n = s->st.n_live;
+    });
+}
+
+int dl_session_finalise(dl_context* ctx, dl_session* s, int64_t* n_tri)
+{
+    return guarded([&] {
+        if (!ctx || !s) throw value_error("null argument");
+        DL_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const size_t px = static_cast<size_t>(s->W) * s->H;
+        s->out.reserve(sizeof(int32_t) * 3 * px);
+        if (s->st.n_live == 0) {
+            fill_u32(s->out.as<uint32_t>(), 0xFFFFFFFFu, 3 * px, st);
+            DL_HIP(hipStreamSynchronize(st));
+            s->n_tri = 0;
+        } else {
+            ProfInstall pi(ctx);
+            StageClock clk(ctx);
+            plan_single(ctx, s->W, s->H, s->P, s->st.n_live);
+            run_pipeline(ctx, s->st.list.as<int32_t>(), s->out.as<int32_t>(), 3, st, clk);
+            s->n_tri = ctx->n_tri;
+            s->rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(std::max<int64_t>(s->n_tri, 1)));
+            if (s->n_tri)
+                DL_HIP(hipMemcpyAsync(s->rows.ptr, ctx->tri.rows.ptr,
+                                      sizeof(int32_t) * 5 * static_cast<size_t>(s->n_tri),
+                                      hipMemcpyDeviceToDevice, st));
+            clk.mark(5);
+            clk.finish(st);
+        }
+        s->done = true;
+        if (n_tri) *n_tri = s->n_tri;
+    });
+}
+
+int dl_session_copy_result(dl_session* s, int32_t* out_hw3, int32_t* out_rows)
+{
+    return guarded([&] {
+        if (!s) throw value_error("null session");
+        if (!s->done) throw value_error("the session has pending changes: finalise first");
+        DL_HIP(hipSetDevice(s->device));
+        if (out_hw3)
+            DL_HIP(hipMemcpy(out_hw3, s->out.ptr, sizeof(int32_t) * 3 * static_cast<size_t>(s->W) * s->H,
+                             hipMemcpyDeviceToHost));
+        if (out_rows && s->n_tri)
+            DL_HIP(hipMemcpy(out_rows, s->rows.ptr, sizeof(int32_t) * 5 * static_cast<size_t>(s->n_tri),
+                             hipMemcpyDeviceToHost));
+    });
+}
+
+int dl_session_copy_seeds(dl_session* s, int32_t* out_xy)
+{
+    return guarded([&] {
+        if (!s || !out_xy) throw value_error("null argument");
+        if (!s->st.n_live) return;
+        DL_HIP(hipSetDevice(s->device));
+        DL_HIP(hipMemcpy(out_xy, s->st.list.ptr, sizeof(int32_t) * 2 * static_cast<size_t>(s->st.n_live),
+                         hipMemcpyDeviceToHost));
+    });
+}
+
+const int32_t* dl_session_device_grid(dl_session* s)
+{
+    return s && s->done ? s->out.as<int32_t>() : nullptr;
 }
 
 }  // extern "C"

@@ -250,6 +250,30 @@ void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// Incremental sessions: the live seed set on the device.
+// ---------------------------------------------------------------------------
+struct SessionState {
+    int W = 0, H = 0;
+    int64_t max_seeds = 0;
+    int64_t words = 0;     // presence bitmap words
+    int64_t n_live = 0;
+    DevBuf bitmap;         // bit x*H + y set for every live seed
+    DevBuf list, spare;    // live (x, y) pairs in insertion order; compaction target
+    DevBuf mine;           // per-call: which bits this call changed (rollback)
+    DevBuf keep;           // compaction flags and their prefix sums
+    DevBuf flags;
+    PinnedBuf host_flags;
+};
+
+void session_alloc(SessionState& st, int W, int H, int64_t max_seeds, hipStream_t s);
+void session_clear(SessionState& st, hipStream_t s);
+// d_xy: n device (x, y) pairs.  Throws value_error (state unchanged) on an
+// out-of-bounds, already-live or repeated position, or on overflow.
+void session_insert(SessionState& st, const int32_t* d_xy, int64_t n, hipStream_t s);
+// Throws value_error (state unchanged) on a position that is not live or repeats.
+void session_remove(SessionState& st, const int32_t* d_xy, int64_t n, DevBuf& tmp, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // Per-kernel timing.  When a profiler is installed for the calling thread,
 // launch sites bracket their kernels with events; collect() turns the pairs
 // into per-kind counts and milliseconds after the stream has been synchronised.

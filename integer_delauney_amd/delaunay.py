@@ -7,7 +7,9 @@ the reference's incremental path breaks Voronoi ties and cocircular quads by
 (tests/test_incremental_deferred.py:333-355).  Here the state after any
 sequence of inserts and removals is, bit for bit, the batch pipeline
 (`triangulate`) on the current seed set; ``remove`` is new (the reference has
-no removal).
+no removal).  The live seeds, the last grid and the last triangle rows stay on
+the device between calls (``dl_session_*`` in include/delauney_hip.h); the
+host keeps copies of the live seed list and the per-seed values for queries.
 
 Seed numbering follows the reference:
 * ``insert`` / ``finalise`` report sorted (x asc, y asc) ids;
@@ -17,12 +19,13 @@ Seed numbering follows the reference:
 """
 from __future__ import annotations
 
+import ctypes
 import time
 
 import numpy as np
 
 from . import _lib
-from .api import DEFAULT_BORDER_PADDING, _pack, as_seed_array, triangulate
+from .api import DEFAULT_BORDER_PADDING, _pack, as_seed_array
 
 
 def _keys(xy: np.ndarray) -> np.ndarray:
@@ -39,14 +42,30 @@ class Delaunay:
         # delaunay.cu:72: any negative value selects the default
         self._P = int(border_padding) if border_padding is not None and border_padding >= 0 \
             else DEFAULT_BORDER_PADDING
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.dl_session_create(_lib.context().handle, self._W, self._H, self._max,
+                                               self._P, ctypes.byref(h)))
+        self._h = h
         self._xy = np.empty((0, 2), dtype=np.int32)
-        self._keys = set()
         self._values = np.empty(0, dtype=np.float32)
         self._pending = True
         self._rows = None
         self._grid = None
         self._rank = None
-        self._last_bfs_ms = 0.0
+        self._last = {"bfs_ms": 0.0, "bfs_iters": 0, "detect_ms": 0.0, "dedup_ms": 0.0,
+                      "assign_ms": 0.0}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.dl_session_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
 
     # -- properties (bindings.cpp:220-227) --------------------------------
     @property
@@ -70,20 +89,10 @@ class Delaunay:
         return bool(self._pending) and self.seed_count > 0
 
     # -- mutation ----------------------------------------------------------
-    def _validate_new(self, xy: np.ndarray) -> None:
-        if self.seed_count + xy.shape[0] > self._max:
-            raise ValueError("insert would exceed max_seeds capacity")
-        if np.any(xy[:, 0] < 0) or np.any(xy[:, 0] >= self._W) or \
-                np.any(xy[:, 1] < 0) or np.any(xy[:, 1] >= self._H):
-            raise ValueError("seed coordinate out of bounds")
-        k = _keys(xy)
-        if np.unique(k).shape[0] != k.shape[0]:
-            raise ValueError("duplicate seed positions within batch")
-        if any(int(v) in self._keys for v in k.tolist()):
-            raise ValueError("seed position already exists")
-
     def insert_deferred(self, seeds, values=None) -> None:
-        """Append a batch (delaunay.cu:274-321); the raster is deferred."""
+        """Append a batch (delaunay.cu:274-321); the raster is deferred.
+        The batch is validated on the device (bounds, already live, repeated
+        within the batch, capacity); on error nothing changes."""
         if seeds is None or (not isinstance(seeds, np.ndarray) and len(seeds) == 0) or \
                 (isinstance(seeds, np.ndarray) and seeds.size == 0):
             return
@@ -94,38 +103,33 @@ class Delaunay:
                 raise ValueError("values must have one entry per seed in the batch")
         else:
             vals = np.zeros(xy.shape[0], dtype=np.float32)
-        self._validate_new(xy)
+        _lib.check(self._lib.dl_session_insert(_lib.context().handle, self._h, _lib.i32p(xy),
+                                               xy.shape[0]))
         self._xy = np.concatenate([self._xy, xy])
-        self._keys.update(int(v) for v in _keys(xy).tolist())
         self._values = np.concatenate([self._values, vals])
         self._pending = True
 
     def remove(self, seeds) -> None:
-        """Remove seeds by position.  Raises ValueError for a position that is
-        not a live seed.  No reference counterpart."""
+        """Remove live seeds by position.  ValueError (nothing removed) for a
+        position that is not live or repeats.  No reference counterpart."""
         xy = as_seed_array(seeds)
-        k = _keys(xy)
-        if np.unique(k).shape[0] != k.shape[0]:
-            raise ValueError("duplicate seed positions within batch")
-        for v in k.tolist():
-            if int(v) not in self._keys:
-                raise ValueError("seed position is not present")
-        drop = np.isin(_keys(self._xy), k)
+        _lib.check(self._lib.dl_session_remove(_lib.context().handle, self._h, _lib.i32p(xy),
+                                               xy.shape[0]))
+        drop = np.isin(_keys(self._xy), _keys(xy))
         self._xy = self._xy[~drop]
         self._values = self._values[~drop]
-        self._keys.difference_update(int(v) for v in k.tolist())
         self._pending = True
 
     def _recompute(self):
-        if self.seed_count == 0:
-            self._rows = np.empty((0, 5), dtype=np.int32)
-            self._grid = np.full((self._H, self._W, 3), -1, dtype=np.int32)
-            self._rank = np.empty(0, dtype=np.int32)
-            return
         t0 = time.perf_counter()
-        rows, grid = triangulate(self._W, self._H, self._xy, self._P, as_arrays=False)
-        self._last_bfs_ms = (time.perf_counter() - t0) * 1e3
-        self._rows = np.array([rows[i] for i in range(len(rows))], dtype=np.int32).reshape(-1, 5)
+        n = ctypes.c_int64(0)
+        _lib.check(self._lib.dl_session_finalise(_lib.context().handle, self._h, ctypes.byref(n)))
+        grid = np.empty((self._H, self._W, 3), dtype=np.int32)
+        rows = np.empty((int(n.value), 5), dtype=np.int32)
+        _lib.check(self._lib.dl_session_copy_result(self._h, _lib.i32p(grid),
+                                                    _lib.i32p(rows) if n.value else None))
+        self._last["bfs_ms"] = (time.perf_counter() - t0) * 1e3
+        self._rows = rows
         self._grid = grid
         order = np.lexsort((self._xy[:, 1], self._xy[:, 0]))
         rank = np.empty(self.seed_count, dtype=np.int32)
@@ -144,8 +148,7 @@ class Delaunay:
         return self.finalise(as_arrays)
 
     def _timings(self):
-        return {"bfs_ms": float(self._last_bfs_ms), "bfs_iters": 0, "detect_ms": 0.0,
-                "dedup_ms": 0.0, "assign_ms": 0.0}
+        return dict(self._last)
 
     def insert_timed(self, seeds, as_arrays: bool = False):
         m, g = self.insert(seeds, as_arrays)
@@ -188,3 +191,10 @@ class Delaunay:
 
     def get_values(self) -> np.ndarray:
         return self._values.copy()
+
+    def device_seeds(self) -> np.ndarray:
+        """The live seed list as held on the device (insertion order)."""
+        out = np.empty((self.seed_count, 2), dtype=np.int32)
+        if self.seed_count:
+            _lib.check(self._lib.dl_session_copy_seeds(self._h, _lib.i32p(out)))
+        return out

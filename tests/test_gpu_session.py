@@ -95,3 +95,42 @@ def test_session_errors(case):
     m, g = s.finalise()
     m2, g2 = dl.triangulate(50, 40, [(1, 1), (10, 10), (20, 5)])
     assert m == m2 and np.array_equal(g, g2)
+
+
+def test_failed_calls_roll_back_on_the_device():
+    s = dl.Delaunay(40, 30, 100)
+    s.insert([(1, 1), (10, 10), (20, 5)])
+    # a batch mixing a fresh position with a live one: the fresh bit set by the
+    # failing call must be cleared again
+    with pytest.raises(ValueError):
+        s.insert([(7, 7), (10, 10)])
+    s.insert([(7, 7)])                       # succeeds: (7, 7) was rolled back
+    # a removal mixing a live position with an absent one: the live bit must
+    # be restored
+    with pytest.raises(ValueError):
+        s.remove([(1, 1), (3, 3)])
+    with pytest.raises(ValueError):
+        s.remove([(20, 5), (20, 5)])
+    s.remove([(1, 1)])                       # still live, so this succeeds
+    np.testing.assert_array_equal(s.device_seeds(), s.get_seeds())
+    np.testing.assert_array_equal(s.get_seeds(), np.array([(10, 10), (20, 5), (7, 7)], np.int32))
+    m, g = s.finalise()
+    m2, g2 = dl.triangulate(40, 30, s.get_seeds())
+    assert m == m2 and np.array_equal(g, g2)
+
+
+def test_remove_everything_then_refill():
+    rng = np.random.default_rng(33)
+    W, H = 64, 48
+    s = dl.Delaunay(W, H, 500)
+    a = _distinct(rng, W, H, 200)
+    s.insert(a)
+    s.remove(a)
+    assert s.seed_count == 0
+    m, g = s.finalise()
+    assert len(m) == 0 and np.all(g == -1)
+    b = _distinct(rng, W, H, 150)
+    m, g = s.insert(b)
+    m2, g2 = dl.triangulate(W, H, b)
+    assert m == m2 and np.array_equal(g, g2)
+    np.testing.assert_array_equal(s.device_seeds(), b)
