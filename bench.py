@@ -139,6 +139,98 @@ def bench_batch(lib, ctx, n_images: int, steps: int):
             "batched_equals_single_bitwise": same}
 
 
+def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_del: int):
+    """Clustered seeds + incremental updates: a side x side session with n0
+    seeds from a 64-component Gaussian mixture (sigma 200 px), then `rounds`
+    rounds of n_ins new mixture samples and n_del deletions drawn uniformly
+    from the live set, each round finalised on the device (seed set, grid and
+    rows stay in HBM).  The final grid is compared bit for bit with a fresh
+    batch run on the final seed set."""
+    import numpy as np
+    import torch
+
+    from integer_delauney_amd import _lib
+    from integer_delauney_amd.workloads import clustered_seeds
+
+    log(f"generating {n0} clustered seeds on {side}x{side}")
+    rng = np.random.default_rng(3)
+    seeds0 = clustered_seeds(side, side, n0, 3)
+    centres = np.random.default_rng(3).uniform([0, 0], [side, side], size=(64, 2))
+    live = seeds0.copy()
+    taken = np.zeros(side * side, dtype=bool)
+    taken[live[:, 0].astype(np.int64) * side + live[:, 1]] = True
+    plan = []
+    for r in range(rounds):
+        ins = []
+        while len(ins) < n_ins:
+            k = 2 * (n_ins - len(ins)) + 64
+            comp = rng.integers(0, 64, size=k)
+            pts = np.rint(centres[comp] + rng.normal(0.0, 200.0, size=(k, 2))).astype(np.int64)
+            ok = (pts[:, 0] >= 0) & (pts[:, 0] < side) & (pts[:, 1] >= 0) & (pts[:, 1] < side)
+            for x, y in pts[ok].tolist():
+                key = x * side + y
+                if not taken[key]:
+                    taken[key] = True
+                    ins.append((x, y))
+                    if len(ins) == n_ins:
+                        break
+        ins = np.asarray(ins, dtype=np.int32)
+        live = np.concatenate([live, ins])
+        drop = rng.choice(live.shape[0], size=n_del, replace=False)
+        dele = np.ascontiguousarray(live[drop])
+        taken[dele[:, 0].astype(np.int64) * side + dele[:, 1]] = False
+        keep = np.ones(live.shape[0], dtype=bool)
+        keep[drop] = False
+        live = live[keep]
+        plan.append((np.ascontiguousarray(ins), dele))
+    log("update rounds generated")
+
+    h = ctypes.c_void_p()
+    _lib.check(lib.dl_session_create(ctx.handle, side, side, n0 + rounds * n_ins, 16,
+                                     ctypes.byref(h)))
+    nt = ctypes.c_int64(0)
+    try:
+        t0 = time.perf_counter()
+        _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(seeds0), seeds0.shape[0]))
+        _lib.check(lib.dl_session_finalise(ctx.handle, h, ctypes.byref(nt)))
+        t_build = time.perf_counter() - t0
+        per_round = []
+        for ins, dele in plan:
+            t1 = time.perf_counter()
+            _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(ins), ins.shape[0]))
+            _lib.check(lib.dl_session_remove(ctx.handle, h, _lib.i32p(dele), dele.shape[0]))
+            _lib.check(lib.dl_session_finalise(ctx.handle, h, ctypes.byref(nt)))
+            per_round.append(time.perf_counter() - t1)
+        log("update rounds done")
+        # bit-exact check against a fresh batch run on the final seed set
+        n_live = ctypes.c_int64(0)
+        _lib.check(lib.dl_session_seed_count(h, ctypes.byref(n_live)))
+        d_xy = torch.from_numpy(live).to("cuda")
+        d_ref = torch.empty((side, side, 3), dtype=torch.int32, device="cuda")
+        nt2 = ctypes.c_int64(0)
+        _lib.check(lib.dl_triangulate_device(ctx.handle, side, side, ctypes.c_void_p(d_xy.data_ptr()),
+                                             live.shape[0], 16, ctypes.c_void_p(d_ref.data_ptr()),
+                                             None, ctypes.byref(nt2)))
+        torch.cuda.synchronize()
+        # the session's grid through the host API (12 B/px device-to-host)
+        host = np.empty((side, side, 3), dtype=np.int32)
+        _lib.check(lib.dl_session_copy_result(h, _lib.i32p(host), None))
+        same = bool(torch.equal(torch.from_numpy(host).to("cuda"), d_ref)) and nt.value == nt2.value \
+            and n_live.value == live.shape[0]
+    finally:
+        lib.dl_session_destroy(h)
+    pr = np.array(per_round)
+    return {"workload": f"{side}x{side}, {n0} seeds from a 64-component Gaussian mixture "
+                        f"(sigma 200 px), {rounds} rounds of {n_ins} inserts + {n_del} deletes, "
+                        f"each round finalised on the device, padding 16",
+            "initial_build_ms": t_build * 1e3, "rounds": rounds,
+            "round_ms_mean": float(pr.mean() * 1e3), "round_ms_p50": float(np.percentile(pr, 50) * 1e3),
+            "round_ms_p99": float(np.percentile(pr, 99) * 1e3),
+            "effective_mpixel_per_s": side * side / pr.mean() / 1e6,
+            "final_seeds": int(live.shape[0]), "final_triangles": int(nt.value),
+            "final_equals_batch_bitwise": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +243,10 @@ def main():
     ap.add_argument("--batch-images", type=int, default=512,
                     help="images of the batched workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--inc-rounds", type=int, default=100,
+                    help="update rounds of the incremental workload (0: skip it)")
+    ap.add_argument("--inc-side", type=int, default=16384)
+    ap.add_argument("--inc-seeds", type=int, default=2_000_000)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -270,6 +366,11 @@ def main():
     if args.batch_images > 0:
         batch = bench_batch(lib, ctx, args.batch_images, args.batch_steps)
 
+    incremental = None
+    if args.inc_rounds > 0:
+        incremental = bench_incremental(lib, ctx, args.inc_side, args.inc_seeds, args.inc_rounds,
+                                        10_000, 5_000)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU oracle sample")
@@ -307,6 +408,7 @@ def main():
                  "config": [cfg_w, cfg_r, cfg_k]},
         "end_to_end_mpixel_per_s": e2e,
         "batched": batch,
+        "incremental": incremental,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
