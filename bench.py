@@ -231,6 +231,148 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             "final_equals_batch_bitwise": same}
 
 
+def bench_serving(n_requests: int, n_sessions: int, loads=(0.3, 0.6, 0.9)):
+    """Serving loop (integer_delauney_amd.serving): requests with sides
+    log-uniform in [64, 8192] (width and height independent) at 1 % seed
+    density, 20 % of them insert/delete batches of 1-5 % of a resident
+    session's seeds (half inserts, half deletes).  First the saturated
+    throughput (all requests queued at once), then Poisson arrivals at three
+    offered loads (fractions of that throughput) with p50/p99 latency from
+    arrival to completion."""
+    import numpy as np
+
+    from integer_delauney_amd.serving import Server, TriangulateRequest, UpdateRequest
+    from integer_delauney_amd.workloads import _uniform_fast
+
+    rng = np.random.default_rng(5)
+
+    def side():
+        return int(round(np.exp(rng.uniform(np.log(64), np.log(8192)))))
+
+    srv = Server(hbm_budget=160 << 30, slots=2)
+    model = {}
+    for _ in range(n_sessions):
+        w, h = side(), side()
+        n0 = max(3, int(0.01 * w * h))
+        s0 = _uniform_fast(w, h, n0, rng)
+        sid = srv.create_session(w, h, 2 * n0 + 16, seeds=s0)
+        taken = np.zeros(w * h, dtype=bool)
+        taken[s0[:, 0].astype(np.int64) * h + s0[:, 1]] = True
+        model[sid] = {"w": w, "h": h, "live": s0, "taken": taken, "cap": 2 * n0 + 16}
+    log(f"serving: {n_sessions} resident sessions created")
+
+    def next_request():
+        if rng.random() < 0.2:
+            sid = int(rng.integers(0, n_sessions))
+            m = model[sid]
+            live = m["live"]
+            k = max(2, int(round(rng.uniform(0.01, 0.05) * live.shape[0])))
+            k_del = min(k // 2, live.shape[0] - 3)
+            k_ins = min(k - k // 2, m["cap"] - live.shape[0] + k_del)
+            idx = rng.choice(live.shape[0], size=k_del, replace=False)
+            drop = np.ascontiguousarray(live[idx])
+            m["taken"][drop[:, 0].astype(np.int64) * m["h"] + drop[:, 1]] = False
+            add = []
+            while len(add) < k_ins:
+                x, y = int(rng.integers(0, m["w"])), int(rng.integers(0, m["h"]))
+                if not m["taken"][x * m["h"] + y]:
+                    m["taken"][x * m["h"] + y] = True
+                    add.append((x, y))
+            add = np.asarray(add, dtype=np.int32).reshape(-1, 2)
+            keep = np.ones(live.shape[0], dtype=bool)
+            keep[idx] = False
+            m["live"] = np.concatenate([live[keep], add])
+            return UpdateRequest(sid, insert=add, remove=drop), m["w"] * m["h"]
+        w, h = side(), side()
+        n = max(3, int(0.01 * w * h))
+        return TriangulateRequest(w, h, _uniform_fast(w, h, n, rng)), w * h
+
+    def run(load_rate=None):
+        reqs = [next_request() for _ in range(n_requests)]
+        px = sum(p for _, p in reqs)
+        futs = []
+        t0 = time.perf_counter()
+        if load_rate is None:
+            arrivals = np.zeros(n_requests)
+        else:
+            arrivals = np.cumsum(rng.exponential(1.0 / load_rate, size=n_requests))
+        for (req, _), a in zip(reqs, arrivals):
+            while time.perf_counter() - t0 < a:
+                time.sleep(min(0.0005, max(0.0, a - (time.perf_counter() - t0))))
+            f = srv.submit(req, keep=False)
+            f.t_arrival = t0 + a
+            futs.append(f)
+        lat = []
+        for f in futs:
+            f.result()
+            lat.append(f.t_done - f.t_arrival)
+        makespan = max(f.t_done for f in futs) - t0
+        del futs
+        lat = np.array(lat) * 1e3
+        return {"requests": n_requests, "mpixel": px / 1e6, "makespan_s": makespan,
+                "throughput_mpixel_per_s": px / makespan / 1e6,
+                "latency_ms_p50": float(np.percentile(lat, 50)),
+                "latency_ms_p99": float(np.percentile(lat, 99)),
+                "latency_ms_max": float(lat.max())}
+
+    def run_plain():
+        """The same kind of stream, one request at a time through the plain
+        host API (triangulate per image; session updates and finalise per
+        request), no batching, no overlap."""
+        import ctypes as ct
+
+        from integer_delauney_amd import _lib, triangulate
+        lib = _lib.load()
+        ctx = _lib.context()
+        reqs = [next_request() for _ in range(n_requests)]
+        px = sum(p for _, p in reqs)
+        t0 = time.perf_counter()
+        for req, _ in reqs:
+            if isinstance(req, UpdateRequest):
+                h = srv._sessions[req.session_id].handle
+                _lib.check(lib.dl_session_remove(ctx.handle, h, _lib.i32p(req.remove),
+                                                 req.remove.shape[0]))
+                _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(req.insert),
+                                                 req.insert.shape[0]))
+                n = ct.c_int64()
+                _lib.check(lib.dl_session_finalise(ctx.handle, h, ct.byref(n)))
+                m = model[req.session_id]
+                g = np.empty((m["h"], m["w"], 3), dtype=np.int32)
+                _lib.check(lib.dl_session_copy_result(h, _lib.i32p(g), None))
+            else:
+                triangulate(req.width, req.height, req.seeds, as_arrays=True)
+        dt = time.perf_counter() - t0
+        return {"requests": n_requests, "mpixel": px / 1e6, "seconds": dt,
+                "throughput_mpixel_per_s": px / dt / 1e6}
+
+    try:
+        run(None)   # warm-up (workspaces grow to their working size)
+        sat = run(None)
+        log(f"serving: saturated {sat['throughput_mpixel_per_s']:.0f} Mpx/s")
+        mean_px = sat["mpixel"] * 1e6 / n_requests
+        out = {"workload": f"{n_requests} requests per phase, sides log-uniform in [64, 8192], "
+                           f"1 % seed density, 20 % updates (1-5 % of the seeds of one of "
+                           f"{n_sessions} resident sessions), padding 16, 2 worker slots, "
+                           f"HBM budget 160 GiB",
+               "saturated": sat, "offered": []}
+        for f in loads:
+            rate = f * sat["throughput_mpixel_per_s"] * 1e6 / mean_px   # requests / s
+            r = run(rate)
+            r["offered_fraction"] = f
+            r["offered_requests_per_s"] = rate
+            out["offered"].append(r)
+            log(f"serving: load {f}: p50 {r['latency_ms_p50']:.1f} ms p99 {r['latency_ms_p99']:.1f} ms")
+        plain = run_plain()
+        out["plain_single_image_path"] = plain
+        out["saturated_vs_plain"] = sat["throughput_mpixel_per_s"] / plain["throughput_mpixel_per_s"]
+        log(f"serving: plain path {plain['throughput_mpixel_per_s']:.0f} Mpx/s")
+        out["batches"] = len(srv.batches)
+        out["max_batch"] = max(n for n, _ in srv.batches)
+        return out
+    finally:
+        srv.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +389,9 @@ def main():
                     help="update rounds of the incremental workload (0: skip it)")
     ap.add_argument("--inc-side", type=int, default=16384)
     ap.add_argument("--inc-seeds", type=int, default=2_000_000)
+    ap.add_argument("--serve-requests", type=int, default=300,
+                    help="requests per phase of the serving workload (0: skip it)")
+    ap.add_argument("--serve-sessions", type=int, default=8)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -371,6 +516,10 @@ def main():
         incremental = bench_incremental(lib, ctx, args.inc_side, args.inc_seeds, args.inc_rounds,
                                         10_000, 5_000)
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = bench_serving(args.serve_requests, args.serve_sessions)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU oracle sample")
@@ -409,6 +558,7 @@ def main():
         "end_to_end_mpixel_per_s": e2e,
         "batched": batch,
         "incremental": incremental,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

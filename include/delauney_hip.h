@@ -28,6 +28,7 @@
 #ifndef DELAUNEY_HIP_H
 #define DELAUNEY_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -132,6 +133,31 @@ int dl_session_copy_result(dl_session* s, int32_t* out_hw3, int32_t* out_rows);
 int dl_session_copy_seeds(dl_session* s, int32_t* out_xy);
 /* Device pointer of the last finalise's (H, W, 3) grid (NULL while pending). */
 const int32_t* dl_session_device_grid(dl_session* s);
+
+/* Page-locked host memory (hipHostMalloc) for results that are copied back
+ * at full link speed; the serving loop pools it.  No reference counterpart. */
+int dl_host_alloc(size_t bytes, void** out);
+void dl_host_free(void* p);
+
+/* One image of dl_run_batch: seeds and an output buffer (device pointers, or
+ * host pointers when host != 0: the library stages them), or a session (its
+ * live seeds are used and its result stays in the session). */
+typedef struct dl_image {
+    int width, height, border_padding;  /* ignored for sessions */
+    int host;                           /* seeds_xy / out are host pointers */
+    int64_t n_seeds;                    /* ignored for sessions */
+    const int32_t* seeds_xy;            /* n_seeds (x, y) pairs */
+    int32_t* out;                       /* (height, width, 3) int32 */
+    dl_session* session;                /* non-NULL: finalise this session */
+    int64_t n_tri;                      /* out: the image's triangle count */
+    int64_t tri_offset;                 /* out: its first row in dl_copy_triangles */
+} dl_image;
+
+/* Runs the pipeline once for a mixed batch of fresh images and sessions (the
+ * serving loop's unit of work); every image's result equals its single-image
+ * result.  Sessions with no live seed get an all -1 grid.  No reference
+ * counterpart. */
+int dl_run_batch(dl_context* ctx, int n, dl_image* images, void* stream);
 
 /* Rows (x, y, id_a, id_b, id_c) in tid order of the last triangulation on this
  * context: the reference's triangle_map values (triangulation.py:157;

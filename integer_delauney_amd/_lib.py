@@ -25,6 +25,15 @@ _I32P = ctypes.POINTER(ctypes.c_int32)
 _I64P = ctypes.POINTER(ctypes.c_int64)
 _VP = ctypes.c_void_p
 
+
+class DLImage(ctypes.Structure):
+    """struct dl_image (include/delauney_hip.h)."""
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("border_padding", ctypes.c_int), ("host", ctypes.c_int), ("n_seeds", ctypes.c_int64),
+                ("seeds_xy", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("session", ctypes.c_void_p), ("n_tri", ctypes.c_int64),
+                ("tri_offset", ctypes.c_int64)]
+
 _lib = None
 _lib_lock = threading.Lock()
 _tls = threading.local()
@@ -61,6 +70,9 @@ _SIGNATURES = {
                                             _I32P, _I64P]),
     "dl_triangulate_batch_device": (ctypes.c_int, [_VP, ctypes.c_int, _I32P, _I32P, _I64P, _VP,
                                                    _I32P, _VP, _VP, _I64P]),
+    "dl_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_VP)]),
+    "dl_host_free": (None, [_VP]),
+    "dl_run_batch": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
     "dl_session_create": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                          ctypes.c_int, ctypes.POINTER(_VP)]),
     "dl_session_destroy": (None, [_VP]),
@@ -143,3 +155,62 @@ def context() -> Context:
 
 def i32p(a: np.ndarray):
     return a.ctypes.data_as(_I32P)
+
+
+class _PinnedBlock:
+    __slots__ = ("pool", "ptr", "size", "__weakref__")
+
+    def __init__(self, pool, ptr, size):
+        self.pool, self.ptr, self.size = pool, ptr, size
+
+    def __del__(self):
+        try:
+            self.pool._give_back(self)
+        except Exception:   # pragma: no cover - interpreter shutdown
+            pass
+
+
+class PinnedPool:
+    """Page-locked host buffers by power-of-two size class, reused: an array
+    from :meth:`array` returns its block to the pool when the array (and every
+    view of it) is gone.  ``limit`` bytes of idle blocks are kept."""
+
+    def __init__(self, limit: int = 8 << 30):
+        self._free = {}
+        self._idle = 0
+        self._limit = int(limit)
+        self._lock = threading.Lock()
+
+    def _give_back(self, blk):
+        with self._lock:
+            if self._idle + blk.size <= self._limit:
+                self._free.setdefault(blk.size, []).append(blk.ptr)
+                self._idle += blk.size
+                return
+        load().dl_host_free(blk.ptr)
+
+    def array(self, shape, dtype=np.int32) -> np.ndarray:
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        size = 1 << max(12, (max(nbytes, 1) - 1).bit_length())
+        with self._lock:
+            lst = self._free.get(size)
+            ptr = lst.pop() if lst else None
+            if ptr is not None:
+                self._idle -= size
+        if ptr is None:
+            p = _VP()
+            check(load().dl_host_alloc(size, ctypes.byref(p)))
+            ptr = p.value
+        blk = _PinnedBlock(self, ptr, size)
+        buf = (ctypes.c_char * nbytes).from_address(ptr) if nbytes else (ctypes.c_char * 1)()
+        buf._dl_block = blk   # the block lives as long as the buffer does
+        return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def clear(self):
+        with self._lock:
+            for size, lst in self._free.items():
+                for ptr in lst:
+                    load().dl_host_free(ptr)
+            self._free.clear()
+            self._idle = 0

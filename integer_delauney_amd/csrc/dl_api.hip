@@ -298,16 +298,21 @@ struct StageClock {
 };
 
 // Plans ctx->plan (images already listed), uploads it and orders the seeds on
-// the device: d_xy -> ctx->sorted (id order per image) and ctx->rank_at.
-void prepare(dl_context* ctx, const int32_t* d_xy, hipStream_t s)
+// the device into ctx->sorted (id order per image) and ctx->rank_at.  Images
+// without their own seed / output pointers read seeds from d_xy and write
+// into d_out at their offsets in the concatenated layout (C channels).
+void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s)
 {
     BatchPlan& plan = ctx->plan;
     plan_batch(plan);
+    for (ImageDesc& d : plan.img) {
+        if (!d.xy) d.xy = d_xy + 2 * d.seed_off;
+        if (!d.out && d_out) d.out = d_out + static_cast<int64_t>(C) * d.pix_off;
+    }
     upload_plan(plan, ctx->tri, s);
     ctx->sorted.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_seeds, 1)));
     ctx->rank_at.reserve(sizeof(int32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_pix, 1)));
-    normalise_seeds(plan, d_xy, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), ctx->tri,
-                    s);
+    normalise_seeds(plan, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), ctx->tri, s);
 }
 
 // Flood fill of every image's detection canvas ((W + 2P) x (H + 2P), seeds
@@ -398,7 +403,7 @@ void fill_unpadded(dl_context* ctx, const std::vector<int>& which, hipStream_t s
 void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s,
                   StageClock& clk)
 {
-    prepare(ctx, d_xy, s);
+    prepare(ctx, d_xy, d_out, C, s);
     clk.mark(0);
     BatchPlan& plan = ctx->plan;
     const int n_img = static_cast<int>(plan.img.size());
@@ -442,7 +447,7 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
     clk.mark(3);
     ctx->n_tri = dr.n_tri;
     assign_tiled(plan, ctx->tri.rows.as<int32_t>(), dr.n_tri, ctx->sorted.as<uint32_t>(),
-                 ctx->rank_at.as<int32_t>(), d_out, C, ctx->tri, s);
+                 ctx->rank_at.as<int32_t>(), C, ctx->tri, s);
     clk.mark(4);
 }
 
@@ -634,7 +639,7 @@ int dl_voronoi(dl_context* ctx, int width, int height, const int32_t* seeds_xy, 
         StageClock clk(ctx);
         upload_seeds(ctx, seeds_xy, n);
         plan_single(ctx, width, height, 0, n);
-        prepare(ctx, ctx->xy.as<int32_t>(), s);
+        prepare(ctx, ctx->xy.as<int32_t>(), nullptr, 0, s);
         clk.mark(0);
         const int plane = fill_detection(ctx, std::vector<int>{-1}, s);
         clk.mark(1);
@@ -662,7 +667,7 @@ int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* s
         hipStream_t s = ctx->stream;
         upload_seeds(ctx, seeds_xy, n);
         plan_single(ctx, width, height, 0, n);
-        prepare(ctx, ctx->xy.as<int32_t>(), s);
+        prepare(ctx, ctx->xy.as<int32_t>(), nullptr, 0, s);
         const int plane = fill_detection(ctx, std::vector<int>{margin}, s);
         DL_HIP(hipMemcpyAsync(ctx->host_small.ptr, ctx->esc.ptr, sizeof(int),
                               hipMemcpyDeviceToHost, s));
@@ -691,7 +696,7 @@ int dl_voronoi_simple(dl_context* ctx, int width, int height, const int32_t* see
         hipStream_t s = ctx->stream;
         upload_seeds(ctx, seeds_xy, n);
         plan_single(ctx, width, height, 0, n);
-        prepare(ctx, ctx->xy.as<int32_t>(), s);
+        prepare(ctx, ctx->xy.as<int32_t>(), nullptr, 0, s);
         const size_t cells = static_cast<size_t>(width) * height;
         ctx->unA.reserve(sizeof(uint32_t) * cells);
         ctx->unB.reserve(sizeof(uint32_t) * cells);
@@ -828,7 +833,7 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
             run_pipeline(ctx, ctx->xy.as<int32_t>(), ctx->tri.tidgrid.as<int32_t>(), 1, s, clk);
         } else {
             if (!voronoi_grid_hw2) throw value_error("voronoi_grid is required at padding 0");
-            prepare(ctx, ctx->xy.as<int32_t>(), s);
+            prepare(ctx, ctx->xy.as<int32_t>(), nullptr, 0, s);
             clk.mark(0);
             ctx->have_padded = false;
             ctx->vgrid.reserve(sizeof(int32_t) * 2 * px);
@@ -845,9 +850,10 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
             clk.mark(2);
             clk.mark(3);
             ctx->n_tri = dr.n_tri;
+            ctx->plan.img[0].out = ctx->tri.tidgrid.as<int32_t>();
+            upload_desc(ctx->plan, ctx->tri, s);
             assign_tiled(ctx->plan, ctx->tri.rows.as<int32_t>(), dr.n_tri,
-                         ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(),
-                         ctx->tri.tidgrid.as<int32_t>(), 1, ctx->tri, s);
+                         ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), 1, ctx->tri, s);
             clk.mark(4);
         }
         DL_HIP(hipMemcpyAsync(out_tid_hw, ctx->tri.tidgrid.ptr, sizeof(int32_t) * px,
@@ -894,6 +900,122 @@ int dl_copy_padded_voronoi(dl_context* ctx, int32_t* out, int* out_w, int* out_h
         DL_HIP(hipMemcpyAsync(out, ctx->out.ptr, sizeof(int32_t) * 2 * cells,
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int dl_host_alloc(size_t bytes, void** out)
+{
+    return guarded([&] {
+        if (!out) throw value_error("null argument");
+        *out = nullptr;
+        DL_HIP(hipHostMalloc(out, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+    });
+}
+
+void dl_host_free(void* p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+int dl_run_batch(dl_context* ctx, int n, dl_image* images, void* stream)
+{
+    return guarded([&] {
+        if (!ctx || !images || n <= 0) throw value_error("null argument");
+        DL_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        ProfDrop drop(ctx, s);
+        ctx->plan = BatchPlan{};
+        std::vector<int> idx;   // images with seeds, in plan order
+        // Host images are staged through ctx->xy (seeds) and ctx->out (grids).
+        int64_t host_seeds = 0, host_px = 0;
+        for (int i = 0; i < n; ++i)
+            if (!images[i].session && images[i].host && images[i].n_seeds > 0 &&
+                images[i].width > 0 && images[i].height > 0) {
+                host_seeds += images[i].n_seeds;
+                host_px += static_cast<int64_t>(images[i].width) * images[i].height;
+            }
+        ctx->xy.reserve(sizeof(int32_t) * 2 * static_cast<size_t>(std::max<int64_t>(host_seeds, 1)));
+        ctx->out.reserve(sizeof(int32_t) * 3 * static_cast<size_t>(std::max<int64_t>(host_px, 1)));
+        int64_t so = 0, po = 0;
+        std::vector<int32_t*> dev_out(n, nullptr);
+        for (int i = 0; i < n; ++i) {
+            dl_image& im = images[i];
+            im.n_tri = 0;
+            im.tri_offset = 0;
+            ImageDesc d{};
+            if (im.session) {
+                dl_session* ss = im.session;
+                if (ss->device != ctx->device) throw value_error("session lives on another device");
+                d.W = ss->W;
+                d.H = ss->H;
+                d.P = ss->P;
+                d.n = ss->st.n_live;
+                d.xy = ss->st.list.as<int32_t>();
+                ss->out.reserve(sizeof(int32_t) * 3 * static_cast<size_t>(d.W) * d.H);
+                d.out = ss->out.as<int32_t>();
+                if (d.n == 0) {   // nothing live: all -1
+                    fill_u32(ss->out.as<uint32_t>(), 0xFFFFFFFFu, 3 * static_cast<size_t>(d.W) * d.H, s);
+                    ss->n_tri = 0;
+                    ss->done = true;
+                    continue;
+                }
+            } else {
+                d.W = im.width;
+                d.H = im.height;
+                d.P = resolve_padding(im.border_padding);
+                d.n = im.n_seeds;
+                try {
+                    check_dims(d.W, d.H, d.P);
+                    if (d.n <= 0) throw value_error("seeds must not be empty");
+                    if (!im.seeds_xy || !im.out) throw value_error("null seeds or output");
+                } catch (const value_error& e) {
+                    throw value_error("image " + std::to_string(i) + ": " + e.what());
+                }
+                if (im.host) {
+                    int32_t* sx = ctx->xy.as<int32_t>() + 2 * so;
+                    DL_HIP(hipMemcpyAsync(sx, im.seeds_xy, sizeof(int32_t) * 2 * static_cast<size_t>(d.n),
+                                          hipMemcpyHostToDevice, s));
+                    d.xy = sx;
+                    d.out = ctx->out.as<int32_t>() + 3 * po;
+                    dev_out[i] = d.out;
+                    so += d.n;
+                    po += static_cast<int64_t>(d.W) * d.H;
+                } else {
+                    d.xy = im.seeds_xy;
+                    d.out = im.out;
+                }
+            }
+            ctx->plan.img.push_back(d);
+            idx.push_back(i);
+        }
+        if (idx.empty()) {
+            DL_HIP(hipStreamSynchronize(s));
+            ctx->n_tri = 0;
+            return;
+        }
+        StageClock clk(ctx);
+        run_pipeline(ctx, nullptr, nullptr, 3, s, clk);
+        const std::vector<int64_t>& tb = ctx->tri.tri_base;
+        for (size_t k = 0; k < idx.size(); ++k) {
+            dl_image& im = images[idx[k]];
+            im.tri_offset = tb[k];
+            im.n_tri = tb[k + 1] - tb[k];
+            if (im.session) {
+                dl_session* ss = im.session;
+                ss->n_tri = im.n_tri;
+                ss->rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(std::max<int64_t>(im.n_tri, 1)));
+                if (im.n_tri)
+                    DL_HIP(hipMemcpyAsync(ss->rows.ptr, ctx->tri.rows.as<int32_t>() + 5 * tb[k],
+                                          sizeof(int32_t) * 5 * static_cast<size_t>(im.n_tri),
+                                          hipMemcpyDeviceToDevice, s));
+                ss->done = true;
+            } else if (im.host) {
+                DL_HIP(hipMemcpyAsync(im.out, dev_out[idx[k]],
+                                      sizeof(int32_t) * 3 * static_cast<size_t>(im.width) * im.height,
+                                      hipMemcpyDeviceToHost, s));
+            }
+        }
+        clk.finish(s);
     });
 }
 

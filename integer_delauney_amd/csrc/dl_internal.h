@@ -157,6 +157,8 @@ struct ImageDesc {
     // minus (out_shift, out_shift).
     const uint32_t* out_lab;
     int out_W, out_off, out_shift;
+    const int32_t* xy;    // the image's n (x, y) seed pairs (device)
+    int32_t* out;         // its output: (H, W, 3) grid or (H, W) tid raster (device)
 };
 
 struct BatchPlan {
@@ -198,11 +200,11 @@ struct TriWork {
 void upload_plan(const BatchPlan& plan, TriWork& work, hipStream_t s);
 void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s);
 
-// Validates (bounds, duplicates) and orders the seeds of every image.
-// d_xy: concatenated (x, y) int32 pairs in caller order.  Throws value_error
-// naming the first failing image when the batch has more than one.
-void normalise_seeds(const BatchPlan& plan, const int32_t* d_xy, uint32_t* d_sorted,
-                     int32_t* d_rank_at, TriWork& work, hipStream_t s);
+// Validates (bounds, duplicates) and orders the seeds of every image (read
+// from each image's xy).  Throws value_error naming the first failing image
+// when the batch has more than one.
+void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_at,
+                     TriWork& work, hipStream_t s);
 
 // Writes kInvalid over both planes of every canvas, then each seed's canvas
 // label (image label + (P, P)) at its cell.  plane_a/plane_b: n_cells each.
@@ -232,13 +234,12 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
                               const int32_t* rank_at, const uint32_t* sorted, TriWork& work,
                               hipStream_t s);
 
-// Pixel assignment binned by 64x64 tiles and fused with the output write.
-// C == 3: out_base + 3 * pix_off is image i's (H, W, 3) grid, channels 0/1
-// from its output label source; C == 1: out_base + pix_off its (H, W) tid
-// raster.
+// Pixel assignment binned by 64x64 tiles and fused with the output write into
+// each image's out: C == 3, the (H, W, 3) grid with channels 0/1 from its
+// output label source; C == 1, the (H, W) tid raster.
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
-                  const uint32_t* sorted, const int32_t* rank_at, int32_t* out_base, int C,
-                  TriWork& work, hipStream_t s);
+                  const uint32_t* sorted, const int32_t* rank_at, int C, TriWork& work,
+                  hipStream_t s);
 
 // (H, W, C) int32 output of one image without triangles (C == 2) or with the
 // tid raster (C == 3).  Output pixel (x, y) reads canvas cell (x + off, y + off);

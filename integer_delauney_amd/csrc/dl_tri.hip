@@ -285,8 +285,7 @@ constexpr uint32_t kFlagBadId = 64u;
 
 // Presence bit of (x, y) in its image's bitmap is x*H + y: bit order is the id
 // order.
-__global__ void seeds_mark_kernel(const int32_t* __restrict__ xy, int64_t n_total,
-                                  const int64_t* __restrict__ seed_seg, int n_img,
+__global__ void seeds_mark_kernel(int64_t n_total, const int64_t* __restrict__ seed_seg, int n_img,
                                   const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
                                   uint32_t* __restrict__ flags)
 {
@@ -294,7 +293,8 @@ __global__ void seeds_mark_kernel(const int32_t* __restrict__ xy, int64_t n_tota
     if (g >= n_total) return;
     const int img = n_img > 1 ? find_seg(seed_seg, n_img, g) : 0;
     const ImageDesc& d = D[img];
-    const int x = xy[2 * g], y = xy[2 * g + 1];
+    const int32_t* p = d.xy + 2 * (g - d.seed_off);
+    const int x = p[0], y = p[1];
     if (x < 0 || x >= d.W) { atomicOr(&flags[img], kFlagXRange); return; }
     if (y < 0 || y >= d.H) { atomicOr(&flags[img], kFlagYRange); return; }
     const uint64_t bit = static_cast<uint64_t>(x) * d.H + y;
@@ -303,8 +303,7 @@ __global__ void seeds_mark_kernel(const int32_t* __restrict__ xy, int64_t n_tota
     if (old & m) atomicOr(&flags[img], kFlagDuplicate);
 }
 
-__global__ void seeds_rank_kernel(const int32_t* __restrict__ xy, int64_t n_total,
-                                  const int64_t* __restrict__ seed_seg, int n_img,
+__global__ void seeds_rank_kernel(int64_t n_total, const int64_t* __restrict__ seed_seg, int n_img,
                                   const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
                                   const uint32_t* __restrict__ pre, uint32_t* __restrict__ sorted,
                                   int32_t* __restrict__ rank_at)
@@ -313,7 +312,8 @@ __global__ void seeds_rank_kernel(const int32_t* __restrict__ xy, int64_t n_tota
     if (g >= n_total) return;
     const int img = n_img > 1 ? find_seg(seed_seg, n_img, g) : 0;
     const ImageDesc& d = D[img];
-    const int x = xy[2 * g], y = xy[2 * g + 1];
+    const int32_t* p = d.xy + 2 * (g - d.seed_off);
+    const int x = p[0], y = p[1];
     const uint64_t bit = static_cast<uint64_t>(x) * d.H + y;
     const int64_t w = d.bm_off + static_cast<int64_t>(bit >> 5);
     const uint32_t below = bm[w] & ((1u << (bit & 31)) - 1u);
@@ -340,8 +340,8 @@ __global__ void canvas_seed_kernel(const uint32_t* __restrict__ sorted, int64_t 
 
 }  // namespace
 
-void normalise_seeds(const BatchPlan& plan, const int32_t* d_xy, uint32_t* d_sorted,
-                     int32_t* d_rank_at, TriWork& work, hipStream_t s)
+void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_at,
+                     TriWork& work, hipStream_t s)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t nw = plan.n_words;
@@ -358,7 +358,7 @@ void normalise_seeds(const BatchPlan& plan, const int32_t* d_xy, uint32_t* d_sor
         ProfScope ps(kKSeeds, s);
         fill_u32(bm, 0u, static_cast<size_t>(nw), s);
         fill_u32(flags, 0u, static_cast<size_t>(n_img + 8), s);
-        seeds_mark_kernel<<<blocks, 256, 0, s>>>(d_xy, plan.n_seeds, sseg, n_img, D, bm, flags);
+        seeds_mark_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm, flags);
         DL_LAUNCH_CHECK();
     }
     DL_HIP(hipMemcpyAsync(work.host_flags.ptr, flags, sizeof(uint32_t) * n_img,
@@ -375,7 +375,7 @@ void normalise_seeds(const BatchPlan& plan, const int32_t* d_xy, uint32_t* d_sor
     }
     ProfScope ps(kKSeeds, s);
     popcount_scan(bm, nw, work.scan.as<uint32_t>(), work.scan_tmp, s);
-    seeds_rank_kernel<<<blocks, 256, 0, s>>>(d_xy, plan.n_seeds, sseg, n_img, D, bm,
+    seeds_rank_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm,
                                              work.scan.as<uint32_t>(), d_sorted, d_rank_at);
     DL_LAUNCH_CHECK();
 }
@@ -1181,8 +1181,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int64_t* __restrict__ tile_seg, const int64_t* __restrict__ tri_base, int n_img,
     const ImageDesc* __restrict__ D, const int32_t* __restrict__ rows,
     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ list, const int32_t* __restrict__ rank_at,
-    int32_t* __restrict__ out_base)
+    const uint32_t* __restrict__ list, const int32_t* __restrict__ rank_at)
 {
     __shared__ int32_t tid_tile[kTS * kTS];
     const int64_t gt = blockIdx.x;
@@ -1237,12 +1236,12 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     for (int r = threadIdx.x >> 4; r < kTS; r += blockDim.x / 16) {
         const int y = Y0 + r;
         if (y > Y1 || x > X1) continue;
-        const int64_t p = d.pix_off + static_cast<int64_t>(y) * W + x;
+        const int64_t p = static_cast<int64_t>(y) * W + x;   // pixel within the image
         const int4 t4 = *reinterpret_cast<const int4*>(&tid_tile[r * kTS + 4 * q]);
         const int32_t tv[4] = {t4.x, t4.y, t4.z, t4.w};
         const int n = min(4, X1 - x + 1);
         if (C == 1) {
-            int32_t* dst = out_base + p;
+            int32_t* dst = d.out + p;
             if (n == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
                 *reinterpret_cast<int4*>(dst) = t4;
             } else {
@@ -1270,7 +1269,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
             o[3 * k + 1] = dd;
             o[3 * k + 2] = tv[k];
         }
-        int32_t* dst = out_base + 3 * p;
+        int32_t* dst = d.out + 3 * p;
         if (n == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
             int4* d4 = reinterpret_cast<int4*>(dst);
             d4[0] = make_int4(o[0], o[1], o[2], o[3]);
@@ -1292,8 +1291,8 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
 }  // namespace
 
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
-                  const uint32_t* sorted, const int32_t* rank_at, int32_t* out_base, int C,
-                  TriWork& work, hipStream_t s)
+                  const uint32_t* sorted, const int32_t* rank_at, int C, TriWork& work,
+                  hipStream_t s)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t ntiles = plan.n_tiles;
@@ -1336,12 +1335,10 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
     ProfScope ps(kKOutput, s);
     if (C == 3)
         tile_assign_kernel<3><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
-            out_base);
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at);
     else
         tile_assign_kernel<1><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
-            out_base);
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at);
     DL_LAUNCH_CHECK();
 }
 
