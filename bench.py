@@ -231,7 +231,7 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             "final_equals_batch_bitwise": same}
 
 
-def bench_serving(n_requests: int, n_sessions: int, loads=(0.3, 0.6, 0.9)):
+def bench_serving(n_requests: int, n_sessions: int, slots: int = 3, loads=(0.3, 0.6, 0.9)):
     """Serving loop (integer_delauney_amd.serving): requests with sides
     log-uniform in [64, 8192] (width and height independent) at 1 % seed
     density, 20 % of them insert/delete batches of 1-5 % of a resident
@@ -249,7 +249,7 @@ def bench_serving(n_requests: int, n_sessions: int, loads=(0.3, 0.6, 0.9)):
     def side():
         return int(round(np.exp(rng.uniform(np.log(64), np.log(8192)))))
 
-    srv = Server(hbm_budget=160 << 30, slots=2)
+    srv = Server(hbm_budget=160 << 30, slots=slots)
     model = {}
     for _ in range(n_sessions):
         w, h = side(), side()
@@ -352,7 +352,7 @@ def bench_serving(n_requests: int, n_sessions: int, loads=(0.3, 0.6, 0.9)):
         mean_px = sat["mpixel"] * 1e6 / n_requests
         out = {"workload": f"{n_requests} requests per phase, sides log-uniform in [64, 8192], "
                            f"1 % seed density, 20 % updates (1-5 % of the seeds of one of "
-                           f"{n_sessions} resident sessions), padding 16, 2 worker slots, "
+                           f"{n_sessions} resident sessions), padding 16, {slots} worker slots, "
                            f"HBM budget 160 GiB",
                "saturated": sat, "offered": []}
         for f in loads:
@@ -392,6 +392,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests per phase of the serving workload (0: skip it)")
     ap.add_argument("--serve-sessions", type=int, default=8)
+    ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -518,7 +519,7 @@ def main():
 
     serving = None
     if args.serve_requests > 0:
-        serving = bench_serving(args.serve_requests, args.serve_sessions)
+        serving = bench_serving(args.serve_requests, args.serve_sessions, args.serve_slots)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -23,8 +23,8 @@ workers overlap on the device (the library releases the GIL).
 Admission: a request's device footprint is estimated from its size
 (:func:`footprint`).  A worker admits waiting requests in arrival order (those
 of a busy session are skipped) while the bytes in flight over all workers stay
-within ``hbm_budget - resident session bytes``; a request that does not fit
-waits.  One request runs alone even if it exceeds the budget, so nothing
+within ``hbm_budget - resident session bytes`` and its batch within
+``max_batch_bytes``; a request that does not fit waits.  One request runs alone even if it exceeds the budget, so nothing
 starves.  Each result is the same as the request's result computed alone:
 batching never changes a result (tests/test_gpu_serving.py).
 
@@ -81,11 +81,14 @@ class _Session:
 
 
 class Server:
-    def __init__(self, hbm_budget: int = 64 << 30, slots: int = 2, max_batch: int = 256,
-                 device: int = 0):
+    def __init__(self, hbm_budget: int = 64 << 30, slots: int = 3, max_batch: int = 256,
+                 max_batch_bytes: int = 2 << 30, device: int = 0):
         self._lib = _lib.load()
         self._budget = int(hbm_budget)
         self._max_batch = int(max_batch)
+        # Batches stay small enough that one slot's results travel back to
+        # the host while another slot's batch computes.
+        self._max_batch_bytes = int(max_batch_bytes)
         self._ctxs = [_lib.Context(device) for _ in range(slots)]
         # results land in page-locked memory: the copy back runs at link speed
         self._pool = _lib.PinnedPool()
@@ -171,9 +174,10 @@ class Server:
                     blocked.add(req.session_id)
                     rest.append(item)
                     continue
-            fits = self._inflight + total + cost <= limit
-            alone = not batch and self._inflight == 0
-            if len(batch) >= self._max_batch or not (fits or alone):
+            over_budget = self._inflight + total + cost > limit
+            over_batch = bool(batch) and total + cost > self._max_batch_bytes
+            alone = not batch and self._inflight == 0   # runs even above the budget
+            if len(batch) >= self._max_batch or over_batch or (over_budget and not alone):
                 rest.append(item)
                 rest.extend(self._queue)
                 self._queue.clear()
