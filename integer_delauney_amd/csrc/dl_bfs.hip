@@ -74,7 +74,9 @@ typedef short s16x2 __attribute__((ext_vector_type(2)));
 constexpr uint32_t kFarInv = 0x7FFF7FFFu;
 constexpr uint32_t kFarD = 2u * 32767u * 32767u;
 
-__device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t ninv, uint32_t cxy1)
+// A cell's key lives in one 64-bit register pair, so the compare needs no
+// operand assembly; only the candidate pair (ninv, c) is formed.
+__device__ __forceinline__ bool relax_key(uint64_t& key, uint32_t ninv, uint32_t cxy1)
 {
     const s16x2 dd = __builtin_elementwise_add_sat(__builtin_bit_cast(s16x2, ninv),
                                                    __builtin_bit_cast(s16x2, cxy1));
@@ -82,12 +84,12 @@ __device__ __forceinline__ bool relax_key(uint32_t& inv, uint32_t& d, uint32_t n
     // three-operand dot2 with an inline zero accumulator.
     const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_sdot2(dd, dd, 0, true));
     const uint64_t ck = (static_cast<uint64_t>(c) << 32) | ninv;
-    const uint64_t ok = (static_cast<uint64_t>(d) << 32) | inv;
-    const bool take = ck < ok;
-    inv = take ? ninv : inv;
-    d = take ? c : d;
+    const bool take = ck < key;
+    key = take ? ck : key;
     return take;
 }
+
+__device__ __forceinline__ uint32_t inv_of(uint64_t key) { return static_cast<uint32_t>(key); }
 
 __device__ __forceinline__ uint32_t from_left(uint32_t v)   // lane l <- lane l-1; lane 0 <- far
 {
@@ -134,8 +136,8 @@ struct Watch {
 // A tile where no wave has q set is a fixed point: the remaining iterations
 // are skipped.
 template <int NW, int R, int K>
-__device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], uint32_t (&D0)[R],
-                                        uint32_t (&D1)[R], int c0, int y0, const Watch& wt,
+__device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], int c0, int y0,
+                                        const Watch& wt,
                                         uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
                                         int (*lds_q)[NW], int wave, int lane, int& esc)
 {
@@ -155,10 +157,10 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             // pass 1: source x-1
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t left = from_left(I1[r]);
-                const uint32_t own0 = I0[r];
-                const bool t1 = relax_key(I1[r], D1[r], own0, DL_P1(r));
-                const bool t0 = relax_key(I0[r], D0[r], left, DL_P0(r));
+                const uint32_t left = from_left(inv_of(K1[r]));
+                const uint32_t own0 = inv_of(K0[r]);
+                const bool t1 = relax_key(K1[r], own0, DL_P1(r));
+                const bool t0 = relax_key(K0[r], left, DL_P0(r));
                 ch |= t0 | t1;
                 const uint64_t rb = ((wt.rows_in >> r) & 1u) ? ~0ull : 0ull;
                 hit |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) & rb;
@@ -166,10 +168,10 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             // pass 2: source x+1
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t right = from_right(I0[r]);
-                const uint32_t own1 = I1[r];
-                const bool t0 = relax_key(I0[r], D0[r], own1, DL_P0(r));
-                const bool t1 = relax_key(I1[r], D1[r], right, DL_P1(r));
+                const uint32_t right = from_right(inv_of(K0[r]));
+                const uint32_t own1 = inv_of(K1[r]);
+                const bool t0 = relax_key(K0[r], own1, DL_P0(r));
+                const bool t1 = relax_key(K1[r], right, DL_P1(r));
                 ch |= t0 | t1;
                 const uint64_t rb = ((wt.rows_in >> r) & 1u) ? ~0ull : 0ull;
                 hit |= ((__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1)) & rb;
@@ -178,8 +180,8 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             cx = __ballot(ch) != 0ull;
         }
         // pass 3: source y-1.  Publish this wave's last row and its q flag.
-        ldsA[wave][lane][0] = I0[R - 1];
-        ldsA[wave][lane][1] = I1[R - 1];
+        ldsA[wave][lane][0] = inv_of(K0[R - 1]);
+        ldsA[wave][lane][1] = inv_of(K1[R - 1]);
         if (lane == 0) lds_q[k & 1][wave] = (chy || cx) ? 1 : 0;
         __syncthreads();
         int any = 0;
@@ -195,28 +197,28 @@ __device__ __forceinline__ void iterate(uint32_t (&I0)[R], uint32_t (&I1)[R], ui
             const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : kFarInv;
 #pragma unroll
             for (int r = R - 1; r >= 0; --r) {
-                const uint32_t a0 = r > 0 ? I0[r > 0 ? r - 1 : 0] : top0;
-                const uint32_t a1 = r > 0 ? I1[r > 0 ? r - 1 : 0] : top1;
-                const bool t0 = relax_key(I0[r], D0[r], a0, DL_P0(r));
-                const bool t1 = relax_key(I1[r], D1[r], a1, DL_P1(r));
+                const uint32_t a0 = r > 0 ? inv_of(K0[r > 0 ? r - 1 : 0]) : top0;
+                const uint32_t a1 = r > 0 ? inv_of(K1[r > 0 ? r - 1 : 0]) : top1;
+                const bool t0 = relax_key(K0[r], a0, DL_P0(r));
+                const bool t1 = relax_key(K1[r], a1, DL_P1(r));
                 ch |= t0 | t1;
                 const uint64_t rb = r == wt.r_top ? ~0ull : 0ull;
                 hit |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) & rb;
             }
         }
         // pass 4: source y+1
-        ldsB[wave][lane][0] = I0[0];
-        ldsB[wave][lane][1] = I1[0];
+        ldsB[wave][lane][0] = inv_of(K0[0]);
+        ldsB[wave][lane][1] = inv_of(K1[0]);
         __syncthreads();
         if (run_y) {
             const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : kFarInv;
             const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : kFarInv;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t b0 = r < R - 1 ? I0[r < R - 1 ? r + 1 : r] : bot0;
-                const uint32_t b1 = r < R - 1 ? I1[r < R - 1 ? r + 1 : r] : bot1;
-                const bool t0 = relax_key(I0[r], D0[r], b0, DL_P0(r));
-                const bool t1 = relax_key(I1[r], D1[r], b1, DL_P1(r));
+                const uint32_t b0 = r < R - 1 ? inv_of(K0[r < R - 1 ? r + 1 : r]) : bot0;
+                const uint32_t b1 = r < R - 1 ? inv_of(K1[r < R - 1 ? r + 1 : r]) : bot1;
+                const bool t0 = relax_key(K0[r], b0, DL_P0(r));
+                const bool t1 = relax_key(K1[r], b1, DL_P1(r));
                 ch |= t0 | t1;
                 const uint64_t rb = r == wt.r_bot ? ~0ull : 0ull;
                 hit |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) & rb;
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
 
     // Per-cell state: ~label and squared distance (see relax_key).  Loads use
     // clamped addresses and selects, so no per-row lane masks stay live.
-    uint32_t I0[R], I1[R], D0[R], D1[R];
+    uint64_t K0[R], K1[R];
     const bool in0 = c0 >= 0 && c0 < W;
     const bool in1 = c0 + 1 >= 0 && c0 + 1 < W;
     const int cc0 = min(max(c0, 0), W - 1);
@@ -318,17 +320,15 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
             const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
             const uint32_t a = src[rowoff + cc0];
             const uint32_t c = src[rowoff + cc1];
-            I0[r] = (in0 && a != kInvalid) ? ~a : kFarInv;
-            I1[r] = (in1 && c != kInvalid) ? ~c : kFarInv;
+            const uint32_t ia = (in0 && a != kInvalid) ? ~a : kFarInv;
+            const uint32_t ic = (in1 && c != kInvalid) ? ~c : kFarInv;
             const uint32_t da = a == kInvalid ? kFarD : static_cast<uint32_t>(dist2_lab(c0, y, a));
             const uint32_t dc = c == kInvalid ? kFarD : static_cast<uint32_t>(dist2_lab(c0 + 1, y, c));
-            D0[r] = in0 ? da : 0u;
-            D1[r] = in1 ? dc : 0u;
+            K0[r] = (static_cast<uint64_t>(in0 ? da : 0u) << 32) | ia;
+            K1[r] = (static_cast<uint64_t>(in1 ? dc : 0u) << 32) | ic;
         } else {
-            I0[r] = kFarInv;
-            I1[r] = kFarInv;
-            D0[r] = 0u;
-            D1[r] = 0u;
+            K0[r] = kFarInv;   // outside: distance 0
+            K1[r] = kFarInv;
         }
     }
 
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r)
         if (on && y0 + r >= P && y0 + r < H - P) wt.rows_in |= 1u << r;
-    iterate<NW, R, K>(I0, I1, D0, D1, c0, y0, wt, ldsA, ldsB, lds_any, wave, lane, esc);
+    iterate<NW, R, K>(K0, K1, c0, y0, wt, ldsA, ldsB, lds_any, wave, lane, esc);
 
     // Write the interior back and note whether it changed.  The row origin is
     // passed through an empty asm statement so that the compiler recomputes
@@ -370,13 +370,13 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
         const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
         if (w0) {
             const size_t i = rowoff + c0;
-            const uint32_t v = I0[r] == kFarInv ? kInvalid : ~I0[r];
+            const uint32_t v = inv_of(K0[r]) == kFarInv ? kInvalid : ~inv_of(K0[r]);
             changed |= src[i] != v;
             dst[i] = v;
         }
         if (w1) {
             const size_t i = rowoff + c0 + 1;
-            const uint32_t v = I1[r] == kFarInv ? kInvalid : ~I1[r];
+            const uint32_t v = inv_of(K1[r]) == kFarInv ? kInvalid : ~inv_of(K1[r]);
             changed |= src[i] != v;
             dst[i] = v;
         }
