@@ -234,6 +234,11 @@ int dl_set_force_unpadded(dl_context* ctx, int enabled);
 int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* seeds_xy,
                         int64_t n, int margin, int32_t* out_hw2, int* escaped);
 
+/* Cross-checking only: makes the speculative sizes of detection and
+ * assignment too small on purpose, so that every call takes the exact-size
+ * rerun.  Results are unchanged.  No reference counterpart. */
+int dl_set_tiny_capacities(dl_context* ctx, int enabled);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,7 @@ _SIGNATURES = {
     "dl_session_copy_result": (ctypes.c_int, [_VP, _I32P, _I32P]),
     "dl_session_copy_seeds": (ctypes.c_int, [_VP, _I32P]),
     "dl_session_device_grid": (_VP, [_VP]),
+    "dl_set_tiny_capacities": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_debug_fill_watch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                            ctypes.c_int, _I32P, ctypes.POINTER(ctypes.c_int)]),
 }

@@ -397,9 +397,15 @@ void fill_unpadded(dl_context* ctx, const std::vector<int>& which, hipStream_t s
 }
 
 // The batch pipeline on device-resident seeds (ctx->plan lists the images):
-// seeds -> padded flood fill -> [unpadded fill where the padding reached the
-// image] -> detection -> assignment fused with the (H, W, 3) output, or the
-// bare tid raster when C == 1.
+// seeds -> padded flood fill -> detection -> assignment fused with the
+// (H, W, 3) output (or the bare tid raster when C == 1).
+//
+// After the flood fill nothing waits for the host: detection and assignment
+// are sized from the seed count, and a single read-back at the end checks
+// what the sizes and the padding watch assumed.  A capacity overflow or hash
+// collision (rare) reruns detection and assignment with exact sizes; an image
+// whose padding reached its interior (never seen in practice) gets its
+// unpadded fill and its assignment again.
 void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s,
                   StageClock& clk)
 {
@@ -424,31 +430,44 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
         d.out_off = d.P;
         d.out_shift = d.P;
     }
+    upload_desc(plan, ctx->tri, s);
+    clk.mark(1);
+    const int32_t* rows = nullptr;
+    detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(),
+                     ctx->tri, s, true);
+    clk.mark(2);
+    clk.mark(3);
+    rows = ctx->tri.rows.as<int32_t>();
+    assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), C,
+                 ctx->tri, s, true);
+    clk.mark(4);
+    SpeculativeStatus st = read_status(plan, ctx->tri, any_pad ? ctx->esc.as<int>() : nullptr,
+                                       any_pad ? n_img : 0, s);
+    if (st.bad_id) throw runtime_error("flood fill left an undefined cell");
+    if (st.redo) {
+        const DetectResult dr = detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(),
+                                                 ctx->sorted.as<uint32_t>(), ctx->tri, s, false);
+        rows = ctx->tri.rows.as<int32_t>();
+        assign_tiled(plan, rows, dr.n_tri, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(),
+                     C, ctx->tri, s, false);
+        st.n_tri = dr.n_tri;
+    }
     if (labels_out) {
         // Did a padding cell ever update an image cell?  If not, the image's
         // part of the padded canvas evolved exactly as the unpadded fill.
         std::vector<int> which;
-        if (any_pad) {
-            ctx->host_small.reserve(sizeof(int) * static_cast<size_t>(n_img));
-            DL_HIP(hipMemcpyAsync(ctx->host_small.ptr, ctx->esc.ptr, sizeof(int) * n_img,
-                                  hipMemcpyDeviceToHost, s));
-            DL_HIP(hipStreamSynchronize(s));
-            const int* esc = ctx->host_small.as<int>();
-            for (int i = 0; i < n_img; ++i)
-                if (plan.img[i].P > 0 && (esc[i] || ctx->force_unpadded)) which.push_back(i);
+        for (int i = 0; i < n_img; ++i)
+            if (plan.img[i].P > 0 &&
+                ((!st.escaped.empty() && st.escaped[i]) || ctx->force_unpadded))
+                which.push_back(i);
+        if (!which.empty()) {
+            fill_unpadded(ctx, which, s);
+            upload_desc(plan, ctx->tri, s);
+            assign_tiled(plan, rows, st.n_tri, ctx->sorted.as<uint32_t>(),
+                         ctx->rank_at.as<int32_t>(), C, ctx->tri, s, false);
         }
-        if (!which.empty()) fill_unpadded(ctx, which, s);
-        upload_desc(plan, ctx->tri, s);
     }
-    clk.mark(1);
-    const DetectResult dr = detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(),
-                                             ctx->sorted.as<uint32_t>(), ctx->tri, s);
-    clk.mark(2);
-    clk.mark(3);
-    ctx->n_tri = dr.n_tri;
-    assign_tiled(plan, ctx->tri.rows.as<int32_t>(), dr.n_tri, ctx->sorted.as<uint32_t>(),
-                 ctx->rank_at.as<int32_t>(), C, ctx->tri, s);
-    clk.mark(4);
+    ctx->n_tri = st.n_tri;
 }
 
 void upload_seeds(dl_context* ctx, const int32_t* xy, int64_t n)
@@ -623,6 +642,14 @@ int dl_set_force_unpadded(dl_context* ctx, int enabled)
     return guarded([&] {
         if (!ctx) throw value_error("null context");
         ctx->force_unpadded = enabled != 0;
+    });
+}
+
+int dl_set_tiny_capacities(dl_context* ctx, int enabled)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->tri.spec_tiny = enabled != 0;
     });
 }
 
@@ -846,14 +873,16 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
             clk.mark(1);
             const DetectResult dr =
                 detect_triangles(ctx->plan, ctx->ids.as<uint32_t>(), false,
-                                 ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(), ctx->tri, s);
+                                 ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(), ctx->tri, s,
+                                 false);
             clk.mark(2);
             clk.mark(3);
             ctx->n_tri = dr.n_tri;
             ctx->plan.img[0].out = ctx->tri.tidgrid.as<int32_t>();
             upload_desc(ctx->plan, ctx->tri, s);
             assign_tiled(ctx->plan, ctx->tri.rows.as<int32_t>(), dr.n_tri,
-                         ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), 1, ctx->tri, s);
+                         ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), 1, ctx->tri, s,
+                         false);
             clk.mark(4);
         }
         DL_HIP(hipMemcpyAsync(out_tid_hw, ctx->tri.tidgrid.ptr, sizeof(int32_t) * px,

@@ -193,6 +193,9 @@ struct TriWork {
     DevBuf segs;        // device copy of the offset tables + tri_base
     PinnedBuf host_stage;   // staging for the uploads above
     std::vector<int64_t> tri_base;   // host copy: first row of each image, then the total
+    uint32_t rows_cap = 0;           // capacity of rows (detection)
+    int64_t list_cap = 0;            // capacity of bin_list (assignment)
+    bool spec_tiny = false;          // test hook: speculative capacities of one
 };
 
 // Uploads the plan's descriptors and offset tables (the output sources may be
@@ -230,16 +233,35 @@ struct DetectResult {
 // rank_at) or seed ids (labels = false).  Writes rows
 // (x - P, y - P, a, b, c) of all images, image after image, into work.rows and
 // the first row of each image into work.tri_base (host) and the device table.
+//
+// speculative = false sizes every buffer exactly, reading counts back (host
+// syncs), and fills work.tri_base.  speculative = true sizes them from the
+// seed count without any sync; capacity overflows, hash collisions and the
+// triangle counts are then checked by read_status after the assignment.
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
                               const int32_t* rank_at, const uint32_t* sorted, TriWork& work,
+                              hipStream_t s, bool speculative);
+
+struct SpeculativeStatus {
+    bool bad_id = false;   // a canvas cell without owner / an id out of range
+    bool redo = false;     // a speculative capacity was exceeded or a hash collided
+    int64_t n_tri = 0;
+    std::vector<int> escaped;   // the padding watch's flags, per image
+};
+
+// One sync: status word, triangle offsets (into work.tri_base), bin-list
+// total, and n_esc padding-watch flags from d_esc (may be null).
+SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
                               hipStream_t s);
 
 // Pixel assignment binned by 64x64 tiles and fused with the output write into
 // each image's out: C == 3, the (H, W, 3) grid with channels 0/1 from its
 // output label source; C == 1, the (H, W) tid raster.
+// n_tri < 0: the count is on the device (after a speculative detection);
+// speculative = true sizes the bin list without a sync (checked by read_status).
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
                   const uint32_t* sorted, const int32_t* rank_at, int C, TriWork& work,
-                  hipStream_t s);
+                  hipStream_t s, bool speculative);
 
 // (H, W, C) int32 output of one image without triangles (C == 2) or with the
 // tid raster (C == 3).  Output pixel (x, y) reads canvas cell (x + off, y + off);

@@ -282,6 +282,7 @@ constexpr uint32_t kFlagDuplicate = 4u;
 constexpr uint32_t kFlagTableOverflow = 16u;
 constexpr uint32_t kFlagCollision = 32u;
 constexpr uint32_t kFlagBadId = 64u;
+constexpr uint32_t kFlagOverflow = 128u;   // a speculative capacity was too small
 
 // Presence bit of (x, y) in its image's bitmap is x*H + y: bit order is the id
 // order.
@@ -567,6 +568,9 @@ struct DetectArgs {
     Cand* cand;
     uint32_t* flags;              // batch-wide status word
     Table table;
+    const uint32_t* npos;         // candidate positions actually listed (device)
+    uint32_t pos_cap;             // capacity of positions (cand holds 2 per position)
+    uint32_t rows_cap;            // capacity of rows
 };
 
 __device__ __forceinline__ int32_t to_id(const DetectArgs& A, const ImageDesc& d, uint32_t v)
@@ -679,6 +683,10 @@ __global__ __launch_bounds__(kDetThreads) void detect_collect_kernel(DetectArgs 
         const int j = __ffs(mask) - 1;
         mask &= mask - 1;
         const uint32_t rx = static_cast<uint32_t>(bx * kDetChunk + j * kDetThreads + tid);
+        if (out >= A.pos_cap) {   // capacity too small: flagged, the host reruns exactly
+            atomicOr(A.flags, kFlagOverflow);
+            return;
+        }
         A.positions[out++] = make_uint2(static_cast<uint32_t>(img), static_cast<uint32_t>(ry) * bw + rx);
     }
 }
@@ -706,12 +714,23 @@ __device__ __forceinline__ void add_candidate(const DetectArgs& A, const ImageDe
     A.cand[out++] = cd;
 }
 
-// One thread per listed position.  A position's candidates go to slots 2i and
-// 2i+1 of the candidate array; an unused slot keeps key 0xFFFFFFFF.
-__global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A, uint32_t npos)
+// Listed positions, grid-stride over the device count.  A position's
+// candidates go to slots 2i and 2i+1 of the candidate array; an unused slot
+// keeps key 0xFFFFFFFF.
+__device__ void insert_position(const DetectArgs& A, uint32_t i);
+
+__global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npos) return;
+    // Sizes already known to be too small: the host reruns with exact ones,
+    // so skip the (then long) probing.
+    if (*A.flags & kFlagOverflow) return;
+    const uint32_t n = *A.npos;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        insert_position(A, i);
+}
+
+__device__ void insert_position(const DetectArgs& A, uint32_t i)
+{
     const uint2 ip = A.positions[i];
     const uint32_t img = ip.x, pos = ip.y;
     const ImageDesc& d = A.D[img];
@@ -767,12 +786,25 @@ __device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
 }
 
 // Bit key_off * 32 + key of the registration bitmap marks a first registration.
-__global__ void mark_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
-                            const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
+__device__ void mark_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
+                         const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
+                         uint32_t* __restrict__ flags);
+
+// Candidates, grid-stride over twice the device position count.
+__global__ void mark_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ npos,
+                            Table t, const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
                             uint32_t* __restrict__ flags)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (*flags & kFlagOverflow) return;   // the insert did not run: candidates unset
+    const uint32_t n = 2 * *npos;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        mark_one(cand, i, t, D, bm, flags);
+}
+
+__device__ void mark_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
+                         const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
+                         uint32_t* __restrict__ flags)
+{
     const Cand cd = cand[i];
     if (cd.key == 0xFFFFFFFFu) return;
     int32_t a = cd.a, b = cd.b, c = cd.c;
@@ -798,18 +830,37 @@ __global__ void tri_base_kernel(const ImageDesc* __restrict__ D, int n_img,
     tri_base[i] = i < n_img ? pre[D[i].key_off] : pre[n_words];
 }
 
-__global__ void emit_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
-                            const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
-                            const uint32_t* __restrict__ pre, int32_t* __restrict__ rows)
+__device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
+                         const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
+                         const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
+                         uint32_t rows_cap, uint32_t* __restrict__ flags);
+
+__global__ void emit_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ npos,
+                            Table t, const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
+                            const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
+                            uint32_t rows_cap, uint32_t* __restrict__ flags)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (*flags & kFlagOverflow) return;   // rerun with exact sizes follows
+    const uint32_t n = 2 * *npos;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        emit_one(cand, i, t, D, bm, pre, rows, rows_cap, flags);
+}
+
+__device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
+                         const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
+                         const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
+                         uint32_t rows_cap, uint32_t* __restrict__ flags)
+{
     const Cand cd = cand[i];
     if (cd.key == 0xFFFFFFFFu || !is_first(cd, t)) return;
     const ImageDesc& d = D[cd.img];
     const int64_t w = d.key_off + (cd.key >> 5);
     // global row = first registrations of earlier images and earlier keys
     const uint32_t row = pre[w] + static_cast<uint32_t>(__popc(bm[w] & ((1u << (cd.key & 31)) - 1u)));
+    if (row >= rows_cap) {
+        atomicOr(flags, kFlagOverflow);
+        return;
+    }
     const uint32_t nb = static_cast<uint32_t>(d.nb);
     const uint32_t scan = cd.key / nb;
     const uint32_t rem = cd.key - scan * nb;
@@ -825,6 +876,21 @@ __global__ void emit_kernel(const Cand* __restrict__ cand, uint32_t n, Table t,
     r[2] = cd.a;
     r[3] = cd.b;
     r[4] = cd.c;
+}
+
+// Positions actually listed (capped) and the capacity checks of a
+// speculatively sized detection: more positions than the list holds, or more
+// distinct triplets than three quarters of the table, raise kFlagOverflow.
+__global__ void det_bounds_kernel(const uint32_t* __restrict__ npos_total,
+                                  const uint32_t* __restrict__ quads_total, uint32_t pos_cap,
+                                  uint64_t table_cap, uint32_t* __restrict__ npos,
+                                  uint32_t* __restrict__ flags)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t n = *npos_total, q = *quads_total;
+    *npos = min(n, pos_cap);
+    if (n > pos_cap || 4 * (static_cast<uint64_t>(n) + 2ull * q) > 3 * table_cap)
+        atomicOr(flags, kFlagOverflow);
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n)
@@ -844,7 +910,7 @@ uint64_t next_pow2(uint64_t v)
 
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
                               const int32_t* rank_at, const uint32_t* sorted, TriWork& work,
-                              hipStream_t s)
+                              hipStream_t s, bool speculative)
 {
     DetectResult res;
     const int n_img = static_cast<int>(plan.img.size());
@@ -854,12 +920,20 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     uint32_t* hflags = work.host_flags.as<uint32_t>();
     int64_t* tri_base = seg_ptr(work, 3, n_img);
     work.tri_base.assign(n_img + 1, 0);
+    work.rows_cap = 0;
     const int64_t nblocks = plan.n_det_blocks;
+    // flags layout: [0] status bits, [2] quad count, [4] candidate positions,
+    // [6] positions listed (device count read by the later kernels).
+    {
+        ProfScope ps(kKDetect, s);
+        zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
+        DL_LAUNCH_CHECK();
+    }
     if (nblocks == 0) {
         work.rows.reserve(sizeof(int32_t) * 5);
+        fill_u32(reinterpret_cast<uint32_t*>(tri_base), 0u, 2 * static_cast<size_t>(n_img + 1), s);
         return res;
     }
-    // flags layout: [0] status bits, [2] quad count, [4] candidate positions.
     const int64_t nthr = nblocks * kDetThreads;
     work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 2 * (nblocks + 1)) +
                         static_cast<size_t>(nthr));
@@ -878,30 +952,51 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     uint32_t* quad_scan = A.block_quads + nblocks + 1;
     A.masks = reinterpret_cast<uint8_t*>(quad_scan + nblocks + 1);
     A.flags = dflags;
+    A.npos = dflags + 6;
     {
         ProfScope ps(kKDetect, s);
-        zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
         detect_count_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
         exclusive_scan_u32(A.counts, nthr, work.offsets.as<uint32_t>(), work.scan_tmp, s);
         exclusive_scan_u32(A.block_quads, nblocks, quad_scan, work.scan_tmp, s);
     }
-    DL_HIP(hipMemcpyAsync(hflags + 2, quad_scan + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    DL_HIP(hipMemcpyAsync(hflags + 4, work.offsets.as<uint32_t>() + nthr, sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-    DL_HIP(hipStreamSynchronize(s));
-    const uint64_t npos = hflags[4];
-    const uint64_t nquads = hflags[2];
-    const uint64_t ncand = 2 * npos;   // two slots per position
-    // Distinct triplets <= 2 per position (a quad adds two blocked ones): at
-    // most half full.
-    const uint64_t table_cap = next_pow2(2 * (npos + 2 * nquads) + 1024);
-    work.cand.reserve(sizeof(Cand) * std::max<uint64_t>(ncand, 1));
+    // Capacities: exact (read back the counts) or speculative (from the seed
+    // count: positions ~ triangles ~ 2 per seed; checked on the device).
+    uint64_t pos_cap, table_cap;
+    if (speculative) {
+        const uint64_t n = static_cast<uint64_t>(plan.n_seeds);
+        pos_cap = 3 * n + 4096;
+        table_cap = next_pow2(4 * n + 4096);
+        if (work.spec_tiny) {   // test hook: capacities certain to overflow
+            pos_cap = 1;
+            table_cap = 1024;
+        }
+    } else {
+        DL_HIP(hipMemcpyAsync(hflags + 2, quad_scan + nblocks, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        DL_HIP(hipMemcpyAsync(hflags + 4, work.offsets.as<uint32_t>() + nthr, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        const uint64_t npos = hflags[4], nquads = hflags[2];
+        pos_cap = std::max<uint64_t>(npos, 1);
+        // distinct triplets <= 2 per position (a quad adds two blocked ones):
+        // at most half full
+        table_cap = next_pow2(2 * (npos + 2 * nquads) + 1024);
+        res.n_quads = static_cast<int64_t>(nquads);
+    }
+    A.pos_cap = static_cast<uint32_t>(std::min<uint64_t>(pos_cap, 0xFFFFFFF0ull / 2));
+    A.rows_cap = 2 * A.pos_cap;
+    work.rows_cap = A.rows_cap;
+    work.cand.reserve(sizeof(Cand) * 2 * static_cast<size_t>(A.pos_cap));
     work.table_tag.reserve(sizeof(Slot) * table_cap);
-    work.positions.reserve(sizeof(uint2) * std::max<uint64_t>(npos, 1));
+    work.positions.reserve(sizeof(uint2) * static_cast<size_t>(A.pos_cap));
+    work.rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(A.rows_cap));
     A.positions = work.positions.as<uint2>();
-    if (npos > 0) {
+    {
         ProfScope ps(kKDetect, s);
+        det_bounds_kernel<<<1, 64, 0, s>>>(work.offsets.as<uint32_t>() + nthr, quad_scan + nblocks,
+                                           A.pos_cap, table_cap, dflags + 6, dflags);
+        DL_LAUNCH_CHECK();
         detect_collect_kernel<<<static_cast<unsigned>((nthr + 255) / 256), 256, 0, s>>>(A, nthr);
         DL_LAUNCH_CHECK();
     }
@@ -913,6 +1008,11 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     uint64_t salt = 0x243F6A8885A308D3ull;
     const int img_bits = n_img > 1 ? bits_for(n_img - 1) : 0;
     const int exact = 3 * plan.max_seed_bits + img_bits <= 63 ? 1 : 0;
+    // grid-stride launches sized for the capacity, at most 16 blocks per CU
+    const unsigned pblocks = static_cast<unsigned>(
+        std::min<uint64_t>((A.pos_cap + kDetThreads - 1) / kDetThreads, 4096));
+    const unsigned cblocks = static_cast<unsigned>(
+        std::min<uint64_t>((2ull * A.pos_cap + 255) / 256, 4096));
     for (int attempt = 0; attempt < 4; ++attempt) {
         Table t;
         t.slot = work.table_tag.as<Slot>();
@@ -931,52 +1031,80 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         }
         {
             ProfScope ps(kKDetect, s);
-            if (npos > 0) {
-                detect_insert_kernel<<<static_cast<unsigned>((npos + kDetThreads - 1) / kDetThreads),
-                                       kDetThreads, 0, s>>>(A, static_cast<uint32_t>(npos));
-                DL_LAUNCH_CHECK();
-            }
+            detect_insert_kernel<<<pblocks, kDetThreads, 0, s>>>(A);
+            DL_LAUNCH_CHECK();
         }
-        const unsigned cblocks = static_cast<unsigned>(std::max<uint64_t>(1, (ncand + 255) / 256));
         {
             ProfScope ps(kKRegister, s);
-            mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, A.D, bm,
-                                                dflags);
+            mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.npos, t, A.D, bm, dflags);
             DL_LAUNCH_CHECK();
             popcount_scan(bm, nwords, pre, work.scan_tmp, s);
             tri_base_kernel<<<static_cast<unsigned>((n_img + 256) / 256), 256, 0, s>>>(
                 A.D, n_img, pre, nwords, tri_base);
             DL_LAUNCH_CHECK();
         }
-        int64_t* htb = reinterpret_cast<int64_t*>(hflags + 8);
-        DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        DL_HIP(hipMemcpyAsync(htb, tri_base, sizeof(int64_t) * (n_img + 1), hipMemcpyDeviceToHost, s));
-        DL_HIP(hipStreamSynchronize(s));
-        const uint32_t f = hflags[0];
-        if (f & kFlagBadId) {
-            if (labels) throw runtime_error("flood fill left an undefined cell");
-            throw value_error("voronoi_grid holds a seed id outside [0, n_seeds)");
+        if (!speculative) {
+            int64_t* htb = reinterpret_cast<int64_t*>(hflags + 8);
+            DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            DL_HIP(hipMemcpyAsync(htb, tri_base, sizeof(int64_t) * (n_img + 1),
+                                  hipMemcpyDeviceToHost, s));
+            DL_HIP(hipStreamSynchronize(s));
+            const uint32_t f = hflags[0];
+            if (f & kFlagBadId) {
+                if (labels) throw runtime_error("flood fill left an undefined cell");
+                throw value_error("voronoi_grid holds a seed id outside [0, n_seeds)");
+            }
+            if (f & (kFlagTableOverflow | kFlagOverflow))
+                throw runtime_error("triangle registration table overflow");
+            if (f & kFlagCollision) {
+                salt = salt * 6364136223846793005ull + 1442695040888963407ull;
+                continue;
+            }
+            work.tri_base.assign(htb, htb + n_img + 1);
+            res.n_tri = htb[n_img];
+        } else {
+            res.n_tri = -1;   // known on the device only (tri_base[n_img])
         }
-        if (f & kFlagTableOverflow) throw runtime_error("triangle registration table overflow");
-        if (f & kFlagCollision) {
-            salt = salt * 6364136223846793005ull + 1442695040888963407ull;
-            continue;
-        }
-        work.tri_base.assign(htb, htb + n_img + 1);
-        const int64_t T = htb[n_img];
-        work.rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(std::max<int64_t>(T, 1)));
         {
             ProfScope ps(kKRegister, s);
-            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, static_cast<uint32_t>(ncand), t, A.D, bm,
-                                                pre, work.rows.as<int32_t>());
+            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.npos, t, A.D, bm, pre,
+                                                work.rows.as<int32_t>(), A.rows_cap, dflags);
             DL_LAUNCH_CHECK();
         }
-        res.n_tri = T;
-        res.n_quads = static_cast<int64_t>(nquads);
-        res.n_cand = static_cast<int64_t>(ncand);
         return res;
     }
     throw runtime_error("triangle registration hash collided repeatedly");
+}
+
+SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
+                              hipStream_t s)
+{
+    const int n_img = static_cast<int>(plan.img.size());
+    const size_t bytes = 64 + sizeof(int64_t) * (n_img + 1) + sizeof(int) * std::max(n_esc, 1);
+    work.host_flags.reserve(bytes);
+    char* h = static_cast<char*>(work.host_flags.ptr);
+    uint32_t* hf = reinterpret_cast<uint32_t*>(h);
+    int64_t* htb = reinterpret_cast<int64_t*>(h + 64);
+    int* hesc = reinterpret_cast<int*>(h + 64 + sizeof(int64_t) * (n_img + 1));
+    DL_HIP(hipMemcpyAsync(hf, work.flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(htb, seg_ptr(work, 3, n_img), sizeof(int64_t) * (n_img + 1),
+                          hipMemcpyDeviceToHost, s));
+    hf[2] = 0;
+    if (plan.n_tiles > 0 && work.list_cap > 0)
+        DL_HIP(hipMemcpyAsync(hf + 2, work.bin_offsets.as<uint32_t>() + plan.n_tiles,
+                              sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (d_esc && n_esc > 0)
+        DL_HIP(hipMemcpyAsync(hesc, d_esc, sizeof(int) * n_esc, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    SpeculativeStatus st;
+    const uint32_t f = hf[0];
+    st.bad_id = (f & kFlagBadId) != 0;
+    st.redo = (f & (kFlagOverflow | kFlagTableOverflow | kFlagCollision)) != 0 ||
+              static_cast<int64_t>(hf[2]) > work.list_cap;
+    st.n_tri = htb[n_img];
+    work.tri_base.assign(htb, htb + n_img + 1);
+    st.escaped.assign(hesc, hesc + (d_esc ? n_esc : 0));
+    return st;
 }
 
 // ===========================================================================
@@ -1138,42 +1266,50 @@ __device__ __forceinline__ void row_span(const EdgeRows& E, double y, double& lo
     }
 }
 
-__global__ void bin_count_kernel(const int32_t* __restrict__ rows, int64_t T,
+// Triangles t < min(device count, rows_cap) (the rows buffer's capacity).
+__global__ void bin_count_kernel(const int32_t* __restrict__ rows, int64_t rows_cap,
                                  const int64_t* __restrict__ tri_base, int n_img,
                                  const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
                                  uint32_t* __restrict__ counts)
 {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
-    const ImageDesc& d = D[img];
-    const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
-    int x0, y0, x1, y1;
-    if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) return;
-    uint32_t* c = counts + d.tile_off;
-    for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
-        for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) atomicAdd(&c[ty * d.tiles_x + tx], 1u);
+    const int64_t T = min(tri_base[n_img], rows_cap);
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
+        const ImageDesc& d = D[img];
+        const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
+        int x0, y0, x1, y1;
+        if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) continue;
+        uint32_t* c = counts + d.tile_off;
+        for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
+            for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) atomicAdd(&c[ty * d.tiles_x + tx], 1u);
+    }
 }
 
-__global__ void bin_fill_kernel(const int32_t* __restrict__ rows, int64_t T,
+// List entries beyond cap are dropped and flagged (speculative sizing).
+__global__ void bin_fill_kernel(const int32_t* __restrict__ rows, int64_t rows_cap,
                                 const int64_t* __restrict__ tri_base, int n_img,
                                 const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
                                 const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
-                                uint32_t* __restrict__ list)
+                                uint32_t* __restrict__ list, uint32_t cap,
+                                uint32_t* __restrict__ flags)
 {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
-    const ImageDesc& d = D[img];
-    const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
-    int x0, y0, x1, y1;
-    if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) return;
-    for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
-        for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) {
-            const int64_t tile = d.tile_off + ty * d.tiles_x + tx;
-            const uint32_t pos = atomicAdd(&cursor[tile], 1u);
-            list[offsets[tile] + pos] = static_cast<uint32_t>(t);
-        }
+    const int64_t T = min(tri_base[n_img], rows_cap);
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
+        const ImageDesc& d = D[img];
+        const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
+        int x0, y0, x1, y1;
+        if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) continue;
+        for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
+            for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) {
+                const int64_t tile = d.tile_off + ty * d.tiles_x + tx;
+                const uint32_t k = offsets[tile] + atomicAdd(&cursor[tile], 1u);
+                if (k < cap) list[k] = static_cast<uint32_t>(t);
+                else atomicOr(flags, kFlagOverflow);
+            }
+    }
 }
 
 template <int C>
@@ -1181,7 +1317,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int64_t* __restrict__ tile_seg, const int64_t* __restrict__ tri_base, int n_img,
     const ImageDesc* __restrict__ D, const int32_t* __restrict__ rows,
     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ list, const int32_t* __restrict__ rank_at)
+    const uint32_t* __restrict__ list, const int32_t* __restrict__ rank_at, uint32_t cap)
 {
     __shared__ int32_t tid_tile[kTS * kTS];
     const int64_t gt = blockIdx.x;
@@ -1196,7 +1332,8 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int64_t t0 = tri_base[img];
     for (int i = threadIdx.x; i < kTS * kTS; i += blockDim.x) tid_tile[i] = -1;
     __syncthreads();
-    const uint32_t b = offsets[gt], e = offsets[gt + 1];
+    // an overflowed list (speculative sizing) is redone by the host
+    const uint32_t b = min(offsets[gt], cap), e = min(offsets[gt + 1], cap);
     for (uint32_t k = b + threadIdx.x; k < e; k += blockDim.x) {
         const uint32_t t = list[k];
         const TriVerts v = load_tri(rows, srt, t);
@@ -1292,10 +1429,11 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
 
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
                   const uint32_t* sorted, const int32_t* rank_at, int C, TriWork& work,
-                  hipStream_t s)
+                  hipStream_t s, bool speculative)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t ntiles = plan.n_tiles;
+    work.list_cap = 0;
     if (ntiles == 0) return;
     work.bin_counts.reserve(sizeof(uint32_t) * static_cast<size_t>(2 * ntiles + 1));
     work.bin_offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(ntiles + 1));
@@ -1306,39 +1444,54 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
     const ImageDesc* D = work.desc.as<ImageDesc>();
     const int64_t* tri_base = seg_ptr(work, 3, n_img);
     const int64_t* tile_seg = seg_ptr(work, 2, n_img);
-    uint32_t total = 0;
+    // Triangle count: exact, or on the device (speculative: bounded by rows_cap).
+    const int64_t rows_cap = static_cast<int64_t>(work.rows_cap);
+    const int64_t t_bound = n_tri >= 0 ? std::min(n_tri, rows_cap) : rows_cap;
+    const unsigned tblocks = static_cast<unsigned>(std::max<int64_t>(
+        1, std::min<int64_t>((t_bound + 255) / 256, 8192)));
     {
         ProfScope ps(kKAssign, s);
         fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
-        if (n_tri > 0) {
-            bin_count_kernel<<<static_cast<unsigned>((n_tri + 255) / 256), 256, 0, s>>>(
-                rows, n_tri, tri_base, n_img, D, sorted, counts);
+        if (t_bound > 0) {
+            bin_count_kernel<<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
+                                                     counts);
             DL_LAUNCH_CHECK();
         }
         exclusive_scan_u32(counts, ntiles, offs, work.scan_tmp, s);
     }
-    if (n_tri > 0) {
+    int64_t cap;
+    if (speculative) {
+        // a small triangle's box covers up to four tiles
+        cap = work.spec_tiny ? 1 : 4 * t_bound + ntiles + 1024;
+    } else {
         DL_HIP(hipMemcpyAsync(work.host_flags.ptr, offs + ntiles, sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
-        total = work.host_flags.as<uint32_t>()[0];
+        cap = std::max<int64_t>(work.host_flags.as<uint32_t>()[0], 1);
     }
-    work.bin_list.reserve(sizeof(uint32_t) * std::max<uint32_t>(total, 1));
+    cap = std::min<int64_t>(cap, 0xFFFFFFF0ll);
+    work.list_cap = cap;
+    work.bin_list.reserve(sizeof(uint32_t) * static_cast<size_t>(cap));
     {
         ProfScope ps(kKAssign, s);
-        if (total > 0) {
-            bin_fill_kernel<<<static_cast<unsigned>((n_tri + 255) / 256), 256, 0, s>>>(
-                rows, n_tri, tri_base, n_img, D, sorted, offs, cursor, work.bin_list.as<uint32_t>());
+        if (t_bound > 0) {
+            bin_fill_kernel<<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted, offs,
+                                                    cursor,
+                                                    work.bin_list.as<uint32_t>(),
+                                                    static_cast<uint32_t>(cap),
+                                                    work.flags.as<uint32_t>());
             DL_LAUNCH_CHECK();
         }
     }
     ProfScope ps(kKOutput, s);
     if (C == 3)
         tile_assign_kernel<3><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at);
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
+            static_cast<uint32_t>(cap));
     else
         tile_assign_kernel<1><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at);
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
+            static_cast<uint32_t>(cap));
     DL_LAUNCH_CHECK();
 }
 

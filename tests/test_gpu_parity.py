@@ -267,3 +267,33 @@ def test_fill_watch_on_padded_canvases(cfg):
         assert flagged <= cases // 2, f"{flagged} of {cases} padded fills flagged"
     finally:
         _lib.check(lib.dl_set_bfs_config(ctx.handle, 0, 0, 0))
+
+
+def test_speculative_sizes_overflowing_fall_back_exactly():
+    """Detection and assignment are sized from the seed count without a host
+    round trip; when the sizes are too small the pipeline reruns with exact
+    ones.  Forcing that path must not change any result."""
+    ctx = _lib.context()
+    lib = _lib.load()
+    rng = np.random.default_rng(44)
+    cases = []
+    for W, H, n in [(300, 200, 900), (64, 64, 3), (512, 40, 2000), (17, 300, 40)]:
+        idx = rng.choice(W * H, size=n, replace=False)
+        cases.append((W, H, np.stack([idx % W, idx // W], 1).astype(np.int32)))
+    base = [dl.triangulate(W, H, s) for W, H, s in cases]
+    batch = dl.triangulate_batch(cases)
+    _lib.check(lib.dl_set_tiny_capacities(ctx.handle, 1))
+    try:
+        tiny = [dl.triangulate(W, H, s) for W, H, s in cases]
+        tiny_batch = dl.triangulate_batch(cases)
+    finally:
+        _lib.check(lib.dl_set_tiny_capacities(ctx.handle, 0))
+    for (m0, g0), (m1, g1), (m2, g2), (m3, g3) in zip(base, tiny, batch, tiny_batch):
+        assert m0 == m1 == m2 == m3
+        np.testing.assert_array_equal(g0, g1)
+        np.testing.assert_array_equal(g0, g2)
+        np.testing.assert_array_equal(g0, g3)
+    for (W, H, s), (m, g) in zip(cases, base):
+        rows, ref = O.triangulate(W, H, s, None, return_rows=True)
+        np.testing.assert_array_equal(g, ref)
+        np.testing.assert_array_equal(_rows(m), rows)
