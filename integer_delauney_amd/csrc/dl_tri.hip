@@ -162,12 +162,13 @@ void exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, DevBuf& tm
 // ===========================================================================
 namespace {
 
-// Detection thread blocks cover kDetChunk consecutive 2x2 blocks of one row of
-// blocks; each thread owns kDetItems of them, strided by the thread count so
-// that a wave reads consecutive cells.
+// Detection thread blocks cover kDetChunk consecutive 2x2 blocks in each of
+// kDetRows rows of blocks; each thread owns kDetItems of them per row, strided
+// by the thread count so that a wave reads consecutive cells.
 constexpr int kDetThreads = 256;
 constexpr int kDetItems = 4;
 constexpr int kDetChunk = kDetThreads * kDetItems;
+constexpr int kDetRows = 8;
 constexpr int kTileSide = 64;   // assignment tile side in pixels
 
 // Largest i in [0, n) with off[i] <= g (off is non-decreasing, off[0] = 0).
@@ -214,7 +215,7 @@ void plan_batch(BatchPlan& plan)
         d.nb = (bw > 0 && bh > 0) ? static_cast<int64_t>(bw) * bh : 0;
         d.bx_per_row = d.nb ? (bw + kDetChunk - 1) / kDetChunk : 0;
         d.det_off = det;
-        det += d.nb ? static_cast<int64_t>(d.bx_per_row) * bh : 0;
+        det += d.nb ? static_cast<int64_t>(d.bx_per_row) * ((bh + kDetRows - 1) / kDetRows) : 0;
         d.key_off = keyw;
         keyw += (4 * d.nb + 31) / 32;
         d.tiles_x = (d.W + kTileSide - 1) / kTileSide;
@@ -560,11 +561,9 @@ struct DetectArgs {
     int n_img;
     const int32_t* rank_at;       // image-space id maps (labels mode)
     const uint32_t* sorted;       // id -> packed image position, per image
-    uint32_t* counts;             // per-thread candidate counts (count pass)
-    uint32_t* block_quads;        // per-thread-block quad counts (count pass)
-    uint8_t* masks;               // per-thread masks of items holding candidates
-    uint2* positions;             // compact list (image, block) of those blocks
-    const uint32_t* offsets;      // exclusive prefix sums of counts
+    uint2* positions;             // list of (image, block) of blocks holding candidates
+    uint32_t* n_listed;           // positions listed so far (device counter)
+    uint32_t* n_quads;            // four-region blocks seen (device counter)
     Cand* cand;
     uint32_t* flags;              // batch-wide status word
     Table table;
@@ -628,66 +627,67 @@ __device__ __forceinline__ uint32_t block_candidates(const BlockCorners& c)
     return (c.v[0] | c.v[1] | c.v[2] | c.v[3]) ? 1 : 0;
 }
 
-__global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
+// One pass over the canvas: each thread block finds the 2x2 blocks holding
+// candidates in its kDetRows x kDetChunk region, reserves room for them with a
+// single atomic add and writes their positions.  The list's order does not
+// matter: registration order comes from the keys, first registrations from the
+// table's minimum key.
+__global__ __launch_bounds__(kDetThreads) void detect_list_kernel(DetectArgs A)
 {
+    __shared__ uint32_t part[kDetThreads];
+    __shared__ uint32_t base_s, quads_s;
     const int64_t gb = blockIdx.x;
     const int img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
     const ImageDesc& d = A.D[img];
     const int64_t lb = gb - A.det_seg[img];
-    const int ry = static_cast<int>(lb / d.bx_per_row);
-    const int bx = static_cast<int>(lb - static_cast<int64_t>(ry) * d.bx_per_row);
+    const int ry0 = static_cast<int>(lb / d.bx_per_row) * kDetRows;
+    const int bx = static_cast<int>(lb - static_cast<int64_t>(ry0 / kDetRows) * d.bx_per_row);
     const int rx0 = bx * kDetChunk + threadIdx.x;
-    const int bw = d.Wc - 1;
+    const int bw = d.Wc - 1, bh = d.Hc - 1;
     const uint32_t* g = A.grid + d.cv_off;
-    uint32_t mask = 0, quads = 0;
+    uint32_t mask = 0, quads = 0;   // bit r * kDetItems + j
 #pragma unroll
-    for (int j = 0; j < kDetItems; ++j) {
-        const int rx = rx0 + j * kDetThreads;
-        if (rx >= bw) break;
-        const BlockCorners c = block_corners(g, d.Wc, rx, ry);
-        if (block_candidates(c)) mask |= 1u << j;
-        quads += c.quad;
+    for (int r = 0; r < kDetRows; ++r) {
+        const int ry = ry0 + r;
+        if (ry >= bh) break;
+#pragma unroll
+        for (int j = 0; j < kDetItems; ++j) {
+            const int rx = rx0 + j * kDetThreads;
+            if (rx >= bw) break;
+            const BlockCorners c = block_corners(g, d.Wc, rx, ry);
+            if (block_candidates(c)) mask |= 1u << (r * kDetItems + j);
+            quads += c.quad;
+        }
     }
-    A.counts[gb * kDetThreads + threadIdx.x] = static_cast<uint32_t>(__popc(mask));
-    A.masks[gb * kDetThreads + threadIdx.x] = static_cast<uint8_t>(mask);
-    // Quads per thread block (a single counter would serialise ~10^4 atomics).
-    __shared__ uint32_t red[kDetThreads / 64];
-    for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = quads;
+    // exclusive scan of the per-thread counts over the block
+    const uint32_t cnt = static_cast<uint32_t>(__popc(mask));
+    part[threadIdx.x] = cnt;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int w = 0; w < kDetThreads / 64; ++w) t += red[w];
-        A.block_quads[gb] = t;
+    for (int off = 1; off < kDetThreads; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
     }
-}
-
-// Compact list of the block positions that hold candidates, from the count
-// pass's per-thread masks: the insert pass then runs one thread per position.
-__global__ __launch_bounds__(kDetThreads) void detect_collect_kernel(DetectArgs A, int64_t nthr)
-{
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nthr) return;
-    uint32_t mask = A.masks[t];
-    if (!mask) return;
-    uint32_t out = A.offsets[t];
-    const int64_t gb = t / kDetThreads;
-    const int tid = static_cast<int>(t - gb * kDetThreads);
-    const int img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
-    const ImageDesc& d = A.D[img];
-    const int64_t lb = gb - A.det_seg[img];
-    const int ry = static_cast<int>(lb / d.bx_per_row);
-    const int bx = static_cast<int>(lb - static_cast<int64_t>(ry) * d.bx_per_row);
-    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
+    for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
+    if (threadIdx.x == 0) quads_s = 0;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && quads) atomicAdd(&quads_s, quads);
+    if (threadIdx.x == kDetThreads - 1) base_s = part[kDetThreads - 1] ? atomicAdd(A.n_listed, part[kDetThreads - 1]) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0 && quads_s) atomicAdd(A.n_quads, quads_s);
+    uint32_t out = base_s + part[threadIdx.x] - cnt;
+    const uint32_t bwu = static_cast<uint32_t>(bw);
     while (mask) {
-        const int j = __ffs(mask) - 1;
+        const int b = __ffs(mask) - 1;
         mask &= mask - 1;
-        const uint32_t rx = static_cast<uint32_t>(bx * kDetChunk + j * kDetThreads + tid);
-        if (out >= A.pos_cap) {   // capacity too small: flagged, the host reruns exactly
+        if (out >= A.pos_cap) {   // capacity too small: flagged, the host reruns
             atomicOr(A.flags, kFlagOverflow);
             return;
         }
-        A.positions[out++] = make_uint2(static_cast<uint32_t>(img), static_cast<uint32_t>(ry) * bw + rx);
+        const uint32_t ry = static_cast<uint32_t>(ry0 + b / kDetItems);
+        const uint32_t rx = static_cast<uint32_t>(rx0 + (b % kDetItems) * kDetThreads);
+        A.positions[out++] = make_uint2(static_cast<uint32_t>(img), ry * bwu + rx);
     }
 }
 
@@ -922,8 +922,9 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     work.tri_base.assign(n_img + 1, 0);
     work.rows_cap = 0;
     const int64_t nblocks = plan.n_det_blocks;
-    // flags layout: [0] status bits, [2] quad count, [4] candidate positions,
-    // [6] positions listed (device count read by the later kernels).
+    // flags layout: [0] status bits, [2] four-region blocks (counter),
+    // [4] candidate positions found (counter), [6] positions in the list
+    // (min of [4] and the capacity; read by the later kernels).
     {
         ProfScope ps(kKDetect, s);
         zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
@@ -934,10 +935,6 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         fill_u32(reinterpret_cast<uint32_t*>(tri_base), 0u, 2 * static_cast<size_t>(n_img + 1), s);
         return res;
     }
-    const int64_t nthr = nblocks * kDetThreads;
-    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 2 * (nblocks + 1)) +
-                        static_cast<size_t>(nthr));
-    work.offsets.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 1));
     DetectArgs A{};
     A.grid = grid_base;
     A.labels = labels ? 1 : 0;
@@ -946,58 +943,53 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     A.n_img = n_img;
     A.rank_at = rank_at;
     A.sorted = sorted;
-    A.counts = work.counts.as<uint32_t>();
-    A.offsets = work.offsets.as<uint32_t>();
-    A.block_quads = work.counts.as<uint32_t>() + nthr;
-    uint32_t* quad_scan = A.block_quads + nblocks + 1;
-    A.masks = reinterpret_cast<uint8_t*>(quad_scan + nblocks + 1);
     A.flags = dflags;
+    A.n_quads = dflags + 2;
+    A.n_listed = dflags + 4;
     A.npos = dflags + 6;
-    {
-        ProfScope ps(kKDetect, s);
-        detect_count_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
-        DL_LAUNCH_CHECK();
-        exclusive_scan_u32(A.counts, nthr, work.offsets.as<uint32_t>(), work.scan_tmp, s);
-        exclusive_scan_u32(A.block_quads, nblocks, quad_scan, work.scan_tmp, s);
+    // Capacities: from the seed count (positions ~ triangles ~ 2 per seed),
+    // checked on the device; in exact mode the list is rerun when short and
+    // the table sized from the counts read back.
+    const uint64_t n_seeds = static_cast<uint64_t>(plan.n_seeds);
+    uint64_t pos_cap = 3 * n_seeds + 4096;
+    uint64_t table_cap = next_pow2(4 * n_seeds + 4096);
+    if (speculative && work.spec_tiny) {   // test hook: capacities certain to overflow
+        pos_cap = 1;
+        table_cap = 1024;
     }
-    // Capacities: exact (read back the counts) or speculative (from the seed
-    // count: positions ~ triangles ~ 2 per seed; checked on the device).
-    uint64_t pos_cap, table_cap;
-    if (speculative) {
-        const uint64_t n = static_cast<uint64_t>(plan.n_seeds);
-        pos_cap = 3 * n + 4096;
-        table_cap = next_pow2(4 * n + 4096);
-        if (work.spec_tiny) {   // test hook: capacities certain to overflow
-            pos_cap = 1;
-            table_cap = 1024;
+    for (int pass = 0;; ++pass) {
+        A.pos_cap = static_cast<uint32_t>(std::min<uint64_t>(pos_cap, 0x7FFFFFF0ull));
+        work.positions.reserve(sizeof(uint2) * static_cast<size_t>(A.pos_cap));
+        A.positions = work.positions.as<uint2>();
+        {
+            ProfScope ps(kKDetect, s);
+            if (pass > 0) zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
+            detect_list_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
+            DL_LAUNCH_CHECK();
         }
-    } else {
-        DL_HIP(hipMemcpyAsync(hflags + 2, quad_scan + nblocks, sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
-        DL_HIP(hipMemcpyAsync(hflags + 4, work.offsets.as<uint32_t>() + nthr, sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
+        if (speculative) break;
+        DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
         const uint64_t npos = hflags[4], nquads = hflags[2];
-        pos_cap = std::max<uint64_t>(npos, 1);
+        if (npos > pos_cap) {   // list was short: rerun with room for every position
+            pos_cap = npos;
+            continue;
+        }
         // distinct triplets <= 2 per position (a quad adds two blocked ones):
         // at most half full
         table_cap = next_pow2(2 * (npos + 2 * nquads) + 1024);
         res.n_quads = static_cast<int64_t>(nquads);
+        break;
     }
-    A.pos_cap = static_cast<uint32_t>(std::min<uint64_t>(pos_cap, 0xFFFFFFF0ull / 2));
     A.rows_cap = 2 * A.pos_cap;
     work.rows_cap = A.rows_cap;
     work.cand.reserve(sizeof(Cand) * 2 * static_cast<size_t>(A.pos_cap));
     work.table_tag.reserve(sizeof(Slot) * table_cap);
-    work.positions.reserve(sizeof(uint2) * static_cast<size_t>(A.pos_cap));
     work.rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(A.rows_cap));
-    A.positions = work.positions.as<uint2>();
     {
         ProfScope ps(kKDetect, s);
-        det_bounds_kernel<<<1, 64, 0, s>>>(work.offsets.as<uint32_t>() + nthr, quad_scan + nblocks,
-                                           A.pos_cap, table_cap, dflags + 6, dflags);
-        DL_LAUNCH_CHECK();
-        detect_collect_kernel<<<static_cast<unsigned>((nthr + 255) / 256), 256, 0, s>>>(A, nthr);
+        det_bounds_kernel<<<1, 64, 0, s>>>(A.n_listed, A.n_quads, A.pos_cap, table_cap, dflags + 6,
+                                           dflags);
         DL_LAUNCH_CHECK();
     }
     const int64_t nwords = plan.n_key_words;
