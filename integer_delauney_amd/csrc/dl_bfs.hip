@@ -114,6 +114,11 @@ __device__ __forceinline__ uint32_t from_right(uint32_t v)  // lane l <- lane l+
 // Every mask is zero outside watched tiles.
 using GU32 = __attribute__((address_space(1))) uint32_t;
 
+// Tile change-mask bits: the interior changed somewhere / in its K-wide left,
+// right, top or bottom band / in one of its K x K corners.
+constexpr int kChAny = 1, kChLeft = 2, kChRight = 4, kChTop = 8, kChBottom = 16;
+constexpr int kChTL = 32, kChTR = 64, kChBL = 128, kChBR = 256;
+
 struct Watch {
     uint64_t l0, l1;     // lanes whose column 0 / 1 is the interior's first column
     uint64_t r0, r1;     // ... last column
@@ -285,14 +290,20 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
 
-    // Activity: this tile or one of its 8 neighbours changed last launch.
+    // Activity: a launch reads the tile's interior and a K-cell halo, so it
+    // can change something only if last launch changed a cell of the interior
+    // or of that halo -- the band (or corner) of a neighbour's interior that
+    // faces this tile.  Tile flags are change masks (see the write-back).
     // Every wave reads the nine flags itself, so no barrier is needed.
     if (!first_launch) {
         int f = 0;
         if (lane < 9) {
             const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
+            // bit of neighbour (dx, dy) = (lane % 3 - 1, lane / 3 - 1) that faces us
+            constexpr int kFacing[9] = {kChBR, kChBottom, kChBL, kChRight, kChAny,
+                                        kChLeft, kChTR, kChTop, kChTL};
             if (nx >= 0 && ny >= 0 && nx < tiles_x && ny < tiles_y)
-                f = flags_prev[tile_base + ny * tiles_x + nx];
+                f = flags_prev[tile_base + ny * tiles_x + nx] & kFacing[lane];
         }
         if (__ballot(f != 0) == 0ull) {
             if (tid == 0) flags_cur[tile] = 0;
@@ -358,40 +369,66 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     int yb = y0;
     int Wb = W;
     asm volatile("" : "+s"(yb), "+s"(Wb));
-    bool changed = false;
+    // Change mask: which of the interior, its four K-wide border bands and
+    // its four K x K corners changed (the parts that neighbours' halos read).
     const int lx0 = 2 * lane - K;   // interior-relative column of column 0
     const bool w0 = lx0 >= 0 && lx0 < IW && c0 < Wb;
     const bool w1 = lx0 + 1 >= 0 && lx0 + 1 < IW && c0 + 1 < Wb;
+    bool ch0 = false, ch1 = false;      // column 0 / 1 changed in some row
+    bool top0 = false, top1 = false;    // ... in a row of the top band
+    bool bot0 = false, bot1 = false;    // ... in a row of the bottom band
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ly = wave * R + r - K;
         const int y = yb + r;
         if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
         const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
+        bool c0r = false, c1r = false;
         if (w0) {
             const size_t i = rowoff + c0;
             const uint32_t v = inv_of(K0[r]) == kFarInv ? kInvalid : ~inv_of(K0[r]);
-            changed |= src[i] != v;
+            c0r = src[i] != v;
             dst[i] = v;
         }
         if (w1) {
             const size_t i = rowoff + c0 + 1;
             const uint32_t v = inv_of(K1[r]) == kFarInv ? kInvalid : ~inv_of(K1[r]);
-            changed |= src[i] != v;
+            c1r = src[i] != v;
             dst[i] = v;
+        }
+        ch0 |= c0r;
+        ch1 |= c1r;
+        if (ly < K) {   // wave-uniform
+            top0 |= c0r;
+            top1 |= c1r;
+        }
+        if (ly >= IH - K) {
+            bot0 |= c0r;
+            bot1 |= c1r;
         }
     }
     if (esc && lane == 0 && cv.escape) atomicOr(cv.escape, 1);   // esc is wave-uniform
+    const bool l0 = lx0 < K, l1 = lx0 + 1 < K;                  // left band columns
+    const bool r0 = lx0 >= IW - K, r1 = lx0 + 1 >= IW - K;      // right band columns
+    int m = 0;
+    if (__ballot(ch0 || ch1)) m |= kChAny;
+    if (__ballot((ch0 && l0) || (ch1 && l1))) m |= kChLeft;
+    if (__ballot((ch0 && r0) || (ch1 && r1))) m |= kChRight;
+    if (__ballot(top0 || top1)) m |= kChTop;
+    if (__ballot(bot0 || bot1)) m |= kChBottom;
+    if (__ballot((top0 && l0) || (top1 && l1))) m |= kChTL;
+    if (__ballot((top0 && r0) || (top1 && r1))) m |= kChTR;
+    if (__ballot((bot0 && l0) || (bot1 && l1))) m |= kChBL;
+    if (__ballot((bot0 && r0) || (bot1 && r1))) m |= kChBR;
     __syncthreads();   // every wave is past its last read of lds_any
-    const bool wave_changed = __ballot(changed) != 0ull;   // all lanes take part
-    if (lane == 0) lds_any[0][wave] = wave_changed ? 1 : 0;
+    if (lane == 0) lds_any[0][wave] = m;
     __syncthreads();
     if (tid == 0) {
         int any = 0;
         for (int w = 0; w < NW; ++w) any |= lds_any[0][w];
         flags_cur[tile] = any;
         atomicAdd(&counters[1], 1);
-        if (any) atomicAdd(&counters[0], 1);
+        if (any & kChAny) atomicAdd(&counters[0], 1);
     }
 }
 
@@ -490,7 +527,6 @@ void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n, h
     ProfScope ps(kKInit, s);
     const size_t cells = static_cast<size_t>(cv.W) * static_cast<size_t>(cv.H);
     fill_u32(cv.lab0, kInvalid, cells, s);
-    fill_u32(cv.lab1, kInvalid, cells, s);
     if (n > 0) {
         bfs_seed_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(cv.lab0, cv.lab1,
                                                                               cv.W, d_labels, n);

@@ -115,9 +115,9 @@ inline int bfs_tile_ih(const BfsConfig& c) { return c.waves * c.rows - 2 * c.ite
 
 bool bfs_config_supported(const BfsConfig& c);
 
-// Writes kInvalid everywhere and each seed's own label at its cell, in both
-// label planes of every canvas.  seeds_canvas: packed labels already in canvas
-// coordinates.
+// Writes kInvalid everywhere and each seed's own label at its cell in the
+// first label plane (the second needs none, as for init_canvases).  d_labels:
+// packed labels already in canvas coordinates.
 void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n,
                      hipStream_t s);
 
@@ -209,8 +209,10 @@ void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s);
 void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_at,
                      TriWork& work, hipStream_t s);
 
-// Writes kInvalid over both planes of every canvas, then each seed's canvas
-// label (image label + (P, P)) at its cell.  plane_a/plane_b: n_cells each.
+// Writes kInvalid over the first plane of every canvas, then each seed's
+// canvas label (image label + (P, P)) at its cell.  plane_b needs no
+// initialisation: the flood fill's first launch visits every tile and writes
+// every cell of the second plane.  plane_a/plane_b: n_cells each.
 void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* plane_a,
                    uint32_t* plane_b, struct TriWork& work, hipStream_t s);
 

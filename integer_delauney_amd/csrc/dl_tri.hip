@@ -324,11 +324,10 @@ __global__ void seeds_rank_kernel(int64_t n_total, const int64_t* __restrict__ s
     rank_at[d.pix_off + static_cast<int64_t>(y) * d.W + x] = static_cast<int32_t>(rank);
 }
 
-// Seed cells of every canvas: the image label shifted by (P, P), in both planes.
+// Seed cells of every canvas: the image label shifted by (P, P).
 __global__ void canvas_seed_kernel(const uint32_t* __restrict__ sorted, int64_t n_total,
                                    const int64_t* __restrict__ seed_seg, int n_img,
-                                   const ImageDesc* __restrict__ D, uint32_t* __restrict__ a,
-                                   uint32_t* __restrict__ b)
+                                   const ImageDesc* __restrict__ D, uint32_t* __restrict__ a)
 {
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= n_total) return;
@@ -337,7 +336,6 @@ __global__ void canvas_seed_kernel(const uint32_t* __restrict__ sorted, int64_t 
     const uint32_t l = sorted[g] + ((static_cast<uint32_t>(d.P) << 16) | static_cast<uint32_t>(d.P));
     const int64_t c = d.cv_off + static_cast<int64_t>(l & 0xFFFFu) * d.Wc + (l >> 16);
     a[c] = l;
-    b[c] = l;
 }
 
 }  // namespace
@@ -387,11 +385,13 @@ void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* pl
 {
     const int n_img = static_cast<int>(plan.img.size());
     ProfScope ps(kKInit, s);
+    // Only the first plane: the flood fill's first launch visits every tile
+    // and writes every cell of the second plane.
+    (void)plane_b;
     fill_u32(plane_a, kInvalid, static_cast<size_t>(plan.n_cells), s);
-    fill_u32(plane_b, kInvalid, static_cast<size_t>(plan.n_cells), s);
     canvas_seed_kernel<<<static_cast<unsigned>((plan.n_seeds + 255) / 256), 256, 0, s>>>(
         d_sorted, plan.n_seeds, seg_ptr(work, 0, n_img), n_img, work.desc.as<ImageDesc>(),
-        plane_a, plane_b);
+        plane_a);
     DL_LAUNCH_CHECK();
 }
 
