@@ -562,8 +562,10 @@ struct DetectArgs {
     const int32_t* rank_at;       // image-space id maps (labels mode)
     const uint32_t* sorted;       // id -> packed image position, per image
     uint2* positions;             // list of (image, block) of blocks holding candidates
-    uint32_t* n_listed;           // positions listed so far (device counter)
-    uint32_t* n_quads;            // four-region blocks seen (device counter)
+    uint32_t* masks;              // per thread: its candidate blocks (count pass)
+    uint32_t* block_counts;       // per thread block: candidate blocks / four-region
+    uint32_t* block_quads;        //   blocks (count pass)
+    const uint32_t* block_offsets;   // exclusive scan of block_counts
     Cand* cand;
     uint32_t* flags;              // batch-wide status word
     Table table;
@@ -598,14 +600,13 @@ struct BlockCorners {
     bool quad;
 };
 
-__device__ __forceinline__ BlockCorners block_corners(const uint32_t* g, int Wc, int rx, int ry)
+__device__ __forceinline__ BlockCorners corners_of(uint32_t tl, uint32_t tr, uint32_t bl, uint32_t br)
 {
     BlockCorners c;
-    const size_t o = static_cast<size_t>(ry) * Wc + rx;
-    c.tl = g[o];
-    c.tr = g[o + 1];
-    c.bl = g[o + Wc];
-    c.br = g[o + Wc + 1];
+    c.tl = tl;
+    c.tr = tr;
+    c.bl = bl;
+    c.br = br;
     const bool e_tl_tr = c.tl == c.tr, e_tl_bl = c.tl == c.bl, e_tl_br = c.tl == c.br;
     const bool e_tr_bl = c.tr == c.bl, e_tr_br = c.tr == c.br, e_bl_br = c.bl == c.br;
     c.v[0] = !e_tl_tr && !e_tl_br && !e_tr_br;   // (TL, TR, BR)
@@ -614,6 +615,12 @@ __device__ __forceinline__ BlockCorners block_corners(const uint32_t* g, int Wc,
     c.v[3] = !e_bl_br && !e_tr_bl && !e_tr_br;   // (BL, BR, TR)
     c.quad = c.v[0] && c.v[1] && c.v[2] && c.v[3];
     return c;
+}
+
+__device__ __forceinline__ BlockCorners block_corners(const uint32_t* g, int Wc, int rx, int ry)
+{
+    const size_t o = static_cast<size_t>(ry) * Wc + rx;
+    return corners_of(g[o], g[o + 1], g[o + Wc], g[o + Wc + 1]);
 }
 
 // Candidates a block contributes.  With three distinct regions every valid
@@ -627,39 +634,126 @@ __device__ __forceinline__ uint32_t block_candidates(const BlockCorners& c)
     return (c.v[0] | c.v[1] | c.v[2] | c.v[3]) ? 1 : 0;
 }
 
-// One pass over the canvas: each thread block finds the 2x2 blocks holding
-// candidates in its kDetRows x kDetChunk region, reserves room for them with a
-// single atomic add and writes their positions.  The list's order does not
-// matter: registration order comes from the keys, first registrations from the
-// table's minimum key.
-__global__ __launch_bounds__(kDetThreads) void detect_list_kernel(DetectArgs A)
+// Exclusive scans of two short arrays in one launch of one 1024-thread block:
+// thread t owns a contiguous chunk of each; out_*[n] receives the totals.
+__global__ __launch_bounds__(1024) void scan_two_small_kernel(const uint32_t* __restrict__ a,
+                                                              const uint32_t* __restrict__ b,
+                                                              int64_t n, uint32_t* __restrict__ oa,
+                                                              uint32_t* __restrict__ ob)
 {
-    __shared__ uint32_t part[kDetThreads];
-    __shared__ uint32_t base_s, quads_s;
-    const int64_t gb = blockIdx.x;
-    const int img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
+    __shared__ uint32_t pa[1024], pb[1024];
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t lo = threadIdx.x * per, hi = min(n, lo + per);
+    uint32_t sa = 0, sb = 0;
+    for (int64_t i = lo; i < hi; ++i) {
+        sa += a[i];
+        sb += b[i];
+    }
+    pa[threadIdx.x] = sa;
+    pb[threadIdx.x] = sb;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t xa = threadIdx.x >= off ? pa[threadIdx.x - off] : 0u;
+        const uint32_t xb = threadIdx.x >= off ? pb[threadIdx.x - off] : 0u;
+        __syncthreads();
+        pa[threadIdx.x] += xa;
+        pb[threadIdx.x] += xb;
+        __syncthreads();
+    }
+    uint32_t ra = pa[threadIdx.x] - sa, rb = pb[threadIdx.x] - sb;
+    for (int64_t i = lo; i < hi; ++i) {
+        oa[i] = ra;
+        ob[i] = rb;
+        ra += a[i];
+        rb += b[i];
+    }
+    if (threadIdx.x == 1023) {
+        oa[n] = pa[1023];
+        ob[n] = pb[1023];
+    }
+}
+
+// Candidate blocks in three steps without contended atomics: a count pass
+// keeps each thread's mask of candidate blocks and each thread block's total;
+// the block totals are scanned; a write pass lists the positions at the
+// scanned offsets.  A thread block covers kDetRows x kDetChunk 2x2 blocks.
+__device__ __forceinline__ int64_t det_block_origin(const DetectArgs& A, int64_t gb, int& img,
+                                                    int& ry0, int& rx0)
+{
+    img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
     const ImageDesc& d = A.D[img];
     const int64_t lb = gb - A.det_seg[img];
-    const int ry0 = static_cast<int>(lb / d.bx_per_row) * kDetRows;
-    const int bx = static_cast<int>(lb - static_cast<int64_t>(ry0 / kDetRows) * d.bx_per_row);
-    const int rx0 = bx * kDetChunk + threadIdx.x;
+    const int band = static_cast<int>(lb / d.bx_per_row);
+    const int bx = static_cast<int>(lb - static_cast<int64_t>(band) * d.bx_per_row);
+    ry0 = band * kDetRows;
+    rx0 = bx * kDetChunk + static_cast<int>(threadIdx.x);
+    return lb;
+}
+
+__global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
+{
+    __shared__ uint32_t red[2][kDetThreads / 64];
+    const int64_t gb = blockIdx.x;
+    int img, ry0, rx0;
+    det_block_origin(A, gb, img, ry0, rx0);
+    const ImageDesc& d = A.D[img];
     const int bw = d.Wc - 1, bh = d.Hc - 1;
     const uint32_t* g = A.grid + d.cv_off;
+    const int rows = min(kDetRows, bh - ry0);   // block-uniform
     uint32_t mask = 0, quads = 0;   // bit r * kDetItems + j
 #pragma unroll
-    for (int r = 0; r < kDetRows; ++r) {
-        const int ry = ry0 + r;
-        if (ry >= bh) break;
+    for (int j = 0; j < kDetItems; ++j) {
+        // Walk one column of blocks down the rows, reusing each label row
+        // as the next block's top: two loads per block.
+        const int rx = rx0 + j * kDetThreads;
+        const bool col_ok = rx < bw;
+        const uint32_t* col = g + min(rx, bw - 1);
+        size_t o = static_cast<size_t>(ry0) * d.Wc;
+        uint32_t tl = col[o], tr = col[o + 1];
 #pragma unroll
-        for (int j = 0; j < kDetItems; ++j) {
-            const int rx = rx0 + j * kDetThreads;
-            if (rx >= bw) break;
-            const BlockCorners c = block_corners(g, d.Wc, rx, ry);
-            if (block_candidates(c)) mask |= 1u << (r * kDetItems + j);
-            quads += c.quad;
+        for (int r = 0; r < kDetRows; ++r) {
+            if (r >= rows) break;   // uniform
+            o += d.Wc;
+            const uint32_t bl = col[o], br = col[o + 1];
+            const BlockCorners c = corners_of(tl, tr, bl, br);
+            if (col_ok) {
+                if (block_candidates(c)) mask |= 1u << (r * kDetItems + j);
+                quads += c.quad;
+            }
+            tl = bl;
+            tr = br;
         }
     }
-    // exclusive scan of the per-thread counts over the block
+    A.masks[gb * kDetThreads + threadIdx.x] = mask;
+    uint32_t cnt = static_cast<uint32_t>(__popc(mask));
+    for (int off = 32; off > 0; off >>= 1) {
+        cnt += __shfl_down(cnt, off, 64);
+        quads += __shfl_down(quads, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = cnt;
+        red[1][threadIdx.x >> 6] = quads;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, q = 0;
+        for (int w = 0; w < kDetThreads / 64; ++w) {
+            c += red[0][w];
+            q += red[1][w];
+        }
+        A.block_counts[gb] = c;
+        A.block_quads[gb] = q;
+    }
+}
+
+__global__ __launch_bounds__(kDetThreads) void detect_write_kernel(DetectArgs A)
+{
+    __shared__ uint32_t part[kDetThreads];
+    const int64_t gb = blockIdx.x;
+    int img, ry0, rx0;
+    det_block_origin(A, gb, img, ry0, rx0);
+    const ImageDesc& d = A.D[img];
+    uint32_t mask = A.masks[gb * kDetThreads + threadIdx.x];
     const uint32_t cnt = static_cast<uint32_t>(__popc(mask));
     part[threadIdx.x] = cnt;
     __syncthreads();
@@ -669,15 +763,8 @@ __global__ __launch_bounds__(kDetThreads) void detect_list_kernel(DetectArgs A)
         part[threadIdx.x] += add;
         __syncthreads();
     }
-    for (int off = 32; off > 0; off >>= 1) quads += __shfl_down(quads, off, 64);
-    if (threadIdx.x == 0) quads_s = 0;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0 && quads) atomicAdd(&quads_s, quads);
-    if (threadIdx.x == kDetThreads - 1) base_s = part[kDetThreads - 1] ? atomicAdd(A.n_listed, part[kDetThreads - 1]) : 0u;
-    __syncthreads();
-    if (threadIdx.x == 0 && quads_s) atomicAdd(A.n_quads, quads_s);
-    uint32_t out = base_s + part[threadIdx.x] - cnt;
-    const uint32_t bwu = static_cast<uint32_t>(bw);
+    uint32_t out = A.block_offsets[gb] + part[threadIdx.x] - cnt;
+    const uint32_t bwu = static_cast<uint32_t>(d.Wc - 1);
     while (mask) {
         const int b = __ffs(mask) - 1;
         mask &= mask - 1;
@@ -944,43 +1031,54 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     A.rank_at = rank_at;
     A.sorted = sorted;
     A.flags = dflags;
-    A.n_quads = dflags + 2;
-    A.n_listed = dflags + 4;
     A.npos = dflags + 6;
+    const int64_t nthr = nblocks * kDetThreads;
+    work.counts.reserve(sizeof(uint32_t) * static_cast<size_t>(nthr + 4 * (nblocks + 1)));
+    A.masks = work.counts.as<uint32_t>();
+    A.block_counts = A.masks + nthr;
+    A.block_quads = A.block_counts + (nblocks + 1);
+    uint32_t* block_offsets = A.block_quads + (nblocks + 1);
+    uint32_t* quad_scan = block_offsets + (nblocks + 1);
+    A.block_offsets = block_offsets;
+    {
+        ProfScope ps(kKDetect, s);
+        detect_count_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
+        DL_LAUNCH_CHECK();
+        if (nblocks <= 1024 * 256) {
+            scan_two_small_kernel<<<1, 1024, 0, s>>>(A.block_counts, A.block_quads, nblocks,
+                                                     block_offsets, quad_scan);
+            DL_LAUNCH_CHECK();
+        } else {
+            exclusive_scan_u32(A.block_counts, nblocks, block_offsets, work.scan_tmp, s);
+            exclusive_scan_u32(A.block_quads, nblocks, quad_scan, work.scan_tmp, s);
+        }
+    }
+    const uint32_t* n_listed = block_offsets + nblocks;   // scan totals
+    const uint32_t* n_quads = quad_scan + nblocks;
     // Capacities: from the seed count (positions ~ triangles ~ 2 per seed),
-    // checked on the device; in exact mode the list is rerun when short and
-    // the table sized from the counts read back.
+    // checked on the device; or exact, from the totals read back.
     const uint64_t n_seeds = static_cast<uint64_t>(plan.n_seeds);
     uint64_t pos_cap = 3 * n_seeds + 4096;
     uint64_t table_cap = next_pow2(4 * n_seeds + 4096);
-    if (speculative && work.spec_tiny) {   // test hook: capacities certain to overflow
-        pos_cap = 1;
-        table_cap = 1024;
-    }
-    for (int pass = 0;; ++pass) {
-        A.pos_cap = static_cast<uint32_t>(std::min<uint64_t>(pos_cap, 0x7FFFFFF0ull));
-        work.positions.reserve(sizeof(uint2) * static_cast<size_t>(A.pos_cap));
-        A.positions = work.positions.as<uint2>();
-        {
-            ProfScope ps(kKDetect, s);
-            if (pass > 0) zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
-            detect_list_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
-            DL_LAUNCH_CHECK();
+    if (speculative) {
+        if (work.spec_tiny) {   // test hook: capacities certain to overflow
+            pos_cap = 1;
+            table_cap = 1024;
         }
-        if (speculative) break;
-        DL_HIP(hipMemcpyAsync(hflags, dflags, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost, s));
+    } else {
+        DL_HIP(hipMemcpyAsync(hflags + 2, n_quads, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        DL_HIP(hipMemcpyAsync(hflags + 4, n_listed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
         const uint64_t npos = hflags[4], nquads = hflags[2];
-        if (npos > pos_cap) {   // list was short: rerun with room for every position
-            pos_cap = npos;
-            continue;
-        }
+        pos_cap = std::max<uint64_t>(npos, 1);
         // distinct triplets <= 2 per position (a quad adds two blocked ones):
         // at most half full
         table_cap = next_pow2(2 * (npos + 2 * nquads) + 1024);
         res.n_quads = static_cast<int64_t>(nquads);
-        break;
     }
+    A.pos_cap = static_cast<uint32_t>(std::min<uint64_t>(pos_cap, 0x7FFFFFF0ull));
+    work.positions.reserve(sizeof(uint2) * static_cast<size_t>(A.pos_cap));
+    A.positions = work.positions.as<uint2>();
     A.rows_cap = 2 * A.pos_cap;
     work.rows_cap = A.rows_cap;
     work.cand.reserve(sizeof(Cand) * 2 * static_cast<size_t>(A.pos_cap));
@@ -988,8 +1086,10 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     work.rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(A.rows_cap));
     {
         ProfScope ps(kKDetect, s);
-        det_bounds_kernel<<<1, 64, 0, s>>>(A.n_listed, A.n_quads, A.pos_cap, table_cap, dflags + 6,
+        det_bounds_kernel<<<1, 64, 0, s>>>(n_listed, n_quads, A.pos_cap, table_cap, dflags + 6,
                                            dflags);
+        DL_LAUNCH_CHECK();
+        detect_write_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
     }
     const int64_t nwords = plan.n_key_words;
