@@ -253,7 +253,7 @@ template <int NW, int R, int K>
 __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
-    int* __restrict__ counters)
+    int* __restrict__ changed_any, int* __restrict__ visits)
 {
     constexpr int IW = 128 - 2 * K;
     constexpr int IH = NW * R - 2 * K;
@@ -427,8 +427,11 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
         int any = 0;
         for (int w = 0; w < NW; ++w) any |= lds_any[0][w];
         flags_cur[tile] = any;
-        atomicAdd(&counters[1], 1);
-        if (any & kChAny) atomicAdd(&counters[0], 1);
+        // No atomics here: thousands of blocks would contend for one address.
+        // Only this block owns the tile's visit count in this launch, and the
+        // change flag is an idempotent store.
+        visits[tile] += 1;
+        if (any & kChAny) *changed_any = 1;
     }
 }
 
@@ -480,13 +483,30 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n)
     }
 }
 
+// out[0] = sum of p[0..n) (one block).
+__global__ __launch_bounds__(1024) void sum_ints_kernel(const int* __restrict__ p, int n,
+                                                        int* __restrict__ out)
+{
+    __shared__ int red[16];
+    int v = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) v += p[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 16; ++w) t += red[w];
+        out[0] = t;
+    }
+}
+
 __global__ void zero_ints_kernel(int* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
-using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*);
+using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*, int*);
 
 struct Variant {
     int waves, rows, iters;
@@ -562,10 +582,13 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     // active tile and return at once.
     constexpr int kChunk = 4;
     constexpr int kSlots = 2 * kChunk;  // counters for two chunks in flight
-    work.counters.reserve(sizeof(int) * 2 * kSlots);
-    work.host_counters.reserve(sizeof(int) * 2 * kSlots);
+    work.counters.reserve(sizeof(int) * (2 * kSlots + 2));
+    work.host_counters.reserve(sizeof(int) * (2 * kSlots + 2));
+    work.visits.reserve(sizeof(int) * static_cast<size_t>(total));
     int* dcnt = work.counters.as<int>();
     int* hcnt = work.host_counters.as<int>();
+    int* visits = work.visits.as<int>();
+    fill_u32(reinterpret_cast<uint32_t*>(visits), 0u, static_cast<size_t>(total), s);
     zero_ints_kernel<<<1, 64, 0, s>>>(dcnt, 2 * kSlots);
     DL_LAUNCH_CHECK();
 
@@ -593,7 +616,7 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
                 ProfScope ps(kKFill, s);
                 v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
                     reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
-                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j));
+                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j), visits);
                 DL_LAUNCH_CHECK();
             }
             ++launch;
@@ -611,8 +634,6 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         const int slot0 = (pending_chunk % 2) * kChunk;
         for (int j = 0; j < kChunk; ++j) {
             const int changed = hcnt[2 * (slot0 + j)];
-            const int active = hcnt[2 * (slot0 + j) + 1];
-            st.tile_visits += active;
             const int launch_idx = pending_chunk * kChunk + j;
             if (changed == 0) {
                 // Launch launch_idx wrote its (unchanged) result into plane
@@ -628,10 +649,16 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         }
         pending_chunk = pending_chunk + 1;
     }
-    // Drain whatever is still queued behind the converged launch.
+    // Drain whatever is still queued behind the converged launch, and sum the
+    // tiles' visit counts.
+    sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + 2 * kSlots);
+    DL_LAUNCH_CHECK();
+    DL_HIP(hipMemcpyAsync(hcnt + 2 * kSlots, dcnt + 2 * kSlots, sizeof(int), hipMemcpyDeviceToHost,
+                          s));
     DL_HIP(hipStreamSynchronize(s));
     DL_HIP(hipEventDestroy(ev[0]));
     DL_HIP(hipEventDestroy(ev[1]));
+    st.tile_visits = hcnt[2 * kSlots];
     st.launches = launch;
     if (stats) *stats = st;
     return final_plane;

@@ -103,7 +103,8 @@ struct BfsStats {
 // Workspace that persists across BFS calls of one context (flags, counters).
 struct BfsWork {
     DevBuf flags;     // 2 * total_tiles ints
-    DevBuf counters;  // per-launch changed-tile counters
+    DevBuf counters;  // per-launch "some tile changed" flags, then the visit total
+    DevBuf visits;    // per tile: launches that processed it
     DevBuf canvases;  // device copy of the canvas table
     DevBuf tile_map;  // tile -> canvas index (multi-canvas launches)
     PinnedBuf host_counters;
