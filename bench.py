@@ -193,13 +193,13 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
     nt = ctypes.c_int64(0)
     try:
         t0 = time.perf_counter()
-        _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(seeds0), seeds0.shape[0]))
+        _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(seeds0), None, seeds0.shape[0]))
         _lib.check(lib.dl_session_finalise(ctx.handle, h, ctypes.byref(nt)))
         t_build = time.perf_counter() - t0
         per_round = []
         for ins, dele in plan:
             t1 = time.perf_counter()
-            _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(ins), ins.shape[0]))
+            _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(ins), None, ins.shape[0]))
             _lib.check(lib.dl_session_remove(ctx.handle, h, _lib.i32p(dele), dele.shape[0]))
             _lib.check(lib.dl_session_finalise(ctx.handle, h, ctypes.byref(nt)))
             per_round.append(time.perf_counter() - t1)
@@ -332,10 +332,9 @@ def bench_serving(n_requests: int, n_sessions: int, slots: int = 3, loads=(0.3, 
         for req, _ in reqs:
             if isinstance(req, UpdateRequest):
                 h = srv._sessions[req.session_id].handle
-                _lib.check(lib.dl_session_remove(ctx.handle, h, _lib.i32p(req.remove),
-                                                 req.remove.shape[0]))
-                _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(req.insert),
-                                                 req.insert.shape[0]))
+                _lib.check(lib.dl_session_update(ctx.handle, h, _lib.i32p(req.remove),
+                                                 req.remove.shape[0], _lib.i32p(req.insert),
+                                                 None, req.insert.shape[0]))
                 n = ct.c_int64()
                 _lib.check(lib.dl_session_finalise(ctx.handle, h, ct.byref(n)))
                 m = model[req.session_id]

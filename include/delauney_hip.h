@@ -114,7 +114,8 @@ int dl_triangulate_batch_device(dl_context* ctx, int n_images, const int32_t* wi
  *   create:   border_padding < 0 selects the default (delaunay.cu:72).
  *   insert:   ValueError on a position out of bounds, already live or repeated
  *             in the batch, or when max_seeds would be exceeded
- *             (delaunay.cu:191-211).
+ *             (delaunay.cu:191-211).  values: one float per seed (the scalar
+ *             field of insert_deferred, delaunay.cu:274-309) or NULL (zeros).
  *   remove:   ValueError on a position that is not live or repeats.
  *   finalise: runs the pipeline on the live seeds (finalise,
  *             delaunay.cu:323-345); with none live the grid is all -1.
@@ -125,14 +126,68 @@ typedef struct dl_session dl_session;
 int dl_session_create(dl_context* ctx, int width, int height, int64_t max_seeds,
                       int border_padding, dl_session** out);
 void dl_session_destroy(dl_session* s);
-int dl_session_insert(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, int64_t n);
+int dl_session_insert(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, const float* values,
+                      int64_t n);
 int dl_session_remove(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, int64_t n);
+/* remove then insert as one step: on any failure the session (live set,
+ * insertion order, values) is as before the call.  The serving loop's update
+ * request; no reference counterpart (the reference has no removal). */
+int dl_session_update(dl_context* ctx, dl_session* s, const int32_t* remove_xy, int64_t n_remove,
+                      const int32_t* insert_xy, const float* values, int64_t n_insert);
 int dl_session_seed_count(dl_session* s, int64_t* n);
 int dl_session_finalise(dl_context* ctx, dl_session* s, int64_t* n_tri);
 int dl_session_copy_result(dl_session* s, int32_t* out_hw3, int32_t* out_rows);
 int dl_session_copy_seeds(dl_session* s, int32_t* out_xy);
 /* Device pointer of the last finalise's (H, W, 3) grid (NULL while pending). */
 const int32_t* dl_session_device_grid(dl_session* s);
+
+/* The scalar field: whether any insert carried values (edge_scores and
+ * select_midpoints need it, delaunay_scalar_field.cu:218-220), and the values
+ * of the live seeds in insertion order (get_values, delaunay_output.cu:221-224). */
+int dl_session_has_values(dl_session* s, int* out);
+int dl_session_copy_values(dl_session* s, float* out);
+
+/* Queries.  Each finalises first when changes are pending.
+ *   locate: tid of the triangle containing each of n (x, y) points in image
+ *     coordinates -- the max tid over closed containing triangles, which is the
+ *     grid's tid channel inside the image -- or -1; points beyond the padded
+ *     canvas get -1 (Delaunay::locate, delaunay_query.cu:113-133, 196-227).
+ *   in_circumsphere: also mask[i] = 1 iff dx^2 + dy^2 + t[i]^2 < R^2 about the
+ *     containing triangle's circumcircle, in the reference's double operation
+ *     order on padded-canvas coordinates (delaunay_query.cu:33-100, 135-194). */
+int dl_session_locate(dl_context* ctx, dl_session* s, const int32_t* points_xy, int64_t n,
+                      int32_t* out_tid);
+int dl_session_in_circumsphere(dl_context* ctx, dl_session* s, const int32_t* points_xy,
+                               const double* t, int64_t n, uint8_t* out_mask, int32_t* out_tid);
+
+/* Edges and the scalar field over them (delaunay_scalar_field.cu:35-312).
+ *   edges: builds the distinct undirected edges, (a, b) with a < b in
+ *     insertion-order ids, ascending; *n_edges receives their count.
+ *   copy_edges: the (n_edges, 2) int32 list of the last dl_session_edges.
+ *   edge_scores: n_edges floats |v_a - v_b| * |ab| (0 when |ab| < min_length);
+ *     RuntimeError when no insert carried values.
+ *   select_midpoints: midpoints of the best `count` edges (score descending,
+ *     edge index ascending; only scores > threshold and > 0), positions holding
+ *     a seed dropped, one per pixel (the better edge), in raster order;
+ *     out_xy holds out_cap >= min(count, n_edges) pairs, *n_out receives the
+ *     number written. */
+int dl_session_edges(dl_context* ctx, dl_session* s, int64_t* n_edges);
+int dl_session_copy_edges(dl_session* s, int32_t* out);
+int dl_session_edge_scores(dl_context* ctx, dl_session* s, double min_length, float* out);
+int dl_session_select_midpoints(dl_context* ctx, dl_session* s, double min_length, int64_t count,
+                                float threshold, int32_t* out_xy, int64_t out_cap, int64_t* n_out);
+
+/* finalise with the per-pixel result left on the device
+ * (Delaunay::finalise_device, delaunay.cu:347-360, delaunay_output.cu:119-163):
+ * three H*W arrays owned by the session -- tid (0 where no triangle contains
+ * the pixel), sorted seed id, and the outside mask (1 there) -- valid until the
+ * session's next insert, remove, finalise_device or destroy. */
+int dl_session_finalise_device(dl_context* ctx, dl_session* s, int64_t* n_tri,
+                               int32_t** pixel_tids, int32_t** pixel_seed_ids,
+                               uint8_t** outside_mask);
+/* Blocking copy of device memory to the host (the views' to_host,
+ * py_device_array_view.hpp:49-65). */
+int dl_device_to_host(void* dst, const void* src, size_t bytes);
 
 /* Page-locked host memory (hipHostMalloc) for results that are copied back
  * at full link speed; the serving loop pools it.  No reference counterpart. */

@@ -24,6 +24,9 @@ DL_OK, DL_EVALUE, DL_ERUNTIME = 0, 1, 2
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _I64P = ctypes.POINTER(ctypes.c_int64)
 _VP = ctypes.c_void_p
+_F32P = ctypes.POINTER(ctypes.c_float)
+_F64P = ctypes.POINTER(ctypes.c_double)
+_U8P = ctypes.POINTER(ctypes.c_uint8)
 
 
 class DLImage(ctypes.Structure):
@@ -76,13 +79,28 @@ _SIGNATURES = {
     "dl_session_create": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                          ctypes.c_int, ctypes.POINTER(_VP)]),
     "dl_session_destroy": (None, [_VP]),
-    "dl_session_insert": (ctypes.c_int, [_VP, _VP, _I32P, ctypes.c_int64]),
+    "dl_session_insert": (ctypes.c_int, [_VP, _VP, _I32P, _F32P, ctypes.c_int64]),
     "dl_session_remove": (ctypes.c_int, [_VP, _VP, _I32P, ctypes.c_int64]),
+    "dl_session_update": (ctypes.c_int, [_VP, _VP, _I32P, ctypes.c_int64, _I32P, _F32P,
+                                         ctypes.c_int64]),
     "dl_session_seed_count": (ctypes.c_int, [_VP, _I64P]),
     "dl_session_finalise": (ctypes.c_int, [_VP, _VP, _I64P]),
     "dl_session_copy_result": (ctypes.c_int, [_VP, _I32P, _I32P]),
     "dl_session_copy_seeds": (ctypes.c_int, [_VP, _I32P]),
     "dl_session_device_grid": (_VP, [_VP]),
+    "dl_device_to_host": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
+    "dl_session_has_values": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
+    "dl_session_copy_values": (ctypes.c_int, [_VP, _F32P]),
+    "dl_session_locate": (ctypes.c_int, [_VP, _VP, _I32P, ctypes.c_int64, _I32P]),
+    "dl_session_in_circumsphere": (ctypes.c_int, [_VP, _VP, _I32P, _F64P, ctypes.c_int64, _U8P,
+                                                  _I32P]),
+    "dl_session_edges": (ctypes.c_int, [_VP, _VP, _I64P]),
+    "dl_session_copy_edges": (ctypes.c_int, [_VP, _I32P]),
+    "dl_session_edge_scores": (ctypes.c_int, [_VP, _VP, ctypes.c_double, _F32P]),
+    "dl_session_select_midpoints": (ctypes.c_int, [_VP, _VP, ctypes.c_double, ctypes.c_int64,
+                                                   ctypes.c_float, _I32P, ctypes.c_int64, _I64P]),
+    "dl_session_finalise_device": (ctypes.c_int, [_VP, _VP, _I64P, ctypes.POINTER(_VP),
+                                                  ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "dl_set_tiny_capacities": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_debug_fill_watch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                            ctypes.c_int, _I32P, ctypes.POINTER(ctypes.c_int)]),
@@ -129,6 +147,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self._lib = load()
+        self.device = int(device)
         h = _VP()
         check(self._lib.dl_context_create(int(device), ctypes.byref(h)))
         self.handle = h
@@ -156,6 +175,18 @@ def context() -> Context:
 
 def i32p(a: np.ndarray):
     return a.ctypes.data_as(_I32P)
+
+
+def f32p(a: np.ndarray):
+    return a.ctypes.data_as(_F32P)
+
+
+def f64p(a: np.ndarray):
+    return a.ctypes.data_as(_F64P)
+
+
+def u8p(a: np.ndarray):
+    return a.ctypes.data_as(_U8P)
 
 
 class _PinnedBlock:

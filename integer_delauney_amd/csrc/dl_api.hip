@@ -180,10 +180,17 @@ struct dl_session {
     int W = 0, H = 0, P = 0;
     SessionState st;
     DevBuf stage;        // the call's uploaded (x, y) pairs
+    DevBuf stage_values; // and values
     DevBuf out;          // (H, W, 3) grid of the last finalise
     DevBuf rows;         // its triangle rows
+    DevBuf sorted;       // its seeds' packed labels in sorted id order
     int64_t n_tri = 0;
-    bool done = false;   // out/rows hold the current seed set's result
+    bool done = false;   // out/rows/sorted hold the current seed set's result
+    DevBuf edges;        // distinct edges of the current result
+    int64_t n_edges = -1;   // -1: not built since the last change
+    DevBuf pix_tids, pix_sids, pix_mask;   // per-pixel arrays of dl_session_finalise_device
+    DevBuf q_xy, q_t, q_tid, q_mask, q_out;   // query staging
+    QueryWork qw;
 };
 
 namespace {
@@ -530,6 +537,85 @@ struct ProfDrop {
         set_current_profiler(nullptr);
     }
 };
+
+
+// Keeps the rows and sorted labels of one image of the last pipeline run
+// (seeds from seed_off, rows from row_off) as the session's result.
+void keep_session_result(dl_context* ctx, dl_session* s, int64_t seed_off, int64_t row_off,
+                         hipStream_t st)
+{
+    s->rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(std::max<int64_t>(s->n_tri, 1)));
+    if (s->n_tri)
+        DL_HIP(hipMemcpyAsync(s->rows.ptr, ctx->tri.rows.as<int32_t>() + 5 * row_off,
+                              sizeof(int32_t) * 5 * static_cast<size_t>(s->n_tri),
+                              hipMemcpyDeviceToDevice, st));
+    s->sorted.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(s->st.n_live, 1)));
+    if (s->st.n_live)
+        DL_HIP(hipMemcpyAsync(s->sorted.ptr, ctx->sorted.as<uint32_t>() + seed_off,
+                              sizeof(uint32_t) * static_cast<size_t>(s->st.n_live),
+                              hipMemcpyDeviceToDevice, st));
+    s->n_edges = -1;
+}
+
+// Recomputes the session's result (grid, rows, sorted labels) on the context.
+void session_finalise(dl_context* ctx, dl_session* s)
+{
+    hipStream_t st = ctx->stream;
+    const size_t px = static_cast<size_t>(s->W) * s->H;
+    s->out.reserve(sizeof(int32_t) * 3 * px);
+    if (s->st.n_live == 0) {
+        fill_u32(s->out.as<uint32_t>(), 0xFFFFFFFFu, 3 * px, st);
+        DL_HIP(hipStreamSynchronize(st));
+        s->n_tri = 0;
+        s->n_edges = -1;
+    } else {
+        ProfInstall pi(ctx);
+        StageClock clk(ctx);
+        plan_single(ctx, s->W, s->H, s->P, s->st.n_live);
+        run_pipeline(ctx, s->st.list.as<int32_t>(), s->out.as<int32_t>(), 3, st, clk);
+        s->n_tri = ctx->n_tri;
+        keep_session_result(ctx, s, 0, 0, st);
+        clk.mark(5);
+        clk.finish(st);
+    }
+    s->done = true;
+}
+
+SessionView view_of(const dl_session* s)
+{
+    SessionView v;
+    v.W = s->W;
+    v.H = s->H;
+    v.P = s->P;
+    v.n_live = s->st.n_live;
+    v.list = s->st.list.as<int32_t>();
+    v.values = s->st.values.as<float>();
+    v.bitmap = s->st.bitmap.as<uint32_t>();
+    v.words = s->st.words;
+    v.sorted = s->sorted.as<uint32_t>();
+    v.grid = s->out.as<int32_t>();
+    v.rows = s->rows.as<int32_t>();
+    v.n_tri = s->n_tri;
+    return v;
+}
+
+void ensure_finalised(dl_context* ctx, dl_session* s)
+{
+    if (!s->done) session_finalise(ctx, s);
+}
+
+void ensure_edges(dl_context* ctx, dl_session* s)
+{
+    ensure_finalised(ctx, s);
+    if (s->n_edges < 0) s->n_edges = build_edges(view_of(s), s->edges, s->qw, ctx->stream);
+}
+
+void require_values(const dl_session* s, const char* what)
+{
+    if (!s->st.have_values)
+        throw runtime_error(std::string(what) +
+                            " needs a value per seed; pass values to insert_deferred");
+}
 
 }  // namespace
 
@@ -1032,11 +1118,7 @@ int dl_run_batch(dl_context* ctx, int n, dl_image* images, void* stream)
             if (im.session) {
                 dl_session* ss = im.session;
                 ss->n_tri = im.n_tri;
-                ss->rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(std::max<int64_t>(im.n_tri, 1)));
-                if (im.n_tri)
-                    DL_HIP(hipMemcpyAsync(ss->rows.ptr, ctx->tri.rows.as<int32_t>() + 5 * tb[k],
-                                          sizeof(int32_t) * 5 * static_cast<size_t>(im.n_tri),
-                                          hipMemcpyDeviceToDevice, s));
+                keep_session_result(ctx, ss, ctx->plan.img[k].seed_off, tb[k], s);
                 ss->done = true;
             } else if (im.host) {
                 DL_HIP(hipMemcpyAsync(im.out, dev_out[idx[k]],
@@ -1081,7 +1163,8 @@ void dl_session_destroy(dl_session* s)
     delete s;
 }
 
-int dl_session_insert(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, int64_t n)
+int dl_session_insert(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, const float* values,
+                      int64_t n)
 {
     return guarded([&] {
         if (!ctx || !s) throw value_error("null argument");
@@ -1093,9 +1176,17 @@ int dl_session_insert(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, i
         s->stage.reserve(sizeof(int32_t) * 2 * static_cast<size_t>(n));
         DL_HIP(hipMemcpyAsync(s->stage.ptr, seeds_xy, sizeof(int32_t) * 2 * static_cast<size_t>(n),
                               hipMemcpyHostToDevice, ctx->stream));
-        session_insert(s->st, s->stage.as<int32_t>(), n, ctx->stream);
+        const float* dv = nullptr;
+        if (values) {
+            s->stage_values.reserve(sizeof(float) * static_cast<size_t>(n));
+            DL_HIP(hipMemcpyAsync(s->stage_values.ptr, values, sizeof(float) * static_cast<size_t>(n),
+                                  hipMemcpyHostToDevice, ctx->stream));
+            dv = s->stage_values.as<float>();
+        }
+        session_insert(s->st, s->stage.as<int32_t>(), dv, n, ctx->stream);
         DL_HIP(hipStreamSynchronize(ctx->stream));
         s->done = false;
+        s->n_edges = -1;
     });
 }
 
@@ -1112,6 +1203,42 @@ int dl_session_remove(dl_context* ctx, dl_session* s, const int32_t* seeds_xy, i
         session_remove(s->st, s->stage.as<int32_t>(), n, ctx->tri.scan_tmp, ctx->stream);
         DL_HIP(hipStreamSynchronize(ctx->stream));
         s->done = false;
+        s->n_edges = -1;
+    });
+}
+
+int dl_session_update(dl_context* ctx, dl_session* s, const int32_t* remove_xy, int64_t n_remove,
+                      const int32_t* insert_xy, const float* values, int64_t n_insert)
+{
+    return guarded([&] {
+        if (!ctx || !s) throw value_error("null argument");
+        if (n_remove < 0 || n_insert < 0) throw value_error("negative count");
+        if ((n_remove && !remove_xy) || (n_insert && !insert_xy)) throw value_error("null seeds");
+        if (!n_remove && !n_insert) return;
+        DL_HIP(hipSetDevice(ctx->device));
+        if (s->st.n_live - n_remove + n_insert > s->st.max_seeds)
+            throw value_error("insert would exceed max_seeds capacity");
+        const size_t nr = static_cast<size_t>(n_remove), ni = static_cast<size_t>(n_insert);
+        s->stage.reserve(sizeof(int32_t) * 2 * (nr + ni));
+        int32_t* d_rm = s->stage.as<int32_t>();
+        int32_t* d_ins = d_rm + 2 * nr;
+        if (nr)
+            DL_HIP(hipMemcpyAsync(d_rm, remove_xy, sizeof(int32_t) * 2 * nr, hipMemcpyHostToDevice,
+                                  ctx->stream));
+        if (ni)
+            DL_HIP(hipMemcpyAsync(d_ins, insert_xy, sizeof(int32_t) * 2 * ni, hipMemcpyHostToDevice,
+                                  ctx->stream));
+        const float* dv = nullptr;
+        if (values && ni) {
+            s->stage_values.reserve(sizeof(float) * ni);
+            DL_HIP(hipMemcpyAsync(s->stage_values.ptr, values, sizeof(float) * ni,
+                                  hipMemcpyHostToDevice, ctx->stream));
+            dv = s->stage_values.as<float>();
+        }
+        session_update(s->st, d_rm, n_remove, d_ins, dv, n_insert, ctx->tri.scan_tmp, ctx->stream);
+        DL_HIP(hipStreamSynchronize(ctx->stream));
+        s->done = false;
+        s->n_edges = -1;
     });
 }
 
@@ -1128,28 +1255,7 @@ int dl_session_finalise(dl_context* ctx, dl_session* s, int64_t* n_tri)
     return guarded([&] {
         if (!ctx || !s) throw value_error("null argument");
         DL_HIP(hipSetDevice(ctx->device));
-        hipStream_t st = ctx->stream;
-        const size_t px = static_cast<size_t>(s->W) * s->H;
-        s->out.reserve(sizeof(int32_t) * 3 * px);
-        if (s->st.n_live == 0) {
-            fill_u32(s->out.as<uint32_t>(), 0xFFFFFFFFu, 3 * px, st);
-            DL_HIP(hipStreamSynchronize(st));
-            s->n_tri = 0;
-        } else {
-            ProfInstall pi(ctx);
-            StageClock clk(ctx);
-            plan_single(ctx, s->W, s->H, s->P, s->st.n_live);
-            run_pipeline(ctx, s->st.list.as<int32_t>(), s->out.as<int32_t>(), 3, st, clk);
-            s->n_tri = ctx->n_tri;
-            s->rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(std::max<int64_t>(s->n_tri, 1)));
-            if (s->n_tri)
-                DL_HIP(hipMemcpyAsync(s->rows.ptr, ctx->tri.rows.ptr,
-                                      sizeof(int32_t) * 5 * static_cast<size_t>(s->n_tri),
-                                      hipMemcpyDeviceToDevice, st));
-            clk.mark(5);
-            clk.finish(st);
-        }
-        s->done = true;
+        session_finalise(ctx, s);
         if (n_tri) *n_tri = s->n_tri;
     });
 }
@@ -1183,6 +1289,160 @@ int dl_session_copy_seeds(dl_session* s, int32_t* out_xy)
 const int32_t* dl_session_device_grid(dl_session* s)
 {
     return s && s->done ? s->out.as<int32_t>() : nullptr;
+}
+
+int dl_device_to_host(void* dst, const void* src, size_t bytes)
+{
+    return guarded([&] {
+        if (!bytes) return;
+        if (!dst || !src) throw value_error("null argument");
+        DL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int dl_session_has_values(dl_session* s, int* out)
+{
+    return guarded([&] {
+        if (!s || !out) throw value_error("null argument");
+        *out = s->st.have_values ? 1 : 0;
+    });
+}
+
+int dl_session_copy_values(dl_session* s, float* out)
+{
+    return guarded([&] {
+        if (!s || !out) throw value_error("null argument");
+        if (!s->st.n_live) return;
+        DL_HIP(hipSetDevice(s->device));
+        DL_HIP(hipMemcpy(out, s->st.values.ptr, sizeof(float) * static_cast<size_t>(s->st.n_live),
+                         hipMemcpyDeviceToHost));
+    });
+}
+
+int dl_session_locate(dl_context* ctx, dl_session* s, const int32_t* points_xy, int64_t n,
+                      int32_t* out_tid)
+{
+    return dl_session_in_circumsphere(ctx, s, points_xy, nullptr, n, nullptr, out_tid);
+}
+
+int dl_session_in_circumsphere(dl_context* ctx, dl_session* s, const int32_t* points_xy,
+                               const double* t, int64_t n, uint8_t* out_mask, int32_t* out_tid)
+{
+    return guarded([&] {
+        if (!ctx || !s) throw value_error("null argument");
+        if (n < 0) throw value_error("negative point count");
+        if (n == 0) return;
+        if (!points_xy || !out_tid || (t && !out_mask)) throw value_error("null points or output");
+        DL_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        ensure_finalised(ctx, s);
+        const size_t nn = static_cast<size_t>(n);
+        s->q_xy.reserve(sizeof(int32_t) * 2 * nn);
+        s->q_tid.reserve(sizeof(int32_t) * nn);
+        DL_HIP(hipMemcpyAsync(s->q_xy.ptr, points_xy, sizeof(int32_t) * 2 * nn, hipMemcpyHostToDevice,
+                              st));
+        const double* dt = nullptr;
+        uint8_t* dm = nullptr;
+        if (t) {
+            s->q_t.reserve(sizeof(double) * nn);
+            s->q_mask.reserve(nn);
+            DL_HIP(hipMemcpyAsync(s->q_t.ptr, t, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+            dt = s->q_t.as<double>();
+            dm = s->q_mask.as<uint8_t>();
+        }
+        query_points(view_of(s), s->q_xy.as<int32_t>(), dt, n, s->q_tid.as<int32_t>(), dm, s->qw, st);
+        DL_HIP(hipMemcpyAsync(out_tid, s->q_tid.ptr, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, st));
+        if (t) DL_HIP(hipMemcpyAsync(out_mask, dm, nn, hipMemcpyDeviceToHost, st));
+        DL_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int dl_session_edges(dl_context* ctx, dl_session* s, int64_t* n_edges)
+{
+    return guarded([&] {
+        if (!ctx || !s || !n_edges) throw value_error("null argument");
+        DL_HIP(hipSetDevice(ctx->device));
+        ensure_edges(ctx, s);
+        *n_edges = s->n_edges;
+    });
+}
+
+int dl_session_copy_edges(dl_session* s, int32_t* out)
+{
+    return guarded([&] {
+        if (!s || !out) throw value_error("null argument");
+        if (s->n_edges < 0) throw value_error("the edge list is not built: call dl_session_edges");
+        if (!s->n_edges) return;
+        DL_HIP(hipSetDevice(s->device));
+        DL_HIP(hipMemcpy(out, s->edges.ptr, sizeof(int32_t) * 2 * static_cast<size_t>(s->n_edges),
+                         hipMemcpyDeviceToHost));
+    });
+}
+
+int dl_session_edge_scores(dl_context* ctx, dl_session* s, double min_length, float* out)
+{
+    return guarded([&] {
+        if (!ctx || !s || !out) throw value_error("null argument");
+        DL_HIP(hipSetDevice(ctx->device));
+        ensure_edges(ctx, s);
+        if (!s->n_edges) return;
+        require_values(s, "edge_scores");
+        const size_t ne = static_cast<size_t>(s->n_edges);
+        s->q_out.reserve(sizeof(float) * ne);
+        edge_scores(view_of(s), s->edges.as<int32_t>(), s->n_edges, min_length, s->q_out.as<float>(),
+                    ctx->stream);
+        DL_HIP(hipMemcpyAsync(out, s->q_out.ptr, sizeof(float) * ne, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        DL_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dl_session_select_midpoints(dl_context* ctx, dl_session* s, double min_length, int64_t count,
+                                float threshold, int32_t* out_xy, int64_t out_cap, int64_t* n_out)
+{
+    return guarded([&] {
+        if (!ctx || !s || !n_out) throw value_error("null argument");
+        *n_out = 0;
+        DL_HIP(hipSetDevice(ctx->device));
+        ensure_edges(ctx, s);
+        if (!s->n_edges || count <= 0) return;
+        require_values(s, "select_midpoints");
+        const int64_t cap = std::min(count, s->n_edges);
+        if (!out_xy || out_cap < cap)
+            throw value_error("output capacity below min(count, edge count)");
+        s->q_out.reserve(sizeof(int32_t) * 2 * static_cast<size_t>(cap));
+        const int64_t m = select_midpoints(view_of(s), s->edges.as<int32_t>(), s->n_edges, min_length,
+                                           count, threshold, s->q_out.as<int32_t>(), s->qw,
+                                           ctx->stream);
+        if (m)
+            DL_HIP(hipMemcpyAsync(out_xy, s->q_out.ptr, sizeof(int32_t) * 2 * static_cast<size_t>(m),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        DL_HIP(hipStreamSynchronize(ctx->stream));
+        *n_out = m;
+    });
+}
+
+int dl_session_finalise_device(dl_context* ctx, dl_session* s, int64_t* n_tri,
+                               int32_t** pixel_tids, int32_t** pixel_seed_ids,
+                               uint8_t** outside_mask)
+{
+    return guarded([&] {
+        if (!ctx || !s || !pixel_tids || !pixel_seed_ids || !outside_mask)
+            throw value_error("null argument");
+        DL_HIP(hipSetDevice(ctx->device));
+        ensure_finalised(ctx, s);
+        const size_t px = static_cast<size_t>(s->W) * s->H;
+        s->pix_tids.reserve(sizeof(int32_t) * px);
+        s->pix_sids.reserve(sizeof(int32_t) * px);
+        s->pix_mask.reserve(px);
+        split_grid(s->out.as<int32_t>(), static_cast<int64_t>(px), s->pix_tids.as<int32_t>(),
+                   s->pix_sids.as<int32_t>(), s->pix_mask.as<uint8_t>(), ctx->stream);
+        DL_HIP(hipStreamSynchronize(ctx->stream));
+        *pixel_tids = s->pix_tids.as<int32_t>();
+        *pixel_seed_ids = s->pix_sids.as<int32_t>();
+        *outside_mask = s->pix_mask.as<uint8_t>();
+        if (n_tri) *n_tri = s->n_tri;
+    });
 }
 
 }  // extern "C"

@@ -283,8 +283,11 @@ struct SessionState {
     int64_t max_seeds = 0;
     int64_t words = 0;     // presence bitmap words
     int64_t n_live = 0;
+    bool have_values = false;   // some insert carried per-seed values
     DevBuf bitmap;         // bit x*H + y set for every live seed
     DevBuf list, spare;    // live (x, y) pairs in insertion order; compaction target
+    DevBuf values, values_spare;   // one float per live seed, in list order
+    DevBuf backup_list, backup_values;   // undo copies of an update
     DevBuf mine;           // per-call: which bits this call changed (rollback)
     DevBuf keep;           // compaction flags and their prefix sums
     DevBuf flags;
@@ -293,11 +296,66 @@ struct SessionState {
 
 void session_alloc(SessionState& st, int W, int H, int64_t max_seeds, hipStream_t s);
 void session_clear(SessionState& st, hipStream_t s);
-// d_xy: n device (x, y) pairs.  Throws value_error (state unchanged) on an
-// out-of-bounds, already-live or repeated position, or on overflow.
-void session_insert(SessionState& st, const int32_t* d_xy, int64_t n, hipStream_t s);
+// d_xy: n device (x, y) pairs; d_values: n device floats or null (zeros).
+// Throws value_error (state unchanged) on an out-of-bounds, already-live or
+// repeated position, or on overflow.
+void session_insert(SessionState& st, const int32_t* d_xy, const float* d_values, int64_t n,
+                    hipStream_t s);
 // Throws value_error (state unchanged) on a position that is not live or repeats.
 void session_remove(SessionState& st, const int32_t* d_xy, int64_t n, DevBuf& tmp, hipStream_t s);
+// Removal then insertion as one step: if either fails, the state (bitmap,
+// list order, values) is as before the call.
+void session_update(SessionState& st, const int32_t* d_rm, int64_t n_rm, const int32_t* d_ins,
+                    const float* d_values, int64_t n_ins, DevBuf& tmp, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Queries on a finalised session (dl_query.hip).  All pointers are device
+// pointers; every function runs on stream s and leaves results on the device
+// unless it returns a count.
+// ---------------------------------------------------------------------------
+struct SessionView {
+    int W = 0, H = 0, P = 0;
+    int64_t n_live = 0;
+    const int32_t* list = nullptr;      // live (x, y) pairs, insertion order
+    const float* values = nullptr;      // per live seed, insertion order
+    const uint32_t* bitmap = nullptr;   // live-seed presence bits (x*H + y)
+    int64_t words = 0;
+    const uint32_t* sorted = nullptr;   // packed labels x<<16|y in sorted id order
+    const int32_t* grid = nullptr;      // (H, W, 3) result of the last finalise
+    const int32_t* rows = nullptr;      // (n_tri, 5) triangle rows, sorted ids
+    int64_t n_tri = 0;
+};
+
+struct QueryWork {
+    DevBuf a, b, c, d, e;   // per-query / per-edge scratch
+    DevBuf sort_tmp;        // radix-sort temporary storage
+    DevBuf scan_tmp;
+    DevBuf counter;
+    PinnedBuf host;
+};
+
+// tid of the closed triangle containing each query point (image coordinates),
+// max tid on overlaps; -1 where none does or outside the padded canvas.  With
+// d_t given, also the lifted in-circumsphere mask.
+void query_points(const SessionView& v, const int32_t* d_q, const double* d_t, int64_t n,
+                  int32_t* d_tid, uint8_t* d_mask, QueryWork& w, hipStream_t s);
+// Distinct undirected edges (a < b, insertion-order ids) in ascending (a, b)
+// order, written to edges (2 int32 each, capacity 3 * n_tri); returns their count.
+int64_t build_edges(const SessionView& v, DevBuf& edges, QueryWork& w, hipStream_t s);
+// |value_a - value_b| * |ab| per edge, 0 below min_length.
+void edge_scores(const SessionView& v, const int32_t* d_edges, int64_t n_edges, double min_length,
+                 float* d_scores, hipStream_t s);
+// Midpoints of the best `count` edges (score descending, index ascending),
+// without positions that hold a seed, one per pixel (the better edge), in
+// raster order.  Writes (x, y) pairs to d_out (capacity min(count, n_edges))
+// and returns their number.
+int64_t select_midpoints(const SessionView& v, const int32_t* d_edges, int64_t n_edges,
+                         double min_length, int64_t count, float threshold, int32_t* d_out,
+                         QueryWork& w, hipStream_t s);
+// (H, W, 3) grid -> per-pixel tid (0 outside every triangle), seed id, and
+// outside mask.
+void split_grid(const int32_t* grid, int64_t n_px, int32_t* tids, int32_t* sids, uint8_t* mask,
+                hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Per-kernel timing.  When a profiler is installed for the calling thread,

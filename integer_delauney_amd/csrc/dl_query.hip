@@ -1,0 +1,504 @@
+// Queries on a finalised session: point location, the in-circumsphere
+// predicate lifted by t, the undirected edge list, edge scores over the
+// per-seed values, best-k midpoint selection, and the split of the result grid
+// into per-pixel device arrays.
+//
+// Reference semantics (src/cuda): locate / in_circumsphere follow
+// delaunay_query.cu:33-133; get_edges / edge_scores / select_midpoints follow
+// delaunay_scalar_field.cu:35-312; the grid split follows
+// delaunay_output.cu:119-151.  The differences of layout: triangles are the
+// session's rows (x, y, a, b, c) in sorted seed ids, coordinates come from the
+// session's sorted labels (x << 16 | y), and the per-pixel result is the
+// (H, W, 3) grid of the last finalise, so location inside the image is a
+// gather of its tid channel -- the closed-triangle max-tid rule of the batch
+// pipeline (src/delauney/reference/triangulation.py:193-204), which the
+// reference's locate is tested against (tests/test_scalar_field.py:252-258).
+#include "dl_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace dl {
+namespace {
+
+constexpr int32_t kNoTri = -1;
+constexpr int32_t kRing = -2;   // in the padding ring: resolved by a scan over triangles
+constexpr uint64_t kDeadKey = ~0ull;
+
+unsigned blocks_of(int64_t n, int t = 256)
+{
+    return static_cast<unsigned>(std::max<int64_t>(1, (n + t - 1) / t));
+}
+
+int bits_for(uint64_t v)   // bits needed to hold values 0..v
+{
+    int b = 1;
+    while (b < 64 && (v >> b) != 0) ++b;
+    return b;
+}
+
+__device__ __forceinline__ void tri_xy(const int32_t* __restrict__ rows,
+                                       const uint32_t* __restrict__ sorted, int64_t t,
+                                       int64_t X[3], int64_t Y[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t l = sorted[rows[5 * t + 2 + k]];
+        X[k] = static_cast<int64_t>(l >> 16);
+        Y[k] = static_cast<int64_t>(l & 0xFFFFu);
+    }
+}
+
+// Closed containment of (px, py) in triangle (X, Y), exact in integers
+// (triangulation.py:226-253): not both a negative and a positive edge cross
+// product.  A collinear triple contains exactly the points of its line.
+__device__ __forceinline__ bool contains(int64_t px, int64_t py, const int64_t X[3],
+                                         const int64_t Y[3])
+{
+    bool neg = false, pos = false;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int n = e == 2 ? 0 : e + 1;
+        const int64_t d = (X[e] - px) * (Y[n] - py) - (Y[e] - py) * (X[n] - px);
+        neg |= d < 0;
+        pos |= d > 0;
+    }
+    return !(neg && pos);
+}
+
+// In-image points read the grid's tid channel; ring points (inside the padded
+// canvas, outside the image) are flagged for the triangle scan; anything
+// farther out has no triangle (delaunay_query.cu:128-131).
+__global__ void locate_gather_kernel(const int32_t* __restrict__ q, int64_t n, int W, int H, int P,
+                                     const int32_t* __restrict__ grid, int32_t* __restrict__ tid,
+                                     uint32_t* __restrict__ ring)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = q[2 * i], y = q[2 * i + 1];
+    int32_t t = kNoTri;
+    uint32_t r = 0;
+    if (x >= 0 && x < W && y >= 0 && y < H) {
+        t = grid ? grid[(static_cast<int64_t>(y) * W + x) * 3 + 2] : kNoTri;
+    } else if (x >= -P && x < W + P && y >= -P && y < H + P) {
+        t = kRing;
+        r = 1;
+    }
+    tid[i] = t;
+    ring[i] = r;
+}
+
+__global__ void compact_index_kernel(const uint32_t* __restrict__ flag,
+                                     const uint32_t* __restrict__ pos, int64_t n,
+                                     uint32_t* __restrict__ list)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) list[pos[i]] = static_cast<uint32_t>(i);
+}
+
+// One block per ring point at a time: the max tid over all triangles that
+// contain it.  *count (device) is the number of ring points.
+__global__ __launch_bounds__(256) void locate_ring_kernel(
+    const int32_t* __restrict__ q, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ count, const int32_t* __restrict__ rows, int64_t T,
+    const uint32_t* __restrict__ sorted, int32_t* __restrict__ tid)
+{
+    __shared__ int32_t red[4];
+    const uint32_t m = *count;
+    for (uint32_t k = blockIdx.x; k < m; k += gridDim.x) {
+        const uint32_t i = list[k];
+        const int64_t px = q[2 * i], py = q[2 * i + 1];
+        int32_t best = kNoTri;
+        for (int64_t t = threadIdx.x; t < T; t += blockDim.x) {
+            int64_t X[3], Y[3];
+            tri_xy(rows, sorted, t, X, Y);
+            if (contains(px, py, X, Y)) best = static_cast<int32_t>(t);   // t ascends per thread
+        }
+        for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_down(best, off, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+        __syncthreads();
+        if (threadIdx.x == 0) tid[i] = max(max(red[0], red[1]), max(red[2], red[3]));
+        __syncthreads();
+    }
+}
+
+// dx^2 + dy^2 + t^2 < R^2 about the circumcentre of the containing triangle,
+// in double, in the operation order of delaunay_query.cu:58-100 (the
+// circumcentre as A + (|AC|^2 (N x AB) + |AB|^2 (AC x N)) / (2 |N|^2), the
+// radius as the farthest vertex, distances compared after the square root).
+// Collinear triangles (nz^2 < 1e-12) use the centroid.  Coordinates are taken
+// on the padded canvas (image + P), as the reference holds its seeds, since the
+// rounding depends on the origin.  No contraction into fused multiply-adds, so
+// each operation rounds as written.
+__global__ void circumsphere_kernel(const int32_t* __restrict__ q, const double* __restrict__ tq,
+                                    int64_t n, int P, const int32_t* __restrict__ tid,
+                                    const int32_t* __restrict__ rows,
+                                    const uint32_t* __restrict__ sorted,
+                                    uint8_t* __restrict__ mask)
+{
+#pragma clang fp contract(off)
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t t = tid[i];
+    if (t < 0) {
+        mask[i] = 0;
+        return;
+    }
+    int64_t X[3], Y[3];
+    tri_xy(rows, sorted, t, X, Y);
+    const double ax = static_cast<double>(X[0] + P), ay = static_cast<double>(Y[0] + P);
+    const double bx = static_cast<double>(X[1] + P), by = static_cast<double>(Y[1] + P);
+    const double cx = static_cast<double>(X[2] + P), cy = static_cast<double>(Y[2] + P);
+    const double abx = bx - ax, aby = by - ay;
+    const double acx = cx - ax, acy = cy - ay;
+    const double nz = abx * acy - aby * acx;
+    double ox, oy;
+    if (nz * nz < 1e-12) {
+        ox = (ax + bx + cx) / 3.0;
+        oy = (ay + by + cy) / 3.0;
+    } else {
+        const double ab2 = abx * abx + aby * aby;
+        const double ac2 = acx * acx + acy * acy;
+        const double den = 2.0 * (nz * nz);
+        const double num_x = ac2 * (-(nz * aby)) + ab2 * (acy * nz);
+        const double num_y = ac2 * (nz * abx) + ab2 * (-(acx * nz));
+        ox = ax + num_x / den;
+        oy = ay + num_y / den;
+    }
+    const double ra = (ax - ox) * (ax - ox) + (ay - oy) * (ay - oy);
+    const double rb = (bx - ox) * (bx - ox) + (by - oy) * (by - oy);
+    const double rc = (cx - ox) * (cx - ox) + (cy - oy) * (cy - oy);
+    const double r2 = fmax(fmax(ra, rb), rc);
+    const double dx = static_cast<double>(q[2 * i] + P) - ox;
+    const double dy = static_cast<double>(q[2 * i + 1] + P) - oy;
+    const double tt = tq[i];
+    mask[i] = sqrt(dx * dx + dy * dy + tt * tt) < sqrt(r2) ? 1 : 0;
+}
+
+// sorted id -> insertion index of each live seed: its rank in the presence
+// bitmap is its sorted id.
+__global__ void insertion_of_sorted_kernel(const int32_t* __restrict__ list, int64_t n, int H,
+                                           const uint32_t* __restrict__ bm,
+                                           const uint32_t* __restrict__ pre,
+                                           int32_t* __restrict__ ins)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t bit = static_cast<uint64_t>(list[2 * i]) * H + list[2 * i + 1];
+    const uint32_t below = bm[bit >> 5] & ((1u << (bit & 31)) - 1u);
+    ins[pre[bit >> 5] + __popc(below)] = static_cast<int32_t>(i);
+}
+
+// Three (min, max) edge keys per triangle, in insertion ids, packed as
+// min << b | max so that key order is (min, max) order
+// (delaunay_scalar_field.cu:35-60).
+__global__ void edge_keys_kernel(const int32_t* __restrict__ rows, int64_t T,
+                                 const int32_t* __restrict__ ins, int b,
+                                 uint64_t* __restrict__ keys)
+{
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    int32_t v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = ins[rows[5 * t + 2 + k]];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int32_t p = v[e], r = v[e == 2 ? 0 : e + 1];
+        const uint64_t lo = static_cast<uint64_t>(min(p, r)), hi = static_cast<uint64_t>(max(p, r));
+        keys[3 * t + e] = (lo << b) | hi;
+    }
+}
+
+// flag[i] = 1 for the first of each run of equal keys.
+__global__ void first_of_run_kernel(const uint64_t* __restrict__ k, int64_t n,
+                                    uint32_t* __restrict__ flag)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
+}
+
+__global__ void edge_unpack_kernel(const uint64_t* __restrict__ k, const uint32_t* __restrict__ flag,
+                                   const uint32_t* __restrict__ pos, int64_t n, int b,
+                                   int32_t* __restrict__ edges)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const uint64_t key = k[i];
+    const uint32_t p = pos[i];
+    edges[2 * static_cast<int64_t>(p)] = static_cast<int32_t>(key >> b);
+    edges[2 * static_cast<int64_t>(p) + 1] = static_cast<int32_t>(key & ((1ull << b) - 1));
+}
+
+// |v_a - v_b| * |ab| in double, rounded once to float; 0 when
+// |ab|^2 < min_len2 (delaunay_scalar_field.cu:95-113).
+__global__ void edge_score_kernel(const int32_t* __restrict__ edges, int64_t E,
+                                  const int32_t* __restrict__ list, const float* __restrict__ val,
+                                  double min_len2, float* __restrict__ score)
+{
+#pragma clang fp contract(off)
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int32_t a = edges[2 * e], b = edges[2 * e + 1];
+    const double dx = static_cast<double>(list[2 * a]) - static_cast<double>(list[2 * b]);
+    const double dy = static_cast<double>(list[2 * a + 1]) - static_cast<double>(list[2 * b + 1]);
+    const double len2 = dx * dx + dy * dy;
+    if (len2 < min_len2) {
+        score[e] = 0.0f;
+        return;
+    }
+    const double dv = fabs(static_cast<double>(val[a]) - static_cast<double>(val[b]));
+    score[e] = static_cast<float>(dv * sqrt(len2));
+}
+
+// Best-first key (delaunay_scalar_field.cu:124-139): the complemented bits of
+// a positive float order descending, the edge index breaks ties ascending;
+// ineligible edges (score <= threshold or <= 0) get the largest high half.
+// Also counts the eligible edges (one atomic per block).
+__global__ __launch_bounds__(256) void score_key_kernel(const float* __restrict__ score, int64_t E,
+                                                        float threshold,
+                                                        uint64_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ n_eligible)
+{
+    __shared__ uint32_t red[4];
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint32_t ok = 0;
+    if (e < E) {
+        const float s = score[e];
+        uint32_t hi = 0xFFFFFFFFu;
+        if (s > threshold && s > 0.0f) {
+            hi = ~__float_as_uint(s);
+            ok = 1;
+        }
+        keys[e] = (static_cast<uint64_t>(hi) << 32) | static_cast<uint32_t>(e);
+    }
+    uint32_t c = ok;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = red[0] + red[1] + red[2] + red[3];
+        if (t) atomicAdd(n_eligible, t);
+    }
+}
+
+// Midpoint of the rank-r edge for r < take = min(count, eligible), rounded
+// half to even on the padded canvas (as the reference holds its seeds: with an
+// odd P a tie rounds the other way than in image coordinates) and clamped to
+// the image, keyed (pixel << 32 | r); dropped (kDeadKey) where a live seed sits
+// (delaunay_scalar_field.cu:145-171).
+__global__ void midpoint_kernel(const uint64_t* __restrict__ sorted_keys, int64_t cap,
+                                int64_t count, const uint32_t* __restrict__ n_eligible,
+                                const int32_t* __restrict__ edges, const int32_t* __restrict__ list,
+                                int W, int H, int P, const uint32_t* __restrict__ bm,
+                                uint64_t* __restrict__ out)
+{
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= cap) return;
+    const int64_t take = min(count, static_cast<int64_t>(*n_eligible));
+    if (r >= take) {
+        out[r] = kDeadKey;
+        return;
+    }
+    const int64_t e = static_cast<int64_t>(sorted_keys[r] & 0xFFFFFFFFu);
+    const int32_t a = edges[2 * e], b = edges[2 * e + 1];
+    int mx = static_cast<int>(
+        rint(0.5 * (static_cast<double>(list[2 * a] + P) + (list[2 * b] + P))));
+    int my = static_cast<int>(
+        rint(0.5 * (static_cast<double>(list[2 * a + 1] + P) + (list[2 * b + 1] + P))));
+    mx = min(max(mx, P), W - 1 + P) - P;
+    my = min(max(my, P), H - 1 + P) - P;
+    const uint64_t bit = static_cast<uint64_t>(mx) * H + my;
+    if ((bm[bit >> 5] >> (bit & 31)) & 1u) {
+        out[r] = kDeadKey;
+        return;
+    }
+    const uint64_t pixel = static_cast<uint64_t>(my) * W + mx;
+    out[r] = (pixel << 32) | static_cast<uint64_t>(r);
+}
+
+// First key of each pixel (the lowest rank), dead keys excluded.
+__global__ void first_of_pixel_kernel(const uint64_t* __restrict__ k, int64_t n,
+                                      uint32_t* __restrict__ flag)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t key = k[i];
+    flag[i] = (key != kDeadKey && (i == 0 || (key >> 32) != (k[i - 1] >> 32))) ? 1u : 0u;
+}
+
+__global__ void midpoint_out_kernel(const uint64_t* __restrict__ k, const uint32_t* __restrict__ flag,
+                                    const uint32_t* __restrict__ pos, int64_t n, int W,
+                                    int32_t* __restrict__ out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const uint64_t pixel = k[i] >> 32;
+    const uint32_t p = pos[i];
+    out[2 * static_cast<int64_t>(p)] = static_cast<int32_t>(pixel % static_cast<uint64_t>(W));
+    out[2 * static_cast<int64_t>(p) + 1] = static_cast<int32_t>(pixel / static_cast<uint64_t>(W));
+}
+
+// (H, W, 3) grid -> tid (0 outside every triangle), seed id, outside flag
+// (delaunay_output.cu:119-151).
+__global__ void split_kernel(const int32_t* __restrict__ grid, int64_t n, int32_t* __restrict__ tids,
+                             int32_t* __restrict__ sids, uint8_t* __restrict__ mask)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t sid = grid[3 * i], t = grid[3 * i + 2];
+    const bool outside = t == kNoTri;
+    sids[i] = sid;
+    tids[i] = outside ? 0 : t;
+    mask[i] = outside ? 1 : 0;
+}
+
+// Ascending radix sort of n 64-bit keys over bits [0, end_bit): in -> out.
+void sort_keys(const uint64_t* in, uint64_t* out, int64_t n, int end_bit, DevBuf& tmp,
+               hipStream_t s)
+{
+    if (n <= 0) return;
+    if (n > static_cast<int64_t>(INT32_MAX)) throw value_error("too many keys to sort");
+    size_t bytes = 0;
+    DL_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, in, out, static_cast<int>(n), 0,
+                                             end_bit, s));
+    tmp.reserve(std::max<size_t>(bytes, 16));
+    DL_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.ptr, bytes, in, out, static_cast<int>(n), 0,
+                                             end_bit, s));
+}
+
+uint32_t read_u32(const uint32_t* d, QueryWork& w, hipStream_t s)
+{
+    w.host.reserve(64);
+    DL_HIP(hipMemcpyAsync(w.host.ptr, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    return w.host.as<uint32_t>()[0];
+}
+
+}  // namespace
+
+void query_points(const SessionView& v, const int32_t* d_q, const double* d_t, int64_t n,
+                  int32_t* d_tid, uint8_t* d_mask, QueryWork& w, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (n >= static_cast<int64_t>(UINT32_MAX)) throw value_error("too many query points");
+    w.c.reserve(sizeof(uint32_t) * static_cast<size_t>(n));
+    w.d.reserve(sizeof(uint32_t) * static_cast<size_t>(n + 1));
+    w.e.reserve(sizeof(uint32_t) * static_cast<size_t>(n));
+    uint32_t* ring = w.c.as<uint32_t>();
+    uint32_t* pos = w.d.as<uint32_t>();
+    uint32_t* list = w.e.as<uint32_t>();
+    const bool tris = v.n_tri > 0;
+    // Without triangles every point, ring or not, has none.
+    const int ring_width = tris ? v.P : 0;
+    locate_gather_kernel<<<blocks_of(n), 256, 0, s>>>(d_q, n, v.W, v.H, ring_width,
+                                                      tris ? v.grid : nullptr, d_tid, ring);
+    DL_LAUNCH_CHECK();
+    if (ring_width > 0) {
+        exclusive_scan_u32(ring, n, pos, w.scan_tmp, s);
+        compact_index_kernel<<<blocks_of(n), 256, 0, s>>>(ring, pos, n, list);
+        DL_LAUNCH_CHECK();
+        const unsigned g = static_cast<unsigned>(std::min<int64_t>(n, 2048));
+        locate_ring_kernel<<<g, 256, 0, s>>>(d_q, list, pos + n, v.rows, v.n_tri, v.sorted, d_tid);
+        DL_LAUNCH_CHECK();
+    }
+    if (d_t && d_mask) {
+        circumsphere_kernel<<<blocks_of(n), 256, 0, s>>>(d_q, d_t, n, v.P, d_tid, v.rows, v.sorted,
+                                                         d_mask);
+        DL_LAUNCH_CHECK();
+    }
+}
+
+int64_t build_edges(const SessionView& v, DevBuf& edges, QueryWork& w, hipStream_t s)
+{
+    const int64_t T = v.n_tri;
+    if (T <= 0 || v.n_live <= 0) return 0;
+    const int64_t nk = 3 * T;
+    if (nk >= static_cast<int64_t>(UINT32_MAX)) throw value_error("too many triangles");
+    // sorted id -> insertion index
+    w.d.reserve(sizeof(uint32_t) * static_cast<size_t>(v.words + 1));
+    w.e.reserve(sizeof(int32_t) * static_cast<size_t>(v.n_live));
+    uint32_t* pre = w.d.as<uint32_t>();
+    int32_t* ins = w.e.as<int32_t>();
+    popcount_scan(v.bitmap, v.words, pre, w.scan_tmp, s);
+    insertion_of_sorted_kernel<<<blocks_of(v.n_live), 256, 0, s>>>(v.list, v.n_live, v.H, v.bitmap,
+                                                                    pre, ins);
+    DL_LAUNCH_CHECK();
+    // keys, sort, first of each run
+    const int b = bits_for(static_cast<uint64_t>(v.n_live - 1));
+    w.a.reserve(sizeof(uint64_t) * static_cast<size_t>(nk));
+    w.b.reserve(sizeof(uint64_t) * static_cast<size_t>(nk));
+    uint64_t* k0 = w.a.as<uint64_t>();
+    uint64_t* k1 = w.b.as<uint64_t>();
+    edge_keys_kernel<<<blocks_of(T), 256, 0, s>>>(v.rows, T, ins, b, k0);
+    DL_LAUNCH_CHECK();
+    sort_keys(k0, k1, nk, 2 * b, w.sort_tmp, s);
+    w.c.reserve(sizeof(uint32_t) * static_cast<size_t>(nk));
+    w.d.reserve(sizeof(uint32_t) * static_cast<size_t>(nk + 1));
+    uint32_t* flag = w.c.as<uint32_t>();
+    pre = w.d.as<uint32_t>();
+    first_of_run_kernel<<<blocks_of(nk), 256, 0, s>>>(k1, nk, flag);
+    DL_LAUNCH_CHECK();
+    exclusive_scan_u32(flag, nk, pre, w.scan_tmp, s);
+    edges.reserve(sizeof(int32_t) * 2 * static_cast<size_t>(nk));
+    edge_unpack_kernel<<<blocks_of(nk), 256, 0, s>>>(k1, flag, pre, nk, b, edges.as<int32_t>());
+    DL_LAUNCH_CHECK();
+    return static_cast<int64_t>(read_u32(pre + nk, w, s));
+}
+
+void edge_scores(const SessionView& v, const int32_t* d_edges, int64_t n_edges, double min_length,
+                 float* d_scores, hipStream_t s)
+{
+    if (n_edges <= 0) return;
+    edge_score_kernel<<<blocks_of(n_edges), 256, 0, s>>>(d_edges, n_edges, v.list, v.values,
+                                                         min_length * min_length, d_scores);
+    DL_LAUNCH_CHECK();
+}
+
+int64_t select_midpoints(const SessionView& v, const int32_t* d_edges, int64_t n_edges,
+                         double min_length, int64_t count, float threshold, int32_t* d_out,
+                         QueryWork& w, hipStream_t s)
+{
+    if (n_edges <= 0 || count <= 0) return 0;
+    const int64_t cap = std::min(count, n_edges);
+    // scores, best-first keys, eligible count
+    w.e.reserve(sizeof(float) * static_cast<size_t>(n_edges));
+    float* score = w.e.as<float>();
+    edge_scores(v, d_edges, n_edges, min_length, score, s);
+    w.a.reserve(sizeof(uint64_t) * static_cast<size_t>(n_edges));
+    w.b.reserve(sizeof(uint64_t) * static_cast<size_t>(n_edges));
+    w.counter.reserve(sizeof(uint32_t) * 4);
+    uint64_t* k0 = w.a.as<uint64_t>();
+    uint64_t* k1 = w.b.as<uint64_t>();
+    uint32_t* n_elig = w.counter.as<uint32_t>();
+    fill_u32(n_elig, 0u, 1, s);
+    score_key_kernel<<<blocks_of(n_edges), 256, 0, s>>>(score, n_edges, threshold, k0, n_elig);
+    DL_LAUNCH_CHECK();
+    sort_keys(k0, k1, n_edges, 64, w.sort_tmp, s);
+    // midpoints of the first `take`, keyed by (pixel, rank), sorted
+    midpoint_kernel<<<blocks_of(cap), 256, 0, s>>>(k1, cap, count, n_elig, d_edges, v.list, v.W,
+                                                   v.H, v.P, v.bitmap, k0);
+    DL_LAUNCH_CHECK();
+    sort_keys(k0, k1, cap, 64, w.sort_tmp, s);
+    w.c.reserve(sizeof(uint32_t) * static_cast<size_t>(cap));
+    w.d.reserve(sizeof(uint32_t) * static_cast<size_t>(cap + 1));
+    uint32_t* flag = w.c.as<uint32_t>();
+    uint32_t* pos = w.d.as<uint32_t>();
+    first_of_pixel_kernel<<<blocks_of(cap), 256, 0, s>>>(k1, cap, flag);
+    DL_LAUNCH_CHECK();
+    exclusive_scan_u32(flag, cap, pos, w.scan_tmp, s);
+    midpoint_out_kernel<<<blocks_of(cap), 256, 0, s>>>(k1, flag, pos, cap, v.W, d_out);
+    DL_LAUNCH_CHECK();
+    return static_cast<int64_t>(read_u32(pos + cap, w, s));
+}
+
+void split_grid(const int32_t* grid, int64_t n_px, int32_t* tids, int32_t* sids, uint8_t* mask,
+                hipStream_t s)
+{
+    if (n_px <= 0) return;
+    split_kernel<<<blocks_of(n_px), 256, 0, s>>>(grid, n_px, tids, sids, mask);
+    DL_LAUNCH_CHECK();
+}
+
+}  // namespace dl

@@ -26,6 +26,7 @@ import numpy as np
 
 from . import _lib
 from .api import DEFAULT_BORDER_PADDING, _pack, as_seed_array
+from .device_views import DeviceArrayView
 
 
 def _keys(xy: np.ndarray) -> np.ndarray:
@@ -43,21 +44,26 @@ class Delaunay:
         self._P = int(border_padding) if border_padding is not None and border_padding >= 0 \
             else DEFAULT_BORDER_PADDING
         self._lib = _lib.load()
+        ctx = _lib.context()
+        self._device = ctx.device
         h = ctypes.c_void_p()
-        _lib.check(self._lib.dl_session_create(_lib.context().handle, self._W, self._H, self._max,
+        _lib.check(self._lib.dl_session_create(ctx.handle, self._W, self._H, self._max,
                                                self._P, ctypes.byref(h)))
         self._h = h
         self._xy = np.empty((0, 2), dtype=np.int32)
-        self._values = np.empty(0, dtype=np.float32)
         self._pending = True
-        self._rows = None
-        self._grid = None
+        self._rows = None      # rows of the current result (host copy)
+        self._grid = None      # its (H, W, 3) grid, copied on first use
         self._rank = None
+        # Bumped by every call that changes what finalise_device's arrays show
+        # (delaunay.cu:287-289, 337-339, 349-351).
+        self._generation = 0
         self._last = {"bfs_ms": 0.0, "bfs_iters": 0, "detect_ms": 0.0, "dedup_ms": 0.0,
                       "assign_ms": 0.0}
 
     def close(self):
         if getattr(self, "_h", None):
+            self._generation += 1
             self._lib.dl_session_destroy(self._h)
             self._h = None
 
@@ -66,6 +72,11 @@ class Delaunay:
             self.close()
         except Exception:
             pass
+
+    def _handle(self):
+        if not self._h:
+            raise RuntimeError("the session is closed")
+        return self._h
 
     # -- properties (bindings.cpp:220-227) --------------------------------
     @property
@@ -91,57 +102,98 @@ class Delaunay:
     # -- mutation ----------------------------------------------------------
     def insert_deferred(self, seeds, values=None) -> None:
         """Append a batch (delaunay.cu:274-321); the raster is deferred.
+        ``values``: one float per seed of the batch, the scalar field that
+        edge_scores / select_midpoints read (ValueError on a length mismatch).
         The batch is validated on the device (bounds, already live, repeated
         within the batch, capacity); on error nothing changes."""
         if seeds is None or (not isinstance(seeds, np.ndarray) and len(seeds) == 0) or \
                 (isinstance(seeds, np.ndarray) and seeds.size == 0):
+            if values is not None and np.asarray(values).size != 0:
+                raise ValueError("values must have one entry per seed in the batch")
             return
         xy = as_seed_array(seeds)
+        vp = None
         if values is not None:
-            vals = np.asarray(values, dtype=np.float32).reshape(-1)
+            vals = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(-1))
             if vals.shape[0] != xy.shape[0]:
                 raise ValueError("values must have one entry per seed in the batch")
-        else:
-            vals = np.zeros(xy.shape[0], dtype=np.float32)
-        _lib.check(self._lib.dl_session_insert(_lib.context().handle, self._h, _lib.i32p(xy),
-                                               xy.shape[0]))
+            vp = _lib.f32p(vals)
+        _lib.check(self._lib.dl_session_insert(_lib.context().handle, self._handle(), _lib.i32p(xy),
+                                               vp, xy.shape[0]))
         self._xy = np.concatenate([self._xy, xy])
-        self._values = np.concatenate([self._values, vals])
-        self._pending = True
+        self._mutated()
 
     def remove(self, seeds) -> None:
         """Remove live seeds by position.  ValueError (nothing removed) for a
         position that is not live or repeats.  No reference counterpart."""
         xy = as_seed_array(seeds)
-        _lib.check(self._lib.dl_session_remove(_lib.context().handle, self._h, _lib.i32p(xy),
+        _lib.check(self._lib.dl_session_remove(_lib.context().handle, self._handle(), _lib.i32p(xy),
                                                xy.shape[0]))
         drop = np.isin(_keys(self._xy), _keys(xy))
         self._xy = self._xy[~drop]
-        self._values = self._values[~drop]
+        self._mutated()
+
+    def _mutated(self):
         self._pending = True
+        self._generation += 1
 
     def _recompute(self):
+        """Runs the pipeline on the live seeds; keeps the rows, leaves the grid
+        on the device until asked for."""
         t0 = time.perf_counter()
         n = ctypes.c_int64(0)
-        _lib.check(self._lib.dl_session_finalise(_lib.context().handle, self._h, ctypes.byref(n)))
-        grid = np.empty((self._H, self._W, 3), dtype=np.int32)
+        _lib.check(self._lib.dl_session_finalise(_lib.context().handle, self._handle(),
+                                                 ctypes.byref(n)))
         rows = np.empty((int(n.value), 5), dtype=np.int32)
-        _lib.check(self._lib.dl_session_copy_result(self._h, _lib.i32p(grid),
-                                                    _lib.i32p(rows) if n.value else None))
+        if n.value:
+            _lib.check(self._lib.dl_session_copy_result(self._handle(), None, _lib.i32p(rows)))
         self._last["bfs_ms"] = (time.perf_counter() - t0) * 1e3
         self._rows = rows
-        self._grid = grid
+        self._grid = None
         order = np.lexsort((self._xy[:, 1], self._xy[:, 0]))
         rank = np.empty(self.seed_count, dtype=np.int32)
         rank[order] = np.arange(self.seed_count, dtype=np.int32)
         self._rank = rank
         self._pending = False
 
+    def _ensure(self):
+        if self._pending or self._rows is None:
+            self._recompute()
+
+    def _host_grid(self) -> np.ndarray:
+        self._ensure()
+        if self._grid is None:
+            grid = np.empty((self._H, self._W, 3), dtype=np.int32)
+            _lib.check(self._lib.dl_session_copy_result(self._handle(), _lib.i32p(grid), None))
+            self._grid = grid
+        return self._grid
+
     def finalise(self, as_arrays: bool = False):
         """Materialise (triangle_map, (H, W, 3) grid) in sorted ids (delaunay.cu:323-345)."""
         if self._pending or self._rows is None:
             self._recompute()
-        return _pack(self._rows, as_arrays), self._grid.copy()
+            self._generation += 1
+        return _pack(self._rows, as_arrays), self._host_grid().copy()
+
+    def finalise_device(self):
+        """As finalise(as_arrays=True), with the per-pixel result left on the
+        device (delaunay.cu:347-360, py_delaunay.hpp:65-88): returns
+        (triangle_verts (T, 3), pixel_tids, pixel_seed_ids, outside_hull_mask),
+        the last three as :class:`DeviceArrayView` of H*W elements (tid 0 where
+        the mask is 1).  The views are valid until the next insert,
+        insert_deferred, remove, finalise or finalise_device."""
+        self._generation += 1
+        self._ensure()
+        n = ctypes.c_int64(0)
+        p_t, p_s, p_m = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(self._lib.dl_session_finalise_device(
+            _lib.context().handle, self._handle(), ctypes.byref(n), ctypes.byref(p_t),
+            ctypes.byref(p_s), ctypes.byref(p_m)))
+        count = self._W * self._H
+        g = self._generation
+        views = tuple(DeviceArrayView(self, p.value, count, ts, g, self._device)
+                      for p, ts in ((p_t, "<i4"), (p_s, "<i4"), (p_m, "|u1")))
+        return (_pack(self._rows, True),) + views
 
     def insert(self, seeds, as_arrays: bool = False):
         self.insert_deferred(seeds)
@@ -162,11 +214,7 @@ class Delaunay:
         m, g = self.finalise(as_arrays)
         return m, g, self._timings()
 
-    # -- queries -----------------------------------------------------------
-    def _ensure(self):
-        if self._pending or self._rows is None:
-            self._recompute()
-
+    # -- topology and seeds --------------------------------------------------
     @property
     def sorted_rank(self) -> np.ndarray:
         """insertion-order id -> sorted id (delaunay_output.cu:39-57)."""
@@ -183,18 +231,109 @@ class Delaunay:
 
     def get_voronoi_grid(self) -> np.ndarray:
         """(H, W, 2): sorted seed id and squared distance (delaunay_output.cu:169-189)."""
-        self._ensure()
-        return np.ascontiguousarray(self._grid[:, :, 0:2])
+        return np.ascontiguousarray(self._host_grid()[:, :, 0:2])
 
     def get_seeds(self) -> np.ndarray:
+        """(N, 2) live seeds in insertion order (delaunay_output.cu:209-219)."""
         return self._xy.copy()
 
     def get_values(self) -> np.ndarray:
-        return self._values.copy()
+        """(N,) float32 scalar field in insertion order; empty when no insert
+        carried values (delaunay_output.cu:221-224).  Seeds inserted without
+        values hold 0."""
+        flag = ctypes.c_int(0)
+        _lib.check(self._lib.dl_session_has_values(self._handle(), ctypes.byref(flag)))
+        if not flag.value:
+            return np.empty(0, dtype=np.float32)
+        out = np.empty(self.seed_count, dtype=np.float32)
+        if out.size:
+            _lib.check(self._lib.dl_session_copy_values(self._handle(), _lib.f32p(out)))
+        return out
 
     def device_seeds(self) -> np.ndarray:
         """The live seed list as held on the device (insertion order)."""
-        out = np.empty((self.seed_count, 2), dtype=np.int32)
-        if self.seed_count:
-            _lib.check(self._lib.dl_session_copy_seeds(self._h, _lib.i32p(out)))
+        n = ctypes.c_int64(0)
+        _lib.check(self._lib.dl_session_seed_count(self._handle(), ctypes.byref(n)))
+        out = np.empty((int(n.value), 2), dtype=np.int32)
+        if out.shape[0]:
+            _lib.check(self._lib.dl_session_copy_seeds(self._handle(), _lib.i32p(out)))
         return out
+
+    # -- point queries (delaunay_query.cu) -----------------------------------
+    @staticmethod
+    def _points(points) -> np.ndarray:
+        a = np.asarray(points)
+        if a.ndim != 2 or a.shape[1] != 2:
+            if a.size == 0:
+                return np.empty((0, 2), dtype=np.int32)
+            raise ValueError("points must have shape (N, 2)")
+        return np.ascontiguousarray(a, dtype=np.int32)
+
+    def locate(self, points) -> np.ndarray:
+        """Triangle containing each (x, y) point (image coordinates) as an (N,)
+        int32 array, -1 where none does; ids index get_triangles() and
+        finalise()'s map (bindings.cpp:196-205)."""
+        q = self._points(points)
+        out = np.empty(q.shape[0], dtype=np.int32)
+        if q.shape[0]:
+            _lib.check(self._lib.dl_session_locate(_lib.context().handle, self._handle(),
+                                                   _lib.i32p(q), q.shape[0], _lib.i32p(out)))
+        return out
+
+    def in_circumsphere(self, points, t):
+        """(mask, triangle_ids): mask[i] iff dx^2 + dy^2 + t[i]^2 < R^2 about the
+        circumcircle of the triangle containing point i (bindings.cpp:184-195)."""
+        q = self._points(points)
+        tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(-1))
+        if tt.shape[0] != q.shape[0]:
+            raise ValueError("t must have one entry per point")
+        n = q.shape[0]
+        mask = np.zeros(n, dtype=np.uint8)
+        tids = np.full(n, -1, dtype=np.int32)
+        if n:
+            _lib.check(self._lib.dl_session_in_circumsphere(
+                _lib.context().handle, self._handle(), _lib.i32p(q), _lib.f64p(tt), n,
+                _lib.u8p(mask), _lib.i32p(tids)))
+        return mask.astype(bool), tids
+
+    # -- edges and the scalar field (delaunay_scalar_field.cu) ----------------
+    def _n_edges(self) -> int:
+        n = ctypes.c_int64(0)
+        _lib.check(self._lib.dl_session_edges(_lib.context().handle, self._handle(),
+                                              ctypes.byref(n)))
+        return int(n.value)
+
+    def get_edges(self) -> np.ndarray:
+        """(E, 2) distinct undirected edges (a < b, insertion-order ids),
+        ascending (bindings.cpp:210-219)."""
+        e = self._n_edges()
+        out = np.empty((e, 2), dtype=np.int32)
+        if e:
+            _lib.check(self._lib.dl_session_copy_edges(self._handle(), _lib.i32p(out)))
+        return out
+
+    def edge_scores(self, min_length: float = 0.0) -> np.ndarray:
+        """(E,) float32 |v_a - v_b| * |ab| in get_edges() order, 0 for edges
+        shorter than min_length (bindings.cpp:166-172)."""
+        e = self._n_edges()
+        out = np.empty(e, dtype=np.float32)
+        if e:
+            _lib.check(self._lib.dl_session_edge_scores(_lib.context().handle, self._handle(),
+                                                        float(min_length), _lib.f32p(out)))
+        return out
+
+    def select_midpoints(self, min_length: float, count: int, threshold: float = 0.0) -> np.ndarray:
+        """(M, 2) int32 midpoints of the best `count` edges (score descending,
+        edge index ascending), without positions that hold a seed and one per
+        pixel, in raster order (bindings.cpp:173-183)."""
+        e = self._n_edges()
+        count = int(count)
+        if e == 0 or count <= 0:
+            return np.empty((0, 2), dtype=np.int32)
+        cap = min(count, e)
+        buf = np.empty((cap, 2), dtype=np.int32)
+        n = ctypes.c_int64(0)
+        _lib.check(self._lib.dl_session_select_midpoints(
+            _lib.context().handle, self._handle(), float(min_length), count, float(threshold),
+            _lib.i32p(buf), cap, ctypes.byref(n)))
+        return buf[:int(n.value)].copy()

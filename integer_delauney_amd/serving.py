@@ -124,7 +124,7 @@ class Server:
             self._resident += s.resident
         if seeds is not None:
             xy = as_seed_array(seeds)
-            _lib.check(self._lib.dl_session_insert(ctx.handle, h, _lib.i32p(xy), xy.shape[0]))
+            _lib.check(self._lib.dl_session_insert(ctx.handle, h, _lib.i32p(xy), None, xy.shape[0]))
             s.n_live = xy.shape[0]
             n = ctypes.c_int64()
             _lib.check(self._lib.dl_session_finalise(ctx.handle, h, ctypes.byref(n)))
@@ -220,20 +220,10 @@ class Server:
             if isinstance(req, UpdateRequest):
                 s = self._sessions[req.session_id]
                 try:
-                    if req.remove.shape[0]:
-                        _lib.check(lib.dl_session_remove(ctx.handle, s.handle, _lib.i32p(req.remove),
-                                                         req.remove.shape[0]))
-                    if req.insert.shape[0]:
-                        try:
-                            _lib.check(lib.dl_session_insert(ctx.handle, s.handle,
-                                                             _lib.i32p(req.insert),
-                                                             req.insert.shape[0]))
-                        except Exception:
-                            if req.remove.shape[0]:   # undo the removal: all or nothing
-                                _lib.check(lib.dl_session_insert(ctx.handle, s.handle,
-                                                                 _lib.i32p(req.remove),
-                                                                 req.remove.shape[0]))
-                            raise
+                    # removal then insertion, all or nothing
+                    _lib.check(lib.dl_session_update(ctx.handle, s.handle, _lib.i32p(req.remove),
+                                                     req.remove.shape[0], _lib.i32p(req.insert),
+                                                     None, req.insert.shape[0]))
                 except Exception as e:   # invalid update: this request fails alone
                     fut.set_exception(e)
                     continue
