@@ -233,6 +233,61 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             "final_equals_batch_bitwise": same}
 
 
+def bench_queries(side: int, n_seeds: int, n_points: int, reps: int):
+    """Session queries on one image: locate / in_circumsphere over n_points
+    host points, the edge list, edge scores, best-k midpoint selection and
+    finalise_device, each through the Python API (host arrays in and out)."""
+    import numpy as np
+
+    import integer_delauney_amd as dl
+    from integer_delauney_amd.workloads import uniform_seeds
+
+    log(f"queries: {side}x{side}, {n_seeds} seeds, {n_points} points")
+    rng = np.random.default_rng(7)
+    seeds = uniform_seeds(side, side, n_seeds, 2)
+    vals = rng.random(n_seeds).astype(np.float32)
+    mesh = dl.Delaunay(side, side, n_seeds, 16)
+    mesh.insert_deferred(seeds, vals)
+    mesh.finalise_device()
+    q = rng.integers(0, side, size=(n_points, 2)).astype(np.int32)
+    t = rng.random(n_points) * 4.0
+
+    def timed(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        return (time.perf_counter() - t0) / reps * 1e3, r
+
+    out = {"side": side, "seeds": n_seeds, "points": n_points}
+    ms, _ = timed(lambda: mesh.locate(q))
+    out["locate_ms"] = ms
+    out["locate_mpoints_per_s"] = n_points / ms / 1e3
+    ms, _ = timed(lambda: mesh.in_circumsphere(q, t))
+    out["in_circumsphere_ms"] = ms
+    out["in_circumsphere_mpoints_per_s"] = n_points / ms / 1e3
+
+    # The edge list is built on first use after a change and cached: time the
+    # build (first call) and a cached call.
+    t0 = time.perf_counter()
+    e = mesh.get_edges()
+    out["get_edges_build_ms"] = (time.perf_counter() - t0) * 1e3
+    out["edges"] = int(e.shape[0])
+    ms, _ = timed(lambda: mesh.get_edges())
+    out["get_edges_cached_ms"] = ms
+    ms, _ = timed(lambda: mesh.edge_scores(2.0))
+    out["edge_scores_ms"] = ms
+    for k in (10_000, 100_000):
+        ms, mid = timed(lambda: mesh.select_midpoints(2.0, k))
+        out[f"select_{k}_ms"] = ms
+        out[f"select_{k}_returned"] = int(mid.shape[0])
+    ms, _ = timed(lambda: mesh.finalise_device())
+    out["finalise_device_split_ms"] = ms
+    mesh.close()
+    log("queries done")
+    return out
+
+
 def bench_serving(n_requests: int, n_sessions: int, slots: int = 3, loads=(0.3, 0.6, 0.9)):
     """Serving loop (integer_delauney_amd.serving): requests with sides
     log-uniform in [64, 8192] (width and height independent) at 1 % seed
@@ -394,6 +449,8 @@ def main():
                     help="requests per phase of the serving workload (0: skip it)")
     ap.add_argument("--serve-sessions", type=int, default=8)
     ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--query-points", type=int, default=1_000_000,
+                    help="points per locate / in_circumsphere call in the query section (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -533,6 +590,10 @@ def main():
     if args.serve_requests > 0:
         serving = bench_serving(args.serve_requests, args.serve_sessions, args.serve_slots)
 
+    queries = None
+    if args.query_points > 0:
+        queries = bench_queries(W, N, args.query_points, 5)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU oracle sample")
@@ -573,6 +634,7 @@ def main():
         "batched": batch,
         "incremental": incremental,
         "serving": serving,
+        "queries": queries,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -330,7 +330,6 @@ struct QueryWork {
     DevBuf a, b, c, d, e;   // per-query / per-edge scratch
     DevBuf sort_tmp;        // radix-sort temporary storage
     DevBuf scan_tmp;
-    DevBuf counter;
     PinnedBuf host;
 };
 

@@ -254,33 +254,19 @@ __global__ void edge_score_kernel(const int32_t* __restrict__ edges, int64_t E,
 
 // Best-first key (delaunay_scalar_field.cu:124-139): the complemented bits of
 // a positive float order descending, the edge index breaks ties ascending;
-// ineligible edges (score <= threshold or <= 0) get the largest high half.
-// Also counts the eligible edges (one atomic per block).
-__global__ __launch_bounds__(256) void score_key_kernel(const float* __restrict__ score, int64_t E,
-                                                        float threshold,
-                                                        uint64_t* __restrict__ keys,
-                                                        uint32_t* __restrict__ n_eligible)
+// ineligible edges (score <= threshold or <= 0) get the high half kIneligible,
+// which no eligible edge has (it would need a score of +0), so after sorting
+// the eligible edges are exactly the keys before the first such one.
+constexpr uint32_t kIneligible = 0xFFFFFFFFu;
+
+__global__ void score_key_kernel(const float* __restrict__ score, int64_t E, float threshold,
+                                 uint64_t* __restrict__ keys)
 {
-    __shared__ uint32_t red[4];
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    uint32_t ok = 0;
-    if (e < E) {
-        const float s = score[e];
-        uint32_t hi = 0xFFFFFFFFu;
-        if (s > threshold && s > 0.0f) {
-            hi = ~__float_as_uint(s);
-            ok = 1;
-        }
-        keys[e] = (static_cast<uint64_t>(hi) << 32) | static_cast<uint32_t>(e);
-    }
-    uint32_t c = ok;
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t t = red[0] + red[1] + red[2] + red[3];
-        if (t) atomicAdd(n_eligible, t);
-    }
+    if (e >= E) return;
+    const float s = score[e];
+    const uint32_t hi = (s > threshold && s > 0.0f) ? ~__float_as_uint(s) : kIneligible;
+    keys[e] = (static_cast<uint64_t>(hi) << 32) | static_cast<uint32_t>(e);
 }
 
 // Midpoint of the rank-r edge for r < take = min(count, eligible), rounded
@@ -289,19 +275,18 @@ __global__ __launch_bounds__(256) void score_key_kernel(const float* __restrict_
 // the image, keyed (pixel << 32 | r); dropped (kDeadKey) where a live seed sits
 // (delaunay_scalar_field.cu:145-171).
 __global__ void midpoint_kernel(const uint64_t* __restrict__ sorted_keys, int64_t cap,
-                                int64_t count, const uint32_t* __restrict__ n_eligible,
                                 const int32_t* __restrict__ edges, const int32_t* __restrict__ list,
                                 int W, int H, int P, const uint32_t* __restrict__ bm,
                                 uint64_t* __restrict__ out)
 {
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= cap) return;
-    const int64_t take = min(count, static_cast<int64_t>(*n_eligible));
-    if (r >= take) {
+    if (r >= cap) return;   // cap = min(count, edges)
+    const uint64_t key = sorted_keys[r];
+    if ((key >> 32) == kIneligible) {   // r >= number of eligible edges
         out[r] = kDeadKey;
         return;
     }
-    const int64_t e = static_cast<int64_t>(sorted_keys[r] & 0xFFFFFFFFu);
+    const int64_t e = static_cast<int64_t>(key & 0xFFFFFFFFu);
     const int32_t a = edges[2 * e], b = edges[2 * e + 1];
     int mx = static_cast<int>(
         rint(0.5 * (static_cast<double>(list[2 * a] + P) + (list[2 * b] + P))));
@@ -341,17 +326,45 @@ __global__ void midpoint_out_kernel(const uint64_t* __restrict__ k, const uint32
 }
 
 // (H, W, 3) grid -> tid (0 outside every triangle), seed id, outside flag
-// (delaunay_output.cu:119-151).
+// (delaunay_output.cu:119-151).  Four pixels per thread: three 16-byte loads,
+// two 16-byte stores and one 4-byte store.
+__device__ __forceinline__ void split_one(int32_t sid, int32_t t, int32_t& tid, int32_t& sd,
+                                          uint32_t& m)
+{
+    sd = sid;
+    tid = t == kNoTri ? 0 : t;
+    m = t == kNoTri ? 1u : 0u;
+}
+
 __global__ void split_kernel(const int32_t* __restrict__ grid, int64_t n, int32_t* __restrict__ tids,
                              int32_t* __restrict__ sids, uint8_t* __restrict__ mask)
 {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t sid = grid[3 * i], t = grid[3 * i + 2];
-    const bool outside = t == kNoTri;
-    sids[i] = sid;
-    tids[i] = outside ? 0 : t;
-    mask[i] = outside ? 1 : 0;
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t i0 = 4 * q;
+    if (i0 >= n) return;
+    if (i0 + 4 <= n) {
+        const int4* g = reinterpret_cast<const int4*>(grid + 3 * i0);
+        const int4 a = g[0], b = g[1], c = g[2];
+        // pixels: (a.x a.y a.z) (a.w b.x b.y) (b.z b.w c.x) (c.y c.z c.w)
+        int4 t, sd;
+        uint32_t m0, m1, m2, m3;
+        split_one(a.x, a.z, t.x, sd.x, m0);
+        split_one(a.w, b.y, t.y, sd.y, m1);
+        split_one(b.z, c.x, t.z, sd.z, m2);
+        split_one(c.y, c.w, t.w, sd.w, m3);
+        *reinterpret_cast<int4*>(tids + i0) = t;
+        *reinterpret_cast<int4*>(sids + i0) = sd;
+        *reinterpret_cast<uint32_t*>(mask + i0) = m0 | (m1 << 8) | (m2 << 16) | (m3 << 24);
+        return;
+    }
+    for (int64_t i = i0; i < n; ++i) {
+        int32_t t, sd;
+        uint32_t m;
+        split_one(grid[3 * i], grid[3 * i + 2], t, sd, m);
+        tids[i] = t;
+        sids[i] = sd;
+        mask[i] = static_cast<uint8_t>(m);
+    }
 }
 
 // Ascending radix sort of n 64-bit keys over bits [0, end_bit): in -> out.
@@ -468,17 +481,14 @@ int64_t select_midpoints(const SessionView& v, const int32_t* d_edges, int64_t n
     edge_scores(v, d_edges, n_edges, min_length, score, s);
     w.a.reserve(sizeof(uint64_t) * static_cast<size_t>(n_edges));
     w.b.reserve(sizeof(uint64_t) * static_cast<size_t>(n_edges));
-    w.counter.reserve(sizeof(uint32_t) * 4);
     uint64_t* k0 = w.a.as<uint64_t>();
     uint64_t* k1 = w.b.as<uint64_t>();
-    uint32_t* n_elig = w.counter.as<uint32_t>();
-    fill_u32(n_elig, 0u, 1, s);
-    score_key_kernel<<<blocks_of(n_edges), 256, 0, s>>>(score, n_edges, threshold, k0, n_elig);
+    score_key_kernel<<<blocks_of(n_edges), 256, 0, s>>>(score, n_edges, threshold, k0);
     DL_LAUNCH_CHECK();
     sort_keys(k0, k1, n_edges, 64, w.sort_tmp, s);
     // midpoints of the first `take`, keyed by (pixel, rank), sorted
-    midpoint_kernel<<<blocks_of(cap), 256, 0, s>>>(k1, cap, count, n_elig, d_edges, v.list, v.W,
-                                                   v.H, v.P, v.bitmap, k0);
+    midpoint_kernel<<<blocks_of(cap), 256, 0, s>>>(k1, cap, d_edges, v.list, v.W, v.H, v.P, v.bitmap,
+                                                   k0);
     DL_LAUNCH_CHECK();
     sort_keys(k0, k1, cap, 64, w.sort_tmp, s);
     w.c.reserve(sizeof(uint32_t) * static_cast<size_t>(cap));
@@ -497,7 +507,7 @@ void split_grid(const int32_t* grid, int64_t n_px, int32_t* tids, int32_t* sids,
                 hipStream_t s)
 {
     if (n_px <= 0) return;
-    split_kernel<<<blocks_of(n_px), 256, 0, s>>>(grid, n_px, tids, sids, mask);
+    split_kernel<<<blocks_of((n_px + 3) / 4), 256, 0, s>>>(grid, n_px, tids, sids, mask);
     DL_LAUNCH_CHECK();
 }
 

@@ -304,9 +304,10 @@ def _read(view, dtype):
 
 class TestFinaliseDevice:
 
-    @pytest.mark.parametrize("n", [10, 40, 150, 2000])
-    def test_matches_finalise(self, n):
-        mesh, pts, _ = make_mesh(n, seed=n, w=64 if n < 1000 else 640, h=48 if n < 1000 else 480)
+    @pytest.mark.parametrize("n,w,h", [(10, 64, 48), (40, 63, 47), (150, 65, 49), (2000, 640, 480),
+                                       (300, 201, 3)])
+    def test_matches_finalise(self, n, w, h):
+        mesh, pts, _ = make_mesh(n, seed=n, w=w, h=h)
         verts, tids, sids, mask = mesh.finalise_device()
         ref_verts, grid = mesh.finalise(as_arrays=True)
         # finalise with nothing pending leaves the generation (and views) alone
