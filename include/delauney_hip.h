@@ -243,6 +243,15 @@ int dl_set_bfs_config(dl_context* ctx, int waves, int rows, int iters);
 /* The configuration the last flood fill on this context ran with. */
 int dl_get_bfs_config(dl_context* ctx, int* waves, int* rows, int* iters);
 
+/* Autotuning of the automatic selection (default on).  Every tile shape gives
+ * the same labels, so the shape is chosen per canvas class (device, log2 of
+ * the fill's cells, log4 of cells per seed): the first fill of a class with at
+ * least 2^24 cells times the candidate shapes on its own input and the
+ * fastest is kept for the process; smaller fills use 8,8,8.  dl_autotune_reset
+ * forgets every class.  No reference counterpart. */
+int dl_set_autotune(dl_context* ctx, int enabled);
+int dl_autotune_reset(void);
+
 /* Device-resident triangulate: seeds, output and triangles stay on the device.
  * d_seeds_xy: n (x, y) int32 pairs; d_out_hw3: (H, W, 3) int32.  The triangle
  * rows stay in the context (dl_device_triangles).  No reference counterpart;

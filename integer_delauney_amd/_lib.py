@@ -88,6 +88,8 @@ _SIGNATURES = {
     "dl_session_copy_result": (ctypes.c_int, [_VP, _I32P, _I32P]),
     "dl_session_copy_seeds": (ctypes.c_int, [_VP, _I32P]),
     "dl_session_device_grid": (_VP, [_VP]),
+    "dl_set_autotune": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_autotune_reset": (ctypes.c_int, []),
     "dl_device_to_host": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
     "dl_session_has_values": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
     "dl_session_copy_values": (ctypes.c_int, [_VP, _F32P]),

@@ -10,9 +10,13 @@
 #include "dl_internal.h"
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 
 namespace dl {
 
@@ -168,6 +172,7 @@ struct dl_context {
     double timings[9] = {0};
     BfsConfig cfg{8, 8, 8};
     bool cfg_auto = true;
+    bool autotune = true;   // auto mode may time the candidate shapes on a new canvas class
     BfsConfig last_cfg{0, 0, 0};
     bool force_unpadded = false;
     std::vector<hipEvent_t> events;
@@ -257,12 +262,45 @@ void check_seeds(const void* p, int64_t n)
     if (!p) throw value_error("seeds pointer is null");
 }
 
-BfsConfig auto_config(const dl_context* ctx, int Wc, int Hc)
+// ---------------------------------------------------------------------------
+// Flood-fill autotuner.  Every tile shape computes the same labels (the tiled
+// schedule is exact), so the shape is free to choose per input.  A canvas
+// class is (device, log2 of the fill's total cells, log4 of cells per seed);
+// the first fill of a class large enough to matter times each candidate shape
+// on the live input (re-initialising the canvases between runs; the last
+// run's labels are the result) and keeps the fastest for the process.
+// Smaller fills use the default shape.
+// ---------------------------------------------------------------------------
+constexpr BfsConfig kDefaultCfg{8, 8, 8};
+constexpr BfsConfig kTuneCandidates[] = {{8, 8, 8}, {16, 8, 8}, {16, 8, 12}, {16, 8, 16},
+                                         {16, 4, 8}, {8, 8, 4}};
+constexpr int64_t kTuneMinCells = int64_t(1) << 24;
+
+using TuneKey = std::tuple<int, int, int>;
+std::mutex g_tune_mu;
+std::map<TuneKey, BfsConfig> g_tuned;
+
+TuneKey tune_key(int device, int64_t cells, int64_t seeds)
 {
+    const double c = static_cast<double>(std::max<int64_t>(cells, 1));
+    const double per = c / static_cast<double>(std::max<int64_t>(seeds, 1));
+    return TuneKey{device, static_cast<int>(std::floor(std::log2(c))),
+                   static_cast<int>(std::floor(std::log2(std::max(per, 1.0)) / 2.0))};
+}
+
+// The shape for a fill of `cells` cells and `seeds` seeds; *tune is set when
+// the class should be timed now.
+BfsConfig choose_config(const dl_context* ctx, int64_t cells, int64_t seeds, bool* tune)
+{
+    *tune = false;
     if (!ctx->cfg_auto) return ctx->cfg;
-    (void)Wc;
-    (void)Hc;
-    return BfsConfig{8, 8, 8};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        const auto it = g_tuned.find(tune_key(ctx->device, cells, seeds));
+        if (it != g_tuned.end()) return it->second;
+    }
+    *tune = ctx->autotune && cells >= kTuneMinCells;
+    return kDefaultCfg;
 }
 
 struct StageClock {
@@ -350,9 +388,58 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         cv.escape = watch[i] >= 0 ? ctx->esc.as<int>() + i : nullptr;
     }
     BfsStats st;
-    const BfsConfig cfg = auto_config(ctx, 0, 0);
+    bool tune = false;
+    BfsConfig cfg = choose_config(ctx, plan.n_cells, plan.n_seeds, &tune);
+    int plane = 0;
+    if (!tune) {
+        plane = bfs_run(cvs, cfg, ctx->bfs, s, &st);
+    } else {
+        hipEvent_t ev[2];
+        DL_HIP(hipEventCreate(&ev[0]));
+        DL_HIP(hipEventCreate(&ev[1]));
+        // Two timed runs per candidate (the faster counts); a shape replaces
+        // the default only when it is clearly faster, so timing noise does
+        // not move the choice.
+        float best = 0.f, dflt = 0.f;
+        bool first = true;
+        for (const BfsConfig& cand : kTuneCandidates) {
+            if (!bfs_config_supported(cand)) continue;
+            float t_min = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {
+                if (!first) {   // start again from the seeds
+                    fill_u32(ctx->esc.as<uint32_t>(), 0u, static_cast<size_t>(n_img), s);
+                    init_canvases(plan, ctx->sorted.as<uint32_t>(), ctx->padA.as<uint32_t>(),
+                                  ctx->padB.as<uint32_t>(), ctx->tri, s);
+                }
+                first = false;
+                st = BfsStats{};
+                DL_HIP(hipEventRecord(ev[0], s));
+                plane = bfs_run(cvs, cand, ctx->bfs, s, &st);
+                DL_HIP(hipEventRecord(ev[1], s));
+                DL_HIP(hipEventSynchronize(ev[1]));
+                float ms = 0.f;
+                DL_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                t_min = rep == 0 ? ms : std::min(t_min, ms);
+            }
+            const bool is_default = cand.waves == kDefaultCfg.waves && cand.rows == kDefaultCfg.rows &&
+                                    cand.iters == kDefaultCfg.iters;
+            if (is_default) {
+                dflt = t_min;
+                if (best == 0.f || t_min < best) {
+                    best = t_min;
+                    cfg = cand;
+                }
+            } else if (t_min < 0.97f * dflt && (best == 0.f || t_min < best)) {
+                best = t_min;
+                cfg = cand;
+            }
+        }
+        DL_HIP(hipEventDestroy(ev[0]));
+        DL_HIP(hipEventDestroy(ev[1]));
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(ctx->device, plan.n_cells, plan.n_seeds)] = cfg;
+    }
     ctx->last_cfg = cfg;
-    const int plane = bfs_run(cvs, cfg, ctx->bfs, s, &st);
     ctx->timings[7] += st.launches;
     ctx->timings[8] += st.tile_visits;
     ctx->have_padded = true;
@@ -683,6 +770,22 @@ int dl_set_bfs_config(dl_context* ctx, int waves, int rows, int iters)
         if (!bfs_config_supported(c)) throw value_error("unsupported flood-fill configuration");
         ctx->cfg = c;
         ctx->cfg_auto = false;
+    });
+}
+
+int dl_set_autotune(dl_context* ctx, int enabled)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->autotune = enabled != 0;
+    });
+}
+
+int dl_autotune_reset(void)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned.clear();
     });
 }
 

@@ -82,3 +82,34 @@ def test_single_seed_and_two_seeds():
         m, got = dl.triangulate(10, 10, seeds)
         np.testing.assert_array_equal(got, ref)
         assert len(m) == len(ref_rows)
+
+
+def test_autotuned_fill_equals_fixed_shapes():
+    """The autotuner times every candidate shape on the first fill of a large
+    canvas class and keeps the labels of its last run; the result must equal a
+    run with any fixed shape, bit for bit, and so must the next call of the
+    class (which uses the cached choice)."""
+    import ctypes
+
+    from integer_delauney_amd import _lib
+    lib = _lib.load()
+    ctx = _lib.context()
+    W = H = 4096
+    seeds = clustered_seeds(W, H, 60_000, 9)
+    _lib.check(lib.dl_autotune_reset())
+    _lib.check(lib.dl_set_bfs_config(ctx.handle, 0, 0, 0))
+    rows_a, grid_a = dl.triangulate(W, H, seeds, as_arrays=True)
+    w, r, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _lib.check(lib.dl_get_bfs_config(ctx.handle, ctypes.byref(w), ctypes.byref(r), ctypes.byref(k)))
+    print(f"autotuned shape {w.value},{r.value},{k.value}", flush=True)
+    rows_b, grid_b = dl.triangulate(W, H, seeds, as_arrays=True)
+    np.testing.assert_array_equal(rows_b, rows_a)
+    np.testing.assert_array_equal(grid_b, grid_a)
+    try:
+        for cfg in ((8, 8, 8), (16, 8, 16), (8, 8, 4)):
+            _lib.check(lib.dl_set_bfs_config(ctx.handle, *cfg))
+            rows_f, grid_f = dl.triangulate(W, H, seeds, as_arrays=True)
+            np.testing.assert_array_equal(rows_f, rows_a)
+            np.testing.assert_array_equal(grid_f, grid_a)
+    finally:
+        _lib.check(lib.dl_set_bfs_config(ctx.handle, 0, 0, 0))
