@@ -91,16 +91,20 @@ __device__ __forceinline__ bool relax_key(uint64_t& key, uint32_t ninv, uint32_t
 
 __device__ __forceinline__ uint32_t inv_of(uint64_t key) { return static_cast<uint32_t>(key); }
 
-__device__ __forceinline__ uint32_t from_left(uint32_t v)   // lane l <- lane l-1; lane 0 <- far
+// Whole-wave shifts.  The lane without a source (0 for a right shift, 63 for
+// a left shift) keeps `old`.  Callers chain the shifts of a pass through one
+// variable that starts at kFarInv: that lane never receives data, so it holds
+// kFarInv for every row and no per-row re-initialisation is needed.
+__device__ __forceinline__ uint32_t from_left(uint32_t old, uint32_t v)   // lane l <- lane l-1
 {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
-        static_cast<int>(kFarInv), static_cast<int>(v), 0x138, 0xF, 0xF, false));
+        static_cast<int>(old), static_cast<int>(v), 0x138, 0xF, 0xF, false));
 }
 
-__device__ __forceinline__ uint32_t from_right(uint32_t v)  // lane l <- lane l+1; lane 63 <- far
+__device__ __forceinline__ uint32_t from_right(uint32_t old, uint32_t v)  // lane l <- lane l+1
 {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
-        static_cast<int>(kFarInv), static_cast<int>(v), 0x130, 0xF, 0xF, false));
+        static_cast<int>(old), static_cast<int>(v), 0x130, 0xF, 0xF, false));
 }
 
 // Escape watch for canvases whose image interior must not receive updates
@@ -160,9 +164,10 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
             bool ch = false;
             uint64_t hit = 0ull;
             // pass 1: source x-1
+            uint32_t left = kFarInv;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t left = from_left(inv_of(K1[r]));
+                left = from_left(left, inv_of(K1[r]));
                 const uint32_t own0 = inv_of(K0[r]);
                 const bool t1 = relax_key(K1[r], own0, DL_P1(r));
                 const bool t0 = relax_key(K0[r], left, DL_P0(r));
@@ -171,9 +176,10 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 hit |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) & rb;
             }
             // pass 2: source x+1
+            uint32_t right = kFarInv;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const uint32_t right = from_right(inv_of(K0[r]));
+                right = from_right(right, inv_of(K0[r]));
                 const uint32_t own1 = inv_of(K1[r]);
                 const bool t0 = relax_key(K0[r], own1, DL_P0(r));
                 const bool t1 = relax_key(K1[r], right, DL_P1(r));
