@@ -1457,19 +1457,18 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
         }
     }
     __syncthreads();
-    // Output.  Each thread writes four consecutive pixels of a tile row: for
-    // C == 3 that is 48 bytes, three 16-byte stores when they are aligned.
-    const int q = threadIdx.x & 15;            // quad index within the row
-    const int x = X0 + 4 * q;
-    const int off = d.out_off, shift = d.out_shift, Wl = d.out_W;
-    for (int r = threadIdx.x >> 4; r < kTS; r += blockDim.x / 16) {
-        const int y = Y0 + r;
-        if (y > Y1 || x > X1) continue;
-        const int64_t p = static_cast<int64_t>(y) * W + x;   // pixel within the image
-        const int4 t4 = *reinterpret_cast<const int4*>(&tid_tile[r * kTS + 4 * q]);
-        const int32_t tv[4] = {t4.x, t4.y, t4.z, t4.w};
-        const int n = min(4, X1 - x + 1);
-        if (C == 1) {
+    const int Xw = X1 - X0 + 1;   // pixels per tile row
+    if (C == 1) {
+        // Each thread writes four consecutive pixels of a tile row (16 bytes).
+        const int q = threadIdx.x & 15;
+        const int x = X0 + 4 * q;
+        for (int r = threadIdx.x >> 4; r < kTS; r += blockDim.x / 16) {
+            const int y = Y0 + r;
+            if (y > Y1 || x > X1) continue;
+            const int64_t p = static_cast<int64_t>(y) * W + x;
+            const int4 t4 = *reinterpret_cast<const int4*>(&tid_tile[r * kTS + 4 * q]);
+            const int32_t tv[4] = {t4.x, t4.y, t4.z, t4.w};
+            const int n = min(4, X1 - x + 1);
             int32_t* dst = d.out + p;
             if (n == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
                 *reinterpret_cast<int4*>(dst) = t4;
@@ -1478,40 +1477,65 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
                 for (int k = 0; k < 4; ++k)
                     if (k < n) dst[k] = tv[k];
             }
-            continue;
         }
-        int32_t o[12];
-        const size_t lrow = static_cast<size_t>(y + off) * Wl + off;
+        return;
+    }
+    // C == 3.  Each wave takes four tile rows at a time, one pixel per lane:
+    // the four label loads are issued together, then the four owner-id
+    // gathers.  A row's 3 * Xw output ints are then written as chunks of four
+    // by lanes 0..47 (chunk L = ints 4L .. 4L + 3, channels of pixels 4L / 3
+    // and 4L / 3 + 1, fetched from their lanes by cross-lane reads), so every
+    // store instruction covers one contiguous run of the row.
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int off = d.out_off, shift = d.out_shift, Wl = d.out_W;
+    const int n_rows = Y1 - Y0 + 1;
+    const int pa = (4 * lane) / 3, m = 4 * lane - 3 * pa;   // this lane's chunk: first pixel, channel
+    const int pb = min(pa + 1, 63);
+    const int n_out = min(4, 3 * Xw - 4 * lane);            // ints of the chunk inside the row
+    for (int r0 = 4 * wv; r0 < n_rows; r0 += 16) {
+        uint32_t lab[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int32_t id = -1, dd = -1;
-            if (k < n) {
-                const uint32_t l = d.out_lab[lrow + x + k];
-                if (l != kInvalid) {
-                    const int lx = static_cast<int>(l >> 16), ly = static_cast<int>(l & 0xFFFFu);
-                    const int dx = x + k + off - lx, dy = y + off - ly;
-                    id = rank_at[d.pix_off + static_cast<int64_t>(ly - shift) * W + (lx - shift)];
-                    dd = dx * dx + dy * dy;
-                }
+        for (int i = 0; i < 4; ++i) {
+            const int r = r0 + i;
+            lab[i] = (r < n_rows && lane < Xw)
+                         ? d.out_lab[static_cast<size_t>(Y0 + r + off) * Wl + off + X0 + lane]
+                         : kInvalid;
+        }
+        int32_t id[4], dd[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            id[i] = -1;
+            dd[i] = -1;
+            if (lab[i] != kInvalid) {
+                const int lx = static_cast<int>(lab[i] >> 16), ly = static_cast<int>(lab[i] & 0xFFFFu);
+                const int dx = X0 + lane + off - lx, dy = Y0 + r0 + i + off - ly;
+                dd[i] = dx * dx + dy * dy;
+                id[i] = rank_at[d.pix_off + static_cast<int64_t>(ly - shift) * W + (lx - shift)];
             }
-            o[3 * k] = id;
-            o[3 * k + 1] = dd;
-            o[3 * k + 2] = tv[k];
         }
-        int32_t* dst = d.out + 3 * p;
-        if (n == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
-            int4* d4 = reinterpret_cast<int4*>(dst);
-            d4[0] = make_int4(o[0], o[1], o[2], o[3]);
-            d4[1] = make_int4(o[4], o[5], o[6], o[7]);
-            d4[2] = make_int4(o[8], o[9], o[10], o[11]);
-        } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (k < n) {
-                    dst[3 * k] = o[3 * k];
-                    dst[3 * k + 1] = o[3 * k + 1];
-                    dst[3 * k + 2] = o[3 * k + 2];
-                }
+        for (int i = 0; i < 4; ++i) {
+            const int r = r0 + i;
+            if (r >= n_rows) break;   // uniform across the wave
+            const int32_t tid = tid_tile[r * kTS + lane];
+            const int32_t a0 = __shfl(id[i], pa, 64), a1 = __shfl(dd[i], pa, 64),
+                          a2 = __shfl(tid, pa, 64);
+            const int32_t b0 = __shfl(id[i], pb, 64), b1 = __shfl(dd[i], pb, 64),
+                          b2 = __shfl(tid, pb, 64);
+            // ints of the chunk: (a0 a1 a2 b0) for m = 0, (a1 a2 b0 b1) for 1, (a2 b0 b1 b2) for 2
+            const int32_t o0 = m == 0 ? a0 : (m == 1 ? a1 : a2);
+            const int32_t o1 = m == 0 ? a1 : (m == 1 ? a2 : b0);
+            const int32_t o2 = m == 0 ? a2 : (m == 1 ? b0 : b1);
+            const int32_t o3 = m == 0 ? b0 : (m == 1 ? b1 : b2);
+            if (n_out <= 0) continue;
+            int32_t* dst = d.out + 3 * (static_cast<int64_t>(Y0 + r) * W + X0) + 4 * lane;
+            if (n_out == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+                *reinterpret_cast<int4*>(dst) = make_int4(o0, o1, o2, o3);
+            } else {
+                dst[0] = o0;
+                if (n_out > 1) dst[1] = o1;
+                if (n_out > 2) dst[2] = o2;
+                if (n_out > 3) dst[3] = o3;
             }
         }
     }
