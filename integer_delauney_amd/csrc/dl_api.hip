@@ -346,7 +346,8 @@ struct StageClock {
 // the device into ctx->sorted (id order per image) and ctx->rank_at.  Images
 // without their own seed / output pointers read seeds from d_xy and write
 // into d_out at their offsets in the concatenated layout (C channels).
-void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s)
+void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s,
+             bool defer_seed_check = false)
 {
     BatchPlan& plan = ctx->plan;
     plan_batch(plan);
@@ -357,14 +358,17 @@ void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStr
     upload_plan(plan, ctx->tri, s);
     ctx->sorted.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_seeds, 1)));
     ctx->rank_at.reserve(sizeof(int32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_pix, 1)));
-    normalise_seeds(plan, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), ctx->tri, s);
+    normalise_seeds(plan, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), ctx->tri, s,
+                    defer_seed_check);
 }
 
 // Flood fill of every image's detection canvas ((W + 2P) x (H + 2P), seeds
 // shifted by P).  watch[i] >= 0 records in ctx->esc[i] whether a cell of
 // [m, Wc-m) x [m, Hc-m) ever received an update from outside it.  Returns the
 // plane (0: padA, 1: padB) holding every canvas's result.
-int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s)
+// visits_async: see bfs_run (the autotuning path always synchronises).
+int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s,
+                   int* visits_async = nullptr)
 {
     const BatchPlan& plan = ctx->plan;
     const int n_img = static_cast<int>(plan.img.size());
@@ -392,7 +396,7 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     BfsConfig cfg = choose_config(ctx, plan.n_cells, plan.n_seeds, &tune);
     int plane = 0;
     if (!tune) {
-        plane = bfs_run(cvs, cfg, ctx->bfs, s, &st);
+        plane = bfs_run(cvs, cfg, ctx->bfs, s, &st, visits_async);
     } else {
         hipEvent_t ev[2];
         DL_HIP(hipEventCreate(&ev[0]));
@@ -436,6 +440,8 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         }
         DL_HIP(hipEventDestroy(ev[0]));
         DL_HIP(hipEventDestroy(ev[1]));
+        if (visits_async) *visits_async = st.tile_visits;
+        st.tile_visits = 0;
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[tune_key(ctx->device, plan.n_cells, plan.n_seeds)] = cfg;
     }
@@ -503,7 +509,9 @@ void fill_unpadded(dl_context* ctx, const std::vector<int>& which, hipStream_t s
 void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s,
                   StageClock& clk)
 {
-    prepare(ctx, d_xy, d_out, C, s);
+    // Seed validation is read back with the flood fill's first wait instead of
+    // a round trip of its own.
+    prepare(ctx, d_xy, d_out, C, s, true);
     clk.mark(0);
     BatchPlan& plan = ctx->plan;
     const int n_img = static_cast<int>(plan.img.size());
@@ -515,7 +523,11 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
             watch[i] = plan.img[i].P;
             any_pad = true;
         }
-    const int plane = fill_detection(ctx, watch, s);
+    // The fill's visit total arrives with the final status read below.
+    int* visits_async = ctx->host_small.as<int>() + 64;
+    *visits_async = 0;
+    const int plane = fill_detection(ctx, watch, s, visits_async);
+    check_seed_flags(plan, ctx->tri);   // the fill has waited on the stream at least once
     const uint32_t* base = pad_plane_ptr(ctx, plane);
     for (int i = 0; i < n_img; ++i) {
         ImageDesc& d = plan.img[i];
@@ -537,6 +549,7 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
     clk.mark(4);
     SpeculativeStatus st = read_status(plan, ctx->tri, any_pad ? ctx->esc.as<int>() : nullptr,
                                        any_pad ? n_img : 0, s);
+    ctx->timings[8] += *visits_async;
     if (st.bad_id) throw runtime_error("flood fill left an undefined cell");
     if (st.redo) {
         const DetectResult dr = detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(),

@@ -561,7 +561,7 @@ void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n, h
 }
 
 int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
-            hipStream_t s, BfsStats* stats)
+            hipStream_t s, BfsStats* stats, int* visits_async)
 {
     const Variant* v = find_variant(cfg);
     if (!v) throw runtime_error("unsupported flood-fill tile configuration");
@@ -655,16 +655,23 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         }
         pending_chunk = pending_chunk + 1;
     }
-    // Drain whatever is still queued behind the converged launch, and sum the
-    // tiles' visit counts.
+    // Sum the tiles' visit counts.  Launches still queued behind the converged
+    // one find no active tile; later work on the stream simply follows them
+    // (both label planes hold the converged labels from the converged launch
+    // on), so nothing here needs to wait unless the caller wants the total now.
     sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + 2 * kSlots);
     DL_LAUNCH_CHECK();
-    DL_HIP(hipMemcpyAsync(hcnt + 2 * kSlots, dcnt + 2 * kSlots, sizeof(int), hipMemcpyDeviceToHost,
-                          s));
-    DL_HIP(hipStreamSynchronize(s));
+    if (visits_async) {
+        DL_HIP(hipMemcpyAsync(visits_async, dcnt + 2 * kSlots, sizeof(int), hipMemcpyDeviceToHost,
+                              s));
+    } else {
+        DL_HIP(hipMemcpyAsync(hcnt + 2 * kSlots, dcnt + 2 * kSlots, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        st.tile_visits = hcnt[2 * kSlots];
+    }
     DL_HIP(hipEventDestroy(ev[0]));
     DL_HIP(hipEventDestroy(ev[1]));
-    st.tile_visits = hcnt[2 * kSlots];
     st.launches = launch;
     if (stats) *stats = st;
     return final_plane;

@@ -124,8 +124,12 @@ void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n,
 
 // Runs the flood fill to convergence on the given canvases (all tiles start
 // active).  Returns the plane index (0 or 1) holding the final labels.
+// With visits_async (page-locked host memory) the visit total is copied there
+// asynchronously and the call returns without waiting for the stream; the
+// caller reads it after its next synchronisation (stats->tile_visits is then
+// left at 0).
 int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
-            hipStream_t s, BfsStats* stats);
+            hipStream_t s, BfsStats* stats, int* visits_async = nullptr);
 
 // One-thread-per-cell, one-pass-per-launch version of the same schedule.  Kept
 // as an independent second implementation for cross-checking the tiled kernel.
@@ -207,8 +211,13 @@ void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s);
 // Validates (bounds, duplicates) and orders the seeds of every image (read
 // from each image's xy).  Throws value_error naming the first failing image
 // when the batch has more than one.
+// With defer_check the validation flags are copied back asynchronously and the
+// call returns at once; check_seed_flags must then run after the stream's next
+// synchronisation point.  Until then invalid seeds only produce garbage labels
+// (every kernel in between stays in bounds).
 void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_at,
-                     TriWork& work, hipStream_t s);
+                     TriWork& work, hipStream_t s, bool defer_check = false);
+void check_seed_flags(const BatchPlan& plan, const TriWork& work);
 
 // Writes kInvalid over the first plane of every canvas, then each seed's
 // canvas label (image label + (P, P)) at its cell.  plane_b needs no

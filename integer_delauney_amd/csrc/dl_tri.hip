@@ -316,6 +316,7 @@ __global__ void seeds_rank_kernel(int64_t n_total, const int64_t* __restrict__ s
     const ImageDesc& d = D[img];
     const int32_t* p = d.xy + 2 * (g - d.seed_off);
     const int x = p[0], y = p[1];
+    if (x < 0 || x >= d.W || y < 0 || y >= d.H) return;   // flagged by seeds_mark_kernel
     const uint64_t bit = static_cast<uint64_t>(x) * d.H + y;
     const int64_t w = d.bm_off + static_cast<int64_t>(bit >> 5);
     const uint32_t below = bm[w] & ((1u << (bit & 31)) - 1u);
@@ -333,15 +334,34 @@ __global__ void canvas_seed_kernel(const uint32_t* __restrict__ sorted, int64_t 
     if (g >= n_total) return;
     const int img = n_img > 1 ? find_seg(seed_seg, n_img, g) : 0;
     const ImageDesc& d = D[img];
-    const uint32_t l = sorted[g] + ((static_cast<uint32_t>(d.P) << 16) | static_cast<uint32_t>(d.P));
+    const uint32_t l0 = sorted[g];
+    // Entries of a seed set that failed validation (checked by the host
+    // later) may be stale: keep the write inside the canvas.
+    if ((l0 >> 16) >= static_cast<uint32_t>(d.W) || (l0 & 0xFFFFu) >= static_cast<uint32_t>(d.H))
+        return;
+    const uint32_t l = l0 + ((static_cast<uint32_t>(d.P) << 16) | static_cast<uint32_t>(d.P));
     const int64_t c = d.cv_off + static_cast<int64_t>(l & 0xFFFFu) * d.Wc + (l >> 16);
     a[c] = l;
 }
 
 }  // namespace
 
+void check_seed_flags(const BatchPlan& plan, const TriWork& work)
+{
+    const int n_img = static_cast<int>(plan.img.size());
+    const uint32_t* hf = work.host_flags.as<uint32_t>();
+    for (int i = 0; i < n_img; ++i) {
+        const uint32_t f = hf[i];
+        if (!f) continue;
+        const std::string who = n_img > 1 ? "image " + std::to_string(i) + ": " : "";
+        if (f & kFlagXRange) throw value_error(who + "seed x coordinate out of bounds");
+        if (f & kFlagYRange) throw value_error(who + "seed y coordinate out of bounds");
+        if (f & kFlagDuplicate) throw value_error(who + "duplicate seed positions are not allowed");
+    }
+}
+
 void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_at,
-                     TriWork& work, hipStream_t s)
+                     TriWork& work, hipStream_t s, bool defer_check)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t nw = plan.n_words;
@@ -363,15 +383,9 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_
     }
     DL_HIP(hipMemcpyAsync(work.host_flags.ptr, flags, sizeof(uint32_t) * n_img,
                           hipMemcpyDeviceToHost, s));
-    DL_HIP(hipStreamSynchronize(s));
-    const uint32_t* hf = work.host_flags.as<uint32_t>();
-    for (int i = 0; i < n_img; ++i) {
-        const uint32_t f = hf[i];
-        if (!f) continue;
-        const std::string who = n_img > 1 ? "image " + std::to_string(i) + ": " : "";
-        if (f & kFlagXRange) throw value_error(who + "seed x coordinate out of bounds");
-        if (f & kFlagYRange) throw value_error(who + "seed y coordinate out of bounds");
-        if (f & kFlagDuplicate) throw value_error(who + "duplicate seed positions are not allowed");
+    if (!defer_check) {
+        DL_HIP(hipStreamSynchronize(s));
+        check_seed_flags(plan, work);
     }
     ProfScope ps(kKSeeds, s);
     popcount_scan(bm, nw, work.scan.as<uint32_t>(), work.scan_tmp, s);

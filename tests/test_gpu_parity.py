@@ -172,6 +172,13 @@ def test_value_errors(bad):
         dl.Voronoi().compute(5, 5, bad)
     with pytest.raises(ValueError):
         dl.triangulate(5, 5, bad)
+    # the pipeline reads seed validation back late (with the flood fill's first
+    # wait): the context must be fully usable after the error
+    good = [(0, 0), (4, 1), (2, 4), (1, 2)]
+    rows, ref = O.triangulate(5, 5, good, None, return_rows=True)
+    m, got = dl.triangulate(5, 5, good)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(_rows(m), rows)
 
 
 @pytest.mark.parametrize("W,H", [(0, 10), (10, 0), (-4, 10)])
