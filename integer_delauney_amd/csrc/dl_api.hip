@@ -173,6 +173,11 @@ struct dl_context {
     BfsConfig cfg{8, 8, 8};
     bool cfg_auto = true;
     bool autotune = true;   // auto mode may time the candidate shapes on a new canvas class
+    // Launches the padded fill of a canvas class and shape needed last time
+    // (plus a margin): the next fill of the class issues that many without
+    // waiting (see fill_detection).
+    std::map<std::tuple<int, int, int, int, int, int>, int> fill_pred;
+    std::vector<BfsCanvas> fill_cvs;   // canvases of the last padded fill
     BfsConfig last_cfg{0, 0, 0};
     bool force_unpadded = false;
     std::vector<hipEvent_t> events;
@@ -288,6 +293,30 @@ TuneKey tune_key(int device, int64_t cells, int64_t seeds)
                    static_cast<int>(std::floor(std::log2(std::max(per, 1.0)) / 2.0))};
 }
 
+// A padded fill issued as a fixed number of launches without waiting; the
+// caller settles it after its next synchronisation (finish_predicted_fill).
+struct FillPending {
+    bool active = false;
+    int n = 0;                // launches issued
+    int* host = nullptr;      // n changed flags, then the visit total (page-locked)
+    BfsConfig cfg{};
+    std::tuple<int, int, int, int, int, int> key;
+};
+constexpr int kMaxPredicted = 512;
+
+// Launches to issue for a class whose last fill converged at launch index c:
+// a margin absorbs small changes of the input (launches after convergence find
+// no active tile and cost a few microseconds; a shortfall costs a second
+// detection and assignment).
+int predicted_launches(int c) { return c + 1 + 2 + c / 16; }
+
+std::tuple<int, int, int, int, int, int> pred_key(int device, int64_t cells, int64_t seeds,
+                                                  const BfsConfig& c)
+{
+    const TuneKey k = tune_key(device, cells, seeds);
+    return {std::get<0>(k), std::get<1>(k), std::get<2>(k), c.waves, c.rows, c.iters};
+}
+
 // The shape for a fill of `cells` cells and `seeds` seeds; *tune is set when
 // the class should be timed now.
 BfsConfig choose_config(const dl_context* ctx, int64_t cells, int64_t seeds, bool* tune)
@@ -323,7 +352,7 @@ struct StageClock {
     // Marks the end of stage `idx` (0 seeds, 1 fill, 2 detect, 3 dedup, 4 assign, 5 output).
     void mark(int /*idx*/)
     {
-        if (!ctx->timing) return;
+        if (!ctx->timing || next >= static_cast<int>(ctx->events.size())) return;
         DL_HIP(hipEventRecord(ctx->events[next], ctx->stream));
         ++next;
     }
@@ -367,8 +396,12 @@ void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStr
 // [m, Wc-m) x [m, Hc-m) ever received an update from outside it.  Returns the
 // plane (0: padA, 1: padB) holding every canvas's result.
 // visits_async: see bfs_run (the autotuning path always synchronises).
+//
+// pend: when given and an earlier fill of the same class and shape converged,
+// the fill is issued as that many launches plus a margin (predicted_launches)
+// without any wait, and *pend describes what finish_predicted_fill must check.
 int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s,
-                   int* visits_async = nullptr)
+                   int* visits_async = nullptr, FillPending* pend = nullptr)
 {
     const BatchPlan& plan = ctx->plan;
     const int n_img = static_cast<int>(plan.img.size());
@@ -395,8 +428,25 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     bool tune = false;
     BfsConfig cfg = choose_config(ctx, plan.n_cells, plan.n_seeds, &tune);
     int plane = 0;
-    if (!tune) {
+    const auto pkey = pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg);
+    int pred = 0;
+    if (pend && !tune) {
+        const auto it = ctx->fill_pred.find(pkey);
+        if (it != ctx->fill_pred.end() && it->second <= kMaxPredicted) pred = it->second;
+    }
+    if (pred > 0) {
+        int* host = ctx->host_small.as<int>() + 128;
+        plane = bfs_launch_fixed(cvs, cfg, ctx->bfs, s, pred, host);
+        pend->active = true;
+        pend->n = pred;
+        pend->host = host;
+        pend->cfg = cfg;
+        pend->key = pkey;
+        ctx->fill_cvs = cvs;
+        st.launches = pred;
+    } else if (!tune) {
         plane = bfs_run(cvs, cfg, ctx->bfs, s, &st, visits_async);
+        if (pend) ctx->fill_pred[pkey] = predicted_launches(st.converged);
     } else {
         hipEvent_t ev[2];
         DL_HIP(hipEventCreate(&ev[0]));
@@ -442,6 +492,8 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         DL_HIP(hipEventDestroy(ev[1]));
         if (visits_async) *visits_async = st.tile_visits;
         st.tile_visits = 0;
+        if (pend) ctx->fill_pred[pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg)] =
+            predicted_launches(st.converged);
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[tune_key(ctx->device, plan.n_cells, plan.n_seeds)] = cfg;
     }
@@ -456,6 +508,32 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
 const uint32_t* pad_plane_ptr(const dl_context* ctx, int plane)
 {
     return plane ? ctx->padB.as<uint32_t>() : ctx->padA.as<uint32_t>();
+}
+
+// After the stream's next synchronisation: did the predicted launches reach
+// convergence?  If not, the same fill continues to convergence here (both
+// planes then hold the final labels) and false is returned: everything
+// computed from the labels must be computed again.
+bool finish_predicted_fill(dl_context* ctx, const FillPending& pend, hipStream_t s,
+                           int* visits)
+{
+    int conv = -1;
+    for (int i = 0; i < pend.n; ++i)
+        if (pend.host[i] == 0) {
+            conv = i;
+            break;
+        }
+    *visits += pend.host[pend.n];
+    if (conv >= 0) {
+        ctx->fill_pred[pend.key] = predicted_launches(conv);
+        return true;
+    }
+    BfsStats st;
+    bfs_run(ctx->fill_cvs, pend.cfg, ctx->bfs, s, &st, nullptr, pend.n);
+    ctx->timings[7] += st.launches - pend.n;
+    *visits += st.tile_visits;
+    ctx->fill_pred[pend.key] = predicted_launches(st.converged);
+    return false;
 }
 
 // Unpadded fills (image-sized canvases) for the listed images; sets their
@@ -523,11 +601,14 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
             watch[i] = plan.img[i].P;
             any_pad = true;
         }
-    // The fill's visit total arrives with the final status read below.
+    // The fill's visit total arrives with the final status read below.  A
+    // fill of a class seen before is issued without waiting (predicted launch
+    // count) and settled after that read.
     int* visits_async = ctx->host_small.as<int>() + 64;
     *visits_async = 0;
-    const int plane = fill_detection(ctx, watch, s, visits_async);
-    check_seed_flags(plan, ctx->tri);   // the fill has waited on the stream at least once
+    FillPending pend;
+    const int plane = fill_detection(ctx, watch, s, visits_async, &pend);
+    if (!pend.active) check_seed_flags(plan, ctx->tri);   // the fill has waited at least once
     const uint32_t* base = pad_plane_ptr(ctx, plane);
     for (int i = 0; i < n_img; ++i) {
         ImageDesc& d = plan.img[i];
@@ -538,17 +619,24 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
     }
     upload_desc(plan, ctx->tri, s);
     clk.mark(1);
-    const int32_t* rows = nullptr;
-    detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(),
+    const int32_t* rows = ctx->tri.rows.as<int32_t>();
+    auto detect_and_assign = [&]() {
+        detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(),
+                         ctx->tri, s, true);
+        clk.mark(2);
+        clk.mark(3);
+        rows = ctx->tri.rows.as<int32_t>();
+        assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), C,
                      ctx->tri, s, true);
-    clk.mark(2);
-    clk.mark(3);
-    rows = ctx->tri.rows.as<int32_t>();
-    assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), C,
-                 ctx->tri, s, true);
-    clk.mark(4);
-    SpeculativeStatus st = read_status(plan, ctx->tri, any_pad ? ctx->esc.as<int>() : nullptr,
-                                       any_pad ? n_img : 0, s);
+        clk.mark(4);
+        return read_status(plan, ctx->tri, any_pad ? ctx->esc.as<int>() : nullptr,
+                           any_pad ? n_img : 0, s);
+    };
+    SpeculativeStatus st = detect_and_assign();
+    if (pend.active) {
+        check_seed_flags(plan, ctx->tri);
+        if (!finish_predicted_fill(ctx, pend, s, visits_async)) st = detect_and_assign();
+    }
     ctx->timings[8] += *visits_async;
     if (st.bad_id) throw runtime_error("flood fill left an undefined cell");
     if (st.redo) {

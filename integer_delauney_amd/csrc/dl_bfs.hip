@@ -560,11 +560,11 @@ void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n, h
     }
 }
 
-int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
-            hipStream_t s, BfsStats* stats, int* visits_async)
+// Tiles of every canvas and the device copy of the canvas table; returns the
+// tile count and the launch bound.
+static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+                     hipStream_t s, int64_t* max_iters_out)
 {
-    const Variant* v = find_variant(cfg);
-    if (!v) throw runtime_error("unsupported flood-fill tile configuration");
     const int iw = bfs_tile_iw(cfg), ih = bfs_tile_ih(cfg);
     int total = 0;
     int64_t max_iters = 64;
@@ -575,12 +575,57 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         total += c.tiles_x * c.tiles_y;
         max_iters = std::max<int64_t>(max_iters, 8LL * (c.W + c.H) + 64);
     }
-    const int n_canvas = static_cast<int>(canvases.size());
     if (total == 0) return 0;
+    const int n_canvas = static_cast<int>(canvases.size());
     work.canvases.reserve(sizeof(BfsCanvas) * n_canvas);
     DL_HIP(hipMemcpyAsync(work.canvases.ptr, canvases.data(), sizeof(BfsCanvas) * n_canvas,
                           hipMemcpyHostToDevice, s));
     work.flags.reserve(sizeof(int) * 2 * static_cast<size_t>(total));
+    *max_iters_out = max_iters;
+    return total;
+}
+
+int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+                     hipStream_t s, int n_launches, int* host_changed)
+{
+    const Variant* v = find_variant(cfg);
+    if (!v) throw runtime_error("unsupported flood-fill tile configuration");
+    int64_t max_iters = 0;
+    const int total = bfs_setup(canvases, cfg, work, s, &max_iters);
+    if (total == 0 || n_launches <= 0) return 0;
+    const int n_canvas = static_cast<int>(canvases.size());
+    int* flags = work.flags.as<int>();
+    work.counters.reserve(sizeof(int) * static_cast<size_t>(n_launches + 1));
+    work.visits.reserve(sizeof(int) * static_cast<size_t>(total));
+    int* dcnt = work.counters.as<int>();
+    int* visits = work.visits.as<int>();
+    fill_u32(reinterpret_cast<uint32_t*>(visits), 0u, static_cast<size_t>(total), s);
+    fill_u32(reinterpret_cast<uint32_t*>(dcnt), 0u, static_cast<size_t>(n_launches + 1), s);
+    const int grid_blocks = (total + 7) / 8 * 8;
+    for (int launch = 0; launch < n_launches; ++launch) {
+        ProfScope ps(kKFill, s);
+        v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
+            reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total, launch & 1,
+            flags + ((launch + 1) & 1) * total, flags + (launch & 1) * total,
+            launch == 0 ? 1 : 0, dcnt + launch, visits);
+        DL_LAUNCH_CHECK();
+    }
+    sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + n_launches);
+    DL_LAUNCH_CHECK();
+    DL_HIP(hipMemcpyAsync(host_changed, dcnt, sizeof(int) * static_cast<size_t>(n_launches + 1),
+                          hipMemcpyDeviceToHost, s));
+    return n_launches & 1;
+}
+
+int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+            hipStream_t s, BfsStats* stats, int* visits_async, int start_launch)
+{
+    const Variant* v = find_variant(cfg);
+    if (!v) throw runtime_error("unsupported flood-fill tile configuration");
+    int64_t max_iters = 0;
+    const int total = bfs_setup(canvases, cfg, work, s, &max_iters);
+    const int n_canvas = static_cast<int>(canvases.size());
+    if (total == 0) return 0;
     int* flags = work.flags.as<int>();
 
     // Launches are issued in chunks; the host reads the changed-tile counter of
@@ -603,7 +648,7 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     DL_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
 
     const int max_launches = static_cast<int>((max_iters + cfg.iters - 1) / cfg.iters) + 2;
-    int launch = 0;
+    int launch = start_launch;
     int chunk = 0;
     int final_plane = 0;
     bool done = false;
@@ -640,11 +685,12 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         const int slot0 = (pending_chunk % 2) * kChunk;
         for (int j = 0; j < kChunk; ++j) {
             const int changed = hcnt[2 * (slot0 + j)];
-            const int launch_idx = pending_chunk * kChunk + j;
+            const int launch_idx = start_launch + pending_chunk * kChunk + j;
             if (changed == 0) {
                 // Launch launch_idx wrote its (unchanged) result into plane
                 // (launch_idx + 1) & 1; both planes agree from here on.
                 final_plane = (launch_idx + 1) & 1;
+                st.converged = launch_idx;
                 done = true;
                 break;
             }

@@ -98,6 +98,7 @@ struct BfsConfig {
 struct BfsStats {
     int launches = 0;
     int tile_visits = 0;  // active tiles summed over launches
+    int converged = -1;   // index of the first launch that changed nothing
 };
 
 // Workspace that persists across BFS calls of one context (flags, counters).
@@ -128,8 +129,19 @@ void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n,
 // asynchronously and the call returns without waiting for the stream; the
 // caller reads it after its next synchronisation (stats->tile_visits is then
 // left at 0).
+//
+// start_launch > 0 resumes a fill whose launches 0 .. start_launch - 1 already
+// ran (bfs_launch_fixed): the label planes and tile flags carry over.
 int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
-            hipStream_t s, BfsStats* stats, int* visits_async = nullptr);
+            hipStream_t s, BfsStats* stats, int* visits_async = nullptr, int start_launch = 0);
+
+// Issues exactly n_launches tile launches of a fresh fill without waiting, and
+// copies each launch's "some tile changed" flag, then the visit total, to
+// host_changed (page-locked, n_launches + 1 ints) asynchronously.  The fill
+// has converged iff some launch changed nothing; the labels are then final in
+// both planes.  Returns the plane the last launch wrote.
+int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+                     hipStream_t s, int n_launches, int* host_changed);
 
 // One-thread-per-cell, one-pass-per-launch version of the same schedule.  Kept
 // as an independent second implementation for cross-checking the tiled kernel.
@@ -197,6 +209,7 @@ struct TriWork {
     DevBuf desc;        // device copy of the plan's ImageDesc array
     DevBuf segs;        // device copy of the offset tables + tri_base
     PinnedBuf host_stage;   // staging for the uploads above
+    PinnedBuf host_seed_flags;   // per-image seed validation flags (read back late)
     std::vector<int64_t> tri_base;   // host copy: first row of each image, then the total
     uint32_t rows_cap = 0;           // capacity of rows (detection)
     int64_t list_cap = 0;            // capacity of bin_list (assignment)

@@ -349,7 +349,7 @@ __global__ void canvas_seed_kernel(const uint32_t* __restrict__ sorted, int64_t 
 void check_seed_flags(const BatchPlan& plan, const TriWork& work)
 {
     const int n_img = static_cast<int>(plan.img.size());
-    const uint32_t* hf = work.host_flags.as<uint32_t>();
+    const uint32_t* hf = work.host_seed_flags.as<uint32_t>();
     for (int i = 0; i < n_img; ++i) {
         const uint32_t f = hf[i];
         if (!f) continue;
@@ -381,7 +381,9 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_
         seeds_mark_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm, flags);
         DL_LAUNCH_CHECK();
     }
-    DL_HIP(hipMemcpyAsync(work.host_flags.ptr, flags, sizeof(uint32_t) * n_img,
+    // A buffer of their own: the final status read reuses host_flags.
+    work.host_seed_flags.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max(n_img, 1)));
+    DL_HIP(hipMemcpyAsync(work.host_seed_flags.ptr, flags, sizeof(uint32_t) * n_img,
                           hipMemcpyDeviceToHost, s));
     if (!defer_check) {
         DL_HIP(hipStreamSynchronize(s));
@@ -596,7 +598,13 @@ __device__ __forceinline__ int32_t to_id(const DetectArgs& A, const ImageDesc& d
             return -1;
         }
         const int lx = static_cast<int>(v >> 16) - d.P, ly = static_cast<int>(v & 0xFFFFu) - d.P;
-        return A.rank_at[d.pix_off + static_cast<int64_t>(ly) * d.W + lx];
+        const int32_t id = A.rank_at[d.pix_off + static_cast<int64_t>(ly) * d.W + lx];
+        // Out of range only when the seeds failed validation (read back late).
+        if (id < 0 || id >= d.n) {
+            atomicOr(A.flags, kFlagBadId);
+            return -1;
+        }
+        return id;
     }
     const int32_t id = static_cast<int32_t>(v);
     if (id < 0 || id >= d.n) {

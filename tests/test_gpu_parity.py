@@ -304,3 +304,25 @@ def test_speculative_sizes_overflowing_fall_back_exactly():
         rows, ref = O.triangulate(W, H, s, None, return_rows=True)
         np.testing.assert_array_equal(g, ref)
         np.testing.assert_array_equal(_rows(m), rows)
+
+
+def test_predicted_fill_and_its_shortfall_are_exact():
+    """A padded fill of a canvas class seen before is issued as the launch
+    count the last one needed (plus a margin) without waiting on the host.
+    When a harder input of the same class needs more, the fill continues after
+    the status read and detection and assignment run again.  Every call,
+    predicted, continued or fresh, equals the oracle."""
+    from integer_delauney_amd.workloads import clustered_seeds, uniform_seeds
+    W = H = 512
+    n = 2000
+    easy = uniform_seeds(W, H, n, 11)
+    hard = clustered_seeds(W, H, n, 12)
+    refs = {}
+    for name, seeds in (("easy", easy), ("easy", easy), ("hard", hard), ("hard", hard),
+                        ("easy", easy), ("hard", hard)):
+        if name not in refs:
+            refs[name] = O.triangulate(W, H, seeds, None, return_rows=True)
+        rows, ref = refs[name]
+        m, got = dl.triangulate(W, H, seeds)
+        np.testing.assert_array_equal(got, ref)
+        np.testing.assert_array_equal(_rows(m), rows)
