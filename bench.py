@@ -449,6 +449,8 @@ def main():
                     help="requests per phase of the serving workload (0: skip it)")
     ap.add_argument("--serve-sessions", type=int, default=8)
     ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="0: fixed default flood-fill shape (no first-use timing of the candidates)")
     ap.add_argument("--query-points", type=int, default=1_000_000,
                     help="points per locate / in_circumsphere call in the query section (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -482,6 +484,7 @@ def main():
     seeds = uniform_seeds(W, H, N, 2)
     lib = _lib.load()
     ctx = _lib.context()
+    _lib.check(lib.dl_set_autotune(ctx.handle, 1 if args.autotune else 0))
     if args.bfs_config:
         w, r, k = (int(v) for v in args.bfs_config.split(","))
         _lib.check(lib.dl_set_bfs_config(ctx.handle, w, r, k))

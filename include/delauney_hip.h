@@ -252,6 +252,16 @@ int dl_get_bfs_config(dl_context* ctx, int* waves, int* rows, int* iters);
 int dl_set_autotune(dl_context* ctx, int enabled);
 int dl_autotune_reset(void);
 
+/* Launch-count prediction (default on).  A padded fill of a canvas class
+ * (as above) and shape seen before on this context is issued as the launch
+ * count that fill needed plus a margin, without waiting for the host; the
+ * pipeline checks convergence with its final status read and, on a shortfall,
+ * continues the fill and recomputes detection and assignment.  Results are
+ * the same either way; off trades the saved host wait for a steady latency
+ * when the inputs of a class vary (the serving loop turns it off).  No
+ * reference counterpart. */
+int dl_set_fill_prediction(dl_context* ctx, int enabled);
+
 /* Device-resident triangulate: seeds, output and triangles stay on the device.
  * d_seeds_xy: n (x, y) int32 pairs; d_out_hw3: (H, W, 3) int32.  The triangle
  * rows stay in the context (dl_device_triangles).  No reference counterpart;

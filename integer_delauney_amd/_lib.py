@@ -89,6 +89,7 @@ _SIGNATURES = {
     "dl_session_copy_seeds": (ctypes.c_int, [_VP, _I32P]),
     "dl_session_device_grid": (_VP, [_VP]),
     "dl_set_autotune": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_set_fill_prediction": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_autotune_reset": (ctypes.c_int, []),
     "dl_device_to_host": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
     "dl_session_has_values": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),

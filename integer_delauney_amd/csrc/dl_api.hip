@@ -173,6 +173,7 @@ struct dl_context {
     BfsConfig cfg{8, 8, 8};
     bool cfg_auto = true;
     bool autotune = true;   // auto mode may time the candidate shapes on a new canvas class
+    bool predict = true;    // padded fills of known classes issued without waiting
     // Launches the padded fill of a canvas class and shape needed last time
     // (plus a margin): the next fill of the class issues that many without
     // waiting (see fill_detection).
@@ -430,7 +431,7 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     int plane = 0;
     const auto pkey = pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg);
     int pred = 0;
-    if (pend && !tune) {
+    if (pend && !tune && ctx->predict) {
         const auto it = ctx->fill_pred.find(pkey);
         if (it != ctx->fill_pred.end() && it->second <= kMaxPredicted) pred = it->second;
     }
@@ -879,6 +880,14 @@ int dl_set_autotune(dl_context* ctx, int enabled)
     return guarded([&] {
         if (!ctx) throw value_error("null context");
         ctx->autotune = enabled != 0;
+    });
+}
+
+int dl_set_fill_prediction(dl_context* ctx, int enabled)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->predict = enabled != 0;
     });
 }
 

@@ -90,6 +90,12 @@ class Server:
         # the host while another slot's batch computes.
         self._max_batch_bytes = int(max_batch_bytes)
         self._ctxs = [_lib.Context(device) for _ in range(slots)]
+        for c in self._ctxs:
+            # Mixed request sizes: a steady latency matters more than the
+            # first-use tuning of tile shapes or the launch-count prediction
+            # (both would occasionally spend a request's time on extra work).
+            _lib.check(self._lib.dl_set_autotune(c.handle, 0))
+            _lib.check(self._lib.dl_set_fill_prediction(c.handle, 0))
         # results land in page-locked memory: the copy back runs at link speed
         self._pool = _lib.PinnedPool()
         self._cond = threading.Condition()
