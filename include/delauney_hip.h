@@ -308,6 +308,11 @@ int dl_set_force_unpadded(dl_context* ctx, int enabled);
 int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* seeds_xy,
                         int64_t n, int margin, int32_t* out_hw2, int* escaped);
 
+/* Test hook: fills with at least min_tiles tiles (default 8192; negative
+ * restores it) run their launches after the first over a list of the active
+ * tiles; 0 forces that path on every fill.  Results are the same either way. */
+int dl_set_fill_list_threshold(dl_context* ctx, int min_tiles);
+
 /* Cross-checking only: makes the speculative sizes of detection and
  * assignment too small on purpose, so that every call takes the exact-size
  * rerun.  Results are unchanged.  No reference counterpart. */

@@ -104,6 +104,7 @@ _SIGNATURES = {
                                                    ctypes.c_float, _I32P, ctypes.c_int64, _I64P]),
     "dl_session_finalise_device": (ctypes.c_int, [_VP, _VP, _I64P, ctypes.POINTER(_VP),
                                                   ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
+    "dl_set_fill_list_threshold": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_set_tiny_capacities": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_debug_fill_watch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                            ctypes.c_int, _I32P, ctypes.POINTER(ctypes.c_int)]),

@@ -944,6 +944,14 @@ int dl_set_force_unpadded(dl_context* ctx, int enabled)
     });
 }
 
+int dl_set_fill_list_threshold(dl_context* ctx, int min_tiles)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->bfs.list_min_tiles = min_tiles < 0 ? 8192 : min_tiles;
+    });
+}
+
 int dl_set_tiny_capacities(dl_context* ctx, int enabled)
 {
     return guarded([&] {

@@ -256,8 +256,8 @@ __host__ __device__ inline bool tile_watched(int P, int W, int H, int tx, int ty
 }
 
 template <int NW, int R, int K>
-__global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
-    const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
+__device__ __forceinline__ void tile_body(
+    const BfsCanvas* __restrict__ cvs, int n_canvas, int tile, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
     int* __restrict__ changed_any, int* __restrict__ visits)
 {
@@ -266,15 +266,6 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     __shared__ uint32_t ldsA[NW][64][2];
     __shared__ uint32_t ldsB[NW][64][2];
     __shared__ int lds_any[2][NW];   // per-wave flags (see iterate)
-
-    // Contiguous tile ranges per XCD: consecutive dispatches go round-robin over
-    // the 8 XCDs, so give each XCD a contiguous slice of tiles (neighbouring
-    // tiles share halo lines in the same L2).  The grid is launched as a
-    // multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
-    const int per_xcd = gridDim.x / 8;
-    const int b = blockIdx.x;
-    const int tile = (b % 8) * per_xcd + b / 8;
-    if (tile >= total_tiles) return;
 
     int ci = 0;
     if (n_canvas > 1) {
@@ -441,6 +432,100 @@ __global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
     }
 }
 
+// One block per tile.  Contiguous tile ranges per XCD: consecutive dispatches
+// go round-robin over the 8 XCDs, so give each XCD a contiguous slice of tiles
+// (neighbouring tiles share halo lines in the same L2).  The grid is launched
+// as a multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
+template <int NW, int R, int K>
+__global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
+    const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
+    const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
+    int* __restrict__ changed_any, int* __restrict__ visits)
+{
+    const int per_xcd = gridDim.x / 8;
+    const int b = blockIdx.x;
+    const int tile = (b % 8) * per_xcd + b / 8;
+    if (tile >= total_tiles) return;
+    tile_body<NW, R, K>(cvs, n_canvas, tile, src_plane, flags_prev, flags_cur, first_launch,
+                        changed_any, visits);
+}
+
+// The same launch over a list of the active tiles (bfs_active_list_kernel):
+// a fixed grid strides over the list, each XCD over a contiguous eighth of it.
+// On large canvases late launches have few active tiles, and a block per tile
+// would spend most of the launch starting and retiring idle blocks.
+template <int NW, int R, int K>
+__global__ __launch_bounds__(NW * 64) void bfs_list_kernel(
+    const BfsCanvas* __restrict__ cvs, int n_canvas, const int* __restrict__ list,
+    const int* __restrict__ count, int src_plane, const int* __restrict__ flags_prev,
+    int* __restrict__ flags_cur, int* __restrict__ changed_any, int* __restrict__ visits)
+{
+    const int n = *count;
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, rank = blockIdx.x >> 3;
+    const int lo = static_cast<int>(static_cast<int64_t>(n) * xcd / 8);
+    const int hi = static_cast<int>(static_cast<int64_t>(n) * (xcd + 1) / 8);
+    for (int i = lo + rank; i < hi; i += per) {
+        tile_body<NW, R, K>(cvs, n_canvas, list[i], src_plane, flags_prev, flags_cur, 0,
+                            changed_any, visits);
+        __syncthreads();   // the next tile reuses the LDS exchange buffers
+    }
+}
+
+// Tiles active in the coming launch (some neighbour's last change faces them,
+// as in tile_body's own test), appended to `list` (one atomic per block; the
+// order is irrelevant: tiles of a launch are independent).  Also clears the
+// flags the coming launch writes, since it visits only listed tiles.
+__global__ __launch_bounds__(256) void bfs_active_list_kernel(
+    const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles,
+    const int* __restrict__ flags_prev, int* __restrict__ flags_next, int* __restrict__ list,
+    int* __restrict__ count)
+{
+    __shared__ int wave_n[4];
+    __shared__ int base;
+    const int tile = blockIdx.x * 256 + threadIdx.x;
+    bool act = false;
+    if (tile < total_tiles) {
+        flags_next[tile] = 0;
+        int ci = 0;
+        if (n_canvas > 1) {
+            int lo = 0, hi = n_canvas - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (cvs[mid].tile_base <= tile) lo = mid; else hi = mid - 1;
+            }
+            ci = lo;
+        }
+        const BfsCanvas& cv = cvs[ci];
+        const int local = tile - cv.tile_base;
+        const int tx = local % cv.tiles_x, ty = local / cv.tiles_x;
+        constexpr int kFacing[9] = {kChBR, kChBottom, kChBL, kChRight, kChAny,
+                                    kChLeft, kChTR, kChTop, kChTL};
+        int f = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
+            if (nx >= 0 && ny >= 0 && nx < cv.tiles_x && ny < cv.tiles_y)
+                f |= flags_prev[cv.tile_base + ny * cv.tiles_x + nx] & kFacing[k];
+        }
+        act = f != 0;
+    }
+    const uint64_t m = __ballot(act);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wave_n[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+        base = t ? atomicAdd(count, t) : 0;
+    }
+    __syncthreads();
+    if (act) {
+        int off = base;
+        for (int k = 0; k < w; ++k) off += wave_n[k];
+        off += __popcll(m & ((1ull << lane) - 1ull));
+        list[off] = tile;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Simple version: one thread per cell, one pass per launch, ping-pong planes.
 // ---------------------------------------------------------------------------
@@ -513,13 +598,17 @@ __global__ void zero_ints_kernel(int* p, int n)
 }
 
 using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*, int*);
+using ListKernel = void (*)(const BfsCanvas*, int, const int*, const int*, int, const int*, int*,
+                            int*, int*);
 
 struct Variant {
     int waves, rows, iters;
     TileKernel fn;
+    ListKernel list_fn;
 };
 
-#define DL_VARIANT(W_, R_, K_) {W_, R_, K_, bfs_tile_kernel<W_, R_, K_>}
+#define DL_VARIANT(W_, R_, K_) \
+    {W_, R_, K_, bfs_tile_kernel<W_, R_, K_>, bfs_list_kernel<W_, R_, K_>}
 
 // Instantiated configurations.  The tile is 128 wide; height waves*rows.
 const Variant kVariants[] = {
@@ -533,6 +622,38 @@ const Variant* find_variant(const BfsConfig& c)
     for (const auto& v : kVariants)
         if (v.waves == c.waves && v.rows == c.rows && v.iters == c.iters) return &v;
     return nullptr;
+}
+
+// Launches after the first go through the active-tile list on fills of at
+// least BfsWork::list_min_tiles tiles; the list launch uses a fixed grid.
+constexpr int kListGrid = 2048;
+
+// One tile launch of the fill: launch 0 (and every launch of a small fill)
+// visits every tile; later launches of a large fill list the active tiles
+// first.  changed / listcnt: this launch's zeroed counters.
+void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_canvas, int total,
+                  int launch, int* changed, int* listcnt, hipStream_t s)
+{
+    int* flags = work.flags.as<int>();
+    const int src_plane = launch & 1;
+    int* fprev = flags + ((launch + 1) & 1) * total;
+    int* fcur = flags + (launch & 1) * total;
+    const BfsCanvas* cv = reinterpret_cast<const BfsCanvas*>(work.canvases.ptr);
+    int* visits = work.visits.as<int>();
+    ProfScope ps(kKFill, s);
+    if (launch > 0 && total >= work.list_min_tiles) {
+        int* list = work.list.as<int>();
+        bfs_active_list_kernel<<<(total + 255) / 256, 256, 0, s>>>(cv, n_canvas, total, fprev, fcur,
+                                                                   list, listcnt);
+        DL_LAUNCH_CHECK();
+        v->list_fn<<<kListGrid, cfg.waves * 64, 0, s>>>(cv, n_canvas, list, listcnt, src_plane,
+                                                        fprev, fcur, changed, visits);
+    } else {
+        const int grid_blocks = (total + 7) / 8 * 8;
+        v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(cv, n_canvas, total, src_plane, fprev, fcur,
+                                                    launch == 0 ? 1 : 0, changed, visits);
+    }
+    DL_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -581,6 +702,8 @@ static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     DL_HIP(hipMemcpyAsync(work.canvases.ptr, canvases.data(), sizeof(BfsCanvas) * n_canvas,
                           hipMemcpyHostToDevice, s));
     work.flags.reserve(sizeof(int) * 2 * static_cast<size_t>(total));
+    work.list.reserve(sizeof(int) * static_cast<size_t>(total));
+    work.visits.reserve(sizeof(int) * static_cast<size_t>(total));
     *max_iters_out = max_iters;
     return total;
 }
@@ -594,22 +717,15 @@ int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     const int total = bfs_setup(canvases, cfg, work, s, &max_iters);
     if (total == 0 || n_launches <= 0) return 0;
     const int n_canvas = static_cast<int>(canvases.size());
-    int* flags = work.flags.as<int>();
-    work.counters.reserve(sizeof(int) * static_cast<size_t>(n_launches + 1));
-    work.visits.reserve(sizeof(int) * static_cast<size_t>(total));
+    // counters: n changed flags, the visit total, then n list counts
+    work.counters.reserve(sizeof(int) * 2 * static_cast<size_t>(n_launches + 1));
     int* dcnt = work.counters.as<int>();
     int* visits = work.visits.as<int>();
     fill_u32(reinterpret_cast<uint32_t*>(visits), 0u, static_cast<size_t>(total), s);
-    fill_u32(reinterpret_cast<uint32_t*>(dcnt), 0u, static_cast<size_t>(n_launches + 1), s);
-    const int grid_blocks = (total + 7) / 8 * 8;
-    for (int launch = 0; launch < n_launches; ++launch) {
-        ProfScope ps(kKFill, s);
-        v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
-            reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total, launch & 1,
-            flags + ((launch + 1) & 1) * total, flags + (launch & 1) * total,
-            launch == 0 ? 1 : 0, dcnt + launch, visits);
-        DL_LAUNCH_CHECK();
-    }
+    fill_u32(reinterpret_cast<uint32_t*>(dcnt), 0u, 2 * static_cast<size_t>(n_launches + 1), s);
+    for (int launch = 0; launch < n_launches; ++launch)
+        launch_tiles(v, cfg, work, n_canvas, total, launch, dcnt + launch,
+                     dcnt + n_launches + 1 + launch, s);
     sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + n_launches);
     DL_LAUNCH_CHECK();
     DL_HIP(hipMemcpyAsync(host_changed, dcnt, sizeof(int) * static_cast<size_t>(n_launches + 1),
@@ -626,7 +742,6 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     const int total = bfs_setup(canvases, cfg, work, s, &max_iters);
     const int n_canvas = static_cast<int>(canvases.size());
     if (total == 0) return 0;
-    int* flags = work.flags.as<int>();
 
     // Launches are issued in chunks; the host reads the changed-tile counter of
     // a chunk while the next one runs.  Launches after convergence find no
@@ -655,21 +770,12 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     int pending_chunk = 0;
     BfsStats st;
 
-    const int grid_blocks = (total + 7) / 8 * 8;
     auto issue_chunk = [&](int ch) {
         const int slot0 = (ch % 2) * kChunk;
         zero_ints_kernel<<<1, 64, 0, s>>>(dcnt + 2 * slot0, 2 * kChunk);
         for (int j = 0; j < kChunk; ++j) {
-            const int src_plane = launch & 1;
-            int* fprev = flags + ((launch + 1) & 1) * total;
-            int* fcur = flags + (launch & 1) * total;
-            {
-                ProfScope ps(kKFill, s);
-                v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(
-                    reinterpret_cast<const BfsCanvas*>(work.canvases.ptr), n_canvas, total,
-                    src_plane, fprev, fcur, launch == 0 ? 1 : 0, dcnt + 2 * (slot0 + j), visits);
-                DL_LAUNCH_CHECK();
-            }
+            launch_tiles(v, cfg, work, n_canvas, total, launch, dcnt + 2 * (slot0 + j),
+                         dcnt + 2 * (slot0 + j) + 1, s);
             ++launch;
         }
         DL_HIP(hipMemcpyAsync(hcnt + 2 * slot0, dcnt + 2 * slot0, sizeof(int) * 2 * kChunk,

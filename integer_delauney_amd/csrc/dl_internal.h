@@ -106,6 +106,8 @@ struct BfsWork {
     DevBuf flags;     // 2 * total_tiles ints
     DevBuf counters;  // per-launch "some tile changed" flags, then the visit total
     DevBuf visits;    // per tile: launches that processed it
+    DevBuf list;      // active tiles of the coming launch (large fills)
+    int list_min_tiles = 8192;   // fills with fewer tiles launch a block per tile
     DevBuf canvases;  // device copy of the canvas table
     DevBuf tile_map;  // tile -> canvas index (multi-canvas launches)
     PinnedBuf host_counters;

@@ -326,3 +326,39 @@ def test_predicted_fill_and_its_shortfall_are_exact():
         m, got = dl.triangulate(W, H, seeds)
         np.testing.assert_array_equal(got, ref)
         np.testing.assert_array_equal(_rows(m), rows)
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_active_tile_list_fill_equals_oracle(case):
+    """Large fills run their later launches over a device-built list of the
+    active tiles.  Forced on every fill here (threshold 0), on small cases
+    against the oracle: uniform, clustered and dense seeds, ragged sizes, every
+    padding rule, batches of several canvases."""
+    from integer_delauney_amd import _lib
+    from integer_delauney_amd.workloads import clustered_seeds
+    lib = _lib.load()
+    ctx = _lib.context()
+    rng = np.random.default_rng(900 + case)
+    W, H = int(rng.integers(40, 700)), int(rng.integers(3, 500))
+    n = int(rng.integers(1, max(2, W * H // 40)))
+    if case % 3 == 1 and W * H > 2000:
+        s = clustered_seeds(W, H, min(n, W * H // 4), 900 + case)
+    else:
+        idx = rng.choice(W * H, size=min(n, W * H), replace=False)
+        s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    P = [None, 0, 3, 16][case % 4]
+    _lib.check(lib.dl_set_fill_list_threshold(ctx.handle, 0))
+    try:
+        rows, ref = O.triangulate(W, H, s, P, return_rows=True)
+        m, got = dl.triangulate(W, H, s, -1 if P is None else P)
+        np.testing.assert_array_equal(got, ref, err_msg=f"W={W} H={H} n={len(s)} P={P}")
+        np.testing.assert_array_equal(_rows(m), rows)
+        if case % 4 == 0:
+            imgs = [(W, H, s), (H + 5, W + 2, np.ascontiguousarray(s[:, ::-1]))]
+            res = dl.triangulate_batch(imgs)
+            for (w, h, ss), (mb, gb) in zip(imgs, res):
+                r2, ref2 = O.triangulate(w, h, ss, None, return_rows=True)
+                np.testing.assert_array_equal(gb, ref2)
+                np.testing.assert_array_equal(_rows(mb), r2)
+    finally:
+        _lib.check(lib.dl_set_fill_list_threshold(ctx.handle, -1))
