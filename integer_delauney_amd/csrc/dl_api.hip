@@ -158,7 +158,7 @@ struct dl_context {
     BfsWork bfs;
     TriWork tri;
     BatchPlan plan;   // images of the last call
-    DevBuf xy, sorted, rank_at;
+    DevBuf xy, sorted;
     DevBuf padA, padB, unA, unB;
     DevBuf out, vgrid, ids, esc;
     PinnedBuf host_small;
@@ -373,7 +373,7 @@ struct StageClock {
 };
 
 // Plans ctx->plan (images already listed), uploads it and orders the seeds on
-// the device into ctx->sorted (id order per image) and ctx->rank_at.  Images
+// the device into ctx->sorted (id order per image) and the seed index.  Images
 // without their own seed / output pointers read seeds from d_xy and write
 // into d_out at their offsets in the concatenated layout (C channels).
 void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s,
@@ -387,9 +387,7 @@ void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStr
     }
     upload_plan(plan, ctx->tri, s);
     ctx->sorted.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_seeds, 1)));
-    ctx->rank_at.reserve(sizeof(int32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_pix, 1)));
-    normalise_seeds(plan, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), ctx->tri, s,
-                    defer_seed_check);
+    normalise_seeds(plan, ctx->sorted.as<uint32_t>(), ctx->tri, s, defer_seed_check);
 }
 
 // Flood fill of every image's detection canvas ((W + 2P) x (H + 2P), seeds
@@ -622,13 +620,11 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
     clk.mark(1);
     const int32_t* rows = ctx->tri.rows.as<int32_t>();
     auto detect_and_assign = [&]() {
-        detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(),
-                         ctx->tri, s, true);
+        detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true);
         clk.mark(2);
         clk.mark(3);
         rows = ctx->tri.rows.as<int32_t>();
-        assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), C,
-                     ctx->tri, s, true);
+        assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, true);
         clk.mark(4);
         return read_status(plan, ctx->tri, any_pad ? ctx->esc.as<int>() : nullptr,
                            any_pad ? n_img : 0, s);
@@ -641,11 +637,10 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
     ctx->timings[8] += *visits_async;
     if (st.bad_id) throw runtime_error("flood fill left an undefined cell");
     if (st.redo) {
-        const DetectResult dr = detect_triangles(plan, base, true, ctx->rank_at.as<int32_t>(),
-                                                 ctx->sorted.as<uint32_t>(), ctx->tri, s, false);
+        const DetectResult dr =
+            detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, false);
         rows = ctx->tri.rows.as<int32_t>();
-        assign_tiled(plan, rows, dr.n_tri, ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(),
-                     C, ctx->tri, s, false);
+        assign_tiled(plan, rows, dr.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
         st.n_tri = dr.n_tri;
     }
     if (labels_out) {
@@ -659,8 +654,7 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
         if (!which.empty()) {
             fill_unpadded(ctx, which, s);
             upload_desc(plan, ctx->tri, s);
-            assign_tiled(plan, rows, st.n_tri, ctx->sorted.as<uint32_t>(),
-                         ctx->rank_at.as<int32_t>(), C, ctx->tri, s, false);
+            assign_tiled(plan, rows, st.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
         }
     }
     ctx->n_tri = st.n_tri;
@@ -980,7 +974,7 @@ int dl_voronoi(dl_context* ctx, int width, int height, const int32_t* seeds_xy, 
         const size_t px = static_cast<size_t>(width) * height;
         ctx->out.reserve(sizeof(int32_t) * 2 * px);
         write_output(width, height, pad_plane_ptr(ctx, plane), width, 0, 0,
-                     ctx->rank_at.as<int32_t>(), width, nullptr, ctx->out.as<int32_t>(), 2, s);
+                     seed_index(ctx->tri), 0, 0, height, nullptr, ctx->out.as<int32_t>(), 2, s);
         DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * px,
                               hipMemcpyDeviceToHost, s));
         ctx->have_padded = false;
@@ -1009,7 +1003,7 @@ int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* s
             const size_t px = static_cast<size_t>(width) * height;
             ctx->out.reserve(sizeof(int32_t) * 2 * px);
             write_output(width, height, pad_plane_ptr(ctx, plane), width, 0, 0,
-                         ctx->rank_at.as<int32_t>(), width, nullptr, ctx->out.as<int32_t>(), 2, s);
+                         seed_index(ctx->tri), 0, 0, height, nullptr, ctx->out.as<int32_t>(), 2, s);
             DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * px,
                                   hipMemcpyDeviceToHost, s));
         }
@@ -1043,7 +1037,7 @@ int dl_voronoi_simple(dl_context* ctx, int width, int height, const int32_t* see
         const int plane = bfs_run_simple(cv, ctx->bfs, s, iters);
         ctx->out.reserve(sizeof(int32_t) * 2 * cells);
         write_output(width, height, plane ? cv.lab1 : cv.lab0, width, 0, 0,
-                     ctx->rank_at.as<int32_t>(), width, nullptr, ctx->out.as<int32_t>(), 2, s);
+                     seed_index(ctx->tri), 0, 0, height, nullptr, ctx->out.as<int32_t>(), 2, s);
         DL_HIP(hipMemcpyAsync(out_hw2, ctx->out.ptr, sizeof(int32_t) * 2 * cells,
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
@@ -1178,18 +1172,15 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
                 ctx->vgrid.as<int32_t>(), ctx->ids.as<uint32_t>(), static_cast<int64_t>(px));
             DL_LAUNCH_CHECK();
             clk.mark(1);
-            const DetectResult dr =
-                detect_triangles(ctx->plan, ctx->ids.as<uint32_t>(), false,
-                                 ctx->rank_at.as<int32_t>(), ctx->sorted.as<uint32_t>(), ctx->tri, s,
-                                 false);
+            const DetectResult dr = detect_triangles(ctx->plan, ctx->ids.as<uint32_t>(), false,
+                                                     ctx->sorted.as<uint32_t>(), ctx->tri, s, false);
             clk.mark(2);
             clk.mark(3);
             ctx->n_tri = dr.n_tri;
             ctx->plan.img[0].out = ctx->tri.tidgrid.as<int32_t>();
             upload_desc(ctx->plan, ctx->tri, s);
             assign_tiled(ctx->plan, ctx->tri.rows.as<int32_t>(), dr.n_tri,
-                         ctx->sorted.as<uint32_t>(), ctx->rank_at.as<int32_t>(), 1, ctx->tri, s,
-                         false);
+                         ctx->sorted.as<uint32_t>(), 1, ctx->tri, s, false);
             clk.mark(4);
         }
         DL_HIP(hipMemcpyAsync(out_tid_hw, ctx->tri.tidgrid.ptr, sizeof(int32_t) * px,
@@ -1232,7 +1223,8 @@ int dl_copy_padded_voronoi(dl_context* ctx, int32_t* out, int* out_w, int* out_h
         const size_t cells = static_cast<size_t>(Wc) * Hc;
         ctx->out.reserve(sizeof(int32_t) * 2 * cells);
         write_output(Wc, Hc, pad_plane_ptr(ctx, ctx->pad_plane), Wc, 0, P,
-                     ctx->rank_at.as<int32_t>(), d.W, nullptr, ctx->out.as<int32_t>(), 2, s);
+                     seed_index(ctx->tri), d.bm_off, d.seed_off, d.H, nullptr, ctx->out.as<int32_t>(),
+                     2, s);
         DL_HIP(hipMemcpyAsync(out, ctx->out.ptr, sizeof(int32_t) * 2 * cells,
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));

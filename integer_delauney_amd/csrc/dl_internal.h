@@ -163,7 +163,7 @@ struct ImageDesc {
     int64_t n;            // seeds
     int64_t seed_off;     // first seed: xy pairs, sorted labels
     int64_t bm_off;       // first word of the seed presence bitmap (W*H bits)
-    int64_t pix_off;      // first pixel: rank_at and output (times channels)
+    int64_t pix_off;      // first pixel of the output (times channels)
     int64_t cv_off;       // first cell of the detection canvas planes
     int64_t nb;           // 2x2 blocks of the canvas, (Wc-1)*(Hc-1)
     int64_t det_off;      // first detection thread block
@@ -194,8 +194,34 @@ void plan_batch(BatchPlan& plan);
 
 struct TriWork;
 
+// Sorted seed id of a seed position: the position's rank among the set bits
+// of the presence bitmap (bit x*H + y of an image, words from bm_off on).
+// Each word is stored with the exclusive popcount prefix of all words before
+// it (the seeds of earlier images included), so a lookup is one 8-byte load
+// from an array of W*H/4 bytes that stays in cache, where an image-sized id
+// map would be a scattered HBM read.
+struct SeedIndex {
+    const uint2* wp = nullptr;   // per bitmap word: (prefix, word)
+};
+
+// seed_off: the image's first seed (= the prefix at its first word).  -1
+// where no seed sits (only for a seed set that failed validation).
+__device__ __forceinline__ int32_t seed_id_at(const SeedIndex& si, int64_t bm_off,
+                                             int64_t seed_off, int H, int x, int y)
+{
+    const uint64_t bit = static_cast<uint64_t>(x) * static_cast<uint64_t>(H) +
+                         static_cast<uint64_t>(y);
+    const uint2 e = si.wp[bm_off + static_cast<int64_t>(bit >> 5)];
+    const uint32_t m = 1u << (bit & 31);
+    if (!(e.y & m)) return -1;
+    return static_cast<int32_t>(static_cast<int64_t>(e.x) - seed_off +
+                                __popc(e.y & (m - 1u)));
+}
+
 struct TriWork {
-    DevBuf bitmap;      // seed presence / registration bitmaps
+    DevBuf seed_bm, seed_pre;   // seed presence bitmaps and their popcount prefix
+    DevBuf seed_idx;            // both, interleaved per word (SeedIndex)
+    DevBuf bitmap;      // registration bitmaps
     DevBuf scan;        // word prefix sums
     DevBuf scan_tmp;
     DevBuf flags;       // device status flags (per image, then batch-wide)
@@ -230,8 +256,12 @@ void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s);
 // call returns at once; check_seed_flags must then run after the stream's next
 // synchronisation point.  Until then invalid seeds only produce garbage labels
 // (every kernel in between stays in bounds).
-void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_at,
-                     TriWork& work, hipStream_t s, bool defer_check = false);
+void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, hipStream_t s,
+                     bool defer_check = false);
+inline SeedIndex seed_index(const TriWork& w)
+{
+    return SeedIndex{w.seed_idx.as<uint2>()};
+}
 void check_seed_flags(const BatchPlan& plan, const TriWork& work);
 
 // Writes kInvalid over the first plane of every canvas, then each seed's
@@ -257,7 +287,7 @@ struct DetectResult {
 
 // Triangle extraction on every image's canvas.  grid_base + cv_off is image
 // i's canvas; it holds packed canvas labels (labels = true; ids through
-// rank_at) or seed ids (labels = false).  Writes rows
+// the seed index) or seed ids (labels = false).  Writes rows
 // (x - P, y - P, a, b, c) of all images, image after image, into work.rows and
 // the first row of each image into work.tri_base (host) and the device table.
 //
@@ -266,7 +296,7 @@ struct DetectResult {
 // seed count without any sync; capacity overflows, hash collisions and the
 // triangle counts are then checked by read_status after the assignment.
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
-                              const int32_t* rank_at, const uint32_t* sorted, TriWork& work,
+                              const uint32_t* sorted, TriWork& work,
                               hipStream_t s, bool speculative);
 
 struct SpeculativeStatus {
@@ -287,14 +317,15 @@ SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d
 // n_tri < 0: the count is on the device (after a speculative detection);
 // speculative = true sizes the bin list without a sync (checked by read_status).
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
-                  const uint32_t* sorted, const int32_t* rank_at, int C, TriWork& work,
+                  const uint32_t* sorted, int C, TriWork& work,
                   hipStream_t s, bool speculative);
 
 // (H, W, C) int32 output of one image without triangles (C == 2) or with the
 // tid raster (C == 3).  Output pixel (x, y) reads canvas cell (x + off, y + off);
 // the owner's image position is its canvas position minus shift.
 void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, int shift,
-                  const int32_t* rank_at, int W_img, const int32_t* tidgrid, int32_t* out, int C,
+                  const SeedIndex& si, int64_t bm_off, int64_t seed_off, int H_img,
+                  const int32_t* tidgrid, int32_t* out, int C,
                   hipStream_t s);
 
 void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);

@@ -307,8 +307,7 @@ __global__ void seeds_mark_kernel(int64_t n_total, const int64_t* __restrict__ s
 
 __global__ void seeds_rank_kernel(int64_t n_total, const int64_t* __restrict__ seed_seg, int n_img,
                                   const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
-                                  const uint32_t* __restrict__ pre, uint32_t* __restrict__ sorted,
-                                  int32_t* __restrict__ rank_at)
+                                  const uint32_t* __restrict__ pre, uint32_t* __restrict__ sorted)
 {
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= n_total) return;
@@ -322,7 +321,14 @@ __global__ void seeds_rank_kernel(int64_t n_total, const int64_t* __restrict__ s
     const uint32_t below = bm[w] & ((1u << (bit & 31)) - 1u);
     const uint32_t rank = pre[w] - pre[d.bm_off] + static_cast<uint32_t>(__popc(below));
     sorted[d.seed_off + rank] = pack_label(x, y);
-    rank_at[d.pix_off + static_cast<int64_t>(y) * d.W + x] = static_cast<int32_t>(rank);
+}
+
+__global__ void interleave_index_kernel(const uint32_t* __restrict__ bm,
+                                        const uint32_t* __restrict__ pre, int64_t nw,
+                                        uint2* __restrict__ out)
+{
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (w < nw) out[w] = make_uint2(pre[w], bm[w]);
 }
 
 // Seed cells of every canvas: the image label shifted by (P, P).
@@ -360,16 +366,16 @@ void check_seed_flags(const BatchPlan& plan, const TriWork& work)
     }
 }
 
-void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_at,
-                     TriWork& work, hipStream_t s, bool defer_check)
+void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, hipStream_t s,
+                     bool defer_check)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t nw = plan.n_words;
-    work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(nw, 1)));
-    work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nw + 1));
+    work.seed_bm.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(nw, 1)));
+    work.seed_pre.reserve(sizeof(uint32_t) * static_cast<size_t>(nw + 1));
     work.flags.reserve(sizeof(uint32_t) * (n_img + 8));
     work.host_flags.reserve(sizeof(uint32_t) * (n_img + 8));
-    uint32_t* bm = work.bitmap.as<uint32_t>();
+    uint32_t* bm = work.seed_bm.as<uint32_t>();
     uint32_t* flags = work.flags.as<uint32_t>();
     const ImageDesc* D = work.desc.as<ImageDesc>();
     const int64_t* sseg = seg_ptr(work, 0, n_img);
@@ -390,9 +396,13 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, int32_t* d_rank_
         check_seed_flags(plan, work);
     }
     ProfScope ps(kKSeeds, s);
-    popcount_scan(bm, nw, work.scan.as<uint32_t>(), work.scan_tmp, s);
+    popcount_scan(bm, nw, work.seed_pre.as<uint32_t>(), work.scan_tmp, s);
     seeds_rank_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm,
-                                             work.scan.as<uint32_t>(), d_sorted, d_rank_at);
+                                             work.seed_pre.as<uint32_t>(), d_sorted);
+    DL_LAUNCH_CHECK();
+    work.seed_idx.reserve(sizeof(uint2) * static_cast<size_t>(std::max<int64_t>(nw, 1)));
+    interleave_index_kernel<<<static_cast<unsigned>(std::max<int64_t>(1, (nw + 255) / 256)), 256, 0,
+                              s>>>(bm, work.seed_pre.as<uint32_t>(), nw, work.seed_idx.as<uint2>());
     DL_LAUNCH_CHECK();
 }
 
@@ -575,7 +585,7 @@ struct DetectArgs {
     const ImageDesc* D;
     const int64_t* det_seg;       // first detection thread block of each image
     int n_img;
-    const int32_t* rank_at;       // image-space id maps (labels mode)
+    SeedIndex si;                 // position -> sorted id (labels mode)
     const uint32_t* sorted;       // id -> packed image position, per image
     uint2* positions;             // list of (image, block) of blocks holding candidates
     uint32_t* masks;              // per thread: its candidate blocks (count pass)
@@ -598,8 +608,8 @@ __device__ __forceinline__ int32_t to_id(const DetectArgs& A, const ImageDesc& d
             return -1;
         }
         const int lx = static_cast<int>(v >> 16) - d.P, ly = static_cast<int>(v & 0xFFFFu) - d.P;
-        const int32_t id = A.rank_at[d.pix_off + static_cast<int64_t>(ly) * d.W + lx];
-        // Out of range only when the seeds failed validation (read back late).
+        const int32_t id = seed_id_at(A.si, d.bm_off, d.seed_off, d.H, lx, ly);
+        // No seed there only when the seeds failed validation (read back late).
         if (id < 0 || id >= d.n) {
             atomicOr(A.flags, kFlagBadId);
             return -1;
@@ -1018,7 +1028,7 @@ uint64_t next_pow2(uint64_t v)
 }  // namespace
 
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
-                              const int32_t* rank_at, const uint32_t* sorted, TriWork& work,
+                              const uint32_t* sorted, TriWork& work,
                               hipStream_t s, bool speculative)
 {
     DetectResult res;
@@ -1050,7 +1060,7 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     A.D = work.desc.as<ImageDesc>();
     A.det_seg = seg_ptr(work, 1, n_img);
     A.n_img = n_img;
-    A.rank_at = rank_at;
+    A.si = seed_index(work);
     A.sorted = sorted;
     A.flags = dflags;
     A.npos = dflags + 6;
@@ -1230,8 +1240,9 @@ namespace {
 // (x + off, y + off); the owner's image position is its canvas position - shift.
 template <int C>
 __global__ void output_kernel(int W, int H, const uint32_t* __restrict__ lab, int Wc, int off,
-                              int shift, const int32_t* __restrict__ rank_at, int W_img,
-                              const int32_t* __restrict__ tid, int32_t* __restrict__ out)
+                              int shift, SeedIndex si, int64_t bm_off, int64_t seed_off,
+                              int H_img, const int32_t* __restrict__ tid,
+                              int32_t* __restrict__ out)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
@@ -1245,7 +1256,7 @@ __global__ void output_kernel(int W, int H, const uint32_t* __restrict__ lab, in
         o[0] = -1;
         o[1] = -1;
     } else {
-        o[0] = rank_at[static_cast<size_t>(ly - shift) * W_img + (lx - shift)];
+        o[0] = seed_id_at(si, bm_off, seed_off, H_img, lx - shift, ly - shift);
         o[1] = dx * dx + dy * dy;
     }
     if (C == 3) o[2] = tid[p];
@@ -1254,17 +1265,17 @@ __global__ void output_kernel(int W, int H, const uint32_t* __restrict__ lab, in
 }  // namespace
 
 void write_output(int W, int H, const uint32_t* canvas_labels, int Wc, int off, int shift,
-                  const int32_t* rank_at, int W_img, const int32_t* tidgrid, int32_t* out, int C,
-                  hipStream_t s)
+                  const SeedIndex& si, int64_t bm_off, int64_t seed_off, int H_img,
+                  const int32_t* tidgrid, int32_t* out, int C, hipStream_t s)
 {
     ProfScope ps(kKOutput, s);
     dim3 grid((W + 255) / 256, H);
     if (C == 3)
-        output_kernel<3><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, rank_at, W_img,
-                                              tidgrid, out);
+        output_kernel<3><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, si, bm_off,
+                                              seed_off, H_img, tidgrid, out);
     else
-        output_kernel<2><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, rank_at, W_img,
-                                              tidgrid, out);
+        output_kernel<2><<<grid, 256, 0, s>>>(W, H, canvas_labels, Wc, off, shift, si, bm_off,
+                                              seed_off, H_img, tidgrid, out);
     DL_LAUNCH_CHECK();
 }
 
@@ -1431,7 +1442,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int64_t* __restrict__ tile_seg, const int64_t* __restrict__ tri_base, int n_img,
     const ImageDesc* __restrict__ D, const int32_t* __restrict__ rows,
     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ list, const int32_t* __restrict__ rank_at, uint32_t cap)
+    const uint32_t* __restrict__ list, SeedIndex si, uint32_t cap)
 {
     __shared__ int32_t tid_tile[kTS * kTS];
     const int64_t gt = blockIdx.x;
@@ -1532,7 +1543,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
                 const int lx = static_cast<int>(lab[i] >> 16), ly = static_cast<int>(lab[i] & 0xFFFFu);
                 const int dx = X0 + lane + off - lx, dy = Y0 + r0 + i + off - ly;
                 dd[i] = dx * dx + dy * dy;
-                id[i] = rank_at[d.pix_off + static_cast<int64_t>(ly - shift) * W + (lx - shift)];
+                id[i] = seed_id_at(si, d.bm_off, d.seed_off, H, lx - shift, ly - shift);
             }
         }
 #pragma unroll
@@ -1566,7 +1577,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
 }  // namespace
 
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
-                  const uint32_t* sorted, const int32_t* rank_at, int C, TriWork& work,
+                  const uint32_t* sorted, int C, TriWork& work,
                   hipStream_t s, bool speculative)
 {
     const int n_img = static_cast<int>(plan.img.size());
@@ -1624,11 +1635,11 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
     ProfScope ps(kKOutput, s);
     if (C == 3)
         tile_assign_kernel<3><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), seed_index(work),
             static_cast<uint32_t>(cap));
     else
         tile_assign_kernel<1><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), rank_at,
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), seed_index(work),
             static_cast<uint32_t>(cap));
     DL_LAUNCH_CHECK();
 }
