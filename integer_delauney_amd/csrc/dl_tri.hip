@@ -1391,49 +1391,114 @@ __device__ __forceinline__ void row_span(const EdgeRows& E, double y, double& lo
     }
 }
 
-// Triangles t < min(device count, rows_cap) (the rows buffer's capacity).
-__global__ void bin_count_kernel(const int32_t* __restrict__ rows, int64_t rows_cap,
-                                 const int64_t* __restrict__ tri_base, int n_img,
-                                 const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
-                                 uint32_t* __restrict__ counts)
+// Binning of triangles to 64x64 tiles.  Triangles are taken in chunks of
+// 256 consecutive tids; consecutive tids are neighbours along a scan row, so a
+// chunk touches a few dozen tiles.  A chunk first aggregates its (tile, count)
+// pairs in an LDS hash table and then makes one global atomic per distinct
+// tile, instead of one per (triangle, tile) pair from thousands of threads.
+// A triangle whose box covers more than kBinDirect tiles (long, thin or
+// collinear ones) goes straight to the global counters.
+constexpr int kBinSlots = 1024;   // LDS table: at most 256 triangles x 4 tiles
+constexpr int kBinDirect = 4;
+constexpr uint32_t kBinEmpty = 0xFFFFFFFFu;
+
+struct BinLds {
+    uint32_t key[kBinSlots];     // global tile index, kBinEmpty = free
+    uint32_t count[kBinSlots];   // pairs of this chunk in the tile
+    uint32_t base[kBinSlots];    // (fill) first list position of the chunk's pairs
+};
+
+// Slot of `tile` in the chunk table (inserting it); its count is incremented
+// and the previous count (the pair's rank within the chunk) returned in rank.
+__device__ __forceinline__ int bin_slot(BinLds& L, uint32_t tile, uint32_t& rank)
 {
-    const int64_t T = min(tri_base[n_img], rows_cap);
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
-         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
-        const ImageDesc& d = D[img];
-        const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
-        int x0, y0, x1, y1;
-        if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) continue;
-        uint32_t* c = counts + d.tile_off;
-        for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
-            for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) atomicAdd(&c[ty * d.tiles_x + tx], 1u);
+    uint32_t h = (tile * 2654435761u) & (kBinSlots - 1);
+    for (int probe = 0; probe < kBinSlots; ++probe) {
+        const uint32_t k = atomicCAS(&L.key[h], kBinEmpty, tile);
+        if (k == kBinEmpty || k == tile) {
+            rank = atomicAdd(&L.count[h], 1u);
+            return static_cast<int>(h);
+        }
+        h = (h + 1) & (kBinSlots - 1);
+    }
+    return -1;   // unreachable: at most kBinSlots distinct tiles per chunk
+}
+
+__device__ __forceinline__ void bin_clear(BinLds& L)
+{
+    for (int i = threadIdx.x; i < kBinSlots; i += blockDim.x) {
+        L.key[i] = kBinEmpty;
+        L.count[i] = 0u;
     }
 }
 
-// List entries beyond cap are dropped and flagged (speculative sizing).
-__global__ void bin_fill_kernel(const int32_t* __restrict__ rows, int64_t rows_cap,
-                                const int64_t* __restrict__ tri_base, int n_img,
-                                const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
-                                const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
-                                uint32_t* __restrict__ list, uint32_t cap,
-                                uint32_t* __restrict__ flags)
+// Triangles t < min(device count, rows_cap) (the rows buffer's capacity).
+// FILL = false: counts[tile] += pairs.  FILL = true: list positions from
+// offsets[tile] + cursor[tile]; entries beyond cap are dropped and flagged
+// (speculative sizing).
+template <bool FILL>
+__global__ __launch_bounds__(256) void bin_kernel(
+    const int32_t* __restrict__ rows, int64_t rows_cap, const int64_t* __restrict__ tri_base,
+    int n_img, const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
+    uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
+    uint32_t* __restrict__ list, uint32_t cap, uint32_t* __restrict__ flags)
 {
+    __shared__ BinLds L;
     const int64_t T = min(tri_base[n_img], rows_cap);
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
-         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
-        const ImageDesc& d = D[img];
-        const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
-        int x0, y0, x1, y1;
-        if (!tri_box(v, d.W, d.H, x0, y0, x1, y1)) continue;
-        for (int ty = y0 / kTS; ty <= y1 / kTS; ++ty)
-            for (int tx = x0 / kTS; tx <= x1 / kTS; ++tx) {
-                const int64_t tile = d.tile_off + ty * d.tiles_x + tx;
-                const uint32_t k = offsets[tile] + atomicAdd(&cursor[tile], 1u);
+    bin_clear(L);
+    __syncthreads();
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < T;
+         base += static_cast<int64_t>(gridDim.x) * blockDim.x) {   // uniform across the block
+        const int64_t t = base + threadIdx.x;
+        int slot[kBinDirect];
+        uint32_t rank[kBinDirect];
+        int n_local = 0;
+        if (t < T) {
+            const int img = n_img > 1 ? find_seg(tri_base, n_img, t) : 0;
+            const ImageDesc& d = D[img];
+            const TriVerts v = load_tri(rows, sorted + d.seed_off, t);
+            int x0, y0, x1, y1;
+            if (tri_box(v, d.W, d.H, x0, y0, x1, y1)) {
+                const int tx0 = x0 / kTS, tx1 = x1 / kTS, ty0 = y0 / kTS, ty1 = y1 / kTS;
+                const bool direct = (tx1 - tx0 + 1) * (ty1 - ty0 + 1) > kBinDirect;
+                for (int ty = ty0; ty <= ty1; ++ty)
+                    for (int tx = tx0; tx <= tx1; ++tx) {
+                        const uint32_t tile = static_cast<uint32_t>(d.tile_off + ty * d.tiles_x + tx);
+                        if (direct) {
+                            if (!FILL) {
+                                atomicAdd(&counts[tile], 1u);
+                            } else {
+                                const uint32_t k = offsets[tile] + atomicAdd(&counts[tile], 1u);
+                                if (k < cap) list[k] = static_cast<uint32_t>(t);
+                                else atomicOr(flags, kFlagOverflow);
+                            }
+                        } else {
+                            slot[n_local] = bin_slot(L, tile, rank[n_local]);
+                            ++n_local;
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+        // one global atomic per distinct tile of the chunk
+        for (int i = threadIdx.x; i < kBinSlots; i += blockDim.x) {
+            const uint32_t tile = L.key[i];
+            if (tile == kBinEmpty) continue;
+            const uint32_t c = atomicAdd(&counts[tile], L.count[i]);
+            if (FILL) L.base[i] = offsets[tile] + c;
+        }
+        __syncthreads();
+        if (FILL) {
+            for (int j = 0; j < n_local; ++j) {
+                if (slot[j] < 0) continue;
+                const uint32_t k = L.base[slot[j]] + rank[j];
                 if (k < cap) list[k] = static_cast<uint32_t>(t);
                 else atomicOr(flags, kFlagOverflow);
             }
+        }
+        __syncthreads();
+        bin_clear(L);
+        __syncthreads();
     }
 }
 
@@ -1602,8 +1667,8 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
         ProfScope ps(kKAssign, s);
         fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
         if (t_bound > 0) {
-            bin_count_kernel<<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
-                                                     counts);
+            bin_kernel<false><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
+                                                      counts, nullptr, nullptr, 0u, nullptr);
             DL_LAUNCH_CHECK();
         }
         exclusive_scan_u32(counts, ntiles, offs, work.scan_tmp, s);
@@ -1624,23 +1689,22 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
     {
         ProfScope ps(kKAssign, s);
         if (t_bound > 0) {
-            bin_fill_kernel<<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted, offs,
-                                                    cursor,
-                                                    work.bin_list.as<uint32_t>(),
-                                                    static_cast<uint32_t>(cap),
-                                                    work.flags.as<uint32_t>());
+            bin_kernel<true><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
+                                                     cursor, offs, work.bin_list.as<uint32_t>(),
+                                                     static_cast<uint32_t>(cap),
+                                                     work.flags.as<uint32_t>());
             DL_LAUNCH_CHECK();
         }
     }
     ProfScope ps(kKOutput, s);
     if (C == 3)
         tile_assign_kernel<3><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), seed_index(work),
-            static_cast<uint32_t>(cap));
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(),
+            seed_index(work), static_cast<uint32_t>(cap));
     else
         tile_assign_kernel<1><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
-            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(), seed_index(work),
-            static_cast<uint32_t>(cap));
+            tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(),
+            seed_index(work), static_cast<uint32_t>(cap));
     DL_LAUNCH_CHECK();
 }
 
