@@ -36,6 +36,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def graph_stats(lib, ctx):
+    from integer_delauney_amd import _lib
+    c, r = ctypes.c_int64(0), ctypes.c_int64(0)
+    _lib.check(lib.dl_get_graph_stats(ctx.handle, ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
+
+
 def kernel_stats(lib, ctx):
     counts = (ctypes.c_int64 * 7)()
     ms = (ctypes.c_double * 7)()
@@ -96,16 +103,32 @@ def bench_batch(lib, ctx, n_images: int, steps: int):
             ctypes.c_void_p(d_xy.data_ptr()), _lib.i32p(pads), ctypes.c_void_p(d_out.data_ptr()),
             None, ntri.ctypes.data_as(I64P)))
 
+    # Warm-up: first call (flood-fill shape tuning on a new canvas class),
+    # second (predicted fill, uncaptured), third (captured); then replays.
+    for _ in range(3):
+        batch_step()
+    torch.cuda.synchronize()
+    g0 = graph_stats(lib, ctx)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch_step()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    replays = graph_stats(lib, ctx)[1] - g0[1]
+    log("batched steps done")
+    ref = d_out.clone()
+    ntri_batch = ntri.copy()
+    _lib.check(lib.dl_set_graphs(ctx.handle, 0))
     batch_step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         batch_step()
     torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / steps
-    log("batched steps done")
-    ref = d_out.clone()
-    ntri_batch = ntri.copy()
+    t_uncaptured = (time.perf_counter() - t0) / steps
+    _lib.check(lib.dl_set_graphs(ctx.handle, 1))
+    same_uncaptured = bool(torch.equal(ref, d_out))
+    log("uncaptured batched steps done")
 
     seed_off = np.concatenate([[0], np.cumsum(counts)])
     pix_off = np.concatenate([[0], np.cumsum(px)])
@@ -138,7 +161,10 @@ def bench_batch(lib, ctx, n_images: int, steps: int):
             "single_image_path_ms": t_single * 1e3,
             "single_image_path_mpixel_per_s": total_px / t_single / 1e6,
             "speedup_vs_single_image_path": t_single / t_batch,
-            "batched_equals_single_bitwise": same}
+            "batched_equals_single_bitwise": same,
+            "graph_replays_in_timed_steps": replays,
+            "uncaptured_ms_per_batch": t_uncaptured * 1e3,
+            "uncaptured_equals_replayed_bitwise": same_uncaptured}
 
 
 def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_del: int):
@@ -501,16 +527,33 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
-    # Timed region: the pipeline alone.
+    # Timed region: the pipeline alone.  From the third call of one signature
+    # on, the pipeline's first half is a replayed hipGraph.
+    g_before = graph_stats(lib, ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    g_after = graph_stats(lib, ctx)
     log(f"timed {args.steps} steps")
     ms_per_step = (t1 - t0) / args.steps * 1e3
     value = W * H / (ms_per_step / 1e3) / 1e6
+    # The same steps with graphs off (every launch issued from the host).
+    _lib.check(lib.dl_set_graphs(ctx.handle, 0))
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    uncaptured_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    _lib.check(lib.dl_set_graphs(ctx.handle, 1))
+    log("uncaptured steps done")
+    graphs = {"captures": g_after[0], "replays_in_timed_steps": g_after[1] - g_before[1],
+              "uncaptured_ms_per_step": uncaptured_ms,
+              "uncaptured_mpixel_per_s": W * H / (uncaptured_ms / 1e3) / 1e6}
     # Kernel-timing region: the same steps again with a pair of HIP events
     # around every kernel, on the stream the kernels run on (the events add
     # host work, so this region is not the one `value` comes from).
@@ -627,6 +670,7 @@ def main():
                      "traffic_source": traffic_src,
                      "alg_bytes_per_launch": alg.get(dominant),
                      "avg_launch_us": dom["avg_launch_us"]},
+        "graphs": graphs,
         "kernel_timing_region_ms_per_step": profiled_ms_per_step,
         "pipeline_alg_bytes_per_image": b_alg_image,
         "pipeline_hbm_frac": b_alg_image / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,

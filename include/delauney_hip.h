@@ -262,6 +262,20 @@ int dl_autotune_reset(void);
  * reference counterpart. */
 int dl_set_fill_prediction(dl_context* ctx, int enabled);
 
+/* hipGraph replay of the pipeline (default on).  Everything a triangulation
+ * enqueues before its final status read (seed ordering, the predicted padded
+ * fill, detection, registration, assignment and output, and the status
+ * copies) is captured once per call signature: the images' sizes, seed
+ * counts, padding, seed and output addresses, the stream and the fill's shape
+ * and launch count.  A call matching the last predicted run's signature is
+ * captured; later matching calls replay the graph, with the host planning and
+ * the ~30 launches replaced by one graph launch.  Any buffer reallocation in
+ * the process drops the graph.  Results are identical with and without.
+ * Timing and kernel profiling run uncaptured.  No reference counterpart. */
+int dl_set_graphs(dl_context* ctx, int enabled);
+/* Graphs captured and replayed on this context so far. */
+int dl_get_graph_stats(dl_context* ctx, int64_t* captures, int64_t* replays);
+
 /* Device-resident triangulate: seeds, output and triangles stay on the device.
  * d_seeds_xy: n (x, y) int32 pairs; d_out_hw3: (H, W, 3) int32.  The triangle
  * rows stay in the context (dl_device_triangles).  No reference counterpart;

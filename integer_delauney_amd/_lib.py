@@ -90,6 +90,9 @@ _SIGNATURES = {
     "dl_session_device_grid": (_VP, [_VP]),
     "dl_set_autotune": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_set_fill_prediction": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_set_graphs": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_get_graph_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_int64)]),
     "dl_autotune_reset": (ctypes.c_int, []),
     "dl_device_to_host": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
     "dl_session_has_values": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),

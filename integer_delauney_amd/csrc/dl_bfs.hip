@@ -699,8 +699,9 @@ static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     if (total == 0) return 0;
     const int n_canvas = static_cast<int>(canvases.size());
     work.canvases.reserve(sizeof(BfsCanvas) * n_canvas);
-    DL_HIP(hipMemcpyAsync(work.canvases.ptr, canvases.data(), sizeof(BfsCanvas) * n_canvas,
-                          hipMemcpyHostToDevice, s));
+    work.host_canvases.reserve(sizeof(BfsCanvas) * n_canvas);
+    stage_h2d(work.host_canvases, 0, work.canvases.ptr, canvases.data(),
+              sizeof(BfsCanvas) * n_canvas, s);
     work.flags.reserve(sizeof(int) * 2 * static_cast<size_t>(total));
     work.list.reserve(sizeof(int) * static_cast<size_t>(total));
     work.visits.reserve(sizeof(int) * static_cast<size_t>(total));

@@ -76,6 +76,40 @@ struct PinnedBuf {
 };
 
 // ---------------------------------------------------------------------------
+// Stream capture support.  A captured graph holds raw device and page-locked
+// host addresses, so it stays valid only while no buffer has been reallocated
+// or freed: every reallocation and release of a DevBuf or PinnedBuf advances a
+// process-wide allocation epoch, and a graph records the epoch it was captured
+// at.  While the calling thread captures, a reserve that would reallocate
+// throws capture_abort instead (the pipeline then runs uncaptured).
+// ---------------------------------------------------------------------------
+struct capture_abort : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+uint64_t alloc_epoch();
+
+// Host bytes a captured graph copies to the device from page-locked staging.
+// The graph's copy nodes read the staging buffers when it is replayed, so the
+// bytes seen at capture are put back before every replay.
+struct StagedBytes {
+    void* host;
+    std::vector<char> bytes;
+};
+struct CaptureLog {
+    std::vector<StagedBytes> staged;
+    bool poisoned = false;   // a buffer was freed while capturing
+};
+// Non-null while the calling thread captures a stream.
+CaptureLog*& capture_log();
+
+// Host -> device copy of n bytes through bytes [offset, offset + n) of the
+// page-locked buffer `stage`, which the caller has reserved.  One buffer per
+// upload site: a site is rewritten only after the stream it copied on has been
+// synchronised.
+void stage_h2d(PinnedBuf& stage, size_t offset, void* dst, const void* src, size_t n,
+               hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // BFS (Voronoi flood fill) over one or more canvases.
 // ---------------------------------------------------------------------------
 struct BfsCanvas {
@@ -111,6 +145,7 @@ struct BfsWork {
     DevBuf canvases;  // device copy of the canvas table
     DevBuf tile_map;  // tile -> canvas index (multi-canvas launches)
     PinnedBuf host_counters;
+    PinnedBuf host_canvases;   // staging of the canvas table
 };
 
 // Interior tile size for a configuration.
@@ -236,7 +271,8 @@ struct TriWork {
     DevBuf bin_counts, bin_offsets, bin_list;   // triangle -> tile bins
     DevBuf desc;        // device copy of the plan's ImageDesc array
     DevBuf segs;        // device copy of the offset tables + tri_base
-    PinnedBuf host_stage;   // staging for the uploads above
+    PinnedBuf host_plan;    // staging of upload_plan (descriptors + offset tables)
+    PinnedBuf host_stage;   // staging of upload_desc
     PinnedBuf host_seed_flags;   // per-image seed validation flags (read back late)
     std::vector<int64_t> tri_base;   // host copy: first row of each image, then the total
     uint32_t rows_cap = 0;           // capacity of rows (detection)
@@ -310,6 +346,11 @@ struct SpeculativeStatus {
 // total, and n_esc padding-watch flags from d_esc (may be null).
 SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
                               hipStream_t s);
+// read_status in two halves: the asynchronous copies into work.host_flags,
+// and, after the stream has been synchronised, their parse.
+void issue_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
+                  hipStream_t s);
+SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc);
 
 // Pixel assignment binned by 64x64 tiles and fused with the output write into
 // each image's out: C == 3, the (H, W, 3) grid with channels 0/1 from its

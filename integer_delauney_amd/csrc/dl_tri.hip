@@ -247,28 +247,26 @@ inline int64_t* seg_ptr(const TriWork& w, int which, size_t n_img)
 
 void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s)
 {
-    // The staging buffer is only rewritten after the stream has been
-    // synchronised (every pipeline stage that re-uploads follows a sync).
-    const size_t n = plan.img.size();
-    const size_t db = sizeof(ImageDesc) * n;
+    const size_t db = sizeof(ImageDesc) * plan.img.size();
     work.desc.reserve(db);
-    work.host_stage.reserve(db + sizeof(int64_t) * 4 * (n + 1));
-    std::memcpy(work.host_stage.ptr, plan.img.data(), db);
-    DL_HIP(hipMemcpyAsync(work.desc.ptr, work.host_stage.ptr, db, hipMemcpyHostToDevice, s));
+    work.host_stage.reserve(db);
+    stage_h2d(work.host_stage, 0, work.desc.ptr, plan.img.data(), db, s);
 }
 
 void upload_plan(const BatchPlan& plan, TriWork& work, hipStream_t s)
 {
+    // Descriptors and offset tables in a staging buffer of their own:
+    // upload_desc may rewrite its buffer before these copies have run.
     const size_t n = plan.img.size();
     const size_t db = sizeof(ImageDesc) * n;
-    upload_desc(plan, work, s);
+    const size_t sb = sizeof(int64_t) * (n + 1);
+    work.desc.reserve(db);
     work.segs.reserve(sizeof(int64_t) * 4 * (n + 1));
-    int64_t* h = reinterpret_cast<int64_t*>(static_cast<char*>(work.host_stage.ptr) + db);
-    std::memcpy(h, plan.seed_seg.data(), sizeof(int64_t) * (n + 1));
-    std::memcpy(h + (n + 1), plan.det_seg.data(), sizeof(int64_t) * (n + 1));
-    std::memcpy(h + 2 * (n + 1), plan.tile_seg.data(), sizeof(int64_t) * (n + 1));
-    DL_HIP(hipMemcpyAsync(work.segs.ptr, h, sizeof(int64_t) * 3 * (n + 1), hipMemcpyHostToDevice,
-                          s));
+    work.host_plan.reserve(db + 3 * sb);
+    stage_h2d(work.host_plan, 0, work.desc.ptr, plan.img.data(), db, s);
+    const std::vector<int64_t>* tabs[3] = {&plan.seed_seg, &plan.det_seg, &plan.tile_seg};
+    for (int k = 0; k < 3; ++k)
+        stage_h2d(work.host_plan, db + k * sb, seg_ptr(work, k, n), tabs[k]->data(), sb, s);
 }
 
 // ===========================================================================
@@ -1200,35 +1198,62 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     throw runtime_error("triangle registration hash collided repeatedly");
 }
 
-SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
-                              hipStream_t s)
+namespace {
+// Layout of work.host_flags for the status read: status words, tri_base
+// (n_img + 1), padding-watch flags.
+struct StatusView {
+    uint32_t* hf;
+    int64_t* htb;
+    int* hesc;
+};
+StatusView status_view(const TriWork& work, int n_img)
+{
+    char* h = static_cast<char*>(work.host_flags.ptr);
+    return StatusView{reinterpret_cast<uint32_t*>(h), reinterpret_cast<int64_t*>(h + 64),
+                      reinterpret_cast<int*>(h + 64 + sizeof(int64_t) * (n_img + 1))};
+}
+}  // namespace
+
+void issue_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
+                  hipStream_t s)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const size_t bytes = 64 + sizeof(int64_t) * (n_img + 1) + sizeof(int) * std::max(n_esc, 1);
     work.host_flags.reserve(bytes);
-    char* h = static_cast<char*>(work.host_flags.ptr);
-    uint32_t* hf = reinterpret_cast<uint32_t*>(h);
-    int64_t* htb = reinterpret_cast<int64_t*>(h + 64);
-    int* hesc = reinterpret_cast<int*>(h + 64 + sizeof(int64_t) * (n_img + 1));
-    DL_HIP(hipMemcpyAsync(hf, work.flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    DL_HIP(hipMemcpyAsync(htb, seg_ptr(work, 3, n_img), sizeof(int64_t) * (n_img + 1),
+    const StatusView v = status_view(work, n_img);
+    DL_HIP(hipMemcpyAsync(v.hf, work.flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(v.htb, seg_ptr(work, 3, n_img), sizeof(int64_t) * (n_img + 1),
                           hipMemcpyDeviceToHost, s));
-    hf[2] = 0;
     if (plan.n_tiles > 0 && work.list_cap > 0)
-        DL_HIP(hipMemcpyAsync(hf + 2, work.bin_offsets.as<uint32_t>() + plan.n_tiles,
+        DL_HIP(hipMemcpyAsync(v.hf + 2, work.bin_offsets.as<uint32_t>() + plan.n_tiles,
                               sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     if (d_esc && n_esc > 0)
-        DL_HIP(hipMemcpyAsync(hesc, d_esc, sizeof(int) * n_esc, hipMemcpyDeviceToHost, s));
-    DL_HIP(hipStreamSynchronize(s));
+        DL_HIP(hipMemcpyAsync(v.hesc, d_esc, sizeof(int) * n_esc, hipMemcpyDeviceToHost, s));
+}
+
+SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc)
+{
+    const int n_img = static_cast<int>(plan.img.size());
+    const StatusView v = status_view(work, n_img);
     SpeculativeStatus st;
-    const uint32_t f = hf[0];
+    const uint32_t f = v.hf[0];
     st.bad_id = (f & kFlagBadId) != 0;
+    // bin-list entries needed (copied only when the assignment ran)
+    const int64_t listed = plan.n_tiles > 0 && work.list_cap > 0 ? v.hf[2] : 0;
     st.redo = (f & (kFlagOverflow | kFlagTableOverflow | kFlagCollision)) != 0 ||
-              static_cast<int64_t>(hf[2]) > work.list_cap;
-    st.n_tri = htb[n_img];
-    work.tri_base.assign(htb, htb + n_img + 1);
-    st.escaped.assign(hesc, hesc + (d_esc ? n_esc : 0));
+              listed > work.list_cap;
+    st.n_tri = v.htb[n_img];
+    work.tri_base.assign(v.htb, v.htb + n_img + 1);
+    st.escaped.assign(v.hesc, v.hesc + n_esc);
     return st;
+}
+
+SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
+                              hipStream_t s)
+{
+    issue_status(plan, work, d_esc, n_esc, s);
+    DL_HIP(hipStreamSynchronize(s));
+    return parse_status(plan, work, d_esc ? n_esc : 0);
 }
 
 // ===========================================================================
