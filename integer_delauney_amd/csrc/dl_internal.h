@@ -262,6 +262,7 @@ struct TriWork {
     DevBuf flags;       // device status flags (per image, then batch-wide)
     PinnedBuf host_flags;
     DevBuf table_tag;   // registration hash table slots
+    DevBuf table_tri;   // their triplets (hashed tags only)
     DevBuf cand;        // raw candidate records
     DevBuf counts;      // per-thread / per-block candidate counts
     DevBuf offsets;     // their prefix sums

@@ -164,11 +164,14 @@ namespace {
 
 // Detection thread blocks cover kDetChunk consecutive 2x2 blocks in each of
 // kDetRows rows of blocks; each thread owns kDetItems of them per row, strided
-// by the thread count so that a wave reads consecutive cells.
+// by the thread count so that a wave reads consecutive cells.  Thread blocks
+// are numbered strip-major (a strip is kDetChunk columns wide, top to bottom),
+// so the candidate list runs through the canvas in x order, as seed ids do.
 constexpr int kDetThreads = 256;
-constexpr int kDetItems = 4;
+constexpr int kDetItems = 1;
 constexpr int kDetChunk = kDetThreads * kDetItems;
-constexpr int kDetRows = 8;
+constexpr int kDetRows = 32;
+static_assert(kDetItems * kDetRows <= 32, "one mask bit per 2x2 block of a thread");
 constexpr int kTileSide = 64;   // assignment tile side in pixels
 
 // Largest i in [0, n) with off[i] <= g (off is non-decreasing, off[0] = 0).
@@ -448,21 +451,33 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// One 32-byte slot per distinct (image, triplet): an insert touches one line.
-struct alignas(32) Slot {
+// One 16-byte slot per distinct (image, triplet).
+struct alignas(16) Slot {
     unsigned long long tag;
     uint32_t minkey;   // smallest registration key seen for the triplet
     uint32_t blk;      // 1: blocked by a cocircular quad
-    int32_t a, b, c;   // the sorted triplet (collision check)
+};
+
+// The sorted triplet of a slot, kept only when tags are hashes (the
+// collision check of mark_one).
+struct SlotTri {
+    int32_t a, b, c;
     uint32_t img;
 };
 
+// Open addressing with linear probing.  A triplet's probe starts at
+// stride * (batch-wide id of its smallest seed), so the table is laid out in
+// seed-id order, which is x order on the canvas: detection's candidate list
+// also runs in x order, and an XCD working through a contiguous range of it
+// touches a contiguous part of the table, which stays in its L2.
 struct Table {
     Slot* slot;
+    SlotTri* tri;     // null when tags are exact
     uint64_t mask;
     int exact;        // tags are exact packings of (image, sorted triplet)
     int bits;         // bits per seed id in an exact tag
     uint64_t salt;
+    uint64_t stride;  // slots per seed (>= 1)
 };
 
 __device__ __forceinline__ uint64_t triplet_tag(const Table& t, uint32_t img, int32_t a, int32_t b,
@@ -482,11 +497,12 @@ __device__ __forceinline__ uint64_t triplet_tag(const Table& t, uint32_t img, in
 
 // Returns the slot of (img, sorted triplet), inserting it if absent; -1 (flag
 // raised) if the table is full, which its sizing rules out.
-__device__ int64_t table_insert(const Table& t, uint32_t img, int32_t a, int32_t b, int32_t c,
-                                uint32_t* flags)
+// a < b < c; seed_off: the image's first seed in the batch.
+__device__ int64_t table_insert(const Table& t, uint32_t img, int64_t seed_off, int32_t a,
+                                int32_t b, int32_t c, uint32_t* flags)
 {
     const uint64_t tg = triplet_tag(t, img, a, b, c);
-    uint64_t h = mix64(tg ^ 0x9E3779B97F4A7C15ull) & t.mask;
+    uint64_t h = (static_cast<uint64_t>(seed_off + a) * t.stride) & t.mask;
     for (uint64_t probe = 0; probe <= t.mask; ++probe) {
         Slot& sl = t.slot[h];
         const unsigned long long cur = __atomic_load_n(&sl.tag, __ATOMIC_RELAXED);
@@ -496,10 +512,7 @@ __device__ int64_t table_insert(const Table& t, uint32_t img, int32_t a, int32_t
                 atomicCAS(&sl.tag, static_cast<unsigned long long>(kEmptyTag),
                           static_cast<unsigned long long>(tg));
             if (prev == kEmptyTag) {
-                sl.a = a;
-                sl.b = b;
-                sl.c = c;
-                sl.img = img;
+                if (t.tri) t.tri[h] = SlotTri{a, b, c, img};
                 return static_cast<int64_t>(h);
             }
             if (prev == tg) return static_cast<int64_t>(h);
@@ -514,8 +527,22 @@ __global__ void table_init_kernel(Slot* __restrict__ slots, uint64_t n)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint4* p = reinterpret_cast<uint4*>(slots + i);
-    p[0] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);   // tag, minkey, blk
+    // tag empty, minkey none, not blocked
+    *reinterpret_cast<uint4*>(slots + i) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);
+}
+
+// Block b of a grid that is a multiple of 8 runs on XCD b % 8 (dispatch is
+// round-robin over the XCDs).  Each XCD takes a contiguous eighth of [0, n);
+// the XCD's threads stride over it.
+struct XcdRange {
+    uint32_t first, end, step;
+};
+__device__ __forceinline__ XcdRange xcd_range(uint32_t n)
+{
+    const uint32_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3, rank = blockIdx.x >> 3;
+    const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(n) * xcd / 8);
+    const uint32_t hi = static_cast<uint32_t>(static_cast<uint64_t>(n) * (xcd + 1) / 8);
+    return XcdRange{lo + rank * blockDim.x + threadIdx.x, hi, per * blockDim.x};
 }
 
 // Orientation class of a direction for the reference's arctan2 ordering:
@@ -713,8 +740,9 @@ __device__ __forceinline__ int64_t det_block_origin(const DetectArgs& A, int64_t
     img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
     const ImageDesc& d = A.D[img];
     const int64_t lb = gb - A.det_seg[img];
-    const int band = static_cast<int>(lb / d.bx_per_row);
-    const int bx = static_cast<int>(lb - static_cast<int64_t>(band) * d.bx_per_row);
+    const int n_bands = (d.Hc - 1 + kDetRows - 1) / kDetRows;
+    const int bx = static_cast<int>(lb / n_bands);
+    const int band = static_cast<int>(lb - static_cast<int64_t>(bx) * n_bands);
     ry0 = band * kDetRows;
     rx0 = bx * kDetChunk + static_cast<int>(threadIdx.x);
     return lb;
@@ -814,7 +842,7 @@ __device__ __forceinline__ void add_candidate(const DetectArgs& A, const ImageDe
 {
     int32_t sa = a, sb = b, sc = c;
     sort3(sa, sb, sc);
-    const int64_t sl = table_insert(A.table, img, sa, sb, sc, A.flags);
+    const int64_t sl = table_insert(A.table, img, d.seed_off, sa, sb, sc, A.flags);
     if (sl < 0) {
         A.cand[out++].key = 0xFFFFFFFFu;
         return;
@@ -841,9 +869,8 @@ __global__ __launch_bounds__(kDetThreads) void detect_insert_kernel(DetectArgs A
     // Sizes already known to be too small: the host reruns with exact ones,
     // so skip the (then long) probing.
     if (*A.flags & kFlagOverflow) return;
-    const uint32_t n = *A.npos;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        insert_position(A, i);
+    const XcdRange r = xcd_range(*A.npos);
+    for (uint32_t i = r.first; i < r.end; i += r.step) insert_position(A, i);
 }
 
 __device__ void insert_position(const DetectArgs& A, uint32_t i)
@@ -884,7 +911,7 @@ __device__ void insert_position(const DetectArgs& A, uint32_t i)
     for (int q = 0; q < 2; ++q) {
         int32_t a = blk[q][0], b = blk[q][1], cc = blk[q][2];
         sort3(a, b, cc);
-        const int64_t sl = table_insert(A.table, img, a, b, cc, A.flags);
+        const int64_t sl = table_insert(A.table, img, d.seed_off, a, b, cc, A.flags);
         if (sl >= 0) A.table.slot[sl].blk = 1u;
         for (int k = 0; k < 4; ++k) {
             int32_t x = tri[k][0], y = tri[k][1], z = tri[k][2];
@@ -913,9 +940,8 @@ __global__ void mark_kernel(const Cand* __restrict__ cand, const uint32_t* __res
                             uint32_t* __restrict__ flags)
 {
     if (*flags & kFlagOverflow) return;   // the insert did not run: candidates unset
-    const uint32_t n = 2 * *npos;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        mark_one(cand, i, t, D, bm, flags);
+    const XcdRange r = xcd_range(2 * *npos);
+    for (uint32_t i = r.first; i < r.end; i += r.step) mark_one(cand, i, t, D, bm, flags);
 }
 
 __device__ void mark_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
@@ -924,12 +950,14 @@ __device__ void mark_one(const Cand* __restrict__ cand, uint32_t i, const Table&
 {
     const Cand cd = cand[i];
     if (cd.key == 0xFFFFFFFFu) return;
-    int32_t a = cd.a, b = cd.b, c = cd.c;
-    sort3(a, b, c);
-    const Slot& sl = t.slot[cd.slot];
-    if (sl.a != a || sl.b != b || sl.c != c || sl.img != cd.img) {
-        atomicOr(flags, kFlagCollision);
-        return;
+    if (t.tri) {   // hashed tags: a different triplet with the same tag shares the slot
+        int32_t a = cd.a, b = cd.b, c = cd.c;
+        sort3(a, b, c);
+        const SlotTri st = t.tri[cd.slot];
+        if (st.a != a || st.b != b || st.c != c || st.img != cd.img) {
+            atomicOr(flags, kFlagCollision);
+            return;
+        }
     }
     if (is_first(cd, t)) {
         const int64_t w = D[cd.img].key_off + (cd.key >> 5);
@@ -958,8 +986,8 @@ __global__ void emit_kernel(const Cand* __restrict__ cand, const uint32_t* __res
                             uint32_t rows_cap, uint32_t* __restrict__ flags)
 {
     if (*flags & kFlagOverflow) return;   // rerun with exact sizes follows
-    const uint32_t n = 2 * *npos;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    const XcdRange r = xcd_range(2 * *npos);
+    for (uint32_t i = r.first; i < r.end; i += r.step)
         emit_one(cand, i, t, D, bm, pre, rows, rows_cap, flags);
 }
 
@@ -1131,14 +1159,18 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     const int img_bits = n_img > 1 ? bits_for(n_img - 1) : 0;
     const int exact = 3 * plan.max_seed_bits + img_bits <= 63 ? 1 : 0;
     // grid-stride launches sized for the capacity, at most 16 blocks per CU
+    // (multiples of 8: the kernels split their range over the XCDs)
     const unsigned pblocks = static_cast<unsigned>(
-        std::min<uint64_t>((A.pos_cap + kDetThreads - 1) / kDetThreads, 4096));
+        (std::min<uint64_t>((A.pos_cap + kDetThreads - 1) / kDetThreads, 4096) + 7) / 8 * 8);
     const unsigned cblocks = static_cast<unsigned>(
-        std::min<uint64_t>((2ull * A.pos_cap + 255) / 256, 4096));
+        (std::min<uint64_t>((2ull * A.pos_cap + 255) / 256, 4096) + 7) / 8 * 8);
+    if (!exact) work.table_tri.reserve(sizeof(SlotTri) * table_cap);
     for (int attempt = 0; attempt < 4; ++attempt) {
         Table t;
         t.slot = work.table_tag.as<Slot>();
+        t.tri = exact ? nullptr : work.table_tri.as<SlotTri>();
         t.mask = table_cap - 1;
+        t.stride = std::max<uint64_t>(1, table_cap / std::max<uint64_t>(n_seeds, 1));
         t.exact = exact;
         t.bits = plan.max_seed_bits;
         t.salt = salt;
@@ -1532,10 +1564,15 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int64_t* __restrict__ tile_seg, const int64_t* __restrict__ tri_base, int n_img,
     const ImageDesc* __restrict__ D, const int32_t* __restrict__ rows,
     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ list, SeedIndex si, uint32_t cap)
+    const uint32_t* __restrict__ list, SeedIndex si, uint32_t cap, uint32_t n_tiles)
 {
     __shared__ int32_t tid_tile[kTS * kTS];
-    const int64_t gt = blockIdx.x;
+    // Each XCD takes a contiguous range of tiles (tile rows of one image):
+    // the seed-index words and label lines a tile row reads are then shared
+    // within one L2.  The grid is a multiple of 8 (see xcd_range).
+    const uint32_t per = gridDim.x >> 3;
+    const int64_t gt = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (gt >= n_tiles) return;
     const int img = n_img > 1 ? find_seg(tile_seg, n_img, gt) : 0;
     const ImageDesc& d = D[img];
     const int W = d.W, H = d.H;
@@ -1722,14 +1759,15 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
         }
     }
     ProfScope ps(kKOutput, s);
+    const unsigned tgrid = static_cast<unsigned>((ntiles + 7) / 8 * 8);
     if (C == 3)
-        tile_assign_kernel<3><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
+        tile_assign_kernel<3><<<tgrid, 256, 0, s>>>(
             tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(),
-            seed_index(work), static_cast<uint32_t>(cap));
+            seed_index(work), static_cast<uint32_t>(cap), static_cast<uint32_t>(ntiles));
     else
-        tile_assign_kernel<1><<<static_cast<unsigned>(ntiles), 256, 0, s>>>(
+        tile_assign_kernel<1><<<tgrid, 256, 0, s>>>(
             tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(),
-            seed_index(work), static_cast<uint32_t>(cap));
+            seed_index(work), static_cast<uint32_t>(cap), static_cast<uint32_t>(ntiles));
     DL_LAUNCH_CHECK();
 }
 
