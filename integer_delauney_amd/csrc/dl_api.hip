@@ -206,8 +206,8 @@ namespace {
 // caller settles it after its next synchronisation (finish_predicted_fill).
 struct FillPending {
     bool active = false;
-    int n = 0;                // launches issued
-    int* host = nullptr;      // n changed flags, then the visit total (page-locked)
+    int n = 0;                // launches issued (their n changed flags and the visit
+                              // total come back with the final status read)
     BfsConfig cfg{};
     std::tuple<int, int, int, int, int, int> key;
 };
@@ -468,7 +468,7 @@ struct StageClock {
 // without their own seed / output pointers read seeds from d_xy and write
 // into d_out at their offsets in the concatenated layout (C channels).
 void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStream_t s,
-             bool defer_seed_check = false)
+             SeedCheck check = SeedCheck::kNow)
 {
     BatchPlan& plan = ctx->plan;
     plan_batch(plan);
@@ -478,7 +478,7 @@ void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStr
     }
     upload_plan(plan, ctx->tri, s);
     ctx->sorted.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(plan.n_seeds, 1)));
-    normalise_seeds(plan, ctx->sorted.as<uint32_t>(), ctx->tri, s, defer_seed_check);
+    normalise_seeds(plan, ctx->sorted.as<uint32_t>(), ctx->tri, s, check);
 }
 
 // Launches a padded fill of ctx->plan with shape cfg is issued as without
@@ -532,11 +532,10 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     const auto pkey = pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg);
     const int pred = pend ? fill_prediction(ctx, cfg, tune) : 0;
     if (pred > 0) {
-        int* host = ctx->host_small.as<int>() + 128;
-        plane = bfs_launch_fixed(cvs, cfg, ctx->bfs, s, pred, host);
+        // change flags and visit total stay on the device for the final status read
+        plane = bfs_launch_fixed(cvs, cfg, ctx->bfs, s, pred, nullptr);
         pend->active = true;
         pend->n = pred;
-        pend->host = host;
         pend->cfg = cfg;
         pend->key = pkey;
         ctx->fill_cvs = cvs;
@@ -607,20 +606,20 @@ const uint32_t* pad_plane_ptr(const dl_context* ctx, int plane)
     return plane ? ctx->padB.as<uint32_t>() : ctx->padA.as<uint32_t>();
 }
 
-// After the stream's next synchronisation: did the predicted launches reach
-// convergence?  If not, the same fill continues to convergence here (both
+// After the stream's next synchronisation, with the launches' read-back change
+// flags and visit total: did the predicted launches reach convergence?  If not, the same fill continues to convergence here (both
 // planes then hold the final labels) and false is returned: everything
 // computed from the labels must be computed again.
-bool finish_predicted_fill(dl_context* ctx, const FillPending& pend, hipStream_t s,
-                           int* visits)
+bool finish_predicted_fill(dl_context* ctx, const FillPending& pend, const int* changed,
+                           hipStream_t s, int* visits)
 {
     int conv = -1;
     for (int i = 0; i < pend.n; ++i)
-        if (pend.host[i] == 0) {
+        if (changed[i] == 0) {
             conv = i;
             break;
         }
-    *visits += pend.host[pend.n];
+    *visits += changed[pend.n];
     if (conv >= 0) {
         // Keep the count while it still leaves the margin, so that inputs of
         // one class whose fills converge a launch apart keep one signature
@@ -696,34 +695,50 @@ Issued issue_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
 {
     Issued iss;
     iss.C = C;
-    // Seed validation is read back with the flood fill's first wait instead of
-    // a round trip of its own.
-    prepare(ctx, d_xy, d_out, C, s, true);
-    clk.mark(0);
     BatchPlan& plan = ctx->plan;
     const int n_img = static_cast<int>(plan.img.size());
+    // A fill of a class seen before is issued without waiting (predicted
+    // launch count): its result plane is known now, so the descriptors carry
+    // their output label source from the first upload, and seed validation
+    // comes back with the final status read.  Otherwise the seed flags are
+    // read back with the fill's first wait.
+    plan_batch(plan);
+    bool tune = false;
+    const int pred =
+        fill_prediction(ctx, choose_config(ctx, plan.n_cells, plan.n_seeds, &tune), tune);
+    const size_t cells = static_cast<size_t>(std::max<int64_t>(plan.n_cells, 1));
+    ctx->padA.reserve(sizeof(uint32_t) * cells);
+    ctx->padB.reserve(sizeof(uint32_t) * cells);
+    auto set_out_lab = [&](int plane) {
+        const uint32_t* base = pad_plane_ptr(ctx, plane);
+        for (ImageDesc& d : plan.img) {
+            d.out_lab = base + d.cv_off;
+            d.out_W = d.Wc;
+            d.out_off = d.P;
+            d.out_shift = d.P;
+        }
+    };
+    if (pred > 0) set_out_lab(pred & 1);
+    prepare(ctx, d_xy, d_out, C, s, pred > 0 ? SeedCheck::kWithStatus : SeedCheck::kAfterWait);
+    clk.mark(0);
     std::vector<int> watch(n_img, -1);
     for (int i = 0; i < n_img; ++i)
         if (C == 3 && plan.img[i].P > 0) {
             watch[i] = plan.img[i].P;
             iss.any_pad = true;
         }
-    // The fill's visit total arrives with the final status read.  A fill of a
-    // class seen before is issued without waiting (predicted launch count)
-    // and settled after that read.
+    // The fill's visit total arrives with the final status read (a predicted
+    // fill is settled after that read).
     int* visits_async = ctx->host_small.as<int>() + 64;
     *visits_async = 0;
     iss.plane = fill_detection(ctx, watch, s, visits_async, &iss.pend);
-    if (!iss.pend.active) check_seed_flags(plan, ctx->tri);   // the fill has waited at least once
-    const uint32_t* base = pad_plane_ptr(ctx, iss.plane);
-    for (int i = 0; i < n_img; ++i) {
-        ImageDesc& d = plan.img[i];
-        d.out_lab = base + d.cv_off;
-        d.out_W = d.Wc;
-        d.out_off = d.P;
-        d.out_shift = d.P;
+    if (!iss.pend.active || iss.plane != (pred & 1)) {
+        // the fill has waited at least once
+        if (!iss.pend.active) check_seed_flags(plan, ctx->tri.host_seed_flags.as<uint32_t>());
+        set_out_lab(iss.plane);
+        upload_desc(plan, ctx->tri, s);
     }
-    upload_desc(plan, ctx->tri, s);
+    const uint32_t* base = pad_plane_ptr(ctx, iss.plane);
     clk.mark(1);
     detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true);
     clk.mark(2);
@@ -732,7 +747,9 @@ Issued issue_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
                  true);
     clk.mark(4);
     issue_status(plan, ctx->tri, iss.any_pad ? ctx->esc.as<int>() : nullptr,
-                 iss.any_pad ? n_img : 0, s);
+                 iss.any_pad ? n_img : 0, iss.pend.active,
+                 iss.pend.active ? ctx->bfs.counters.as<int>() : nullptr,
+                 iss.pend.active ? iss.pend.n + 1 : 0, s);
     return iss;
 }
 
@@ -744,7 +761,11 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
     const uint32_t* base = pad_plane_ptr(ctx, iss.plane);
     int* visits_async = ctx->host_small.as<int>() + 64;
     DL_HIP(hipStreamSynchronize(s));
-    SpeculativeStatus st = parse_status(plan, ctx->tri, iss.any_pad ? n_img : 0);
+    const uint32_t* seed_flags = nullptr;
+    const int* fill_changed = nullptr;
+    SpeculativeStatus st =
+        parse_status(plan, ctx->tri, iss.any_pad ? n_img : 0, iss.pend.active,
+                     iss.pend.active ? iss.pend.n + 1 : 0, &seed_flags, &fill_changed);
     const int32_t* rows = ctx->tri.rows.as<int32_t>();
     auto detect_and_assign = [&]() {
         detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true);
@@ -757,8 +778,11 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
                            iss.any_pad ? n_img : 0, s);
     };
     if (iss.pend.active) {
-        check_seed_flags(plan, ctx->tri);
-        if (!finish_predicted_fill(ctx, iss.pend, s, visits_async)) st = detect_and_assign();
+        check_seed_flags(plan, seed_flags);
+        // (copied: a status read below reuses the buffer)
+        const std::vector<int> changed(fill_changed, fill_changed + iss.pend.n + 1);
+        if (!finish_predicted_fill(ctx, iss.pend, changed.data(), s, visits_async))
+            st = detect_and_assign();
     }
     ctx->timings[8] += *visits_async;
     if (st.bad_id) throw runtime_error("flood fill left an undefined cell");

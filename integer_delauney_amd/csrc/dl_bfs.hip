@@ -729,8 +729,10 @@ int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
                      dcnt + n_launches + 1 + launch, s);
     sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + n_launches);
     DL_LAUNCH_CHECK();
-    DL_HIP(hipMemcpyAsync(host_changed, dcnt, sizeof(int) * static_cast<size_t>(n_launches + 1),
-                          hipMemcpyDeviceToHost, s));
+    if (host_changed)
+        DL_HIP(hipMemcpyAsync(host_changed, dcnt,
+                              sizeof(int) * static_cast<size_t>(n_launches + 1),
+                              hipMemcpyDeviceToHost, s));
     return n_launches & 1;
 }
 

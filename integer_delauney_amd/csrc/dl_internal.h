@@ -172,11 +172,12 @@ void bfs_init_canvas(const BfsCanvas& cv, const uint32_t* d_labels, int64_t n,
 int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
             hipStream_t s, BfsStats* stats, int* visits_async = nullptr, int start_launch = 0);
 
-// Issues exactly n_launches tile launches of a fresh fill without waiting, and
-// copies each launch's "some tile changed" flag, then the visit total, to
-// host_changed (page-locked, n_launches + 1 ints) asynchronously.  The fill
-// has converged iff some launch changed nothing; the labels are then final in
-// both planes.  Returns the plane the last launch wrote.
+// Issues exactly n_launches tile launches of a fresh fill without waiting.
+// Each launch's "some tile changed" flag, then the visit total, end up in
+// work.counters (n_launches + 1 ints); with host_changed (page-locked) they
+// are also copied there asynchronously.  The fill has converged iff some
+// launch changed nothing; the labels are then final in both planes.  Returns
+// the plane the last launch wrote.
 int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
                      hipStream_t s, int n_launches, int* host_changed);
 
@@ -270,8 +271,10 @@ struct TriWork {
     DevBuf rows;        // (T, 5) triangle rows of all images, image after image
     DevBuf tidgrid;     // W x H tid raster
     DevBuf bin_counts, bin_offsets, bin_list;   // triangle -> tile bins
-    DevBuf desc;        // device copy of the plan's ImageDesc array
-    DevBuf segs;        // device copy of the offset tables + tri_base
+    DevBuf desc;        // device copy of the plan's ImageDesc array, then (from
+                        // segs_off) the offset tables + tri_base
+    size_t segs_off = 0;
+    DevBuf status_dev;  // status words gathered for the read-back
     PinnedBuf host_plan;    // staging of upload_plan (descriptors + offset tables)
     PinnedBuf host_stage;   // staging of upload_desc
     PinnedBuf host_seed_flags;   // per-image seed validation flags (read back late)
@@ -286,20 +289,31 @@ struct TriWork {
 void upload_plan(const BatchPlan& plan, TriWork& work, hipStream_t s);
 void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s);
 
+// Per-image seed validation flags live in work.flags from this word on (the
+// words before it are detection's status).
+constexpr int kSeedFlagBase = 8;
+
+enum class SeedCheck {
+    kNow,          // read the flags back and check them before returning
+    kAfterWait,    // copy them to work.host_seed_flags; checked after the next wait
+    kWithStatus,   // leave them on the device for the final status read
+};
+
 // Validates (bounds, duplicates) and orders the seeds of every image (read
 // from each image's xy).  Throws value_error naming the first failing image
 // when the batch has more than one.
-// With defer_check the validation flags are copied back asynchronously and the
-// call returns at once; check_seed_flags must then run after the stream's next
-// synchronisation point.  Until then invalid seeds only produce garbage labels
-// (every kernel in between stays in bounds).
+// Unless checked now, the call returns at once and check_seed_flags must run
+// on the read-back flags after the stream's next synchronisation point.
+// Until then invalid seeds only produce garbage labels (every kernel in
+// between stays in bounds).
 void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, hipStream_t s,
-                     bool defer_check = false);
+                     SeedCheck check = SeedCheck::kNow);
 inline SeedIndex seed_index(const TriWork& w)
 {
     return SeedIndex{w.seed_idx.as<uint2>()};
 }
-void check_seed_flags(const BatchPlan& plan, const TriWork& work);
+// hf: the images' read-back seed flags.
+void check_seed_flags(const BatchPlan& plan, const uint32_t* hf);
 
 // Writes kInvalid over the first plane of every canvas, then each seed's
 // canvas label (image label + (P, P)) at its cell.  plane_b needs no
@@ -347,11 +361,16 @@ struct SpeculativeStatus {
 // total, and n_esc padding-watch flags from d_esc (may be null).
 SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
                               hipStream_t s);
-// read_status in two halves: the asynchronous copies into work.host_flags,
-// and, after the stream has been synchronised, their parse.
+// read_status in two halves: one kernel gathers the status words on the
+// device and one copy brings them into work.host_flags; after the stream has
+// been synchronised parse_status reads them.  Optionally the gather also takes
+// the images' seed flags (seed_flags) and n_fill words from d_fill (a
+// predicted fill's change flags and visit total); parse_status then points
+// *seed_out and *fill_out at their copies (valid until the next status read).
 void issue_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
-                  hipStream_t s);
-SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc);
+                  bool seed_flags, const int* d_fill, int n_fill, hipStream_t s);
+SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc, bool seed_flags,
+                               int n_fill, const uint32_t** seed_out, const int** fill_out);
 
 // Pixel assignment binned by 64x64 tiles and fused with the output write into
 // each image's out: C == 3, the (H, W, 3) grid with channels 0/1 from its

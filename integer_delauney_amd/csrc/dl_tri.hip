@@ -244,32 +244,44 @@ void plan_batch(BatchPlan& plan)
 namespace {
 inline int64_t* seg_ptr(const TriWork& w, int which, size_t n_img)
 {
-    return w.segs.as<int64_t>() + static_cast<size_t>(which) * (n_img + 1);
+    return reinterpret_cast<int64_t*>(static_cast<char*>(w.desc.ptr) + w.segs_off) +
+           static_cast<size_t>(which) * (n_img + 1);
+}
+
+// work.desc holds the descriptors, then (from a 256-byte boundary) the offset
+// tables, so one copy uploads a plan.
+size_t plan_bytes(size_t n_img, size_t* segs_off)
+{
+    *segs_off = (sizeof(ImageDesc) * n_img + 255) / 256 * 256;
+    return *segs_off + sizeof(int64_t) * 4 * (n_img + 1);
 }
 }  // namespace
 
 void upload_desc(const BatchPlan& plan, TriWork& work, hipStream_t s)
 {
-    const size_t db = sizeof(ImageDesc) * plan.img.size();
-    work.desc.reserve(db);
+    const size_t n = plan.img.size();
+    const size_t db = sizeof(ImageDesc) * n;
+    work.desc.reserve(plan_bytes(n, &work.segs_off));
     work.host_stage.reserve(db);
     stage_h2d(work.host_stage, 0, work.desc.ptr, plan.img.data(), db, s);
 }
 
 void upload_plan(const BatchPlan& plan, TriWork& work, hipStream_t s)
 {
-    // Descriptors and offset tables in a staging buffer of their own:
-    // upload_desc may rewrite its buffer before these copies have run.
+    // Descriptors and the three host-made offset tables in one copy, from a
+    // staging buffer of their own (upload_desc may rewrite its buffer before
+    // this copy has run).
     const size_t n = plan.img.size();
     const size_t db = sizeof(ImageDesc) * n;
     const size_t sb = sizeof(int64_t) * (n + 1);
-    work.desc.reserve(db);
-    work.segs.reserve(sizeof(int64_t) * 4 * (n + 1));
-    work.host_plan.reserve(db + 3 * sb);
-    stage_h2d(work.host_plan, 0, work.desc.ptr, plan.img.data(), db, s);
+    work.desc.reserve(plan_bytes(n, &work.segs_off));
+    const size_t up = work.segs_off + 3 * sb;
+    std::vector<char> h(up, 0);
+    std::memcpy(h.data(), plan.img.data(), db);
     const std::vector<int64_t>* tabs[3] = {&plan.seed_seg, &plan.det_seg, &plan.tile_seg};
-    for (int k = 0; k < 3; ++k)
-        stage_h2d(work.host_plan, db + k * sb, seg_ptr(work, k, n), tabs[k]->data(), sb, s);
+    for (int k = 0; k < 3; ++k) std::memcpy(h.data() + work.segs_off + k * sb, tabs[k]->data(), sb);
+    work.host_plan.reserve(up);
+    stage_h2d(work.host_plan, 0, work.desc.ptr, h.data(), up, s);
 }
 
 // ===========================================================================
@@ -353,10 +365,9 @@ __global__ void canvas_seed_kernel(const uint32_t* __restrict__ sorted, int64_t 
 
 }  // namespace
 
-void check_seed_flags(const BatchPlan& plan, const TriWork& work)
+void check_seed_flags(const BatchPlan& plan, const uint32_t* hf)
 {
     const int n_img = static_cast<int>(plan.img.size());
-    const uint32_t* hf = work.host_seed_flags.as<uint32_t>();
     for (int i = 0; i < n_img; ++i) {
         const uint32_t f = hf[i];
         if (!f) continue;
@@ -368,7 +379,7 @@ void check_seed_flags(const BatchPlan& plan, const TriWork& work)
 }
 
 void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, hipStream_t s,
-                     bool defer_check)
+                     SeedCheck check)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t nw = plan.n_words;
@@ -385,16 +396,19 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, h
         ProfScope ps(kKSeeds, s);
         fill_u32(bm, 0u, static_cast<size_t>(nw), s);
         fill_u32(flags, 0u, static_cast<size_t>(n_img + 8), s);
-        seeds_mark_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm, flags);
+        seeds_mark_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm,
+                                                 flags + kSeedFlagBase);
         DL_LAUNCH_CHECK();
     }
-    // A buffer of their own: the final status read reuses host_flags.
-    work.host_seed_flags.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max(n_img, 1)));
-    DL_HIP(hipMemcpyAsync(work.host_seed_flags.ptr, flags, sizeof(uint32_t) * n_img,
-                          hipMemcpyDeviceToHost, s));
-    if (!defer_check) {
+    if (check != SeedCheck::kWithStatus) {
+        // A buffer of its own: the final status read reuses host_flags.
+        work.host_seed_flags.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max(n_img, 1)));
+        DL_HIP(hipMemcpyAsync(work.host_seed_flags.ptr, flags + kSeedFlagBase,
+                              sizeof(uint32_t) * n_img, hipMemcpyDeviceToHost, s));
+    }
+    if (check == SeedCheck::kNow) {
         DL_HIP(hipStreamSynchronize(s));
-        check_seed_flags(plan, work);
+        check_seed_flags(plan, work.host_seed_flags.as<uint32_t>());
     }
     ProfScope ps(kKSeeds, s);
     popcount_scan(bm, nw, work.seed_pre.as<uint32_t>(), work.scan_tmp, s);
@@ -1231,61 +1245,105 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
 }
 
 namespace {
-// Layout of work.host_flags for the status read: status words, tri_base
-// (n_img + 1), padding-watch flags.
-struct StatusView {
-    uint32_t* hf;
-    int64_t* htb;
-    int* hesc;
+
+// Words of the status read-back, gathered on the device and copied in one
+// piece into work.host_flags: [0] status bits, [1] bin-list entries needed,
+// [2..) tri_base (n_img + 1 int64), then the padding-watch flags, the seed
+// flags and the fill's per-launch change flags with its visit total.
+struct StatusLayout {
+    uint32_t tb, esc, seed, fill, words;
 };
-StatusView status_view(const TriWork& work, int n_img)
+StatusLayout status_layout(int n_img, int n_esc, bool seed, int n_fill)
 {
-    char* h = static_cast<char*>(work.host_flags.ptr);
-    return StatusView{reinterpret_cast<uint32_t*>(h), reinterpret_cast<int64_t*>(h + 64),
-                      reinterpret_cast<int*>(h + 64 + sizeof(int64_t) * (n_img + 1))};
+    StatusLayout L;
+    L.tb = 2;
+    L.esc = L.tb + 2 * static_cast<uint32_t>(n_img + 1);
+    L.seed = L.esc + static_cast<uint32_t>(n_esc);
+    L.fill = L.seed + (seed ? static_cast<uint32_t>(n_img) : 0u);
+    L.words = L.fill + static_cast<uint32_t>(n_fill);
+    return L;
 }
+
+struct GatherArgs {
+    const uint32_t* src[6];
+    uint32_t words[6];
+    uint32_t dst[6];
+    int n;
+};
+
+__global__ __launch_bounds__(256) void gather_words_kernel(GatherArgs g, uint32_t* __restrict__ out)
+{
+    for (int k = 0; k < g.n; ++k)
+        for (uint32_t i = threadIdx.x; i < g.words[k]; i += blockDim.x)
+            out[g.dst[k] + i] = g.src[k][i];
+}
+
 }  // namespace
 
 void issue_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
-                  hipStream_t s)
+                  bool seed_flags, const int* d_fill, int n_fill, hipStream_t s)
 {
     const int n_img = static_cast<int>(plan.img.size());
-    const size_t bytes = 64 + sizeof(int64_t) * (n_img + 1) + sizeof(int) * std::max(n_esc, 1);
-    work.host_flags.reserve(bytes);
-    const StatusView v = status_view(work, n_img);
-    DL_HIP(hipMemcpyAsync(v.hf, work.flags.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    DL_HIP(hipMemcpyAsync(v.htb, seg_ptr(work, 3, n_img), sizeof(int64_t) * (n_img + 1),
-                          hipMemcpyDeviceToHost, s));
+    if (!d_esc) n_esc = 0;
+    if (!d_fill) n_fill = 0;
+    const StatusLayout L = status_layout(n_img, n_esc, seed_flags, n_fill);
+    work.status_dev.reserve(sizeof(uint32_t) * L.words);
+    work.host_flags.reserve(sizeof(uint32_t) * L.words);
+    GatherArgs g{};
+    auto add = [&](const void* src, uint32_t words, uint32_t dst) {
+        if (!words) return;
+        g.src[g.n] = static_cast<const uint32_t*>(src);
+        g.words[g.n] = words;
+        g.dst[g.n] = dst;
+        ++g.n;
+    };
+    const uint32_t* flags = work.flags.as<uint32_t>();
+    add(flags, 1, 0);
+    // bin-list entries needed: copied only when the assignment ran
     if (plan.n_tiles > 0 && work.list_cap > 0)
-        DL_HIP(hipMemcpyAsync(v.hf + 2, work.bin_offsets.as<uint32_t>() + plan.n_tiles,
-                              sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    if (d_esc && n_esc > 0)
-        DL_HIP(hipMemcpyAsync(v.hesc, d_esc, sizeof(int) * n_esc, hipMemcpyDeviceToHost, s));
+        add(work.bin_offsets.as<uint32_t>() + plan.n_tiles, 1, 1);
+    add(seg_ptr(work, 3, n_img), 2 * static_cast<uint32_t>(n_img + 1), L.tb);
+    add(d_esc, static_cast<uint32_t>(n_esc), L.esc);
+    if (seed_flags) add(flags + kSeedFlagBase, static_cast<uint32_t>(n_img), L.seed);
+    add(d_fill, static_cast<uint32_t>(n_fill), L.fill);
+    uint32_t* dev = work.status_dev.as<uint32_t>();
+    {
+        ProfScope ps(kKRegister, s);
+        gather_words_kernel<<<1, 256, 0, s>>>(g, dev);
+        DL_LAUNCH_CHECK();
+    }
+    DL_HIP(hipMemcpyAsync(work.host_flags.ptr, dev, sizeof(uint32_t) * L.words,
+                          hipMemcpyDeviceToHost, s));
 }
 
-SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc)
+SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc, bool seed_flags,
+                               int n_fill, const uint32_t** seed_out, const int** fill_out)
 {
     const int n_img = static_cast<int>(plan.img.size());
-    const StatusView v = status_view(work, n_img);
+    const StatusLayout L = status_layout(n_img, n_esc, seed_flags, n_fill);
+    const uint32_t* h = work.host_flags.as<uint32_t>();
+    const int64_t* htb = reinterpret_cast<const int64_t*>(h + L.tb);
     SpeculativeStatus st;
-    const uint32_t f = v.hf[0];
+    const uint32_t f = h[0];
     st.bad_id = (f & kFlagBadId) != 0;
-    // bin-list entries needed (copied only when the assignment ran)
-    const int64_t listed = plan.n_tiles > 0 && work.list_cap > 0 ? v.hf[2] : 0;
+    const int64_t listed = plan.n_tiles > 0 && work.list_cap > 0 ? h[1] : 0;
     st.redo = (f & (kFlagOverflow | kFlagTableOverflow | kFlagCollision)) != 0 ||
               listed > work.list_cap;
-    st.n_tri = v.htb[n_img];
-    work.tri_base.assign(v.htb, v.htb + n_img + 1);
-    st.escaped.assign(v.hesc, v.hesc + n_esc);
+    st.n_tri = htb[n_img];
+    work.tri_base.assign(htb, htb + n_img + 1);
+    const int* hesc = reinterpret_cast<const int*>(h + L.esc);
+    st.escaped.assign(hesc, hesc + n_esc);
+    if (seed_out) *seed_out = h + L.seed;
+    if (fill_out) *fill_out = reinterpret_cast<const int*>(h + L.fill);
     return st;
 }
 
 SpeculativeStatus read_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_esc,
                               hipStream_t s)
 {
-    issue_status(plan, work, d_esc, n_esc, s);
+    issue_status(plan, work, d_esc, n_esc, false, nullptr, 0, s);
     DL_HIP(hipStreamSynchronize(s));
-    return parse_status(plan, work, d_esc ? n_esc : 0);
+    return parse_status(plan, work, d_esc ? n_esc : 0, false, 0, nullptr, nullptr);
 }
 
 // ===========================================================================
