@@ -532,7 +532,9 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     const auto pkey = pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg);
     const int pred = pend ? fill_prediction(ctx, cfg, tune) : 0;
     if (pred > 0) {
-        // change flags and visit total stay on the device for the final status read
+        // change flags and visit total stay on the device for the final status
+        // read; visits are counted only for statistics
+        ctx->bfs.count_visits = ctx->timing || ctx->profiling;
         plane = bfs_launch_fixed(cvs, cfg, ctx->bfs, s, pred, nullptr);
         pend->active = true;
         pend->n = pred;
@@ -740,7 +742,8 @@ Issued issue_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
     }
     const uint32_t* base = pad_plane_ptr(ctx, iss.plane);
     clk.mark(1);
-    detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true);
+    // (the status words were cleared with the seed flags)
+    detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true, true);
     clk.mark(2);
     clk.mark(3);
     assign_tiled(plan, ctx->tri.rows.as<int32_t>(), -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s,

@@ -722,13 +722,18 @@ int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     work.counters.reserve(sizeof(int) * 2 * static_cast<size_t>(n_launches + 1));
     int* dcnt = work.counters.as<int>();
     int* visits = work.visits.as<int>();
-    fill_u32(reinterpret_cast<uint32_t*>(visits), 0u, static_cast<size_t>(total), s);
+    // Without visit counting the per-tile counts are left as they are (never
+    // read) and the total stays 0.
+    if (work.count_visits)
+        fill_u32(reinterpret_cast<uint32_t*>(visits), 0u, static_cast<size_t>(total), s);
     fill_u32(reinterpret_cast<uint32_t*>(dcnt), 0u, 2 * static_cast<size_t>(n_launches + 1), s);
     for (int launch = 0; launch < n_launches; ++launch)
         launch_tiles(v, cfg, work, n_canvas, total, launch, dcnt + launch,
                      dcnt + n_launches + 1 + launch, s);
-    sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + n_launches);
-    DL_LAUNCH_CHECK();
+    if (work.count_visits) {
+        sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + n_launches);
+        DL_LAUNCH_CHECK();
+    }
     if (host_changed)
         DL_HIP(hipMemcpyAsync(host_changed, dcnt,
                               sizeof(int) * static_cast<size_t>(n_launches + 1),

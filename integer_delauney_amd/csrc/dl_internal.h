@@ -146,6 +146,8 @@ struct BfsWork {
     DevBuf tile_map;  // tile -> canvas index (multi-canvas launches)
     PinnedBuf host_counters;
     PinnedBuf host_canvases;   // staging of the canvas table
+    // bfs_launch_fixed: count tile visits (statistics only) into the visit total
+    bool count_visits = true;
 };
 
 // Interior tile size for a configuration.
@@ -255,8 +257,8 @@ __device__ __forceinline__ int32_t seed_id_at(const SeedIndex& si, int64_t bm_of
 }
 
 struct TriWork {
-    DevBuf seed_bm, seed_pre;   // seed presence bitmaps and their popcount prefix
-    DevBuf seed_idx;            // both, interleaved per word (SeedIndex)
+    DevBuf seed_bm;     // seed presence bitmaps
+    DevBuf seed_idx;    // per word: (popcount prefix, word) (SeedIndex)
     DevBuf bitmap;      // registration bitmaps
     DevBuf scan;        // word prefix sums
     DevBuf scan_tmp;
@@ -327,6 +329,10 @@ void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* pl
 void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp,
                    hipStream_t s);
 
+// The same scan written as pairs[i] = (prefix before word i, word i).
+void popcount_scan_pairs(const uint32_t* words, int64_t n, uint2* pairs, DevBuf& tmp,
+                         hipStream_t s);
+
 // Exclusive prefix sum of in[0..n) into out[0..n); out[n] receives the total.
 void exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s);
 
@@ -346,9 +352,11 @@ struct DetectResult {
 // syncs), and fills work.tri_base.  speculative = true sizes them from the
 // seed count without any sync; capacity overflows, hash collisions and the
 // triangle counts are then checked by read_status after the assignment.
+// flags_zeroed: the detection status words (work.flags[0 .. 8)) are known to
+// be zero already (normalise_seeds clears them), so no clearing launch.
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
                               const uint32_t* sorted, TriWork& work,
-                              hipStream_t s, bool speculative);
+                              hipStream_t s, bool speculative, bool flags_zeroed = false);
 
 struct SpeculativeStatus {
     bool bad_id = false;   // a canvas cell without owner / an id out of range

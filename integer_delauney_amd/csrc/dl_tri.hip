@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(
 }
 
 // Exclusive scan of bsum[0..nb) in place with one 1024-thread block; writes the
-// grand total to *total.
+// grand total to *total (when given).
 __global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* __restrict__ bsum, int64_t nb,
                                                            uint32_t* __restrict__ total)
 {
@@ -88,23 +88,44 @@ __global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t* __restrict_
         if (threadIdx.x == 1023) carry += incl;
         __syncthreads();
     }
-    if (threadIdx.x == 0) *total = carry;
+    if (threadIdx.x == 0 && total) *total = carry;
 }
 
-template <bool POPC>
+// The scan of each block's kScanBlock items, offset by the block's prefix.
+// SELF: the prefix is the sum of the unscanned partials bsum[0 .. block)
+// (none when bsum is null: a one-block scan), and the last block writes the
+// total to out[n]; otherwise bsum holds the scanned prefixes.  INTER writes
+// (prefix, raw word) pairs to inter instead of prefixes to out (no total).
+template <bool POPC, bool INTER, bool SELF>
 __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
     const uint32_t* __restrict__ w, int64_t n, const uint32_t* __restrict__ bsum,
-    uint32_t* __restrict__ out)
+    uint32_t* __restrict__ out, uint2* __restrict__ inter)
 {
     __shared__ uint32_t part[kScanThreads];
+    __shared__ uint32_t red[kScanThreads / 64];
+    uint32_t prefix;
+    if (SELF) {
+        uint32_t pf = 0;
+        if (bsum)
+            for (int64_t k = threadIdx.x; k < blockIdx.x; k += kScanThreads) pf += bsum[k];
+        for (int off = 32; off > 0; off >>= 1) pf += __shfl_down(pf, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pf;
+        __syncthreads();
+        prefix = 0;
+#pragma unroll
+        for (int k = 0; k < kScanThreads / 64; ++k) prefix += red[k];
+    } else {
+        prefix = bsum[blockIdx.x];
+    }
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanBlock +
                          static_cast<int64_t>(threadIdx.x) * kScanItems;
-    uint32_t c[kScanItems];
+    uint32_t raw[kScanItems], c[kScanItems];
     uint32_t s = 0;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const int64_t i = base + j;
-        c[j] = i < n ? scan_item<POPC>(w[i]) : 0u;
+        raw[j] = i < n ? w[i] : 0u;
+        c[j] = scan_item<POPC>(raw[j]);
         s += c[j];
     }
     part[threadIdx.x] = s;
@@ -115,45 +136,65 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
         part[threadIdx.x] += add;
         __syncthreads();
     }
-    uint32_t run = bsum[blockIdx.x] + part[threadIdx.x] - s;
+    uint32_t run = prefix + part[threadIdx.x] - s;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const int64_t i = base + j;
-        if (i < n) out[i] = run;
+        if (i < n) {
+            if (INTER) inter[i] = make_uint2(run, raw[j]);
+            else out[i] = run;
+        }
         run += c[j];
     }
+    if (!INTER && SELF && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1)
+        out[n] = run;
 }
 
 }  // namespace
 
-template <bool POPC>
-static void scan_impl(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
+// Up to this many partial sums, each block of the down pass adds up the ones
+// before it itself (no separate pass over the partials).
+constexpr int64_t kScanSelfMax = 512;
+
+template <bool POPC, bool INTER>
+static void scan_impl(const uint32_t* words, int64_t n, uint32_t* out, uint2* inter, DevBuf& tmp,
+                      hipStream_t s)
 {
     const int64_t nb = std::max<int64_t>(1, (n + kScanBlock - 1) / kScanBlock);
+    if (nb == 1) {   // one block: no partial sums at all
+        scan_down_kernel<POPC, INTER, true><<<1, kScanThreads, 0, s>>>(words, n, nullptr, out, inter);
+        DL_LAUNCH_CHECK();
+        return;
+    }
     tmp.reserve(sizeof(uint32_t) * static_cast<size_t>(nb));
     uint32_t* bsum = tmp.as<uint32_t>();
-    if (n > 0) {
-        scan_reduce_kernel<POPC><<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum);
-        DL_LAUNCH_CHECK();
-    } else {
-        fill_u32(bsum, 0u, 1, s);
-    }
-    scan_blocks_kernel<<<1, 1024, 0, s>>>(bsum, nb, out + n);
+    scan_reduce_kernel<POPC><<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum);
     DL_LAUNCH_CHECK();
-    if (n > 0) {
-        scan_down_kernel<POPC><<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum, out);
+    if (nb <= kScanSelfMax) {
+        scan_down_kernel<POPC, INTER, true>
+            <<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum, out, inter);
+    } else {
+        scan_blocks_kernel<<<1, 1024, 0, s>>>(bsum, nb, INTER ? nullptr : out + n);
         DL_LAUNCH_CHECK();
+        scan_down_kernel<POPC, INTER, false>
+            <<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(words, n, bsum, out, inter);
     }
+    DL_LAUNCH_CHECK();
 }
 
 void popcount_scan(const uint32_t* words, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
 {
-    scan_impl<true>(words, n, out, tmp, s);
+    scan_impl<true, false>(words, n, out, nullptr, tmp, s);
+}
+
+void popcount_scan_pairs(const uint32_t* words, int64_t n, uint2* pairs, DevBuf& tmp, hipStream_t s)
+{
+    scan_impl<true, true>(words, n, nullptr, pairs, tmp, s);
 }
 
 void exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, DevBuf& tmp, hipStream_t s)
 {
-    scan_impl<false>(in, n, out, tmp, s);
+    scan_impl<false, false>(in, n, out, nullptr, tmp, s);
 }
 
 
@@ -319,8 +360,8 @@ __global__ void seeds_mark_kernel(int64_t n_total, const int64_t* __restrict__ s
 }
 
 __global__ void seeds_rank_kernel(int64_t n_total, const int64_t* __restrict__ seed_seg, int n_img,
-                                  const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
-                                  const uint32_t* __restrict__ pre, uint32_t* __restrict__ sorted)
+                                  const ImageDesc* __restrict__ D, const uint2* __restrict__ idx,
+                                  uint32_t* __restrict__ sorted)
 {
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= n_total) return;
@@ -331,17 +372,12 @@ __global__ void seeds_rank_kernel(int64_t n_total, const int64_t* __restrict__ s
     if (x < 0 || x >= d.W || y < 0 || y >= d.H) return;   // flagged by seeds_mark_kernel
     const uint64_t bit = static_cast<uint64_t>(x) * d.H + y;
     const int64_t w = d.bm_off + static_cast<int64_t>(bit >> 5);
-    const uint32_t below = bm[w] & ((1u << (bit & 31)) - 1u);
-    const uint32_t rank = pre[w] - pre[d.bm_off] + static_cast<uint32_t>(__popc(below));
+    const uint2 e = idx[w];   // (prefix, word)
+    const uint32_t below = e.y & ((1u << (bit & 31)) - 1u);
+    // rank among the image's set bits (< n even when an earlier image holds
+    // duplicates, which are flagged and leave fewer bits than seeds)
+    const uint32_t rank = e.x - idx[d.bm_off].x + static_cast<uint32_t>(__popc(below));
     sorted[d.seed_off + rank] = pack_label(x, y);
-}
-
-__global__ void interleave_index_kernel(const uint32_t* __restrict__ bm,
-                                        const uint32_t* __restrict__ pre, int64_t nw,
-                                        uint2* __restrict__ out)
-{
-    const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (w < nw) out[w] = make_uint2(pre[w], bm[w]);
 }
 
 // Seed cells of every canvas: the image label shifted by (P, P).
@@ -384,7 +420,6 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, h
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t nw = plan.n_words;
     work.seed_bm.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(nw, 1)));
-    work.seed_pre.reserve(sizeof(uint32_t) * static_cast<size_t>(nw + 1));
     work.flags.reserve(sizeof(uint32_t) * (n_img + 8));
     work.host_flags.reserve(sizeof(uint32_t) * (n_img + 8));
     uint32_t* bm = work.seed_bm.as<uint32_t>();
@@ -411,13 +446,10 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, h
         check_seed_flags(plan, work.host_seed_flags.as<uint32_t>());
     }
     ProfScope ps(kKSeeds, s);
-    popcount_scan(bm, nw, work.seed_pre.as<uint32_t>(), work.scan_tmp, s);
-    seeds_rank_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm,
-                                             work.seed_pre.as<uint32_t>(), d_sorted);
-    DL_LAUNCH_CHECK();
     work.seed_idx.reserve(sizeof(uint2) * static_cast<size_t>(std::max<int64_t>(nw, 1)));
-    interleave_index_kernel<<<static_cast<unsigned>(std::max<int64_t>(1, (nw + 255) / 256)), 256, 0,
-                              s>>>(bm, work.seed_pre.as<uint32_t>(), nw, work.seed_idx.as<uint2>());
+    popcount_scan_pairs(bm, nw, work.seed_idx.as<uint2>(), work.scan_tmp, s);
+    seeds_rank_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D,
+                                             work.seed_idx.as<uint2>(), d_sorted);
     DL_LAUNCH_CHECK();
 }
 
@@ -635,6 +667,9 @@ struct DetectArgs {
     uint32_t* flags;              // batch-wide status word
     Table table;
     const uint32_t* npos;         // candidate positions actually listed (device)
+    const uint32_t* n_listed;     // candidate positions found (scan total)
+    const uint32_t* n_quads;      // four-region blocks found (scan total)
+    uint64_t table_cap;
     uint32_t pos_cap;             // capacity of positions (cand holds 2 per position)
     uint32_t rows_cap;            // capacity of rows
 };
@@ -822,6 +857,16 @@ __global__ __launch_bounds__(kDetThreads) void detect_write_kernel(DetectArgs A)
 {
     __shared__ uint32_t part[kDetThreads];
     const int64_t gb = blockIdx.x;
+    if (gb == 0 && threadIdx.x == 0) {
+        // Positions listed (capped) and the capacity checks of a speculatively
+        // sized detection: more positions than the list holds, or more
+        // distinct triplets than three quarters of the table, raise
+        // kFlagOverflow (read by the later kernels and the host).
+        const uint32_t n = *A.n_listed, q = *A.n_quads;
+        *const_cast<uint32_t*>(A.npos) = min(n, A.pos_cap);
+        if (n > A.pos_cap || 4 * (static_cast<uint64_t>(n) + 2ull * q) > 3 * A.table_cap)
+            atomicOr(A.flags, kFlagOverflow);
+    }
     int img, ry0, rx0;
     det_block_origin(A, gb, img, ry0, rx0);
     const ImageDesc& d = A.D[img];
@@ -997,8 +1042,14 @@ __device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table&
 __global__ void emit_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ npos,
                             Table t, const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
                             const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
-                            uint32_t rows_cap, uint32_t* __restrict__ flags)
+                            uint32_t rows_cap, uint32_t* __restrict__ flags, int n_img,
+                            int64_t n_words, int64_t* __restrict__ tri_base)
 {
+    // With tri_base given, block 0 also writes each image's first row (as
+    // tri_base_kernel does).
+    if (tri_base && blockIdx.x == 0)
+        for (int i = threadIdx.x; i <= n_img; i += blockDim.x)
+            tri_base[i] = i < n_img ? pre[D[i].key_off] : pre[n_words];
     if (*flags & kFlagOverflow) return;   // rerun with exact sizes follows
     const XcdRange r = xcd_range(2 * *npos);
     for (uint32_t i = r.first; i < r.end; i += r.step)
@@ -1037,21 +1088,6 @@ __device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table&
     r[4] = cd.c;
 }
 
-// Positions actually listed (capped) and the capacity checks of a
-// speculatively sized detection: more positions than the list holds, or more
-// distinct triplets than three quarters of the table, raise kFlagOverflow.
-__global__ void det_bounds_kernel(const uint32_t* __restrict__ npos_total,
-                                  const uint32_t* __restrict__ quads_total, uint32_t pos_cap,
-                                  uint64_t table_cap, uint32_t* __restrict__ npos,
-                                  uint32_t* __restrict__ flags)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t n = *npos_total, q = *quads_total;
-    *npos = min(n, pos_cap);
-    if (n > pos_cap || 4 * (static_cast<uint64_t>(n) + 2ull * q) > 3 * table_cap)
-        atomicOr(flags, kFlagOverflow);
-}
-
 __global__ void zero_u32_kernel(uint32_t* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1069,7 +1105,7 @@ uint64_t next_pow2(uint64_t v)
 
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
                               const uint32_t* sorted, TriWork& work,
-                              hipStream_t s, bool speculative)
+                              hipStream_t s, bool speculative, bool flags_zeroed)
 {
     DetectResult res;
     const int n_img = static_cast<int>(plan.img.size());
@@ -1084,7 +1120,7 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     // flags layout: [0] status bits, [2] four-region blocks (counter),
     // [4] candidate positions found (counter), [6] positions in the list
     // (min of [4] and the capacity; read by the later kernels).
-    {
+    if (!flags_zeroed) {
         ProfScope ps(kKDetect, s);
         zero_u32_kernel<<<1, 64, 0, s>>>(dflags, 8);
         DL_LAUNCH_CHECK();
@@ -1149,6 +1185,9 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         res.n_quads = static_cast<int64_t>(nquads);
     }
     A.pos_cap = static_cast<uint32_t>(std::min<uint64_t>(pos_cap, 0x7FFFFFF0ull));
+    A.n_listed = n_listed;
+    A.n_quads = n_quads;
+    A.table_cap = table_cap;
     work.positions.reserve(sizeof(uint2) * static_cast<size_t>(A.pos_cap));
     A.positions = work.positions.as<uint2>();
     A.rows_cap = 2 * A.pos_cap;
@@ -1158,9 +1197,6 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     work.rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(A.rows_cap));
     {
         ProfScope ps(kKDetect, s);
-        det_bounds_kernel<<<1, 64, 0, s>>>(n_listed, n_quads, A.pos_cap, table_cap, dflags + 6,
-                                           dflags);
-        DL_LAUNCH_CHECK();
         detect_write_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
     }
@@ -1207,9 +1243,11 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
             mark_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.npos, t, A.D, bm, dflags);
             DL_LAUNCH_CHECK();
             popcount_scan(bm, nwords, pre, work.scan_tmp, s);
-            tri_base_kernel<<<static_cast<unsigned>((n_img + 256) / 256), 256, 0, s>>>(
-                A.D, n_img, pre, nwords, tri_base);
-            DL_LAUNCH_CHECK();
+            if (!speculative) {   // read back below; otherwise emit_kernel writes it
+                tri_base_kernel<<<static_cast<unsigned>((n_img + 256) / 256), 256, 0, s>>>(
+                    A.D, n_img, pre, nwords, tri_base);
+                DL_LAUNCH_CHECK();
+            }
         }
         if (!speculative) {
             int64_t* htb = reinterpret_cast<int64_t*>(hflags + 8);
@@ -1236,7 +1274,8 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         {
             ProfScope ps(kKRegister, s);
             emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.npos, t, A.D, bm, pre,
-                                                work.rows.as<int32_t>(), A.rows_cap, dflags);
+                                                work.rows.as<int32_t>(), A.rows_cap, dflags,
+                                                n_img, nwords, speculative ? tri_base : nullptr);
             DL_LAUNCH_CHECK();
         }
         return res;
