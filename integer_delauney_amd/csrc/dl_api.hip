@@ -510,9 +510,9 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     ctx->padA.reserve(sizeof(uint32_t) * cells);
     ctx->padB.reserve(sizeof(uint32_t) * cells);
     ctx->esc.reserve(sizeof(int) * static_cast<size_t>(n_img));
-    fill_u32(ctx->esc.as<uint32_t>(), 0u, static_cast<size_t>(n_img), s);
+    const FillSeg esc_clear{ctx->esc.as<uint32_t>(), static_cast<size_t>(n_img), 0u};
     init_canvases(plan, ctx->sorted.as<uint32_t>(), ctx->padA.as<uint32_t>(),
-                  ctx->padB.as<uint32_t>(), ctx->tri, s);
+                  ctx->padB.as<uint32_t>(), ctx->tri, s, &esc_clear);
     std::vector<BfsCanvas> cvs(n_img);
     for (int i = 0; i < n_img; ++i) {
         const ImageDesc& d = plan.img[i];
@@ -559,9 +559,8 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
             float t_min = 0.f;
             for (int rep = 0; rep < 2; ++rep) {
                 if (!first) {   // start again from the seeds
-                    fill_u32(ctx->esc.as<uint32_t>(), 0u, static_cast<size_t>(n_img), s);
                     init_canvases(plan, ctx->sorted.as<uint32_t>(), ctx->padA.as<uint32_t>(),
-                                  ctx->padB.as<uint32_t>(), ctx->tri, s);
+                                  ctx->padB.as<uint32_t>(), ctx->tri, s, &esc_clear);
                 }
                 first = false;
                 st = BfsStats{};

@@ -575,6 +575,26 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n)
 }
 
 // out[0] = sum of p[0..n) (one block).
+// Up to four fills in one launch: block b works on the segment whose block
+// range holds b (ranges in `first`, one per segment).
+__global__ void fill_multi_kernel(FillSegs f, FillBlocks first)
+{
+    int k = 0;
+    while (k + 1 < f.k && blockIdx.x >= first.b[k + 1]) ++k;
+    uint32_t* p = f.seg[k].p;
+    const uint32_t v = f.seg[k].v;
+    const size_t n = f.seg[k].n;
+    const size_t stride = static_cast<size_t>(first.b[k + 1] - first.b[k]) * blockDim.x * 4;
+    size_t i = (static_cast<size_t>(blockIdx.x - first.b[k]) * blockDim.x + threadIdx.x) * 4;
+    for (; i < n; i += stride) {
+        if (i + 4 <= n && (reinterpret_cast<uintptr_t>(p + i) & 15) == 0) {
+            *reinterpret_cast<uint4*>(p + i) = make_uint4(v, v, v, v);
+        } else {
+            for (size_t j = i; j < n && j < i + 4; ++j) p[j] = v;
+        }
+    }
+}
+
 __global__ __launch_bounds__(1024) void sum_ints_kernel(const int* __restrict__ p, int n,
                                                         int* __restrict__ out)
 {
@@ -666,6 +686,24 @@ void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
     const size_t threads = (n + 3) / 4;
     const unsigned blocks = static_cast<unsigned>(std::min<size_t>((threads + 255) / 256, 65536));
     fill_u32_kernel<<<blocks, 256, 0, s>>>(p, v, n);
+    DL_LAUNCH_CHECK();
+}
+
+void fill_u32_multi(const FillSegs& segs, hipStream_t s)
+{
+    FillSegs f{};
+    FillBlocks first{};
+    unsigned blocks = 0;
+    for (int k = 0; k < segs.k; ++k) {
+        if (segs.seg[k].n == 0) continue;
+        first.b[f.k] = blocks;
+        f.seg[f.k++] = segs.seg[k];
+        const size_t threads = (segs.seg[k].n + 3) / 4;
+        blocks += static_cast<unsigned>(std::min<size_t>((threads + 255) / 256, 16384));
+    }
+    if (f.k == 0) return;
+    first.b[f.k] = blocks;
+    fill_multi_kernel<<<blocks, 256, 0, s>>>(f, first);
     DL_LAUNCH_CHECK();
 }
 

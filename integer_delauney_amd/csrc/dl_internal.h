@@ -109,6 +109,21 @@ CaptureLog*& capture_log();
 void stage_h2d(PinnedBuf& stage, size_t offset, void* dst, const void* src, size_t n,
                hipStream_t s);
 
+// Several fills (p[0 .. n) = v) in one launch.
+struct FillSeg {
+    uint32_t* p;
+    size_t n;
+    uint32_t v;
+};
+struct FillSegs {
+    FillSeg seg[4];
+    int k;
+};
+struct FillBlocks {
+    unsigned b[5];
+};
+void fill_u32_multi(const FillSegs& segs, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // BFS (Voronoi flood fill) over one or more canvases.
 // ---------------------------------------------------------------------------
@@ -320,9 +335,11 @@ void check_seed_flags(const BatchPlan& plan, const uint32_t* hf);
 // Writes kInvalid over the first plane of every canvas, then each seed's
 // canvas label (image label + (P, P)) at its cell.  plane_b needs no
 // initialisation: the flood fill's first launch visits every tile and writes
-// every cell of the second plane.  plane_a/plane_b: n_cells each.
+// every cell of the second plane.  plane_a/plane_b: n_cells each.  extra: one
+// more fill issued in the same launch as the plane's.
 void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* plane_a,
-                   uint32_t* plane_b, struct TriWork& work, hipStream_t s);
+                   uint32_t* plane_b, struct TriWork& work, hipStream_t s,
+                   const FillSeg* extra = nullptr);
 
 // Exclusive prefix sum of popcount(words[i]) into out[i]; out[n] receives the
 // total.  out has n+1 entries.

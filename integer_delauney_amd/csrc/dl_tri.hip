@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
 
 // Up to this many partial sums, each block of the down pass adds up the ones
 // before it itself (no separate pass over the partials).
-constexpr int64_t kScanSelfMax = 512;
+constexpr int64_t kScanSelfMax = 2048;
 
 template <bool POPC, bool INTER>
 static void scan_impl(const uint32_t* words, int64_t n, uint32_t* out, uint2* inter, DevBuf& tmp,
@@ -429,8 +429,11 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, h
     const unsigned blocks = static_cast<unsigned>((plan.n_seeds + 255) / 256);
     {
         ProfScope ps(kKSeeds, s);
-        fill_u32(bm, 0u, static_cast<size_t>(nw), s);
-        fill_u32(flags, 0u, static_cast<size_t>(n_img + 8), s);
+        FillSegs fs{};
+        fs.seg[0] = FillSeg{bm, static_cast<size_t>(nw), 0u};
+        fs.seg[1] = FillSeg{flags, static_cast<size_t>(n_img + 8), 0u};
+        fs.k = 2;
+        fill_u32_multi(fs, s);
         seeds_mark_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm,
                                                  flags + kSeedFlagBase);
         DL_LAUNCH_CHECK();
@@ -454,14 +457,18 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, h
 }
 
 void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* plane_a,
-                   uint32_t* plane_b, TriWork& work, hipStream_t s)
+                   uint32_t* plane_b, TriWork& work, hipStream_t s, const FillSeg* extra)
 {
     const int n_img = static_cast<int>(plan.img.size());
     ProfScope ps(kKInit, s);
     // Only the first plane: the flood fill's first launch visits every tile
     // and writes every cell of the second plane.
     (void)plane_b;
-    fill_u32(plane_a, kInvalid, static_cast<size_t>(plan.n_cells), s);
+    FillSegs fs{};
+    fs.seg[0] = FillSeg{plane_a, static_cast<size_t>(plan.n_cells), kInvalid};
+    fs.k = 1;
+    if (extra) fs.seg[fs.k++] = *extra;
+    fill_u32_multi(fs, s);
     canvas_seed_kernel<<<static_cast<unsigned>((plan.n_seeds + 255) / 256), 256, 0, s>>>(
         d_sorted, plan.n_seeds, seg_ptr(work, 0, n_img), n_img, work.desc.as<ImageDesc>(),
         plane_a);
@@ -569,12 +576,14 @@ __device__ int64_t table_insert(const Table& t, uint32_t img, int64_t seed_off, 
     return -1;
 }
 
-__global__ void table_init_kernel(Slot* __restrict__ slots, uint64_t n)
+// Empties the table and clears the registration bitmap (bm: nw words).
+__global__ void table_init_kernel(Slot* __restrict__ slots, uint64_t n, uint32_t* __restrict__ bm,
+                                  uint64_t nw)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     // tag empty, minkey none, not blocked
-    *reinterpret_cast<uint4*>(slots + i) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);
+    if (i < n) *reinterpret_cast<uint4*>(slots + i) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u);
+    for (uint64_t w = i; w < nw; w += static_cast<uint64_t>(gridDim.x) * blockDim.x) bm[w] = 0u;
 }
 
 // Block b of a grid that is a multiple of 8 runs on XCD b % 8 (dispatch is
@@ -1229,9 +1238,8 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         {
             ProfScope ps(kKInit, s);
             table_init_kernel<<<static_cast<unsigned>((table_cap + 255) / 256), 256, 0, s>>>(
-                t.slot, table_cap);
+                t.slot, table_cap, bm, static_cast<uint64_t>(nwords));
             DL_LAUNCH_CHECK();
-            fill_u32(bm, 0u, static_cast<size_t>(nwords), s);
         }
         {
             ProfScope ps(kKDetect, s);
