@@ -167,6 +167,52 @@ def bench_batch(lib, ctx, n_images: int, steps: int):
             "uncaptured_equals_replayed_bitwise": same_uncaptured}
 
 
+def bench_small_images(lib, ctx, steps: int):
+    """Small single images through dl_triangulate_device, the regime where the
+    host's issue of ~30 launches per call is comparable to the GPU work:
+    256 x 256 with 1,000 uniform seeds (the reference's CPU-runnable
+    configuration, default_rng(1)) and 1024 x 1024 at the 4096^2 seed density.
+    Calls per second with hipGraph replay and with every launch issued from
+    the host; the two outputs are compared bit for bit."""
+    import torch
+
+    from integer_delauney_amd import _lib
+    from integer_delauney_amd.workloads import uniform_seeds
+
+    res = []
+    for side, n, rng in ((256, 1000, 1), (1024, 12500, 2)):
+        seeds = uniform_seeds(side, side, n, rng)
+        d_xy = torch.from_numpy(seeds).to("cuda")
+        d_out = torch.empty((side, side, 3), dtype=torch.int32, device="cuda")
+        nt = ctypes.c_int64(0)
+
+        def call():
+            _lib.check(lib.dl_triangulate_device(ctx.handle, side, side,
+                                                 ctypes.c_void_p(d_xy.data_ptr()), n, 16,
+                                                 ctypes.c_void_p(d_out.data_ptr()), None,
+                                                 ctypes.byref(nt)))
+
+        out = {}
+        for graphs in (1, 0):
+            _lib.check(lib.dl_set_graphs(ctx.handle, graphs))
+            for _ in range(4):
+                call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                call()
+            torch.cuda.synchronize()
+            out[graphs] = ((time.perf_counter() - t0) / steps * 1e3, d_out.clone())
+        _lib.check(lib.dl_set_graphs(ctx.handle, 1))
+        res.append({"side": side, "seeds": n, "triangles": int(nt.value), "calls": steps,
+                    "graph_ms_per_call": out[1][0], "uncaptured_ms_per_call": out[0][0],
+                    "graph_mpixel_per_s": side * side / out[1][0] / 1e3,
+                    "uncaptured_mpixel_per_s": side * side / out[0][0] / 1e3,
+                    "equal_bitwise": bool(torch.equal(out[1][1], out[0][1]))})
+        log(f"small images {side}^2 done")
+    return res
+
+
 def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_del: int):
     """Clustered seeds + incremental updates: a side x side session with n0
     seeds from a 64-component Gaussian mixture (sigma 200 px), then `rounds`
@@ -467,6 +513,8 @@ def main():
     ap.add_argument("--batch-images", type=int, default=512,
                     help="images of the batched workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--small-steps", type=int, default=200,
+                    help="calls per small-image case, graphs on and off (0: skip)")
     ap.add_argument("--inc-rounds", type=int, default=100,
                     help="update rounds of the incremental workload (0: skip it)")
     ap.add_argument("--inc-side", type=int, default=16384)
@@ -623,6 +671,8 @@ def main():
         e2e = W * H / ((t3 - t2) / args.e2e_steps) / 1e6
         log("end-to-end steps done")
 
+    small = bench_small_images(lib, ctx, args.small_steps) if args.small_steps > 0 else None
+
     batch = None
     if args.batch_images > 0:
         batch = bench_batch(lib, ctx, args.batch_images, args.batch_steps)
@@ -678,6 +728,7 @@ def main():
         "fill": {"launches": fill_launches, "tile_visits": fill_visits,
                  "config": [cfg_w, cfg_r, cfg_k]},
         "end_to_end_mpixel_per_s": e2e,
+        "small_images": small,
         "batched": batch,
         "incremental": incremental,
         "serving": serving,
