@@ -556,20 +556,19 @@ __device__ int64_t table_insert(const Table& t, uint32_t img, int64_t seed_off, 
 {
     const uint64_t tg = triplet_tag(t, img, a, b, c);
     uint64_t h = (static_cast<uint64_t>(seed_off + a) * t.stride) & t.mask;
+    // One compare-and-swap per probed slot (its return value says whether the
+    // slot was empty, holds this tag, or holds another): device-scope atomics
+    // and loads are executed past the L2, so a load before the swap would
+    // cost as much as the swap.
     for (uint64_t probe = 0; probe <= t.mask; ++probe) {
         Slot& sl = t.slot[h];
-        const unsigned long long cur = __atomic_load_n(&sl.tag, __ATOMIC_RELAXED);
-        if (cur == tg) return static_cast<int64_t>(h);
-        if (cur == kEmptyTag) {
-            const unsigned long long prev =
-                atomicCAS(&sl.tag, static_cast<unsigned long long>(kEmptyTag),
-                          static_cast<unsigned long long>(tg));
-            if (prev == kEmptyTag) {
-                if (t.tri) t.tri[h] = SlotTri{a, b, c, img};
-                return static_cast<int64_t>(h);
-            }
-            if (prev == tg) return static_cast<int64_t>(h);
+        const unsigned long long prev = atomicCAS(&sl.tag, static_cast<unsigned long long>(kEmptyTag),
+                                                  static_cast<unsigned long long>(tg));
+        if (prev == kEmptyTag) {
+            if (t.tri) t.tri[h] = SlotTri{a, b, c, img};
+            return static_cast<int64_t>(h);
         }
+        if (prev == tg) return static_cast<int64_t>(h);
         h = (h + 1) & t.mask;
     }
     atomicOr(flags, kFlagTableOverflow);
