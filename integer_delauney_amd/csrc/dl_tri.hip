@@ -1524,7 +1524,16 @@ __device__ __forceinline__ EdgeRows edge_rows(const TriVerts& v)
         E.al[e] = sg * (ay - by);
         E.k0[e] = sg * (ax * by - bx * ay);
         E.k1[e] = sg * (bx - ax);
-        E.inv[e] = E.al[e] != 0.0 ? 1.0 / E.al[e] : 0.0;
+        // Hardware reciprocal estimate refined by two Newton steps (each
+        // squares the relative error); row_span corrects the rounded bound
+        // with exact integer tests, so only its estimate depends on this.
+        if (E.al[e] != 0.0) {
+            double r = __builtin_amdgcn_rcp(E.al[e]);
+            r = fma(r, fma(-E.al[e], r, 1.0), r);
+            E.inv[e] = fma(r, fma(-E.al[e], r, 1.0), r);
+        } else {
+            E.inv[e] = 0.0;
+        }
     }
     return E;
 }
@@ -1550,6 +1559,39 @@ __device__ __forceinline__ void row_span(const EdgeRows& E, double y, double& lo
             hi = 0.0;
         }
     }
+}
+
+// A triangle of a tile's raster round (see tile_assign_kernel).
+struct RasterEntry {
+    uint16_t x[3], y[3];   // vertices (image positions, below 2^15)
+    int32_t ya;            // first row inside the tile
+    int32_t id;            // the image's own tid
+};
+
+// Exclusive scan of one value per thread of a 256-thread block into
+// pref[0 .. 256); pref[256] and the return value are the total.  tmp: 4 words.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* pref, uint32_t* tmp)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) tmp[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t tw = tmp[w];
+        if (w < wv) before += tw;
+        total += tw;
+    }
+    pref[threadIdx.x] = before + incl - v;
+    if (threadIdx.x == 0) pref[256] = total;
+    __syncthreads();
+    return total;
 }
 
 // Binning of triangles to 64x64 tiles.  Triangles are taken in chunks of
@@ -1670,7 +1712,10 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ list, SeedIndex si, uint32_t cap, uint32_t n_tiles)
 {
-    __shared__ int32_t tid_tile[kTS * kTS];
+    __shared__ __align__(16) int32_t tid_tile[kTS * kTS];
+    __shared__ RasterEntry ent[256];
+    __shared__ uint32_t pref[257];
+    __shared__ uint32_t scan_tmp[4];
     // Each XCD takes a contiguous range of tiles (tile rows of one image):
     // the seed-index words and label lines a tile row reads are then shared
     // within one L2.  The grid is a multiple of 8 (see xcd_range).
@@ -1690,35 +1735,68 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     __syncthreads();
     // an overflowed list (speculative sizing) is redone by the host
     const uint32_t b = min(offsets[gt], cap), e = min(offsets[gt + 1], cap);
-    for (uint32_t k = b + threadIdx.x; k < e; k += blockDim.x) {
-        const uint32_t t = list[k];
-        const TriVerts v = load_tri(rows, srt, t);
-        const int32_t id = static_cast<int32_t>(t - t0);   // the image's own tid
-        if (v.S == 0) {
-            const int64_t dx = v.x[1] - v.x[0], dy = v.y[1] - v.y[0];
-            for (int y = Y0; y <= Y1; ++y) {
-                if (dy == 0) {
-                    if (y != v.y[0]) continue;
-                    for (int x = X0; x <= X1; ++x) atomicMax(&tid_tile[(y - Y0) * kTS + (x - X0)], id);
-                    continue;
+    // The tile's triangles in rounds of one per thread.  A thread loads its
+    // triangle; a collinear one (rare) it rasterises itself, otherwise it
+    // records the vertices, first row and row count.  After a block scan of
+    // the row counts every thread takes (triangle, row) pairs, so long and
+    // short triangles spread evenly over the block.
+    for (uint32_t base = b; base < e; base += blockDim.x) {   // block-uniform
+        const uint32_t k = base + threadIdx.x;
+        int nrows = 0;
+        if (k < e) {
+            const uint32_t t = list[k];
+            const TriVerts v = load_tri(rows, srt, t);
+            const int32_t id = static_cast<int32_t>(t - t0);   // the image's own tid
+            if (v.S == 0) {
+                const int64_t dx = v.x[1] - v.x[0], dy = v.y[1] - v.y[0];
+                for (int y = Y0; y <= Y1; ++y) {
+                    if (dy == 0) {
+                        if (y != v.y[0]) continue;
+                        for (int x = X0; x <= X1; ++x) atomicMax(&tid_tile[(y - Y0) * kTS + (x - X0)], id);
+                        continue;
+                    }
+                    const int64_t num = (y - v.y[0]) * dx;
+                    if (num % dy != 0) continue;
+                    const int64_t x = v.x[0] + num / dy;
+                    if (x >= X0 && x <= X1) atomicMax(&tid_tile[(y - Y0) * kTS + (x - X0)], id);
                 }
-                const int64_t num = (y - v.y[0]) * dx;
-                if (num % dy != 0) continue;
-                const int64_t x = v.x[0] + num / dy;
-                if (x >= X0 && x <= X1) atomicMax(&tid_tile[(y - Y0) * kTS + (x - X0)], id);
+            } else {
+                const int ya = static_cast<int>(max<int64_t>(Y0, min(v.y[0], min(v.y[1], v.y[2]))));
+                const int yb = static_cast<int>(min<int64_t>(Y1, max(v.y[0], max(v.y[1], v.y[2]))));
+                nrows = max(0, yb - ya + 1);
+                RasterEntry& re = ent[threadIdx.x];
+                for (int c = 0; c < 3; ++c) {
+                    re.x[c] = static_cast<uint16_t>(v.x[c]);
+                    re.y[c] = static_cast<uint16_t>(v.y[c]);
+                }
+                re.ya = ya;
+                re.id = id;
             }
-            continue;
         }
-        const int ya = static_cast<int>(max<int64_t>(Y0, min(v.y[0], min(v.y[1], v.y[2]))));
-        const int yb = static_cast<int>(min<int64_t>(Y1, max(v.y[0], max(v.y[1], v.y[2]))));
-        const EdgeRows E = edge_rows(v);
-        for (int y = ya; y <= yb; ++y) {
+        const uint32_t total = block_exclusive_scan(static_cast<uint32_t>(nrows), pref, scan_tmp);
+        // pref[i]: first pair of entry i (pref[blockDim.x] = total)
+        for (uint32_t p = threadIdx.x; p < total; p += blockDim.x) {
+            int lo_e = 0, hi_e = static_cast<int>(blockDim.x) - 1;   // last entry with pref <= p
+            while (lo_e < hi_e) {
+                const int mid = (lo_e + hi_e + 1) >> 1;
+                if (pref[mid] <= p) lo_e = mid; else hi_e = mid - 1;
+            }
+            const RasterEntry& re = ent[lo_e];
+            TriVerts v;
+            for (int c = 0; c < 3; ++c) {
+                v.x[c] = re.x[c];
+                v.y[c] = re.y[c];
+            }
+            v.S = (v.x[1] - v.x[0]) * (v.y[2] - v.y[0]) - (v.y[1] - v.y[0]) * (v.x[2] - v.x[0]);
+            const int y = re.ya + static_cast<int>(p - pref[lo_e]);
+            const EdgeRows E = edge_rows(v);
             double lo = X0, hi = X1;
             row_span(E, static_cast<double>(y), lo, hi);
-            const int base = (y - Y0) * kTS - X0;
+            const int rb = (y - Y0) * kTS - X0;
             const int l = static_cast<int>(lo), h = static_cast<int>(hi);
-            for (int x = l; x <= h; ++x) atomicMax(&tid_tile[base + x], id);
+            for (int x = l; x <= h; ++x) atomicMax(&tid_tile[rb + x], re.id);
         }
+        __syncthreads();   // the next round rewrites ent and pref
     }
     __syncthreads();
     const int Xw = X1 - X0 + 1;   // pixels per tile row
