@@ -436,8 +436,15 @@ __device__ __forceinline__ void tile_body(
 // go round-robin over the 8 XCDs, so give each XCD a contiguous slice of tiles
 // (neighbouring tiles share halo lines in the same L2).  The grid is launched
 // as a multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
+// Tile-kernel shapes of at most 8 waves x 8 rows fit 64 VGPRs without
+// spilling (the compiler's own choice is 67): 8 waves per SIMD instead of 7,
+// i.e. four resident blocks per CU instead of three, to hide each tile's
+// loads.  (The list kernel would spill under the same bound.)
+template <int NW, int R>
+constexpr int kFillWavesPerEu = (NW <= 8 && R <= 8) ? 8 : 1;
+
 template <int NW, int R, int K>
-__global__ __launch_bounds__(NW * 64) void bfs_tile_kernel(
+__global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_tile_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
     int* __restrict__ changed_any, int* __restrict__ visits)
