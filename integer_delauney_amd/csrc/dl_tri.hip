@@ -1713,7 +1713,8 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const uint32_t* __restrict__ list, SeedIndex si, uint32_t cap, uint32_t n_tiles)
 {
     __shared__ __align__(16) int32_t tid_tile[kTS * kTS];
-    __shared__ RasterEntry ent[256];
+    __shared__ __align__(16) RasterEntry ent[256];   // (also the output row buffers)
+    static_assert(sizeof(RasterEntry) * 256 >= 4 * 3 * kTS * sizeof(int32_t), "row buffers");
     __shared__ uint32_t pref[257];
     __shared__ uint32_t scan_tmp[4];
     // Each XCD takes a contiguous range of tiles (tile rows of one image):
@@ -1824,16 +1825,16 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     }
     // C == 3.  Each wave takes four tile rows at a time, one pixel per lane:
     // the four label loads are issued together, then the four owner-id
-    // gathers.  A row's 3 * Xw output ints are then written as chunks of four
-    // by lanes 0..47 (chunk L = ints 4L .. 4L + 3, channels of pixels 4L / 3
-    // and 4L / 3 + 1, fetched from their lanes by cross-lane reads), so every
-    // store instruction covers one contiguous run of the row.
+    // gathers.  A row's 3 * Xw output ints are then staged in the wave's own
+    // LDS row buffer (each lane writes its pixel's three channels) and written
+    // as chunks of four by lanes 0..47 (chunk L = ints 4L .. 4L + 3), so every
+    // store instruction covers one contiguous run of the row.  The buffer
+    // reuses the raster entries, which are dead by now.
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int off = d.out_off, shift = d.out_shift, Wl = d.out_W;
     const int n_rows = Y1 - Y0 + 1;
-    const int pa = (4 * lane) / 3, m = 4 * lane - 3 * pa;   // this lane's chunk: first pixel, channel
-    const int pb = min(pa + 1, 63);
     const int n_out = min(4, 3 * Xw - 4 * lane);            // ints of the chunk inside the row
+    int32_t* rowbuf = reinterpret_cast<int32_t*>(ent) + wv * (3 * kTS);   // 3 ints per pixel
     for (int r0 = 4 * wv; r0 < n_rows; r0 += 16) {
         uint32_t lab[4];
 #pragma unroll
@@ -1860,25 +1861,25 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
             const int r = r0 + i;
             if (r >= n_rows) break;   // uniform across the wave
             const int32_t tid = tid_tile[r * kTS + lane];
-            const int32_t a0 = __shfl(id[i], pa, 64), a1 = __shfl(dd[i], pa, 64),
-                          a2 = __shfl(tid, pa, 64);
-            const int32_t b0 = __shfl(id[i], pb, 64), b1 = __shfl(dd[i], pb, 64),
-                          b2 = __shfl(tid, pb, 64);
-            // ints of the chunk: (a0 a1 a2 b0) for m = 0, (a1 a2 b0 b1) for 1, (a2 b0 b1 b2) for 2
-            const int32_t o0 = m == 0 ? a0 : (m == 1 ? a1 : a2);
-            const int32_t o1 = m == 0 ? a1 : (m == 1 ? a2 : b0);
-            const int32_t o2 = m == 0 ? a2 : (m == 1 ? b0 : b1);
-            const int32_t o3 = m == 0 ? b0 : (m == 1 ? b1 : b2);
+            // LDS operations of one wave complete in order: the chunk reads
+            // below see these writes, and the next row's writes come after
+            // the reads.
+            rowbuf[3 * lane] = id[i];
+            rowbuf[3 * lane + 1] = dd[i];
+            rowbuf[3 * lane + 2] = tid;
+            __builtin_amdgcn_wave_barrier();
             if (n_out <= 0) continue;
+            const int4 o = *reinterpret_cast<const int4*>(rowbuf + 4 * lane);
             int32_t* dst = d.out + 3 * (static_cast<int64_t>(Y0 + r) * W + X0) + 4 * lane;
             if (n_out == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
-                *reinterpret_cast<int4*>(dst) = make_int4(o0, o1, o2, o3);
+                *reinterpret_cast<int4*>(dst) = o;
             } else {
-                dst[0] = o0;
-                if (n_out > 1) dst[1] = o1;
-                if (n_out > 2) dst[2] = o2;
-                if (n_out > 3) dst[3] = o3;
+                dst[0] = o.x;
+                if (n_out > 1) dst[1] = o.y;
+                if (n_out > 2) dst[2] = o.z;
+                if (n_out > 3) dst[3] = o.w;
             }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
