@@ -63,11 +63,22 @@ void stage_h2d(PinnedBuf& stage, size_t offset, void* dst, const void* src, size
     if (!stage.ptr || offset + n > stage.bytes) throw runtime_error("staging buffer too small");
     char* h = static_cast<char*>(stage.ptr) + offset;
     std::memcpy(h, src, n);
-    DL_HIP(hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, s));
-    if (g_capture_log) {
+    if (g_capture_log) {   // issued before each replay instead (CaptureLog)
         const char* b = static_cast<const char*>(src);
-        g_capture_log->staged.push_back(StagedBytes{h, std::vector<char>(b, b + n)});
+        g_capture_log->staged.push_back(StagedBytes{h, dst, std::vector<char>(b, b + n)});
+        return;
     }
+    DL_HIP(hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, s));
+}
+
+void copy_back_async(void* host, const void* dev, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    if (g_capture_log) {   // issued after each replay instead (CaptureLog)
+        g_capture_log->copy_back.push_back(CopyBack{host, dev, n});
+        return;
+    }
+    DL_HIP(hipMemcpyAsync(host, dev, n, hipMemcpyDeviceToHost, s));
 }
 
 void DevBuf::release()
@@ -232,7 +243,8 @@ struct PipelineGraph {
     BfsConfig last_cfg{};
     uint32_t rows_cap = 0;
     int64_t list_cap = 0;
-    std::vector<StagedBytes> staged;   // staging bytes the copy nodes read
+    std::vector<StagedBytes> staged;    // uploads issued before each replay
+    std::vector<CopyBack> copy_back;    // read-backs issued after it
 
     void reset()
     {
@@ -240,6 +252,7 @@ struct PipelineGraph {
         exec = nullptr;
         sig.clear();
         staged.clear();
+        copy_back.clear();
     }
 };
 
@@ -889,6 +902,7 @@ bool capture_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
     g.rows_cap = ctx->tri.rows_cap;
     g.list_cap = ctx->tri.list_cap;
     g.staged = std::move(log.staged);
+    g.copy_back = std::move(log.copy_back);
     ++ctx->graph_captures;
     return true;
 }
@@ -899,7 +913,10 @@ void replay_pipeline(dl_context* ctx, hipStream_t s)
     PipelineGraph& g = ctx->graph;
     // The previous call on this context ended with a synchronisation of its
     // stream, so no copy still reads the staging buffers.
-    for (const StagedBytes& b : g.staged) std::memcpy(b.host, b.bytes.data(), b.bytes.size());
+    for (const StagedBytes& b : g.staged) {
+        std::memcpy(b.host, b.bytes.data(), b.bytes.size());
+        DL_HIP(hipMemcpyAsync(b.dev, b.host, b.bytes.size(), hipMemcpyHostToDevice, s));
+    }
     ctx->plan = g.plan;
     ctx->fill_cvs = g.fill_cvs;
     ctx->last_cfg = g.last_cfg;
@@ -910,6 +927,8 @@ void replay_pipeline(dl_context* ctx, hipStream_t s)
     ctx->host_small.as<int>()[64] = 0;   // visit total (see issue_pipeline)
     ctx->timings[7] += g.iss.pend.n;
     DL_HIP(hipGraphLaunch(g.exec, s));
+    for (const CopyBack& c : g.copy_back)
+        DL_HIP(hipMemcpyAsync(c.host, c.dev, c.n, hipMemcpyDeviceToHost, s));
     ++ctx->graph_replays;
 }
 

@@ -88,19 +88,32 @@ struct capture_abort : std::runtime_error {
 };
 uint64_t alloc_epoch();
 
-// Host bytes a captured graph copies to the device from page-locked staging.
-// The graph's copy nodes read the staging buffers when it is replayed, so the
-// bytes seen at capture are put back before every replay.
+// A captured graph holds kernels only.  The host -> device uploads of the
+// captured stretch are recorded instead (staging address, device address,
+// the bytes seen at capture) and issued as ordinary stream copies before
+// every replay, and its device -> host copies are issued after it: no kernel
+// of the stretch writes an uploaded region or reads a copied-back one
+// before the end, so stream order makes this equivalent.
 struct StagedBytes {
     void* host;
+    void* dev;
     std::vector<char> bytes;
+};
+struct CopyBack {
+    void* host;
+    const void* dev;
+    size_t n;
 };
 struct CaptureLog {
     std::vector<StagedBytes> staged;
+    std::vector<CopyBack> copy_back;
     bool poisoned = false;   // a buffer was freed while capturing
 };
 // Non-null while the calling thread captures a stream.
 CaptureLog*& capture_log();
+
+// Device -> host copy on s; recorded for after the replay while capturing.
+void copy_back_async(void* host, const void* dev, size_t n, hipStream_t s);
 
 // Host -> device copy of n bytes through bytes [offset, offset + n) of the
 // page-locked buffer `stage`, which the caller has reserved.  One buffer per

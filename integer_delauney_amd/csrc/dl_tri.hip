@@ -1358,8 +1358,7 @@ void issue_status(const BatchPlan& plan, TriWork& work, const int* d_esc, int n_
         gather_words_kernel<<<1, 256, 0, s>>>(g, dev);
         DL_LAUNCH_CHECK();
     }
-    DL_HIP(hipMemcpyAsync(work.host_flags.ptr, dev, sizeof(uint32_t) * L.words,
-                          hipMemcpyDeviceToHost, s));
+    copy_back_async(work.host_flags.ptr, dev, sizeof(uint32_t) * L.words, s);
 }
 
 SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc, bool seed_flags,
