@@ -819,19 +819,19 @@ __global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
 #pragma unroll
     for (int j = 0; j < kDetItems; ++j) {
         // Walk one column of blocks down the rows, reusing each label row
-        // as the next block's top: two loads per block.
+        // as the next block's top: two loads per block.  Rows past the
+        // canvas (last band) read the last row again and count nothing, so
+        // the walk has no branch and its loads can be issued ahead.
         const int rx = rx0 + j * kDetThreads;
         const bool col_ok = rx < bw;
         const uint32_t* col = g + min(rx, bw - 1);
-        size_t o = static_cast<size_t>(ry0) * d.Wc;
-        uint32_t tl = col[o], tr = col[o + 1];
+        uint32_t tl = col[static_cast<size_t>(ry0) * d.Wc], tr = col[static_cast<size_t>(ry0) * d.Wc + 1];
 #pragma unroll
         for (int r = 0; r < kDetRows; ++r) {
-            if (r >= rows) break;   // uniform
-            o += d.Wc;
+            const size_t o = static_cast<size_t>(min(ry0 + r + 1, bh)) * d.Wc;
             const uint32_t bl = col[o], br = col[o + 1];
             const BlockCorners c = corners_of(tl, tr, bl, br);
-            if (col_ok) {
+            if (col_ok && r < rows) {
                 if (block_candidates(c)) mask |= 1u << (r * kDetItems + j);
                 quads += c.quad;
             }
