@@ -119,12 +119,26 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
     }
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanBlock +
                          static_cast<int64_t>(threadIdx.x) * kScanItems;
+    // A thread's kScanItems consecutive items move as 16-byte vectors when
+    // they are all inside and aligned (every thread but the tail's).
+    const bool full = base + kScanItems <= n;
     uint32_t raw[kScanItems], c[kScanItems];
+    if (full && (reinterpret_cast<uintptr_t>(w + base) & 15) == 0) {
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const uint4 v = reinterpret_cast<const uint4*>(w + base)[q];
+            raw[4 * q] = v.x;
+            raw[4 * q + 1] = v.y;
+            raw[4 * q + 2] = v.z;
+            raw[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) raw[j] = base + j < n ? w[base + j] : 0u;
+    }
     uint32_t s = 0;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        const int64_t i = base + j;
-        raw[j] = i < n ? w[i] : 0u;
         c[j] = scan_item<POPC>(raw[j]);
         s += c[j];
     }
@@ -137,14 +151,34 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(
         __syncthreads();
     }
     uint32_t run = prefix + part[threadIdx.x] - s;
+    uint32_t pre[kScanItems];
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        const int64_t i = base + j;
-        if (i < n) {
-            if (INTER) inter[i] = make_uint2(run, raw[j]);
-            else out[i] = run;
-        }
+        pre[j] = run;
         run += c[j];
+    }
+    if (INTER) {
+        if (full && (reinterpret_cast<uintptr_t>(inter + base) & 15) == 0) {
+#pragma unroll
+            for (int q = 0; q < kScanItems / 2; ++q)
+                reinterpret_cast<uint4*>(inter + base)[q] =
+                    make_uint4(pre[2 * q], raw[2 * q], pre[2 * q + 1], raw[2 * q + 1]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j)
+                if (base + j < n) inter[base + j] = make_uint2(pre[j], raw[j]);
+        }
+    } else {
+        if (full && (reinterpret_cast<uintptr_t>(out + base) & 15) == 0) {
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q)
+                reinterpret_cast<uint4*>(out + base)[q] =
+                    make_uint4(pre[4 * q], pre[4 * q + 1], pre[4 * q + 2], pre[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j)
+                if (base + j < n) out[base + j] = pre[j];
+        }
     }
     if (!INTER && SELF && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1)
         out[n] = run;
