@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <tuple>
 
@@ -229,6 +230,7 @@ struct Issued {
     int plane = 0;          // label plane holding the padded fill's result
     bool any_pad = false;   // some image's padding is watched
     int C = 3;              // output channels
+    bool fixed_bins = true; // the speculative assignment may bin into fixed slices
 };
 
 // The first half of the pipeline captured as a graph, with the host state its
@@ -290,6 +292,9 @@ struct dl_context {
     std::vector<hipEvent_t> events;
     bool profiling = false;
     KernelProfiler prof;
+    // Canvas classes (as for the fill autotuner) whose fixed bin slices
+    // overflowed once: their assignment bins with count + scan + fill.
+    std::set<std::tuple<int, int, int>> bins_exact;
     // hipGraph replay of the pipeline (see run_pipeline)
     bool graphs = true;
     PipelineGraph graph;
@@ -733,6 +738,7 @@ Issued issue_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
         }
     };
     if (pred > 0) set_out_lab(pred & 1);
+    iss.fixed_bins = !ctx->bins_exact.count(tune_key(ctx->device, plan.n_cells, plan.n_seeds));
     prepare(ctx, d_xy, d_out, C, s, pred > 0 ? SeedCheck::kWithStatus : SeedCheck::kAfterWait);
     clk.mark(0);
     std::vector<int> watch(n_img, -1);
@@ -759,7 +765,7 @@ Issued issue_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
     clk.mark(2);
     clk.mark(3);
     assign_tiled(plan, ctx->tri.rows.as<int32_t>(), -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s,
-                 true);
+                 true, iss.fixed_bins);
     clk.mark(4);
     issue_status(plan, ctx->tri, iss.any_pad ? ctx->esc.as<int>() : nullptr,
                  iss.any_pad ? n_img : 0, iss.pend.active,
@@ -787,7 +793,8 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
         clk.mark(2);
         clk.mark(3);
         rows = ctx->tri.rows.as<int32_t>();
-        assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, true);
+        assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, true,
+                     iss.fixed_bins);
         clk.mark(4);
         return read_status(plan, ctx->tri, iss.any_pad ? ctx->esc.as<int>() : nullptr,
                            iss.any_pad ? n_img : 0, s);
@@ -801,12 +808,16 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
     }
     ctx->timings[8] += *visits_async;
     if (st.bad_id) throw runtime_error("flood fill left an undefined cell");
+    if (st.bin_redo)   // this class bins exactly from now on
+        ctx->bins_exact.insert(tune_key(ctx->device, plan.n_cells, plan.n_seeds));
     if (st.redo) {
         const DetectResult dr =
             detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, false);
         rows = ctx->tri.rows.as<int32_t>();
         assign_tiled(plan, rows, dr.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
         st.n_tri = dr.n_tri;
+    } else if (st.bin_redo) {   // detection stands; bin and assign exactly
+        assign_tiled(plan, rows, st.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
     }
     if (C == 3) {
         // Did a padding cell ever update an image cell?  If not, the image's
@@ -843,7 +854,11 @@ std::vector<int64_t> pipeline_sig(const dl_context* ctx, const int32_t* d_xy, co
                            pred,
                            ctx->bfs.list_min_tiles,
                            ctx->tri.spec_tiny ? 1 : 0,
-                           ctx->force_unpadded ? 1 : 0};
+                           ctx->force_unpadded ? 1 : 0,
+                           ctx->bins_exact.count(tune_key(ctx->device, ctx->plan.n_cells,
+                                                          ctx->plan.n_seeds))
+                               ? 0
+                               : 1};
     v.reserve(v.size() + 6 * ctx->plan.img.size());
     for (const ImageDesc& d : ctx->plan.img) {
         v.push_back(d.W);

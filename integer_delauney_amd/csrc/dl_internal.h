@@ -391,6 +391,7 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
 struct SpeculativeStatus {
     bool bad_id = false;   // a canvas cell without owner / an id out of range
     bool redo = false;     // a speculative capacity was exceeded or a hash collided
+    bool bin_redo = false; // a tile's fixed bin slice overflowed (assignment only)
     int64_t n_tri = 0;
     std::vector<int> escaped;   // the padding watch's flags, per image
 };
@@ -414,10 +415,12 @@ SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc, 
 // each image's out: C == 3, the (H, W, 3) grid with channels 0/1 from its
 // output label source; C == 1, the (H, W) tid raster.
 // n_tri < 0: the count is on the device (after a speculative detection);
-// speculative = true sizes the bin list without a sync (checked by read_status).
+// speculative = true sizes the bin list without a sync (checked by read_status);
+// with allow_fixed it may bin in one pass into fixed per-tile slices, whose
+// overflow the status reports as bin_redo (the assignment alone runs again).
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
                   const uint32_t* sorted, int C, TriWork& work,
-                  hipStream_t s, bool speculative);
+                  hipStream_t s, bool speculative, bool allow_fixed = true);
 
 // (H, W, C) int32 output of one image without triangles (C == 2) or with the
 // tid raster (C == 3).  Output pixel (x, y) reads canvas cell (x + off, y + off);

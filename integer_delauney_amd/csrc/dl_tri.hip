@@ -372,6 +372,7 @@ constexpr uint32_t kFlagTableOverflow = 16u;
 constexpr uint32_t kFlagCollision = 32u;
 constexpr uint32_t kFlagBadId = 64u;
 constexpr uint32_t kFlagOverflow = 128u;   // a speculative capacity was too small
+constexpr uint32_t kFlagBinOverflow = 256u;   // a tile's fixed bin slice was too small
 
 // Presence bit of (x, y) in its image's bitmap is x*H + y: bit order is the id
 // order.
@@ -1408,6 +1409,7 @@ SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc, 
     const int64_t listed = plan.n_tiles > 0 && work.list_cap > 0 ? h[1] : 0;
     st.redo = (f & (kFlagOverflow | kFlagTableOverflow | kFlagCollision)) != 0 ||
               listed > work.list_cap;
+    st.bin_redo = (f & kFlagBinOverflow) != 0;
     st.n_tri = htb[n_img];
     work.tri_base.assign(htb, htb + n_img + 1);
     const int* hesc = reinterpret_cast<const int*>(h + L.esc);
@@ -1669,10 +1671,15 @@ __device__ __forceinline__ void bin_clear(BinLds& L)
 }
 
 // Triangles t < min(device count, rows_cap) (the rows buffer's capacity).
-// FILL = false: counts[tile] += pairs.  FILL = true: list positions from
-// offsets[tile] + cursor[tile]; entries beyond cap are dropped and flagged
-// (speculative sizing).
-template <bool FILL>
+// kBinCount: counts[tile] += pairs.  kBinFill: list positions from
+// offsets[tile] + cursor[tile] (counts is the cursor); entries beyond cap are
+// dropped and flagged (speculative sizing).  kBinFixed: tile t owns list
+// entries [t * cap, (t + 1) * cap) and counts[t] is its cursor; entries past
+// a tile's capacity are dropped and flagged with kFlagBinOverflow (only the
+// assignment needs to run again).
+constexpr int kBinCount = 0, kBinFill = 1, kBinFixed = 2;
+
+template <int MODE>
 __global__ __launch_bounds__(256) void bin_kernel(
     const int32_t* __restrict__ rows, int64_t rows_cap, const int64_t* __restrict__ tri_base,
     int n_img, const ImageDesc* __restrict__ D, const uint32_t* __restrict__ sorted,
@@ -1701,12 +1708,16 @@ __global__ __launch_bounds__(256) void bin_kernel(
                     for (int tx = tx0; tx <= tx1; ++tx) {
                         const uint32_t tile = static_cast<uint32_t>(d.tile_off + ty * d.tiles_x + tx);
                         if (direct) {
-                            if (!FILL) {
+                            if (MODE == kBinCount) {
                                 atomicAdd(&counts[tile], 1u);
-                            } else {
+                            } else if (MODE == kBinFill) {
                                 const uint32_t k = offsets[tile] + atomicAdd(&counts[tile], 1u);
                                 if (k < cap) list[k] = static_cast<uint32_t>(t);
                                 else atomicOr(flags, kFlagOverflow);
+                            } else {
+                                const uint32_t k = atomicAdd(&counts[tile], 1u);
+                                if (k < cap) list[static_cast<size_t>(tile) * cap + k] = static_cast<uint32_t>(t);
+                                else atomicOr(flags, kFlagBinOverflow);
                             }
                         } else {
                             slot[n_local] = bin_slot(L, tile, rank[n_local]);
@@ -1721,15 +1732,21 @@ __global__ __launch_bounds__(256) void bin_kernel(
             const uint32_t tile = L.key[i];
             if (tile == kBinEmpty) continue;
             const uint32_t c = atomicAdd(&counts[tile], L.count[i]);
-            if (FILL) L.base[i] = offsets[tile] + c;
+            if (MODE == kBinFill) L.base[i] = offsets[tile] + c;
+            if (MODE == kBinFixed) L.base[i] = c;
         }
         __syncthreads();
-        if (FILL) {
+        if (MODE != kBinCount) {
             for (int j = 0; j < n_local; ++j) {
                 if (slot[j] < 0) continue;
                 const uint32_t k = L.base[slot[j]] + rank[j];
-                if (k < cap) list[k] = static_cast<uint32_t>(t);
-                else atomicOr(flags, kFlagOverflow);
+                if (MODE == kBinFill) {
+                    if (k < cap) list[k] = static_cast<uint32_t>(t);
+                    else atomicOr(flags, kFlagOverflow);
+                } else {
+                    if (k < cap) list[static_cast<size_t>(L.key[slot[j]]) * cap + k] = static_cast<uint32_t>(t);
+                    else atomicOr(flags, kFlagBinOverflow);
+                }
             }
         }
         __syncthreads();
@@ -1743,7 +1760,8 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int64_t* __restrict__ tile_seg, const int64_t* __restrict__ tri_base, int n_img,
     const ImageDesc* __restrict__ D, const int32_t* __restrict__ rows,
     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ list, SeedIndex si, uint32_t cap, uint32_t n_tiles)
+    const uint32_t* __restrict__ list, SeedIndex si, uint32_t cap, uint32_t n_tiles,
+    const uint32_t* __restrict__ fixed_counts)
 {
     __shared__ __align__(16) int32_t tid_tile[kTS * kTS];
     __shared__ __align__(16) RasterEntry ent[256];   // (also the output row buffers)
@@ -1767,8 +1785,17 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int64_t t0 = tri_base[img];
     for (int i = threadIdx.x; i < kTS * kTS; i += blockDim.x) tid_tile[i] = -1;
     __syncthreads();
-    // an overflowed list (speculative sizing) is redone by the host
-    const uint32_t b = min(offsets[gt], cap), e = min(offsets[gt + 1], cap);
+    // This tile's list entries: its fixed slice (fixed_counts given: cap per
+    // tile) or its range from the offsets.  An overflowed list (speculative
+    // sizing) is redone by the host.
+    uint32_t b, e;
+    if (fixed_counts) {
+        b = static_cast<uint32_t>(gt) * cap;
+        e = b + min(fixed_counts[gt], cap);
+    } else {
+        b = min(offsets[gt], cap);
+        e = min(offsets[gt + 1], cap);
+    }
     // The tile's triangles in rounds of one per thread.  A thread loads its
     // triangle; a collinear one (rare) it rasterises itself, otherwise it
     // records the vertices, first row and row count.  After a block scan of
@@ -1919,9 +1946,33 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
 
 }  // namespace
 
+namespace {
+// Bin-list entries per tile for the speculative fixed-slice layout: three
+// times the triangles a tile of the densest image expects (2 per seed, by
+// area), plus a margin.  0 when that layout would take much more memory than
+// the exact one (very dense or tiny images): those use count + scan + fill.
+uint32_t fixed_bin_capacity(const BatchPlan& plan, int64_t t_bound)
+{
+    double per_tile = 0.0;
+    for (const ImageDesc& d : plan.img) {
+        const double area = static_cast<double>(d.W) * d.H;
+        const double tile = static_cast<double>(std::min(d.W, kTS)) * std::min(d.H, kTS);
+        per_tile = std::max(per_tile, 2.0 * static_cast<double>(d.n) * tile / area);
+    }
+    const double capd = std::ceil((3.0 * per_tile + 128.0) / 32.0) * 32.0;
+    if (capd > 4096.0) return 0;
+    const uint32_t cap = static_cast<uint32_t>(capd);
+    const double entries = static_cast<double>(cap) * static_cast<double>(plan.n_tiles);
+    // (slice offsets are 32-bit in the kernels)
+    if (entries > 16.0 * static_cast<double>(t_bound) + (1 << 22) || entries >= 2147483648.0)
+        return 0;
+    return cap;
+}
+}  // namespace
+
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
                   const uint32_t* sorted, int C, TriWork& work,
-                  hipStream_t s, bool speculative)
+                  hipStream_t s, bool speculative, bool allow_fixed)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t ntiles = plan.n_tiles;
@@ -1941,49 +1992,70 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
     const int64_t t_bound = n_tri >= 0 ? std::min(n_tri, rows_cap) : rows_cap;
     const unsigned tblocks = static_cast<unsigned>(std::max<int64_t>(
         1, std::min<int64_t>((t_bound + 255) / 256, 8192)));
-    {
+    // Speculative: one pass into fixed per-tile slices when they fit (no
+    // count, no scan); an overflowing tile raises kFlagOverflow and the host
+    // reruns exactly.
+    const uint32_t fixed = speculative && allow_fixed
+                               ? (work.spec_tiny ? 1u : fixed_bin_capacity(plan, t_bound))
+                               : 0u;
+    int64_t cap;
+    if (fixed) {
+        cap = fixed;
+        work.bin_list.reserve(sizeof(uint32_t) * static_cast<size_t>(ntiles) * fixed);
         ProfScope ps(kKAssign, s);
-        fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
+        fill_u32(counts, 0u, static_cast<size_t>(ntiles), s);
         if (t_bound > 0) {
-            bin_kernel<false><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
-                                                      counts, nullptr, nullptr, 0u, nullptr);
+            bin_kernel<kBinFixed><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
+                                                          counts, nullptr, work.bin_list.as<uint32_t>(),
+                                                          fixed, work.flags.as<uint32_t>());
             DL_LAUNCH_CHECK();
         }
-        exclusive_scan_u32(counts, ntiles, offs, work.scan_tmp, s);
-    }
-    int64_t cap;
-    if (speculative) {
-        // a small triangle's box covers up to four tiles
-        cap = work.spec_tiny ? 1 : 4 * t_bound + ntiles + 1024;
     } else {
-        DL_HIP(hipMemcpyAsync(work.host_flags.ptr, offs + ntiles, sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
-        DL_HIP(hipStreamSynchronize(s));
-        cap = std::max<int64_t>(work.host_flags.as<uint32_t>()[0], 1);
-    }
-    cap = std::min<int64_t>(cap, 0xFFFFFFF0ll);
-    work.list_cap = cap;
-    work.bin_list.reserve(sizeof(uint32_t) * static_cast<size_t>(cap));
-    {
+        {
+            ProfScope ps(kKAssign, s);
+            fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
+            if (t_bound > 0) {
+                bin_kernel<kBinCount><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D,
+                                                              sorted, counts, nullptr, nullptr, 0u,
+                                                              nullptr);
+                DL_LAUNCH_CHECK();
+            }
+            exclusive_scan_u32(counts, ntiles, offs, work.scan_tmp, s);
+        }
+        if (speculative) {
+            // a small triangle's box covers up to four tiles
+            cap = work.spec_tiny ? 1 : 4 * t_bound + ntiles + 1024;
+        } else {
+            DL_HIP(hipMemcpyAsync(work.host_flags.ptr, offs + ntiles, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s));
+            DL_HIP(hipStreamSynchronize(s));
+            cap = std::max<int64_t>(work.host_flags.as<uint32_t>()[0], 1);
+        }
+        cap = std::min<int64_t>(cap, 0xFFFFFFF0ll);
+        work.list_cap = cap;   // the status read checks the list's total against it
+        work.bin_list.reserve(sizeof(uint32_t) * static_cast<size_t>(cap));
         ProfScope ps(kKAssign, s);
         if (t_bound > 0) {
-            bin_kernel<true><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
-                                                     cursor, offs, work.bin_list.as<uint32_t>(),
-                                                     static_cast<uint32_t>(cap),
-                                                     work.flags.as<uint32_t>());
+            bin_kernel<kBinFill><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
+                                                         cursor, offs, work.bin_list.as<uint32_t>(),
+                                                         static_cast<uint32_t>(cap),
+                                                         work.flags.as<uint32_t>());
             DL_LAUNCH_CHECK();
         }
     }
     ProfScope ps(kKOutput, s);
     const unsigned tgrid = static_cast<unsigned>((ntiles + 7) / 8 * 8);
+    const uint32_t* fixed_counts = fixed ? counts : nullptr;
     if (C == 3)
         tile_assign_kernel<3><<<tgrid, 256, 0, s>>>(
             tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(),
-            seed_index(work), static_cast<uint32_t>(cap), static_cast<uint32_t>(ntiles));
+            seed_index(work), static_cast<uint32_t>(cap), static_cast<uint32_t>(ntiles),
+            fixed_counts);
     else
         tile_assign_kernel<1><<<tgrid, 256, 0, s>>>(
             tile_seg, tri_base, n_img, D, rows, sorted, offs, work.bin_list.as<uint32_t>(),
-            seed_index(work), static_cast<uint32_t>(cap), static_cast<uint32_t>(ntiles));
+            seed_index(work), static_cast<uint32_t>(cap), static_cast<uint32_t>(ntiles),
+            fixed_counts);
     DL_LAUNCH_CHECK();
 }
 

@@ -362,3 +362,26 @@ def test_active_tile_list_fill_equals_oracle(case):
                 np.testing.assert_array_equal(_rows(mb), r2)
     finally:
         _lib.check(lib.dl_set_fill_list_threshold(ctx.handle, -1))
+
+
+def test_dense_cluster_bins_fall_back_exactly():
+    """Uniform seeds plus one dense cluster: the cluster's tiles hold far more
+    triangles than the one-pass binning's fixed per-tile slices (sized from
+    the image's average density), so the first call reruns the assignment
+    with exact bins, and later calls of the class bin exactly from the start.
+    Every call equals the oracle."""
+    rng = np.random.default_rng(4242)
+    W, H = 640, 480
+    idx = rng.choice(W * H, size=400, replace=False)
+    pts = {(int(i % W), int(i // W)) for i in idx}
+    cx, cy = 300, 200
+    for x in range(cx, cx + 120):
+        for y in range(cy, cy + 90):
+            if rng.random() < 0.5:
+                pts.add((x, y))
+    seeds = np.array(sorted(pts), dtype=np.int32)
+    rows, ref = O.triangulate(W, H, seeds, None, return_rows=True)
+    for _ in range(3):
+        m, got = dl.triangulate(W, H, seeds)
+        np.testing.assert_array_equal(got, ref)
+        np.testing.assert_array_equal(_rows(m), rows)
