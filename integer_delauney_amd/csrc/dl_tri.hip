@@ -1104,8 +1104,10 @@ __device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table&
                          const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
                          uint32_t rows_cap, uint32_t* __restrict__ flags)
 {
+    // Most positions fill one of their two slots: read the key alone first.
+    if (cand[i].key == 0xFFFFFFFFu) return;
     const Cand cd = cand[i];
-    if (cd.key == 0xFFFFFFFFu || !is_first(cd, t)) return;
+    if (!is_first(cd, t)) return;
     const ImageDesc& d = D[cd.img];
     const int64_t w = d.key_off + (cd.key >> 5);
     // global row = first registrations of earlier images and earlier keys
