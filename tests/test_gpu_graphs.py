@@ -187,3 +187,27 @@ def test_graph_dropped_after_reallocation():
         _check(W, H, s, -1, m, g)
     c1, r1 = _stats(ctx)
     assert c1 > c0 and r1 > r0
+
+
+@pytest.mark.parametrize("bad", ["duplicate", "bounds"])
+def test_invalid_seeds_on_predicted_runs(bad):
+    """A seed set that fails validation on a run of a known class (predicted
+    fill, seeds validated beside the fill on the second stream, a replayed
+    graph once the signature repeats): ValueError, and the calls after it
+    equal the oracle."""
+    W, H, n = 120, 90, 150
+    for i in range(4):
+        s = uniform_seeds(W, H, n, 900 + i)
+        m, g = dl.triangulate(W, H, s)
+        _check(W, H, s, -1, m, g)
+    s = uniform_seeds(W, H, n, 950).copy()
+    if bad == "duplicate":
+        s[7] = s[3]
+    else:
+        s[7] = (W + 5, 2)
+    with pytest.raises(ValueError):
+        dl.triangulate(W, H, s)
+    for i in range(3):
+        s = uniform_seeds(W, H, n, 960 + i)
+        m, g = dl.triangulate(W, H, s)
+        _check(W, H, s, -1, m, g)
