@@ -853,25 +853,29 @@ __global__ __launch_bounds__(kDetThreads) void detect_count_kernel(DetectArgs A)
     uint32_t mask = 0, quads = 0;   // bit r * kDetItems + j
 #pragma unroll
     for (int j = 0; j < kDetItems; ++j) {
-        // Walk one column of blocks down the rows, reusing each label row
-        // as the next block's top: two loads per block.  Rows past the
-        // canvas (last band) read the last row again and count nothing, so
-        // the walk has no branch and its loads can be issued ahead.
+        // One column of blocks down the rows: each label row is the bottom
+        // of one block and the top of the next, two loads per block.  All
+        // of the column's loads are issued before the first block is
+        // classified, so a wave waits for memory once, not once per row.
+        // Rows past the canvas (last band) read its last row again and count
+        // nothing, so the loads need no branch.
         const int rx = rx0 + j * kDetThreads;
         const bool col_ok = rx < bw;
         const uint32_t* col = g + min(rx, bw - 1);
-        uint32_t tl = col[static_cast<size_t>(ry0) * d.Wc], tr = col[static_cast<size_t>(ry0) * d.Wc + 1];
+        uint32_t lv[kDetRows + 1], rv[kDetRows + 1];
+#pragma unroll
+        for (int r = 0; r <= kDetRows; ++r) {
+            const size_t o = static_cast<size_t>(min(ry0 + r, bh)) * d.Wc;
+            lv[r] = col[o];
+            rv[r] = col[o + 1];
+        }
 #pragma unroll
         for (int r = 0; r < kDetRows; ++r) {
-            const size_t o = static_cast<size_t>(min(ry0 + r + 1, bh)) * d.Wc;
-            const uint32_t bl = col[o], br = col[o + 1];
-            const BlockCorners c = corners_of(tl, tr, bl, br);
+            const BlockCorners c = corners_of(lv[r], rv[r], lv[r + 1], rv[r + 1]);
             if (col_ok && r < rows) {
                 if (block_candidates(c)) mask |= 1u << (r * kDetItems + j);
                 quads += c.quad;
             }
-            tl = bl;
-            tr = br;
         }
     }
     A.masks[gb * kDetThreads + threadIdx.x] = mask;
