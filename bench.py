@@ -104,7 +104,21 @@ def bench_batch(lib, ctx, n_images: int, steps: int):
             None, ntri.ctypes.data_as(I64P)))
 
     # Warm-up: first call (flood-fill shape tuning on a new canvas class),
-    # second (predicted fill, uncaptured), third (captured); then replays.
+    # then predicted fills; the default graph mode (auto: uncaptured at this
+    # size).
+    for _ in range(3):
+        batch_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch_step()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    log("batched steps done")
+    ref = d_out.clone()
+    ntri_batch = ntri.copy()
+    # The same batches as hipGraph replays (captured on the third call).
+    _lib.check(lib.dl_set_graphs(ctx.handle, 1))
     for _ in range(3):
         batch_step()
     torch.cuda.synchronize()
@@ -113,22 +127,11 @@ def bench_batch(lib, ctx, n_images: int, steps: int):
     for _ in range(steps):
         batch_step()
     torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / steps
+    t_replayed = (time.perf_counter() - t0) / steps
     replays = graph_stats(lib, ctx)[1] - g0[1]
-    log("batched steps done")
-    ref = d_out.clone()
-    ntri_batch = ntri.copy()
-    _lib.check(lib.dl_set_graphs(ctx.handle, 0))
-    batch_step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch_step()
-    torch.cuda.synchronize()
-    t_uncaptured = (time.perf_counter() - t0) / steps
-    _lib.check(lib.dl_set_graphs(ctx.handle, 1))
-    same_uncaptured = bool(torch.equal(ref, d_out))
-    log("uncaptured batched steps done")
+    _lib.check(lib.dl_set_graphs(ctx.handle, 2))
+    same_replayed = bool(torch.equal(ref, d_out))
+    log("replayed batched steps done")
 
     seed_off = np.concatenate([[0], np.cumsum(counts)])
     pix_off = np.concatenate([[0], np.cumsum(px)])
@@ -163,8 +166,8 @@ def bench_batch(lib, ctx, n_images: int, steps: int):
             "speedup_vs_single_image_path": t_single / t_batch,
             "batched_equals_single_bitwise": same,
             "graph_replays_in_timed_steps": replays,
-            "uncaptured_ms_per_batch": t_uncaptured * 1e3,
-            "uncaptured_equals_replayed_bitwise": same_uncaptured}
+            "replayed_ms_per_batch": t_replayed * 1e3,
+            "replayed_equals_uncaptured_bitwise": same_replayed}
 
 
 def bench_small_images(lib, ctx, steps: int):
@@ -203,7 +206,7 @@ def bench_small_images(lib, ctx, steps: int):
                 call()
             torch.cuda.synchronize()
             out[graphs] = ((time.perf_counter() - t0) / steps * 1e3, d_out.clone())
-        _lib.check(lib.dl_set_graphs(ctx.handle, 1))
+        _lib.check(lib.dl_set_graphs(ctx.handle, 2))
         res.append({"side": side, "seeds": n, "triangles": int(nt.value), "calls": steps,
                     "graph_ms_per_call": out[1][0], "uncaptured_ms_per_call": out[0][0],
                     "graph_mpixel_per_s": side * side / out[1][0] / 1e3,
@@ -575,33 +578,39 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
-    # Timed region: the pipeline alone.  From the third call of one signature
-    # on, the pipeline's first half is a replayed hipGraph.
-    g_before = graph_stats(lib, ctx)
+    # Timed region: the pipeline alone, in the library's default graph mode
+    # (auto: at this size every launch is issued from the host).
+    g_main = graph_stats(lib, ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    g_after = graph_stats(lib, ctx)
     log(f"timed {args.steps} steps")
+    default_replays = graph_stats(lib, ctx)[1] - g_main[1]
     ms_per_step = (t1 - t0) / args.steps * 1e3
     value = W * H / (ms_per_step / 1e3) / 1e6
-    # The same steps with graphs off (every launch issued from the host).
-    _lib.check(lib.dl_set_graphs(ctx.handle, 0))
-    step()
+    # The same steps as hipGraph replays (graphs on: the third call of the
+    # signature is captured, the timed ones replay it).
+    _lib.check(lib.dl_set_graphs(ctx.handle, 1))
+    for _ in range(3):
+        step()
+    g_before = graph_stats(lib, ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
     torch.cuda.synchronize()
-    uncaptured_ms = (time.perf_counter() - t0) / args.steps * 1e3
-    _lib.check(lib.dl_set_graphs(ctx.handle, 1))
-    log("uncaptured steps done")
-    graphs = {"captures": g_after[0], "replays_in_timed_steps": g_after[1] - g_before[1],
-              "uncaptured_ms_per_step": uncaptured_ms,
-              "uncaptured_mpixel_per_s": W * H / (uncaptured_ms / 1e3) / 1e6}
+    replayed_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    g_after = graph_stats(lib, ctx)
+    _lib.check(lib.dl_set_graphs(ctx.handle, 2))
+    log("replayed steps done")
+    graphs = {"default_mode": "auto (capture below 2^22 canvas cells)",
+              "replays_in_default_steps": default_replays,
+              "captures": g_after[0], "replays_in_timed_steps": g_after[1] - g_before[1],
+              "replayed_ms_per_step": replayed_ms,
+              "replayed_mpixel_per_s": W * H / (replayed_ms / 1e3) / 1e6}
     # Kernel-timing region: the same steps again with a pair of HIP events
     # around every kernel, on the stream the kernels run on (the events add
     # host work, so this region is not the one `value` comes from).

@@ -262,7 +262,8 @@ int dl_autotune_reset(void);
  * reference counterpart. */
 int dl_set_fill_prediction(dl_context* ctx, int enabled);
 
-/* hipGraph replay of the pipeline (default on).  Everything a triangulation
+/* hipGraph replay of the pipeline: 0 off, 1 on, 2 auto (the default: on for
+ * pipelines of fewer than 2^22 canvas cells).  Everything a triangulation
  * enqueues before its final status read (seed ordering, the predicted padded
  * fill, detection, registration, assignment and output, and the status
  * copies) is captured once per call signature: the images' sizes, seed
@@ -271,7 +272,11 @@ int dl_set_fill_prediction(dl_context* ctx, int enabled);
  * captured; later matching calls replay the graph, with the host planning and
  * the ~30 launches replaced by one graph launch.  Any buffer reallocation in
  * the process drops the graph.  Results are identical with and without.
- * Timing and kernel profiling run uncaptured.  No reference counterpart. */
+ * Timing and kernel profiling run uncaptured.  A replay saves the host's
+ * issue work, which bounds small pipelines, but on ROCm 7 costs about 0.5 us
+ * of GPU time per kernel node more than host-issued launches, which shows on
+ * large ones; hence the size threshold.  Other values: ValueError.
+ * No reference counterpart. */
 int dl_set_graphs(dl_context* ctx, int enabled);
 /* Graphs captured and replayed on this context so far. */
 int dl_get_graph_stats(dl_context* ctx, int64_t* captures, int64_t* replays);

@@ -24,6 +24,16 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def _graphs_on():
+    """These tests turn graphs on (mode 1) for the shared context and restore
+    the default (2, auto by size) afterwards."""
+    ctx = _lib.context()
+    _lib.check(_lib.load().dl_set_graphs(ctx.handle, 1))
+    yield
+    _lib.check(_lib.load().dl_set_graphs(ctx.handle, 2))
+
+
 def _rows(m):
     return np.array([m[i] for i in range(len(m))], dtype=np.int32).reshape(-1, 5)
 
@@ -127,6 +137,7 @@ def test_device_path_with_and_without_graphs():
 
         W, H, n = 1024, 768, 15000
         ctxg, ctxe = _lib.Context(0), _lib.Context(0)
+        _lib.check(lib.dl_set_graphs(ctxg.handle, 1))
         _lib.check(lib.dl_set_graphs(ctxe.handle, 0))
         dev = torch.device("cuda:0")
         out_g = torch.empty((H, W, 3), dtype=torch.int32, device=dev)
@@ -211,3 +222,30 @@ def test_invalid_seeds_on_predicted_runs(bad):
         s = uniform_seeds(W, H, n, 960 + i)
         m, g = dl.triangulate(W, H, s)
         _check(W, H, s, -1, m, g)
+
+
+def test_auto_mode_captures_small_pipelines_only():
+    """Mode 2 (the default): a repeated small signature is captured and
+    replayed; a repeated signature of at least 2^22 canvas cells is not.
+    Both equal the oracle; other modes are rejected."""
+    ctx = _lib.context()
+    lib = _lib.load()
+    _lib.check(lib.dl_set_graphs(ctx.handle, 2))
+    c0, r0 = _stats(ctx)
+    for i in range(5):
+        s = uniform_seeds(200, 150, 400, 1200 + i)
+        m, g = dl.triangulate(200, 150, s)
+        _check(200, 150, s, -1, m, g)
+    c1, r1 = _stats(ctx)
+    assert c1 > c0 and r1 - r0 >= 2
+    W, H = 2100, 2020          # (2100 + 32) * (2020 + 32) > 2^22 cells
+    big = uniform_seeds(W, H, 30000, 1300)
+    m0, g0 = dl.triangulate(W, H, big)
+    _check(W, H, big, -1, m0, g0)
+    for _ in range(3):
+        m, g = dl.triangulate(W, H, big)
+        assert m == m0 and np.array_equal(g, g0)
+    assert _stats(ctx) == (c1, r1)
+    for bad in (-1, 3):
+        with pytest.raises(ValueError):
+            _lib.check(lib.dl_set_graphs(ctx.handle, bad))
