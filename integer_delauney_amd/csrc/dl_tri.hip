@@ -1889,9 +1889,10 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
         }
         return;
     }
-    // C == 3.  Each wave takes four tile rows at a time, one pixel per lane:
-    // the four label loads are issued together, then the four owner-id
-    // gathers.  A row's 3 * Xw output ints are then staged in the wave's own
+    // C == 3.  Each wave takes kOutRows tile rows at a time, one pixel per
+    // lane: the rows' label loads are issued together, then their owner-id
+    // gathers, so a wave waits for memory twice per batch (occupancy here is
+    // bound by LDS, not registers).  A row's 3 * Xw output ints are then staged in the wave's own
     // LDS row buffer (each lane writes its pixel's three channels) and written
     // as chunks of four by lanes 0..47 (chunk L = ints 4L .. 4L + 3), so every
     // store instruction covers one contiguous run of the row.  The buffer
@@ -1901,18 +1902,19 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     const int n_rows = Y1 - Y0 + 1;
     const int n_out = min(4, 3 * Xw - 4 * lane);            // ints of the chunk inside the row
     int32_t* rowbuf = reinterpret_cast<int32_t*>(ent) + wv * (3 * kTS);   // 3 ints per pixel
-    for (int r0 = 4 * wv; r0 < n_rows; r0 += 16) {
-        uint32_t lab[4];
+    constexpr int kOutRows = kTS / 4;   // four waves cover the tile in one batch each
+    for (int r0 = kOutRows * wv; r0 < n_rows; r0 += 4 * kOutRows) {
+        uint32_t lab[kOutRows];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < kOutRows; ++i) {
             const int r = r0 + i;
             lab[i] = (r < n_rows && lane < Xw)
                          ? d.out_lab[static_cast<size_t>(Y0 + r + off) * Wl + off + X0 + lane]
                          : kInvalid;
         }
-        int32_t id[4], dd[4];
+        int32_t id[kOutRows], dd[kOutRows];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < kOutRows; ++i) {
             id[i] = -1;
             dd[i] = -1;
             if (lab[i] != kInvalid) {
@@ -1923,7 +1925,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
             }
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < kOutRows; ++i) {
             const int r = r0 + i;
             if (r >= n_rows) break;   // uniform across the wave
             const int32_t tid = tid_tile[r * kTS + lane];
