@@ -544,8 +544,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": 4 * n, "dispatches": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the pipeline "
-              f"shards by image or by row strip; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (a batch would "
+              f"shard by image, one large image by row strips; neither is built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
