@@ -444,11 +444,17 @@ template <int NW, int R>
 constexpr int kFillWavesPerEu = (NW <= 8 && R <= 8) ? 8 : 1;
 
 template <int NW, int R, int K>
+//
+// prev_changed (nullable): the previous launch's change counter.  When it is
+// 0 the fill has converged (no cell changed anywhere, so both planes hold the
+// final labels) and every block returns at once, writing nothing: the next
+// launch sees this launch's counter still 0 and returns too.
 __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_tile_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
-    int* __restrict__ changed_any, int* __restrict__ visits)
+    int* __restrict__ changed_any, int* __restrict__ visits, const int* __restrict__ prev_changed)
 {
+    if (prev_changed && *prev_changed == 0) return;   // uniform across the grid
     const int per_xcd = gridDim.x / 8;
     const int b = blockIdx.x;
     const int tile = (b % 8) * per_xcd + b / 8;
@@ -624,7 +630,8 @@ __global__ void zero_ints_kernel(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
-using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*, int*);
+using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*, int*,
+                           const int*);
 using ListKernel = void (*)(const BfsCanvas*, int, const int*, const int*, int, const int*, int*,
                             int*, int*);
 
@@ -658,8 +665,10 @@ constexpr int kListGrid = 2048;
 // One tile launch of the fill: launch 0 (and every launch of a small fill)
 // visits every tile; later launches of a large fill list the active tiles
 // first.  changed / listcnt: this launch's zeroed counters.
+// prev_changed: see bfs_tile_kernel (nullptr: always run).
 void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_canvas, int total,
-                  int launch, int* changed, int* listcnt, hipStream_t s)
+                  int launch, int* changed, int* listcnt, hipStream_t s,
+                  const int* prev_changed = nullptr)
 {
     int* flags = work.flags.as<int>();
     const int src_plane = launch & 1;
@@ -678,7 +687,8 @@ void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_c
     } else {
         const int grid_blocks = (total + 7) / 8 * 8;
         v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(cv, n_canvas, total, src_plane, fprev, fcur,
-                                                    launch == 0 ? 1 : 0, changed, visits);
+                                                    launch == 0 ? 1 : 0, changed, visits,
+                                                    launch > 0 ? prev_changed : nullptr);
     }
     DL_LAUNCH_CHECK();
 }
@@ -773,8 +783,10 @@ int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
         fill_u32(reinterpret_cast<uint32_t*>(visits), 0u, static_cast<size_t>(total), s);
     fill_u32(reinterpret_cast<uint32_t*>(dcnt), 0u, 2 * static_cast<size_t>(n_launches + 1), s);
     for (int launch = 0; launch < n_launches; ++launch)
+        // (the counters were zeroed above, so a launch's counter stays 0
+        // unless it changed something)
         launch_tiles(v, cfg, work, n_canvas, total, launch, dcnt + launch,
-                     dcnt + n_launches + 1 + launch, s);
+                     dcnt + n_launches + 1 + launch, s, launch > 0 ? dcnt + launch - 1 : nullptr);
     if (work.count_visits) {
         sum_ints_kernel<<<1, 1024, 0, s>>>(visits, total, dcnt + n_launches);
         DL_LAUNCH_CHECK();
