@@ -533,8 +533,22 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     ctx->padB.reserve(sizeof(uint32_t) * cells);
     ctx->esc.reserve(sizeof(int) * static_cast<size_t>(n_img));
     const FillSeg esc_clear{ctx->esc.as<uint32_t>(), static_cast<size_t>(n_img), 0u};
+    BfsStats st;
+    bool tune = false;
+    BfsConfig cfg = choose_config(ctx, plan.n_cells, plan.n_seeds, &tune);
+    int plane = 0;
+    const auto pkey = pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg);
+    const int pred = pend ? fill_prediction(ctx, cfg, tune) : 0;
+    // A predicted fill's launch counters are cleared in the canvases' clear
+    // launch.
+    FillSeg cnt_clear{};
+    if (pred > 0) {
+        ctx->bfs.counters.reserve(sizeof(int) * bfs_fixed_counter_words(pred));
+        cnt_clear = FillSeg{ctx->bfs.counters.as<uint32_t>(), bfs_fixed_counter_words(pred), 0u};
+    }
     init_canvases(plan, ctx->sorted.as<uint32_t>(), ctx->padA.as<uint32_t>(),
-                  ctx->padB.as<uint32_t>(), ctx->tri, s, &esc_clear);
+                  ctx->padB.as<uint32_t>(), ctx->tri, s, &esc_clear,
+                  pred > 0 ? &cnt_clear : nullptr);
     std::vector<BfsCanvas> cvs(n_img);
     for (int i = 0; i < n_img; ++i) {
         const ImageDesc& d = plan.img[i];
@@ -547,17 +561,11 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         cv.P = watch[i] >= 0 ? watch[i] : 0;
         cv.escape = watch[i] >= 0 ? ctx->esc.as<int>() + i : nullptr;
     }
-    BfsStats st;
-    bool tune = false;
-    BfsConfig cfg = choose_config(ctx, plan.n_cells, plan.n_seeds, &tune);
-    int plane = 0;
-    const auto pkey = pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg);
-    const int pred = pend ? fill_prediction(ctx, cfg, tune) : 0;
     if (pred > 0) {
         // change flags and visit total stay on the device for the final status
         // read; visits are counted only for statistics
         ctx->bfs.count_visits = ctx->timing || ctx->profiling;
-        plane = bfs_launch_fixed(cvs, cfg, ctx->bfs, s, pred, nullptr);
+        plane = bfs_launch_fixed(cvs, cfg, ctx->bfs, s, pred, nullptr, true);
         pend->active = true;
         pend->n = pred;
         pend->cfg = cfg;

@@ -765,7 +765,7 @@ static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
 }
 
 int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
-                     hipStream_t s, int n_launches, int* host_changed)
+                     hipStream_t s, int n_launches, int* host_changed, bool counters_zeroed)
 {
     const Variant* v = find_variant(cfg);
     if (!v) throw runtime_error("unsupported flood-fill tile configuration");
@@ -774,14 +774,15 @@ int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     if (total == 0 || n_launches <= 0) return 0;
     const int n_canvas = static_cast<int>(canvases.size());
     // counters: n changed flags, the visit total, then n list counts
-    work.counters.reserve(sizeof(int) * 2 * static_cast<size_t>(n_launches + 1));
+    work.counters.reserve(sizeof(int) * bfs_fixed_counter_words(n_launches));
     int* dcnt = work.counters.as<int>();
     int* visits = work.visits.as<int>();
     // Without visit counting the per-tile counts are left as they are (never
     // read) and the total stays 0.
     if (work.count_visits)
         fill_u32(reinterpret_cast<uint32_t*>(visits), 0u, static_cast<size_t>(total), s);
-    fill_u32(reinterpret_cast<uint32_t*>(dcnt), 0u, 2 * static_cast<size_t>(n_launches + 1), s);
+    if (!counters_zeroed)
+        fill_u32(reinterpret_cast<uint32_t*>(dcnt), 0u, bfs_fixed_counter_words(n_launches), s);
     for (int launch = 0; launch < n_launches; ++launch)
         // (the counters were zeroed above, so a launch's counter stays 0
         // unless it changed something)

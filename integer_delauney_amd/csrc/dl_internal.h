@@ -207,9 +207,18 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
 // work.counters (n_launches + 1 ints); with host_changed (page-locked) they
 // are also copied there asynchronously.  The fill has converged iff some
 // launch changed nothing; the labels are then final in both planes.  Returns
-// the plane the last launch wrote.
+// the plane the last launch wrote.  counters_zeroed: the caller already
+// cleared the first bfs_fixed_counter_words(n_launches) words of
+// work.counters on the stream.
 int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
-                     hipStream_t s, int n_launches, int* host_changed);
+                     hipStream_t s, int n_launches, int* host_changed,
+                     bool counters_zeroed = false);
+// Counter words bfs_launch_fixed uses (n change flags, the visit total, n list
+// counts and one spare).
+inline size_t bfs_fixed_counter_words(int n_launches)
+{
+    return 2 * static_cast<size_t>(n_launches + 1);
+}
 
 // One-thread-per-cell, one-pass-per-launch version of the same schedule.  Kept
 // as an independent second implementation for cross-checking the tiled kernel.
@@ -352,7 +361,7 @@ void check_seed_flags(const BatchPlan& plan, const uint32_t* hf);
 // more fill issued in the same launch as the plane's.
 void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* plane_a,
                    uint32_t* plane_b, struct TriWork& work, hipStream_t s,
-                   const FillSeg* extra = nullptr);
+                   const FillSeg* extra = nullptr, const FillSeg* extra2 = nullptr);
 
 // Exclusive prefix sum of popcount(words[i]) into out[i]; out[n] receives the
 // total.  out has n+1 entries.

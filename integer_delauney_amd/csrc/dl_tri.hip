@@ -492,7 +492,8 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, h
 }
 
 void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* plane_a,
-                   uint32_t* plane_b, TriWork& work, hipStream_t s, const FillSeg* extra)
+                   uint32_t* plane_b, TriWork& work, hipStream_t s, const FillSeg* extra,
+                   const FillSeg* extra2)
 {
     const int n_img = static_cast<int>(plan.img.size());
     ProfScope ps(kKInit, s);
@@ -503,6 +504,7 @@ void init_canvases(const BatchPlan& plan, const uint32_t* d_sorted, uint32_t* pl
     fs.seg[0] = FillSeg{plane_a, static_cast<size_t>(plan.n_cells), kInvalid};
     fs.k = 1;
     if (extra) fs.seg[fs.k++] = *extra;
+    if (extra2) fs.seg[fs.k++] = *extra2;
     fill_u32_multi(fs, s);
     canvas_seed_kernel<<<static_cast<unsigned>((plan.n_seeds + 255) / 256), 256, 0, s>>>(
         d_sorted, plan.n_seeds, seg_ptr(work, 0, n_img), n_img, work.desc.as<ImageDesc>(),
