@@ -776,8 +776,9 @@ Issued issue_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
     detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true, true);
     clk.mark(2);
     clk.mark(3);
+    // (the bin counts were cleared with the seed bitmap, in prepare)
     assign_tiled(plan, ctx->tri.rows.as<int32_t>(), -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s,
-                 true, iss.fixed_bins);
+                 true, iss.fixed_bins, true);
     clk.mark(4);
     issue_status(plan, ctx->tri, iss.any_pad ? ctx->esc.as<int>() : nullptr,
                  iss.any_pad ? n_img : 0, iss.pend.active,

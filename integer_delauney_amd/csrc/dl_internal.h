@@ -427,9 +427,12 @@ SpeculativeStatus parse_status(const BatchPlan& plan, TriWork& work, int n_esc, 
 // speculative = true sizes the bin list without a sync (checked by read_status);
 // with allow_fixed it may bin in one pass into fixed per-tile slices, whose
 // overflow the status reports as bin_redo (the assignment alone runs again).
+// counts_zeroed: work.bin_counts (2 * n_tiles + 1 words) is already clear on
+// the stream (normalise_seeds clears it with the seed bitmap).
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
                   const uint32_t* sorted, int C, TriWork& work,
-                  hipStream_t s, bool speculative, bool allow_fixed = true);
+                  hipStream_t s, bool speculative, bool allow_fixed = true,
+                  bool counts_zeroed = false);
 
 // (H, W, C) int32 output of one image without triangles (C == 2) or with the
 // tid raster (C == 3).  Output pixel (x, y) reads canvas cell (x + off, y + off);

@@ -468,6 +468,12 @@ void normalise_seeds(const BatchPlan& plan, uint32_t* d_sorted, TriWork& work, h
         fs.seg[0] = FillSeg{bm, static_cast<size_t>(nw), 0u};
         fs.seg[1] = FillSeg{flags, static_cast<size_t>(n_img + 8), 0u};
         fs.k = 2;
+        // and the assignment's bin counts (see assign_tiled's counts_zeroed)
+        if (plan.n_tiles > 0) {
+            const size_t nb = static_cast<size_t>(2 * plan.n_tiles + 1);
+            work.bin_counts.reserve(sizeof(uint32_t) * nb);
+            fs.seg[fs.k++] = FillSeg{work.bin_counts.as<uint32_t>(), nb, 0u};
+        }
         fill_u32_multi(fs, s);
         seeds_mark_kernel<<<blocks, 256, 0, s>>>(plan.n_seeds, sseg, n_img, D, bm,
                                                  flags + kSeedFlagBase);
@@ -1982,7 +1988,7 @@ uint32_t fixed_bin_capacity(const BatchPlan& plan, int64_t t_bound)
 
 void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
                   const uint32_t* sorted, int C, TriWork& work,
-                  hipStream_t s, bool speculative, bool allow_fixed)
+                  hipStream_t s, bool speculative, bool allow_fixed, bool counts_zeroed)
 {
     const int n_img = static_cast<int>(plan.img.size());
     const int64_t ntiles = plan.n_tiles;
@@ -2013,7 +2019,7 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
         cap = fixed;
         work.bin_list.reserve(sizeof(uint32_t) * static_cast<size_t>(ntiles) * fixed);
         ProfScope ps(kKAssign, s);
-        fill_u32(counts, 0u, static_cast<size_t>(ntiles), s);
+        if (!counts_zeroed) fill_u32(counts, 0u, static_cast<size_t>(ntiles), s);
         if (t_bound > 0) {
             bin_kernel<kBinFixed><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D, sorted,
                                                           counts, nullptr, work.bin_list.as<uint32_t>(),
@@ -2023,7 +2029,7 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
     } else {
         {
             ProfScope ps(kKAssign, s);
-            fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
+            if (!counts_zeroed) fill_u32(counts, 0u, static_cast<size_t>(2 * ntiles), s);
             if (t_bound > 0) {
                 bin_kernel<kBinCount><<<tblocks, 256, 0, s>>>(rows, rows_cap, tri_base, n_img, D,
                                                               sorted, counts, nullptr, nullptr, 0u,
