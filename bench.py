@@ -423,6 +423,7 @@ def bench_serving(n_requests: int, n_sessions: int, slots: int = 3, loads=(0.3, 
         reqs = [next_request() for _ in range(n_requests)]
         px = sum(p for _, p in reqs)
         futs = []
+        b0 = len(srv.batches)
         t0 = time.perf_counter()
         if load_rate is None:
             arrivals = np.zeros(n_requests)
@@ -440,12 +441,20 @@ def bench_serving(n_requests: int, n_sessions: int, slots: int = 3, loads=(0.3, 
             lat.append(f.t_done - f.t_arrival)
         makespan = max(f.t_done for f in futs) - t0
         del futs
+        time.sleep(0.05)   # the last workers file their batch records
+        recs = srv.batches[b0:]
+        dev = sum(b.gpu_ms for b in recs)
+        cpy = sum(b.copy_ms for b in recs)
         lat = np.array(lat) * 1e3
         return {"requests": n_requests, "mpixel": px / 1e6, "makespan_s": makespan,
                 "throughput_mpixel_per_s": px / makespan / 1e6,
                 "latency_ms_p50": float(np.percentile(lat, 50)),
                 "latency_ms_p99": float(np.percentile(lat, 99)),
-                "latency_ms_max": float(lat.max())}
+                "latency_ms_max": float(lat.max()),
+                "batches": len(recs),
+                "pipeline_device_ms_sum": dev, "copy_back_ms_sum": cpy,
+                "copy_back_share": cpy / (dev + cpy) if dev + cpy else None,
+                "rejected": srv.rejected}
 
     def run_plain():
         """The same kind of stream, one request at a time through the plain
@@ -498,7 +507,7 @@ def bench_serving(n_requests: int, n_sessions: int, slots: int = 3, loads=(0.3, 
         out["saturated_vs_plain"] = sat["throughput_mpixel_per_s"] / plain["throughput_mpixel_per_s"]
         log(f"serving: plain path {plain['throughput_mpixel_per_s']:.0f} Mpx/s")
         out["batches"] = len(srv.batches)
-        out["max_batch"] = max(n for n, _ in srv.batches)
+        out["max_batch"] = max(b.requests for b in srv.batches)
         return out
     finally:
         srv.close()

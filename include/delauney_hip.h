@@ -20,7 +20,11 @@
  *   - border_padding: -1 selects DL_DEFAULT_BORDER_PADDING (the reference's
  *     border_padding=None / AUTO_BORDER_PADDING, binding_helpers.hpp:18-20);
  *     other negative values clamp to 0 (triangulation.py:75-78).
- *   - Canvas sides (image side + 2 * padding) are limited to DL_MAX_CANVAS.
+ *   - Canvas sides (image side + 2 * padding) are limited to DL_MAX_CANVAS;
+ *     a larger canvas is DL_ERUNTIME, as the reference's failing allocation
+ *     of an oversized canvas is (tests/test_cuda_error_checking.py:28-58).
+ *     A failing device allocation is DL_ERUNTIME naming hipMalloc, and the
+ *     context stays usable.
  *   - Calls on one context are serialised by the caller (one context per
  *     thread); they block until their results are ready and may be made with
  *     the Python GIL released.
@@ -210,9 +214,24 @@ typedef struct dl_image {
 
 /* Runs the pipeline once for a mixed batch of fresh images and sessions (the
  * serving loop's unit of work); every image's result equals its single-image
- * result.  Sessions with no live seed get an all -1 grid.  No reference
- * counterpart. */
+ * result.  Sessions with no live seed get an all -1 grid.  With timing on
+ * (dl_set_timing), dl_get_timings' output stage [5] is the copy of host
+ * images' grids back to the host.  No reference counterpart. */
 int dl_run_batch(dl_context* ctx, int n, dl_image* images, void* stream);
+
+/* Device memory accounting for admission control (the serving loop); no
+ * reference counterpart.
+ *   context_device_bytes: bytes of device workspace the context holds now
+ *     (grow-only between trims).
+ *   context_trim: releases every device workspace of the context (the next
+ *     call allocates again); results of earlier calls are no longer
+ *     available from it (dl_copy_triangles, dl_device_triangles).
+ *   session_device_bytes: bytes of device memory the session holds.
+ *   device_memory: hipMemGetInfo of a device. */
+int dl_context_device_bytes(dl_context* ctx, size_t* bytes);
+int dl_context_trim(dl_context* ctx);
+int dl_session_device_bytes(dl_session* s, size_t* bytes);
+int dl_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 
 /* Rows (x, y, id_a, id_b, id_c) in tid order of the last triangulation on this
  * context: the reference's triangle_map values (triangulation.py:157;
@@ -231,6 +250,18 @@ int dl_copy_padded_voronoi(dl_context* ctx, int32_t* out, int* out_w, int* out_h
  * [8] flood-fill tile visits.  Mirrors the *_timed dictionaries
  * (py_grid_triangulation.hpp:39-46, triangulation.cuh:29-34). */
 int dl_get_timings(dl_context* ctx, double* out9);
+
+/* Flood-fill statistics of the last call: launches issued, launches up to
+ * and including the first that changed nothing (later ones return at once),
+ * tile visits (counted while timing or kernel profiling is on, else 0), and
+ * iterations per launch; and, over the context's life, the predicted fills
+ * (see dl_set_fill_prediction) that fell short and were continued.
+ * bfs_iters of the session *_timed dictionaries is converged *
+ * iters_per_launch, the iterations the fill ran (the reference counts its
+ * BFS passes the same way, delaunay_voronoi.cu:57-108,
+ * py_delaunay.hpp:268-277).  Any pointer may be NULL. */
+int dl_get_fill_stats(dl_context* ctx, int64_t* launches, int64_t* converged,
+                      int64_t* tile_visits, int* iters_per_launch, int64_t* shortfalls);
 
 /* Enables event timing of stages for subsequent calls (adds synchronisation). */
 int dl_set_timing(dl_context* ctx, int enabled);

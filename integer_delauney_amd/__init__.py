@@ -12,7 +12,8 @@ and creates a HIP context, and raises if either is unavailable.
 from .api import (AUTO_BORDER_PADDING, DEFAULT_BORDER_PADDING, GridTriangulation, Voronoi,
                   max_boundary_spacing, triangulate, triangulate_batch)
 from .delaunay import Delaunay
+from .device_views import DeviceArrayView
 
-__all__ = ["Voronoi", "GridTriangulation", "Delaunay", "triangulate", "triangulate_batch",
-           "DEFAULT_BORDER_PADDING", "max_boundary_spacing"]
+__all__ = ["Voronoi", "GridTriangulation", "Delaunay", "DeviceArrayView", "triangulate",
+           "triangulate_batch", "DEFAULT_BORDER_PADDING", "max_boundary_spacing"]
 __version__ = "0.2.0"

@@ -57,6 +57,8 @@ _SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_int)]),
     "dl_get_timings": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "dl_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_get_fill_stats": (ctypes.c_int, [_VP, _I64P, _I64P, _I64P, ctypes.POINTER(ctypes.c_int),
+                                         _I64P]),
     "dl_set_bfs_config": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "dl_get_bfs_config": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]),
@@ -76,6 +78,11 @@ _SIGNATURES = {
     "dl_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_VP)]),
     "dl_host_free": (None, [_VP]),
     "dl_run_batch": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
+    "dl_context_device_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t)]),
+    "dl_context_trim": (ctypes.c_int, [_VP]),
+    "dl_session_device_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t)]),
+    "dl_device_memory": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
+                                        ctypes.POINTER(ctypes.c_size_t)]),
     "dl_session_create": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                          ctypes.c_int, ctypes.POINTER(_VP)]),
     "dl_session_destroy": (None, [_VP]),

@@ -278,6 +278,11 @@ struct dl_context {
 
     bool timing = false;
     double timings[9] = {0};
+    // Launches of the last call's fills up to and including the first that
+    // changed nothing (launches issued past convergence return at once).
+    int64_t fill_converged = 0;
+    // Predicted fills on this context that fell short and were continued.
+    int64_t fill_shortfalls = 0;
     BfsConfig cfg{8, 8, 8};
     bool cfg_auto = true;
     bool autotune = true;   // auto mode may time the candidate shapes on a new canvas class
@@ -375,13 +380,18 @@ int resolve_padding(int p)
     return p < 0 ? 0 : p;
 }
 
+// A canvas beyond the library's limit is a RuntimeError, as the reference's
+// failing device allocation of an oversized canvas is
+// (tests/test_cuda_error_checking.py:28-58); non-positive sides are a
+// ValueError.
 void check_dims(int W, int H, int P)
 {
     if (W <= 0 || H <= 0) throw value_error("width and height must be positive");
     if (static_cast<int64_t>(W) + 2LL * P > DL_MAX_CANVAS ||
         static_cast<int64_t>(H) + 2LL * P > DL_MAX_CANVAS)
-        throw value_error("canvas (image plus twice the border padding) exceeds " +
-                          std::to_string(DL_MAX_CANVAS) + " cells per side");
+        throw runtime_error("canvas (image plus twice the border padding) exceeds " +
+                            std::to_string(DL_MAX_CANVAS) +
+                            " cells per side, the limit of 16-bit packed seed positions");
 }
 
 void check_seeds(const void* p, int64_t n)
@@ -446,38 +456,46 @@ BfsConfig choose_config(const dl_context* ctx, int64_t cells, int64_t seeds, boo
     return kDefaultCfg;
 }
 
+// Stage timing.  With ctx->timing on, every mark(idx) records an event on the
+// context's stream and the interval since the previous mark is added to
+// stage idx (0 seeds, 1 fill, 2 detect, 3 dedup/rank, 4 assign, 5 output), so
+// a stage run twice (a continued fill, an exact rerun) accumulates.
 struct StageClock {
     dl_context* ctx;
     std::chrono::steady_clock::time_point t0;
-    int next = 0;
+    std::vector<int> stage;   // stage of the interval ending at event k + 1
     explicit StageClock(dl_context* c) : ctx(c), t0(std::chrono::steady_clock::now())
     {
-        if (ctx->timing) {
-            while (ctx->events.size() < 8) {
-                hipEvent_t e;
-                DL_HIP(hipEventCreate(&e));
-                ctx->events.push_back(e);
-            }
-            DL_HIP(hipEventRecord(ctx->events[0], ctx->stream));
-            next = 1;
-        }
         for (double& v : ctx->timings) v = 0.0;
+        ctx->fill_converged = 0;
+        if (ctx->timing) {
+            event(0);
+            DL_HIP(hipEventRecord(ctx->events[0], ctx->stream));
+        }
     }
-    // Marks the end of stage `idx` (0 seeds, 1 fill, 2 detect, 3 dedup, 4 assign, 5 output).
-    void mark(int /*idx*/)
+    hipEvent_t event(size_t k)
     {
-        if (!ctx->timing || next >= static_cast<int>(ctx->events.size())) return;
-        DL_HIP(hipEventRecord(ctx->events[next], ctx->stream));
-        ++next;
+        while (ctx->events.size() <= k) {
+            hipEvent_t e;
+            DL_HIP(hipEventCreate(&e));
+            ctx->events.push_back(e);
+        }
+        return ctx->events[k];
+    }
+    void mark(int idx)
+    {
+        if (!ctx->timing) return;
+        DL_HIP(hipEventRecord(event(stage.size() + 1), ctx->stream));
+        stage.push_back(idx);
     }
     void finish(hipStream_t s)
     {
         DL_HIP(hipStreamSynchronize(s));
         if (ctx->timing) {
-            for (int i = 1; i < next && i <= 6; ++i) {
+            for (size_t k = 0; k < stage.size(); ++k) {
                 float ms = 0.f;
-                DL_HIP(hipEventElapsedTime(&ms, ctx->events[i - 1], ctx->events[i]));
-                ctx->timings[i - 1] = ms;
+                DL_HIP(hipEventElapsedTime(&ms, ctx->events[k], ctx->events[k + 1]));
+                ctx->timings[stage[k]] += ms;
             }
         }
         ctx->timings[6] =
@@ -574,6 +592,7 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         st.launches = pred;
     } else if (!tune) {
         plane = bfs_run(cvs, cfg, ctx->bfs, s, &st, visits_async);
+        ctx->fill_converged += st.converged + 1;
         if (pend) ctx->fill_pred[pkey] = predicted_launches(st.converged);
     } else {
         hipEvent_t ev[2];
@@ -619,6 +638,7 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         DL_HIP(hipEventDestroy(ev[1]));
         if (visits_async) *visits_async = st.tile_visits;
         st.tile_visits = 0;
+        ctx->fill_converged += st.converged + 1;
         if (pend) ctx->fill_pred[pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg)] =
             predicted_launches(st.converged);
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -652,6 +672,7 @@ bool finish_predicted_fill(dl_context* ctx, const FillPending& pend, const int* 
         }
     *visits += changed[pend.n];
     if (conv >= 0) {
+        ctx->fill_converged += conv + 1;
         // Keep the count while it still leaves the margin, so that inputs of
         // one class whose fills converge a launch apart keep one signature
         // (and a captured graph); shrink it once it is far too generous.
@@ -661,8 +682,10 @@ bool finish_predicted_fill(dl_context* ctx, const FillPending& pend, const int* 
         return true;
     }
     BfsStats st;
+    ++ctx->fill_shortfalls;
     bfs_run(ctx->fill_cvs, pend.cfg, ctx->bfs, s, &st, nullptr, pend.n);
     ctx->timings[7] += st.launches - pend.n;
+    ctx->fill_converged += st.converged + 1;
     *visits += st.tile_visits;
     ctx->fill_pred[pend.key] = predicted_launches(st.converged);
     return false;
@@ -697,6 +720,7 @@ void fill_unpadded(dl_context* ctx, const std::vector<int>& which, hipStream_t s
     const int plane = bfs_run(cvs, ctx->last_cfg, ctx->bfs, s, &st);
     ctx->timings[7] += st.launches;
     ctx->timings[8] += st.tile_visits;
+    ctx->fill_converged += st.converged + 1;
     for (size_t k = 0; k < which.size(); ++k) {
         ImageDesc& d = plan.img[which[k]];
         d.out_lab = (plane ? ctx->unB.as<uint32_t>() : ctx->unA.as<uint32_t>()) + off[k];
@@ -773,8 +797,8 @@ Issued issue_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int 
     const uint32_t* base = pad_plane_ptr(ctx, iss.plane);
     clk.mark(1);
     // (the status words were cleared with the seed flags)
-    detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true, true);
-    clk.mark(2);
+    detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true, true,
+                     [&] { clk.mark(2); });
     clk.mark(3);
     // (the bin counts were cleared with the seed bitmap, in prepare)
     assign_tiled(plan, ctx->tri.rows.as<int32_t>(), -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s,
@@ -802,8 +826,8 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
                      iss.pend.active ? iss.pend.n + 1 : 0, &seed_flags, &fill_changed);
     const int32_t* rows = ctx->tri.rows.as<int32_t>();
     auto detect_and_assign = [&]() {
-        detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true);
-        clk.mark(2);
+        detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, true, false,
+                         [&] { clk.mark(2); });
         clk.mark(3);
         rows = ctx->tri.rows.as<int32_t>();
         assign_tiled(plan, rows, -1, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, true,
@@ -816,8 +840,10 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
         check_seed_flags(plan, seed_flags);
         // (copied: a status read below reuses the buffer)
         const std::vector<int> changed(fill_changed, fill_changed + iss.pend.n + 1);
-        if (!finish_predicted_fill(ctx, iss.pend, changed.data(), s, visits_async))
+        if (!finish_predicted_fill(ctx, iss.pend, changed.data(), s, visits_async)) {
+            clk.mark(1);
             st = detect_and_assign();
+        }
     }
     ctx->timings[8] += *visits_async;
     if (st.bad_id) throw runtime_error("flood fill left an undefined cell");
@@ -825,12 +851,16 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
         ctx->bins_exact.insert(tune_key(ctx->device, plan.n_cells, plan.n_seeds));
     if (st.redo) {
         const DetectResult dr =
-            detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, false);
+            detect_triangles(plan, base, true, ctx->sorted.as<uint32_t>(), ctx->tri, s, false,
+                             false, [&] { clk.mark(2); });
+        clk.mark(3);
         rows = ctx->tri.rows.as<int32_t>();
         assign_tiled(plan, rows, dr.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
+        clk.mark(4);
         st.n_tri = dr.n_tri;
     } else if (st.bin_redo) {   // detection stands; bin and assign exactly
         assign_tiled(plan, rows, st.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
+        clk.mark(4);
     }
     if (C == 3) {
         // Did a padding cell ever update an image cell?  If not, the image's
@@ -842,8 +872,10 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
                 which.push_back(i);
         if (!which.empty()) {
             fill_unpadded(ctx, which, s);
+            clk.mark(1);
             upload_desc(plan, ctx->tri, s);
             assign_tiled(plan, rows, st.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
+            clk.mark(4);
         }
     }
     ctx->n_tri = st.n_tri;
@@ -1101,8 +1133,10 @@ void session_finalise(dl_context* ctx, dl_session* s)
     const size_t px = static_cast<size_t>(s->W) * s->H;
     s->out.reserve(sizeof(int32_t) * 3 * px);
     if (s->st.n_live == 0) {
+        StageClock clk(ctx);
         fill_u32(s->out.as<uint32_t>(), 0xFFFFFFFFu, 3 * px, st);
-        DL_HIP(hipStreamSynchronize(st));
+        clk.mark(5);
+        clk.finish(st);
         s->n_tri = 0;
         s->n_edges = -1;
     } else {
@@ -1145,6 +1179,38 @@ void ensure_edges(dl_context* ctx, dl_session* s)
 {
     ensure_finalised(ctx, s);
     if (s->n_edges < 0) s->n_edges = build_edges(view_of(s), s->edges, s->qw, ctx->stream);
+}
+
+// Every device workspace of a context / session (for the byte counts and
+// the trim).
+template <class F>
+void for_each_devbuf(dl_context* c, F&& f)
+{
+    BfsWork& b = c->bfs;
+    for (DevBuf* d : {&b.flags, &b.counters, &b.visits, &b.list, &b.canvases, &b.tile_map}) f(*d);
+    TriWork& t = c->tri;
+    for (DevBuf* d : {&t.seed_bm, &t.seed_idx, &t.bitmap, &t.scan, &t.scan_tmp, &t.flags,
+                      &t.table_tag, &t.table_tri, &t.cand, &t.counts, &t.offsets, &t.positions,
+                      &t.rows, &t.tidgrid, &t.bin_counts, &t.bin_offsets, &t.bin_list, &t.desc,
+                      &t.status_dev})
+        f(*d);
+    for (DevBuf* d : {&c->xy, &c->sorted, &c->padA, &c->padB, &c->unA, &c->unB, &c->out,
+                      &c->vgrid, &c->ids, &c->esc})
+        f(*d);
+}
+
+template <class F>
+void for_each_devbuf(dl_session* s, F&& f)
+{
+    SessionState& st = s->st;
+    for (DevBuf* d : {&st.bitmap, &st.list, &st.spare, &st.values, &st.values_spare,
+                      &st.backup_list, &st.backup_values, &st.mine, &st.keep, &st.flags})
+        f(*d);
+    for (DevBuf* d : {&s->stage, &s->stage_values, &s->out, &s->rows, &s->sorted, &s->edges,
+                      &s->pix_tids, &s->pix_sids, &s->pix_mask, &s->q_xy, &s->q_t, &s->q_tid,
+                      &s->q_mask, &s->q_out, &s->qw.a, &s->qw.b, &s->qw.c, &s->qw.d, &s->qw.e,
+                      &s->qw.sort_tmp, &s->qw.scan_tmp})
+        f(*d);
 }
 
 void require_values(const dl_session* s, const char* what)
@@ -1229,6 +1295,62 @@ int dl_get_timings(dl_context* ctx, double* out9)
     return guarded([&] {
         if (!ctx || !out9) throw value_error("null argument");
         for (int i = 0; i < 9; ++i) out9[i] = ctx->timings[i];
+    });
+}
+
+int dl_context_device_bytes(dl_context* ctx, size_t* bytes)
+{
+    return guarded([&] {
+        if (!ctx || !bytes) throw value_error("null argument");
+        size_t n = 0;
+        for_each_devbuf(ctx, [&](DevBuf& d) { n += d.bytes; });
+        *bytes = n;
+    });
+}
+
+int dl_context_trim(dl_context* ctx)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        DL_HIP(hipSetDevice(ctx->device));
+        DL_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->graph.reset();
+        for_each_devbuf(ctx, [](DevBuf& d) { d.release(); });
+        ctx->n_tri = 0;
+        ctx->have_padded = false;
+        ctx->tri.tri_base.clear();
+    });
+}
+
+int dl_session_device_bytes(dl_session* s, size_t* bytes)
+{
+    return guarded([&] {
+        if (!s || !bytes) throw value_error("null argument");
+        size_t n = 0;
+        for_each_devbuf(s, [&](DevBuf& d) { n += d.bytes; });
+        *bytes = n;
+    });
+}
+
+int dl_device_memory(int device, size_t* free_bytes, size_t* total_bytes)
+{
+    return guarded([&] {
+        if (!free_bytes || !total_bytes) throw value_error("null argument");
+        DL_HIP(hipSetDevice(device));
+        DL_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    });
+}
+
+int dl_get_fill_stats(dl_context* ctx, int64_t* launches, int64_t* converged,
+                      int64_t* tile_visits, int* iters_per_launch, int64_t* shortfalls)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        if (launches) *launches = static_cast<int64_t>(ctx->timings[7]);
+        if (converged) *converged = ctx->fill_converged;
+        if (tile_visits) *tile_visits = static_cast<int64_t>(ctx->timings[8]);
+        if (iters_per_launch) *iters_per_launch = ctx->last_cfg.iters;
+        if (shortfalls) *shortfalls = ctx->fill_shortfalls;
     });
 }
 
@@ -1551,8 +1673,8 @@ int dl_grid_triangulation(dl_context* ctx, const int32_t* voronoi_grid_hw2, int 
             DL_LAUNCH_CHECK();
             clk.mark(1);
             const DetectResult dr = detect_triangles(ctx->plan, ctx->ids.as<uint32_t>(), false,
-                                                     ctx->sorted.as<uint32_t>(), ctx->tri, s, false);
-            clk.mark(2);
+                                                     ctx->sorted.as<uint32_t>(), ctx->tri, s, false,
+                                                     false, [&] { clk.mark(2); });
             clk.mark(3);
             ctx->n_tri = dr.n_tri;
             ctx->plan.img[0].out = ctx->tri.tidgrid.as<int32_t>();
@@ -1717,6 +1839,7 @@ int dl_run_batch(dl_context* ctx, int n, dl_image* images, void* stream)
                                       hipMemcpyDeviceToHost, s));
             }
         }
+        clk.mark(5);
         clk.finish(s);
     });
 }

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -393,9 +394,14 @@ struct DetectResult {
 // triangle counts are then checked by read_status after the assignment.
 // flags_zeroed: the detection status words (work.flags[0 .. 8)) are known to
 // be zero already (normalise_seeds clears them), so no clearing launch.
+// after_detect (optional) runs on the host once the candidate kernels (count,
+// scan, write) are issued and before the registration kernels (insert, mark,
+// scan, emit): the stage clock's boundary between detection and
+// deduplication.
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
                               const uint32_t* sorted, TriWork& work,
-                              hipStream_t s, bool speculative, bool flags_zeroed = false);
+                              hipStream_t s, bool speculative, bool flags_zeroed = false,
+                              const std::function<void()>& after_detect = {});
 
 struct SpeculativeStatus {
     bool bad_id = false;   // a canvas cell without owner / an id out of range

@@ -1162,7 +1162,8 @@ uint64_t next_pow2(uint64_t v)
 
 DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, bool labels,
                               const uint32_t* sorted, TriWork& work,
-                              hipStream_t s, bool speculative, bool flags_zeroed)
+                              hipStream_t s, bool speculative, bool flags_zeroed,
+                              const std::function<void()>& after_detect)
 {
     DetectResult res;
     const int n_img = static_cast<int>(plan.img.size());
@@ -1257,6 +1258,7 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         detect_write_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
     }
+    if (after_detect) after_detect();
     const int64_t nwords = plan.n_key_words;
     work.bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(std::max<int64_t>(nwords, 1)));
     work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords + 1));

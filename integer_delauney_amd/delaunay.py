@@ -29,6 +29,26 @@ from .api import DEFAULT_BORDER_PADDING, _pack, as_seed_array
 from .device_views import DeviceArrayView
 
 
+def _stage_timings(ctx, wall_ms: float) -> dict:
+    """The *_timed dictionary of the context's last timed call (all zero
+    without one)."""
+    d = {"bfs_ms": 0.0, "bfs_iters": 0, "detect_ms": 0.0, "dedup_ms": 0.0, "assign_ms": 0.0,
+         "seeds_ms": 0.0, "output_ms": 0.0, "total_ms": wall_ms, "fill_launches": 0}
+    if ctx is None:
+        return d
+    lib = _lib.load()
+    t = np.zeros(9, dtype=np.float64)
+    _lib.check(lib.dl_get_timings(ctx.handle, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    issued, conv, visits = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    k = ctypes.c_int(0)
+    _lib.check(lib.dl_get_fill_stats(ctx.handle, ctypes.byref(issued), ctypes.byref(conv),
+                                     ctypes.byref(visits), ctypes.byref(k), None))
+    d.update(bfs_ms=float(t[0] + t[1]), bfs_iters=int(conv.value) * int(k.value),
+             detect_ms=float(t[2]), dedup_ms=float(t[3]), assign_ms=float(t[4]),
+             seeds_ms=float(t[0]), output_ms=float(t[5]), fill_launches=int(issued.value))
+    return d
+
+
 def _keys(xy: np.ndarray) -> np.ndarray:
     return (xy[:, 0].astype(np.int64) << 32) | (xy[:, 1].astype(np.int64) & 0xFFFFFFFF)
 
@@ -58,8 +78,6 @@ class Delaunay:
         # Bumped by every call that changes what finalise_device's arrays show
         # (delaunay.cu:287-289, 337-339, 349-351).
         self._generation = 0
-        self._last = {"bfs_ms": 0.0, "bfs_iters": 0, "detect_ms": 0.0, "dedup_ms": 0.0,
-                      "assign_ms": 0.0}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -137,17 +155,25 @@ class Delaunay:
         self._pending = True
         self._generation += 1
 
-    def _recompute(self):
+    def _recompute(self, timed: bool = False):
         """Runs the pipeline on the live seeds; keeps the rows, leaves the grid
-        on the device until asked for."""
-        t0 = time.perf_counter()
+        on the device until asked for.  With ``timed`` the stages are timed
+        with HIP events and the timings dictionary is returned."""
+        ctx = _lib.context()
         n = ctypes.c_int64(0)
-        _lib.check(self._lib.dl_session_finalise(_lib.context().handle, self._handle(),
-                                                 ctypes.byref(n)))
+        t0 = time.perf_counter()
+        if timed:
+            _lib.check(self._lib.dl_set_timing(ctx.handle, 1))
+        try:
+            _lib.check(self._lib.dl_session_finalise(ctx.handle, self._handle(), ctypes.byref(n)))
+        finally:
+            if timed:
+                _lib.check(self._lib.dl_set_timing(ctx.handle, 0))
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        timings = _stage_timings(ctx, wall_ms) if timed else None
         rows = np.empty((int(n.value), 5), dtype=np.int32)
         if n.value:
             _lib.check(self._lib.dl_session_copy_result(self._handle(), None, _lib.i32p(rows)))
-        self._last["bfs_ms"] = (time.perf_counter() - t0) * 1e3
         self._rows = rows
         self._grid = None
         order = np.lexsort((self._xy[:, 1], self._xy[:, 0]))
@@ -155,6 +181,7 @@ class Delaunay:
         rank[order] = np.arange(self.seed_count, dtype=np.int32)
         self._rank = rank
         self._pending = False
+        return timings
 
     def _ensure(self):
         if self._pending or self._rows is None:
@@ -168,12 +195,19 @@ class Delaunay:
             self._grid = grid
         return self._grid
 
+    def _finalise(self, as_arrays: bool, timed: bool):
+        timings = None
+        if self._pending or self._rows is None:
+            timings = self._recompute(timed)
+            self._generation += 1
+        if timed and timings is None:   # nothing pending: no stage ran
+            timings = _stage_timings(None, 0.0)
+        return _pack(self._rows, as_arrays), self._host_grid().copy(), timings
+
     def finalise(self, as_arrays: bool = False):
         """Materialise (triangle_map, (H, W, 3) grid) in sorted ids (delaunay.cu:323-345)."""
-        if self._pending or self._rows is None:
-            self._recompute()
-            self._generation += 1
-        return _pack(self._rows, as_arrays), self._host_grid().copy()
+        m, g, _ = self._finalise(as_arrays, False)
+        return m, g
 
     def finalise_device(self):
         """As finalise(as_arrays=True), with the per-pixel result left on the
@@ -199,20 +233,27 @@ class Delaunay:
         self.insert_deferred(seeds)
         return self.finalise(as_arrays)
 
-    def _timings(self):
-        return dict(self._last)
-
+    # The timings dictionaries of py_delaunay.hpp:268-277 (delaunay.cuh:8-14):
+    # bfs_ms (seed ordering + flood fill), bfs_iters, detect_ms, dedup_ms
+    # (registration and tid ranking) and assign_ms, from HIP events on the
+    # context's stream, plus seeds_ms, output_ms, total_ms and fill_launches.
     def insert_timed(self, seeds, as_arrays: bool = False):
-        m, g = self.insert(seeds, as_arrays)
-        return m, g, self._timings()
+        self.insert_deferred(seeds)
+        return self._finalise(as_arrays, True)
 
     def insert_deferred_timed(self, seeds):
+        """insert_deferred with its timings.  Here a deferred insert only
+        updates the live seed set on the device (validation, presence bits,
+        list); the flood fill, detection and assignment all run at finalise,
+        so those keys are 0 and ``update_ms`` holds the update's own time."""
+        t0 = time.perf_counter()
         self.insert_deferred(seeds)
-        return {"bfs_ms": 0.0, "bfs_iters": 0, "detect_ms": 0.0, "dedup_ms": 0.0, "assign_ms": 0.0}
+        t = _stage_timings(None, 0.0)
+        t["update_ms"] = t["total_ms"] = (time.perf_counter() - t0) * 1e3
+        return t
 
     def finalise_timed(self, as_arrays: bool = False):
-        m, g = self.finalise(as_arrays)
-        return m, g, self._timings()
+        return self._finalise(as_arrays, True)
 
     # -- topology and seeds --------------------------------------------------
     @property

@@ -41,12 +41,20 @@
  *   orc_assign_*       brute-force definition test_pixel_assignment_correctness
  *                      .py:16-32 and golden vectors
  *
+ * Threads: the flood-fill passes (rows / column strips) and the raster (row
+ * bands) run under OpenMP; orc_set_threads sets the count.  The result does
+ * not depend on it (each pass is a Jacobi sweep along one axis, and raster
+ * bands are disjoint).
+ *
  * Integer arithmetic: squared distances are int32 exactly as NumPy computes
  * them (the caller keeps (W-1)^2+(H-1)^2 < 2^31).  The containment predicate
  * uses int64; the reference evaluates it in float64 on integer coordinates
  * below 2^26, where float64 is exact, so the two agree bit for bit.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -155,33 +163,50 @@ int orc_voronoi(int W, int H, const int32_t* sorted_xy, int64_t n,
     }
     int64_t iters = 0;
     W_cur = W;
+    /* Each pass is a Jacobi sweep along one axis: a row (x passes) or a
+     * column (y passes) depends on nothing outside itself within the pass, so
+     * rows / column strips run on separate threads with the same result as
+     * the sequential visit order.  The watched variant counts crossings in a
+     * shared counter and stays on one thread. */
+    const int par = g_watch_P < 0;
+    const int strips = (W + 63) / 64;
     for (;;) {
         int changed = 0;
         ++iters;
         /* source x-1 (voronoi.py:157-160) */
+#pragma omp parallel for schedule(static) reduction(|:changed) if (par)
         for (int y = 0; y < H; ++y)
             for (int x = W - 1; x >= 1; --x) {
                 const size_t c = (size_t)y * W + x;
                 changed |= relax(&s, c, c - 1, x, y);
             }
         /* source x+1 (voronoi.py:161-164) */
+#pragma omp parallel for schedule(static) reduction(|:changed) if (par)
         for (int y = 0; y < H; ++y)
             for (int x = 0; x + 1 < W; ++x) {
                 const size_t c = (size_t)y * W + x;
                 changed |= relax(&s, c, c + 1, x, y);
             }
         /* source y-1 (voronoi.py:165-168) */
-        for (int y = H - 1; y >= 1; --y)
-            for (int x = 0; x < W; ++x) {
-                const size_t c = (size_t)y * W + x;
-                changed |= relax(&s, c, c - (size_t)W, x, y);
-            }
+#pragma omp parallel for schedule(static) reduction(|:changed) if (par)
+        for (int st = 0; st < strips; ++st) {
+            const int x1 = st * 64 + 64 < W ? st * 64 + 64 : W;
+            for (int y = H - 1; y >= 1; --y)
+                for (int x = st * 64; x < x1; ++x) {
+                    const size_t c = (size_t)y * W + x;
+                    changed |= relax(&s, c, c - (size_t)W, x, y);
+                }
+        }
         /* source y+1 (voronoi.py:169-172) */
-        for (int y = 0; y + 1 < H; ++y)
-            for (int x = 0; x < W; ++x) {
-                const size_t c = (size_t)y * W + x;
-                changed |= relax(&s, c, c + (size_t)W, x, y);
-            }
+#pragma omp parallel for schedule(static) reduction(|:changed) if (par)
+        for (int st = 0; st < strips; ++st) {
+            const int x1 = st * 64 + 64 < W ? st * 64 + 64 : W;
+            for (int y = 0; y + 1 < H; ++y)
+                for (int x = st * 64; x < x1; ++x) {
+                    const size_t c = (size_t)y * W + x;
+                    changed |= relax(&s, c, c + (size_t)W, x, y);
+                }
+        }
         if (!changed) break;
     }
     for (size_t i = 0; i < N; ++i) {
@@ -475,6 +500,17 @@ void orc_assign_raster(int W, int H, const int32_t* seeds_xy, const int32_t* tri
                        int64_t T, int32_t* out_tid)
 {
     for (size_t i = 0; i < (size_t)W * H; ++i) out_tid[i] = -1;
+    /* Each thread owns a band of rows and writes only there; triangles are
+     * visited in tid order within every band, so the max is the same as in a
+     * sequential visit. */
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+    const int nthr = omp_get_num_threads(), me = omp_get_thread_num();
+#else
+    const int nthr = 1, me = 0;
+#endif
+    const int64_t band_lo = (int64_t)H * me / nthr, band_hi = (int64_t)H * (me + 1) / nthr;
     for (int64_t t = 0; t < T; ++t) {
         const int32_t* r = tris + 5 * t;
         const int32_t* V[3] = {seeds_xy + 2 * r[2], seeds_xy + 2 * r[3], seeds_xy + 2 * r[4]};
@@ -482,7 +518,7 @@ void orc_assign_raster(int W, int H, const int32_t* seeds_xy, const int32_t* tri
         if (S == 0) {
             const int64_t ax = V[0][0], ay = V[0][1];
             const int64_t dx = V[1][0] - ax, dy = V[1][1] - ay;
-            for (int y = 0; y < H; ++y) {
+            for (int64_t y = band_lo; y < band_hi; ++y) {
                 int64_t x;
                 if (dy == 0) {
                     if (y != ay) continue;
@@ -492,7 +528,7 @@ void orc_assign_raster(int W, int H, const int32_t* seeds_xy, const int32_t* tri
                     }
                     continue;
                 }
-                int64_t num = (int64_t)(y - ay) * dx;
+                int64_t num = (y - ay) * dx;
                 if (num % dy != 0) continue;
                 x = ax + num / dy;
                 if (x < 0 || x >= W) continue;
@@ -507,8 +543,8 @@ void orc_assign_raster(int W, int H, const int32_t* seeds_xy, const int32_t* tri
             if (V[k][1] < ymin) ymin = V[k][1];
             if (V[k][1] > ymax) ymax = V[k][1];
         }
-        if (ymin < 0) ymin = 0;
-        if (ymax > H - 1) ymax = H - 1;
+        if (ymin < band_lo) ymin = band_lo;
+        if (ymax > band_hi - 1) ymax = band_hi - 1;
         for (int64_t y = ymin; y <= ymax; ++y) {
             int64_t lo = 0, hi = W - 1;
             for (int e = 0; e < 3; ++e) {
@@ -533,4 +569,20 @@ void orc_assign_raster(int W, int H, const int32_t* seeds_xy, const int32_t* tri
             }
         }
     }
+    }
+}
+
+/* Thread count of the parallel parts (flood-fill passes, raster bands);
+ * n <= 0 restores the OpenMP default.  Returns the count now in use. */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    static int dflt = 0;
+    if (!dflt) dflt = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : dflt);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

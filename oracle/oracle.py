@@ -61,12 +61,21 @@ def _load():
     lib.orc_voronoi_watch.restype = ctypes.c_int
     lib.orc_free.argtypes = [ctypes.c_void_p]
     lib.orc_free.restype = None
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_set_threads.restype = ctypes.c_int
     for name in ("orc_assign_brute", "orc_assign_raster"):
         fn = getattr(lib, name)
         fn.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, ctypes.c_int64, _I32P]
         fn.restype = None
     _lib = lib
     return lib
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's parallel parts (flood-fill passes, raster row
+    bands; results do not depend on it).  n <= 0 restores the OpenMP default
+    (OMP_NUM_THREADS, else all cores).  Returns the count now in use."""
+    return int(_load().orc_set_threads(int(n)))
 
 
 def _p(a: np.ndarray):

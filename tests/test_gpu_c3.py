@@ -1,0 +1,220 @@
+"""The clustered incremental workload: seeds from a Gaussian mixture, then
+rounds of inserts and deletes against a resident device session.
+
+The reference's session semantics are insert_deferred / finalise
+(src/cuda/delaunay.cu:184-372); its incremental test holds each round to the
+batch result (tests/test_incremental_cuda.py:232-313).  Here each finalise
+must equal the batch pipeline on the live seeds, bit for bit.
+
+* At 1536 x 1536 (16 components, sigma 20 px -- the 16384^2 workload's
+  sigma 200 px scaled to the canvas -- 20,000 seeds, ten rounds of 1,000
+  inserts and 500 deletes) every round is compared with the oracle.
+* At the full size (16384 x 16384, 2,000,000 seeds, 64 components, sigma
+  200 px, two rounds of 10,000 inserts and 5,000 deletes; a canvas of
+  16416^2 = 269,485,056 cells, above 2^28) the oracle's flood fill would
+  take hours, so the fill is checked by what the mathematics fixes: every
+  cell is defined, its d^2 is the squared distance to its labelled seed, the
+  grid is a fixed point of every one of the four directional passes (the
+  reference stops only after an iteration in which no pass changed a cell,
+  voronoi.py:110-163), and d^2 is at least the exact nearest-seed distance.
+  Detection and assignment are then checked bit for bit with the oracle on
+  the GPU's own padded canvas, and the session's result with the batch
+  path's.
+"""
+import ctypes
+import time
+
+import numpy as np
+import pytest
+
+import integer_delauney_amd as dl
+from integer_delauney_amd import _lib
+from integer_delauney_amd.workloads import clustered_more, clustered_seeds
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _rows(m):
+    return np.array([m[i] for i in range(len(m))], dtype=np.int32).reshape(-1, 5)
+
+
+def _keys(xy, H):
+    return xy[:, 0].astype(np.int64) * H + xy[:, 1]
+
+
+def _round_plan(W, H, live, n_ins, n_del, rng, seed, components, sigma, centres_seed):
+    ins = clustered_more(W, H, live, n_ins, seed, components=components, sigma=sigma,
+                         centres_seed=centres_seed)
+    live = np.concatenate([live, ins])
+    drop = rng.choice(live.shape[0], size=n_del, replace=False)
+    dele = np.ascontiguousarray(live[drop])
+    keep = np.ones(live.shape[0], dtype=bool)
+    keep[drop] = False
+    return ins, dele, np.ascontiguousarray(live[keep])
+
+
+def test_clustered_session_rounds_equal_oracle():
+    W = H = 1536
+    comps, sigma = 16, 20.0
+    O.set_threads(0)
+    seeds = clustered_seeds(W, H, 20_000, 3, components=comps, sigma=sigma)
+    rng = np.random.default_rng(30)
+    s = dl.Delaunay(W, H, 40_000, 16)
+    s.insert_deferred(seeds)
+    live = seeds
+    for r in range(10):
+        ins, dele, live = _round_plan(W, H, live, 1_000, 500, rng, 100 + r, comps, sigma, 3)
+        s.insert_deferred(ins)
+        s.remove(dele)
+        assert s.seed_count == live.shape[0]
+        t0 = time.perf_counter()
+        m, grid = s.finalise()
+        t_gpu = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref_rows, ref_grid = O.triangulate(W, H, live, 16, return_rows=True)
+        _say(f"round {r}: {live.shape[0]} seeds, {len(ref_rows)} triangles, "
+             f"gpu {t_gpu * 1e3:.1f} ms, oracle {time.perf_counter() - t0:.1f} s")
+        np.testing.assert_array_equal(grid, ref_grid, err_msg=f"round {r}")
+        np.testing.assert_array_equal(_rows(m), ref_rows, err_msg=f"round {r}")
+        np.testing.assert_array_equal(np.sort(_keys(s.get_seeds(), H)), np.sort(_keys(live, H)))
+
+
+# --- full size -----------------------------------------------------------------
+
+def _check_fill(ids, d2, sx, sy, off, name, rows_per_chunk=512):
+    """ids, d2: (h, w) owner ids and squared distances of a flood fill whose
+    seeds sit at (sx + off, sy + off) in canvas coordinates.  Asserts every
+    cell is defined, d2 is the distance to the owner, and no cell would change
+    in any of the four passes (a candidate from a neighbour is never strictly
+    closer, nor equally close with a higher id)."""
+    h, w = ids.shape
+    n = sx.shape[0]
+    X = np.arange(w, dtype=np.int64)[None, :]
+    for y0 in range(0, h, rows_per_chunk):
+        y1 = min(h, y0 + rows_per_chunk)
+        Y = np.arange(y0, y1, dtype=np.int64)[:, None]
+        idc = ids[y0:y1]
+        assert idc.min() >= 0 and idc.max() < n, f"{name}: undefined cell in rows {y0}-{y1}"
+        ox = sx[idc] + off
+        oy = sy[idc] + off
+        dd = (X - ox) ** 2 + (Y - oy) ** 2
+        np.testing.assert_array_equal(d2[y0:y1], dd, err_msg=f"{name}: d^2 rows {y0}-{y1}")
+        # neighbours along x (passes 1 and 2)
+        for a, b in ((slice(1, None), slice(None, -1)), (slice(None, -1), slice(1, None))):
+            nid = idc[:, b]
+            cand = (X[:, a] - sx[nid] - off) ** 2 + (Y - sy[nid] - off) ** 2
+            cur = dd[:, a]
+            bad = (cand < cur) | ((cand == cur) & (nid > idc[:, a]))
+            assert not bad.any(), f"{name}: an x pass would change rows {y0}-{y1}"
+        # neighbours along y (passes 3 and 4), rows y0-1 and y1 included
+        lo, hi = max(0, y0 - 1), min(h, y1 + 1)
+        ext = ids[lo:hi]
+        for dy in (-1, 1):
+            ys = np.arange(y0, y1)
+            ok = (ys + dy >= 0) & (ys + dy < h)
+            ys = ys[ok]
+            nid = ext[ys + dy - lo]
+            cur_id = ids[ys]
+            Yv = ys.astype(np.int64)[:, None]
+            cand = (X - sx[nid] - off) ** 2 + (Yv - sy[nid] - off) ** 2
+            cur = dd[ys - y0]
+            bad = (cand < cur) | ((cand == cur) & (nid > cur_id))
+            assert not bad.any(), f"{name}: a y pass would change rows {y0}-{y1}"
+
+
+def _triangulate_c_abi(W, H, seeds, P):
+    """triangulate through the C ABI: grid, rows and the padded canvas."""
+    lib = _lib.load()
+    ctx = _lib.context()
+    grid = np.empty((H, W, 3), dtype=np.int32)
+    n = ctypes.c_int64(0)
+    _lib.check(lib.dl_triangulate(ctx.handle, W, H, _lib.i32p(seeds), seeds.shape[0], P,
+                                  _lib.i32p(grid), ctypes.byref(n)))
+    rows = np.empty((int(n.value), 5), dtype=np.int32)
+    _lib.check(lib.dl_copy_triangles(ctx.handle, _lib.i32p(rows)))
+    w, h = ctypes.c_int(0), ctypes.c_int(0)
+    _lib.check(lib.dl_copy_padded_voronoi(ctx.handle, None, ctypes.byref(w), ctypes.byref(h)))
+    pad = np.empty((h.value, w.value, 2), dtype=np.int32)
+    _lib.check(lib.dl_copy_padded_voronoi(ctx.handle, _lib.i32p(pad), ctypes.byref(w),
+                                          ctypes.byref(h)))
+    return grid, rows, pad
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(1200)
+def test_clustered_16384_2m_full_size():
+    W = H = 16384
+    P = 16
+    O.set_threads(0)
+    t0 = time.perf_counter()
+    seeds0 = clustered_seeds(W, H, 2_000_000, 3)
+    rng = np.random.default_rng(3)
+    plan = []
+    live = seeds0
+    for r in range(2):
+        ins, dele, live = _round_plan(W, H, live, 10_000, 5_000, rng, 1000 + r, 64, 200.0, 3)
+        plan.append((ins, dele))
+    _say(f"seeds generated in {time.perf_counter() - t0:.1f} s")
+
+    # the session: initial build, then two rounds of updates
+    s = dl.Delaunay(W, H, 2_020_000, P)
+    s.insert_deferred(seeds0)
+    s.finalise(as_arrays=True)
+    for ins, dele in plan:
+        s.insert_deferred(ins)
+        s.remove(dele)
+    t0 = time.perf_counter()
+    verts_s, grid_s = s.finalise(as_arrays=True)
+    _say(f"session round finalised in {time.perf_counter() - t0:.2f} s")
+    assert s.seed_count == live.shape[0]
+    s.close()
+
+    # the batch path on the live seeds
+    t0 = time.perf_counter()
+    grid, rows, pad = _triangulate_c_abi(W, H, live, P)
+    _say(f"batch triangulate + copies {time.perf_counter() - t0:.1f} s, {len(rows)} triangles")
+    np.testing.assert_array_equal(grid_s, grid)
+    np.testing.assert_array_equal(verts_s, rows[:, 2:5])
+    del grid_s, verts_s
+    assert pad.shape == (H + 2 * P, W + 2 * P, 2)
+
+    srt = O.sort_seeds(W, H, live)
+    sx = srt[:, 0].astype(np.int64)
+    sy = srt[:, 1].astype(np.int64)
+    t0 = time.perf_counter()
+    _check_fill(np.ascontiguousarray(grid[:, :, 0]), np.ascontiguousarray(grid[:, :, 1]),
+                sx, sy, 0, "image fill")
+    _say(f"image fill invariants {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    _check_fill(np.ascontiguousarray(pad[:, :, 0]), np.ascontiguousarray(pad[:, :, 1]),
+                sx, sy, P, "padded fill")
+    _say(f"padded fill invariants {time.perf_counter() - t0:.1f} s")
+
+    # d^2 >= the exact nearest-seed distance, on sampled rows
+    spatial = pytest.importorskip("scipy.spatial")
+    tree = spatial.cKDTree(np.stack([sx, sy], 1).astype(np.float64))
+    ys = np.random.default_rng(7).choice(H, size=48, replace=False)
+    xs = np.arange(W)
+    q = np.stack([np.tile(xs, ys.size), np.repeat(ys, W)], 1)
+    _, nn = tree.query(q.astype(np.float64))
+    near = (q[:, 0] - sx[nn]) ** 2 + (q[:, 1] - sy[nn]) ** 2
+    got = grid[ys][:, :, 1].reshape(-1).astype(np.int64)
+    assert np.all(got >= near)
+    _say(f"sampled rows: {np.mean(got == near) * 100:.4f} % of cells at the nearest distance")
+
+    # detection on the GPU's padded canvas and assignment, bit for bit
+    t0 = time.perf_counter()
+    ref_rows, _ = O.detect(pad[:, :, 0], srt, P)
+    _say(f"oracle detection {time.perf_counter() - t0:.1f} s")
+    np.testing.assert_array_equal(rows, ref_rows)
+    t0 = time.perf_counter()
+    tid = O.assign(W, H, srt, ref_rows, "raster")
+    _say(f"oracle raster {time.perf_counter() - t0:.1f} s")
+    np.testing.assert_array_equal(grid[:, :, 2], tid)
+    # give back the default context's 16384^2 workspaces
+    _lib.check(_lib.load().dl_context_trim(_lib.context().handle))

@@ -183,6 +183,8 @@ def test_graph_dropped_after_reallocation():
     earlier signature is dropped (its addresses are gone) and the next calls
     recapture.  Every call equals the oracle."""
     ctx = _lib.context()
+    # start from empty workspaces, whatever earlier tests grew them to
+    _lib.check(_lib.load().dl_context_trim(ctx.handle))
     W, H, n = 200, 160, 300
     for i in range(4):
         s = uniform_seeds(W, H, n, 60 + i)

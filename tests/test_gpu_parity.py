@@ -188,8 +188,86 @@ def test_non_positive_dimensions(W, H):
 
 
 def test_oversized_canvas_rejected():
-    with pytest.raises(ValueError):
+    # A canvas the library cannot hold is a RuntimeError, as the reference's
+    # failing allocation of one is (tests/test_cuda_error_checking.py:36-46).
+    with pytest.raises(RuntimeError, match="exceeds"):
         dl.triangulate(32000, 10, [(0, 0), (5, 5), (9, 1)])
+    with pytest.raises(RuntimeError):
+        dl.Delaunay(200_000, 200_000, 1_000, 0)
+
+
+def test_device_allocation_failure_is_a_runtime_error_and_recoverable():
+    """A real hipMalloc failure (a seed list of 2^40 entries, 8 TiB) raises
+    RuntimeError naming hipMalloc, leaks nothing that breaks later work, and
+    the process and a new session work afterwards
+    (tests/test_cuda_error_checking.py:28-58)."""
+    for _ in range(2):
+        with pytest.raises(RuntimeError, match="hipMalloc") as ei:
+            dl.Delaunay(64, 64, 1 << 40, 0)
+        assert "dl_api.hip" in str(ei.value)
+    inc = dl.Delaunay(64, 64, 256, 0)
+    tri_map, tgrid = inc.insert([(10, 10), (50, 10), (30, 50)])
+    assert len(tri_map) > 0 and tgrid.shape == (64, 64, 3)
+    rows, ref = O.triangulate(64, 64, [(10, 10), (50, 10), (30, 50)], 0, return_rows=True)
+    np.testing.assert_array_equal(tgrid, ref)
+    m, g = dl.triangulate(64, 64, [(10, 10), (50, 10), (30, 50)], 0)
+    np.testing.assert_array_equal(g, ref)
+
+
+@pytest.mark.parametrize("P", [-5, -2, -1000])
+def test_negative_padding_clamps_to_zero(P):
+    """border_padding below -1 clamps to 0 (triangulation.py:75-78; the
+    reference test tests/test_border_padding.py:91-96), -1 selects the
+    default; a session takes the default for any negative value
+    (delaunay.cu:72)."""
+    rng = np.random.default_rng(5000 - P)
+    for W, H, n in [(96, 64, 20), (200, 150, 300), (31, 257, 60)]:
+        idx = rng.choice(W * H, size=n, replace=False)
+        seeds = np.stack([idx % W, idx // W], 1).astype(np.int32)
+        rows0, ref0 = O.triangulate(W, H, seeds, 0, return_rows=True)
+        m, got = dl.triangulate(W, H, seeds, P)
+        np.testing.assert_array_equal(got, ref0)
+        np.testing.assert_array_equal(_rows(m), rows0)
+        vg = dl.Voronoi().compute(W, H, seeds)
+        a, tga = dl.GridTriangulation().compute(vg, seeds, border_padding=P)
+        b, tgb = dl.GridTriangulation().compute(vg, seeds, border_padding=0)
+        assert a == b and np.array_equal(tga, tgb) and len(a) == len(rows0)
+    s = dl.Delaunay(96, 64, 100, border_padding=P)
+    assert s.border_padding == dl.DEFAULT_BORDER_PADDING
+
+
+def test_timed_dictionaries():
+    """compute_timed and the session's *_timed methods carry the reference's
+    keys (py_grid_triangulation.hpp:39-46, py_delaunay.hpp:268-277) with
+    stage times measured on the device, and leave the results unchanged."""
+    rng = np.random.default_rng(60)
+    W, H = 640, 480
+    idx = rng.choice(W * H, size=4000, replace=False)
+    seeds = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    rows, ref = O.triangulate(W, H, seeds, None, return_rows=True)
+    vg = dl.Voronoi().compute(W, H, seeds)
+    m, tg, t = dl.GridTriangulation().compute_timed(vg, seeds)
+    assert set(t) >= {"detect_ms", "dedup_ms", "assign_ms"}
+    assert all(t[k] > 0 for k in ("detect_ms", "dedup_ms", "assign_ms"))
+    np.testing.assert_array_equal(tg, ref)
+    s = dl.Delaunay(W, H, 10_000)
+    t1 = s.insert_deferred_timed(seeds[:2000])
+    assert set(t1) >= {"bfs_ms", "bfs_iters", "detect_ms", "dedup_ms", "assign_ms"}
+    assert t1["update_ms"] > 0
+    m1, g1, t2 = s.insert_timed(seeds[2000:])
+    keys = ("bfs_ms", "detect_ms", "dedup_ms", "assign_ms")
+    assert all(t2[k] > 0 for k in keys), t2
+    assert t2["bfs_iters"] > 0 and t2["fill_launches"] > 0
+    assert sum(t2[k] for k in keys) <= t2["total_ms"] + 1e-3
+    np.testing.assert_array_equal(g1, ref)
+    np.testing.assert_array_equal(_rows(m1), rows)
+    s.remove(seeds[:10])
+    m2, g2, t3 = s.finalise_timed()
+    assert all(t3[k] > 0 for k in keys)
+    rows2, ref2 = O.triangulate(W, H, seeds[10:], None, return_rows=True)
+    np.testing.assert_array_equal(g2, ref2)
+    _, _, t4 = s.finalise_timed()   # nothing pending: nothing ran
+    assert t4["bfs_iters"] == 0 and t4["bfs_ms"] == 0.0
 
 
 def test_reusing_the_context_across_sizes():
