@@ -321,6 +321,27 @@ int dl_triangulate_device(dl_context* ctx, int width, int height, const int32_t*
                           int64_t n, int border_padding, int32_t* d_out_hw3, void* stream,
                           int64_t* n_tri);
 
+/* dl_triangulate_device that allocates nothing: no hipMalloc, hipHostMalloc,
+ * hipFree, or event or stream creation, for callers that cannot allocate
+ * (tight serving loops, code that must not touch the allocator).  No
+ * counterpart in the reference.  Its workspaces are carved from one static
+ * device array of DL_NOALLOC_SCRATCH_BYTES per GPU (held from the library's
+ * load on that GPU until the process ends; its address is looked up once per
+ * device) and a static host block.  Limit: images for which
+ * 10 * (W + 2P)(H + 2P) + 13 * W * H + 600 * n + 2 MiB <= DL_NOALLOC_SCRATCH_BYTES
+ * (e.g. 1024 x 1024 at padding 16 with up to 60,000 seeds); larger inputs are
+ * DL_EVALUE before anything runs.  The grid goes to d_out_hw3 and the
+ * (n_tri, 5) rows to d_rows (capacity rows_cap rows; NULL: not copied;
+ * DL_EVALUE if too small, with *n_tri set), both device memory.  The call
+ * waits on the stream before it returns.  Every call on a GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (another thread, another
+ * stream) race.  Results are those of dl_triangulate_device. */
+#define DL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dl_triangulate_device_noalloc(dl_context* ctx, int width, int height,
+                                  const int32_t* d_seeds_xy, int64_t n, int border_padding,
+                                  int32_t* d_out_hw3, int32_t* d_rows, int64_t rows_cap,
+                                  void* stream, int64_t* n_tri);
+
 /* Device pointer to the (n_tri, 5) rows of the last triangulation. */
 const int32_t* dl_device_triangles(dl_context* ctx);
 

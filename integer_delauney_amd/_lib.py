@@ -64,6 +64,9 @@ _SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_int)]),
     "dl_triangulate_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP,
                                              ctypes.c_int64, ctypes.c_int, _VP, _VP, _I64P]),
+    "dl_triangulate_device_noalloc": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP,
+                                                     ctypes.c_int64, ctypes.c_int, _VP, _VP,
+                                                     ctypes.c_int64, _VP, _I64P]),
     "dl_device_triangles": (_VP, [_VP]),
     "dl_voronoi_simple": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                          _I32P, _I64P]),

@@ -42,11 +42,25 @@ void note_realloc()
     if (g_capture_log) throw capture_abort("buffer reallocation during stream capture");
     g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
 }
+
+thread_local Arena* g_arena = nullptr;
+
+// Bytes [used, used + n) of an arena region, 256-byte aligned.
+void* carve(char* base, size_t cap, size_t& used, size_t n, const char* what)
+{
+    const size_t at = (used + 255) & ~size_t(255);
+    if (at > cap || n > cap - at)
+        throw runtime_error(std::string("the no-allocation scratch is exhausted (") + what + ")");
+    used = at + n;
+    return base + at;
+}
 }  // namespace
+
+Arena*& current_arena() { return g_arena; }
 
 DevBuf::~DevBuf()
 {
-    if (!ptr) return;
+    if (!ptr || arena) return;
     // No throw from a destructor: a capture that outlives a buffer is dropped.
     if (g_capture_log) g_capture_log->poisoned = true;
     g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
@@ -84,17 +98,26 @@ void copy_back_async(void* host, const void* dev, size_t n, hipStream_t s)
 
 void DevBuf::release()
 {
-    if (ptr) {
+    if (ptr && !arena) {
         note_realloc();
         (void)hipFree(ptr);
     }
     ptr = nullptr;
     bytes = 0;
+    arena = false;
 }
 
 void DevBuf::reserve(size_t n)
 {
     if (n <= bytes && ptr) return;
+    if (g_arena) {
+        if (ptr && !arena) throw runtime_error("an allocated buffer grew inside an arena");
+        const size_t want = std::max<size_t>(n, 256);
+        ptr = carve(g_arena->dev, g_arena->dev_cap, g_arena->dev_used, want, "device");
+        bytes = want;
+        arena = true;
+        return;
+    }
     if (g_capture_log) throw capture_abort("buffer growth during stream capture");
     release();
     g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
@@ -114,7 +137,7 @@ void DevBuf::reserve(size_t n)
 
 PinnedBuf::~PinnedBuf()
 {
-    if (ptr) {
+    if (ptr && !arena) {
         if (g_capture_log) g_capture_log->poisoned = true;
         g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
         (void)hipHostFree(ptr);
@@ -124,6 +147,14 @@ PinnedBuf::~PinnedBuf()
 void PinnedBuf::reserve(size_t n)
 {
     if (n <= bytes && ptr) return;
+    if (g_arena) {
+        if (ptr && !arena) throw runtime_error("an allocated buffer grew inside an arena");
+        const size_t want = std::max<size_t>(n, 256);
+        ptr = carve(g_arena->host, g_arena->host_cap, g_arena->host_used, want, "host");
+        bytes = want;
+        arena = true;
+        return;
+    }
     note_realloc();
     if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
@@ -209,6 +240,9 @@ __global__ void extract_ids_kernel(const int32_t* __restrict__ hw2, uint32_t* __
 
 }  // namespace
 }  // namespace dl
+
+// Device scratch of dl_triangulate_device_noalloc (see g_scratch_dev).
+static __device__ __align__(256) unsigned char scratch[DL_NOALLOC_SCRATCH_BYTES];
 
 using namespace dl;
 
@@ -1220,6 +1254,55 @@ void require_values(const dl_session* s, const char* what)
                             " needs a value per seed; pass values to insert_deferred");
 }
 
+// ---------------------------------------------------------------------------
+// The no-allocation entry point (dl_triangulate_device_noalloc).  Its device
+// workspace is one static array per GPU, `scratch`; host code never uses the
+// array's name as an address (in host code it is only a stand-in for the
+// symbol): the device address comes from hipGetSymbolAddress, once per device
+// with that device current, and is kept in g_scratch_dev.  A call carves every
+// workspace of a call-local context from that address (an Arena), and its
+// small status read-backs from a static host block per device, so it calls no
+// hipMalloc, hipHostMalloc, hipFree or event / stream creation.  The
+// pipeline clears everything it reads before reading it, so the previous
+// call's contents do not matter; two calls that overlap on one GPU share the
+// array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = DL_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kNoAllocHostBytes = size_t(1) << 16;
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_dev[kNoAllocDevices] = {};
+alignas(256) char g_noalloc_host[kNoAllocDevices][kNoAllocHostBytes];
+
+// An upper bound on the bytes one image's pipeline carves: per padded canvas
+// cell the two label planes (8 B), the registration bitmap and its scan
+// (1 B) and the detection counts; per pixel the seed index, the bins' fixed
+// slices (at most 4 B) and the planes of the unpadded fill run after a
+// padding crossing (8 B); per seed the candidate, hash-table and row
+// capacities (about 560 B); and the capacities' fixed terms.
+size_t noalloc_bound(int W, int H, int P, int64_t n)
+{
+    const size_t C = static_cast<size_t>(W + 2 * P) * static_cast<size_t>(H + 2 * P);
+    const size_t px = static_cast<size_t>(W) * static_cast<size_t>(H);
+    return 10 * C + 13 * px + 600 * static_cast<size_t>(n) + (size_t(2) << 20);
+}
+
+// The device address of `scratch` on the current device (set up on first use).
+char* scratch_on(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices)
+        throw runtime_error("device index beyond the no-allocation scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) throw_hip(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        if (!p) throw runtime_error("hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_dev[device] = static_cast<char*>(p);
+    }
+    return g_scratch_dev[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1587,6 +1670,68 @@ int dl_triangulate_device(dl_context* ctx, int width, int height, const int32_t*
         run_pipeline(ctx, d_seeds_xy, d_out_hw3, 3, s, clk);
         clk.finish(s);
         if (n_tri) *n_tri = ctx->n_tri;
+    });
+}
+
+int dl_triangulate_device_noalloc(dl_context* ctx, int width, int height,
+                                  const int32_t* d_seeds_xy, int64_t n, int border_padding,
+                                  int32_t* d_out_hw3, int32_t* d_rows, int64_t rows_cap,
+                                  void* stream, int64_t* n_tri)
+{
+    return guarded([&] {
+        if (!ctx || !d_out_hw3) throw value_error("null argument");
+        const int P = resolve_padding(border_padding);
+        check_dims(width, height, P);
+        check_seeds(d_seeds_xy, n);
+        if (rows_cap < 0 || (rows_cap > 0 && !d_rows)) throw value_error("bad rows buffer");
+        const size_t need = noalloc_bound(width, height, P, n);
+        if (need > kNoAllocBytes)
+            throw value_error("a " + std::to_string(width) + "x" + std::to_string(height) +
+                              " image with " + std::to_string(n) + " seeds at padding " +
+                              std::to_string(P) + " may need " + std::to_string(need) +
+                              " bytes of scratch; the no-allocation entry point has " +
+                              std::to_string(kNoAllocBytes) +
+                              " (10 per canvas cell + 13 per pixel + 600 per seed + 2 MiB)");
+        DL_HIP(hipSetDevice(ctx->device));
+        char* dev = scratch_on(ctx->device);
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        Arena arena;
+        arena.dev = dev;
+        arena.dev_cap = kNoAllocBytes;
+        arena.host = g_noalloc_host[ctx->device];
+        arena.host_cap = kNoAllocHostBytes;
+        struct Install {
+            explicit Install(Arena* a) { current_arena() = a; }
+            ~Install() { current_arena() = nullptr; }
+        } install(&arena);
+        // A call-local context: every workspace it reserves is carved from
+        // the arena; no graphs, prediction, tuning or timing (they keep state
+        // or create events).
+        dl_context nc;
+        nc.device = ctx->device;
+        nc.stream = s;
+        nc.graph_mode = 0;
+        nc.predict = false;
+        nc.autotune = false;
+        nc.bfs.use_events = false;
+        try {
+            nc.host_small.reserve(4096);
+            StageClock clk(&nc);
+            plan_single(&nc, width, height, P, n);
+            run_pipeline(&nc, d_seeds_xy, d_out_hw3, 3, s, clk);
+            if (n_tri) *n_tri = nc.n_tri;
+            if (d_rows && nc.n_tri > rows_cap)
+                throw value_error("rows_cap " + std::to_string(rows_cap) + " is below the " +
+                                  std::to_string(nc.n_tri) + " triangles");
+            if (d_rows && nc.n_tri)
+                DL_HIP(hipMemcpyAsync(d_rows, nc.tri.rows.ptr,
+                                      sizeof(int32_t) * 5 * static_cast<size_t>(nc.n_tri),
+                                      hipMemcpyDeviceToDevice, s));
+            DL_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);   // nothing may still use the scratch
+            throw;
+        }
     });
 }
 

@@ -824,9 +824,13 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     zero_ints_kernel<<<1, 64, 0, s>>>(dcnt, 2 * kSlots);
     DL_LAUNCH_CHECK();
 
-    hipEvent_t ev[2];
-    DL_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    DL_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    // Without events (work.use_events false) a chunk's counters are waited for
+    // with a stream synchronisation, which also waits for the chunk behind it.
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (work.use_events) {
+        DL_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        DL_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
 
     const int max_launches = static_cast<int>((max_iters + cfg.iters - 1) / cfg.iters) + 2;
     int launch = start_launch;
@@ -846,14 +850,17 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         }
         DL_HIP(hipMemcpyAsync(hcnt + 2 * slot0, dcnt + 2 * slot0, sizeof(int) * 2 * kChunk,
                               hipMemcpyDeviceToHost, s));
-        DL_HIP(hipEventRecord(ev[ch % 2], s));
+        if (ev[ch % 2]) DL_HIP(hipEventRecord(ev[ch % 2], s));
     };
 
     issue_chunk(chunk++);
     pending_chunk = 0;
     while (!done) {
         if (launch < max_launches) issue_chunk(chunk++);  // keep one chunk in flight
-        DL_HIP(hipEventSynchronize(ev[pending_chunk % 2]));
+        if (ev[pending_chunk % 2])
+            DL_HIP(hipEventSynchronize(ev[pending_chunk % 2]));
+        else
+            DL_HIP(hipStreamSynchronize(s));
         const int slot0 = (pending_chunk % 2) * kChunk;
         for (int j = 0; j < kChunk; ++j) {
             const int changed = hcnt[2 * (slot0 + j)];
@@ -888,8 +895,8 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         DL_HIP(hipStreamSynchronize(s));
         st.tile_visits = hcnt[2 * kSlots];
     }
-    DL_HIP(hipEventDestroy(ev[0]));
-    DL_HIP(hipEventDestroy(ev[1]));
+    if (ev[0]) DL_HIP(hipEventDestroy(ev[0]));
+    if (ev[1]) DL_HIP(hipEventDestroy(ev[1]));
     st.launches = launch;
     if (stats) *stats = st;
     return final_plane;

@@ -56,6 +56,7 @@ __host__ __device__ inline uint32_t pack_label(int x, int y)
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool arena = false;   // carved from the calling thread's arena (never freed)
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
@@ -68,6 +69,7 @@ struct DevBuf {
 struct PinnedBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool arena = false;   // carved from the calling thread's host arena
     PinnedBuf() = default;
     PinnedBuf(const PinnedBuf&) = delete;
     PinnedBuf& operator=(const PinnedBuf&) = delete;
@@ -75,6 +77,21 @@ struct PinnedBuf {
     void reserve(size_t n);
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
+
+// ---------------------------------------------------------------------------
+// Arenas.  While an arena is installed for the calling thread, every DevBuf /
+// PinnedBuf that grows takes its bytes from the arena's device / host region
+// with a bump pointer (256-byte aligned) instead of allocating, and a carved
+// buffer is never freed.  The no-allocation entry point installs one over its
+// static scratch for the duration of a call; running out is a runtime_error.
+// ---------------------------------------------------------------------------
+struct Arena {
+    char* dev = nullptr;
+    size_t dev_cap = 0, dev_used = 0;
+    char* host = nullptr;
+    size_t host_cap = 0, host_used = 0;
+};
+Arena*& current_arena();
 
 // ---------------------------------------------------------------------------
 // Stream capture support.  A captured graph holds raw device and page-locked
@@ -177,6 +194,9 @@ struct BfsWork {
     PinnedBuf host_canvases;   // staging of the canvas table
     // bfs_launch_fixed: count tile visits (statistics only) into the visit total
     bool count_visits = true;
+    // bfs_run waits for a chunk's counters with an event; false: with a stream
+    // synchronisation (no event objects are created)
+    bool use_events = true;
 };
 
 // Interior tile size for a configuration.
