@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
 """Benchmark of the batch pipeline on one MI355X.
 
-Workload: 4096 x 4096 image, 200,000 distinct uniform seeds (default_rng(2)),
-default border padding 16, full pipeline per step -- seed ordering, the padded
+Workload: 4096 x 4096 image, 200,000 distinct uniform seeds (four seed sets,
+default_rng 2, 12, 22 and 32, rotated across the steps), default border
+padding 16, full pipeline per step -- seed ordering, the padded
 Voronoi flood fill, triangle extraction with first-registration numbering,
 per-pixel triangle assignment and the (H, W, 3) int32 output -- with the seeds
 already resident in HBM and the output left in HBM
@@ -29,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4   # wave64 VALU instructions, G/s
 KINDS = ["fill", "seeds", "detect", "register", "bin", "assign_output", "init"]
 
 
@@ -50,23 +52,56 @@ def kernel_stats(lib, ctx):
     return {KINDS[i]: (int(counts[i]), float(ms[i])) for i in range(7)}
 
 
-def cpu_baseline(side: int):
-    """The C oracle (1 thread) on a side x side image at the same seed density."""
-    import numpy as np
-
+def _oracle_rate(side: int, n: int, rng: int, threads: int, min_seconds: float):
+    """Mpixel/s of the C oracle's full pipeline on a side x side image with n
+    uniform seeds, repeated until min_seconds of work."""
     from integer_delauney_amd.workloads import uniform_seeds
     from oracle import oracle as O
 
-    n = int(round(200_000 * side * side / (4096 * 4096)))
-    seeds = uniform_seeds(side, side, n, 2)
-    t0 = time.perf_counter()
-    O.triangulate(side, side, seeds, None, assign_method="raster")
-    dt = time.perf_counter() - t0
-    return {"value": side * side / dt / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": "port",
-            "sample": f"{side}x{side} image, {n} uniform seeds (same density as 4096x4096/200k), "
-                      f"full pipeline in the C oracle: unpadded + padded flood fill, detection, "
-                      f"per-triangle exact assignment; {dt:.2f} s",
-            "seconds": dt, "host_cpu_count": os.cpu_count()}
+    seeds = uniform_seeds(side, side, n, rng)
+    used = O.set_threads(threads)
+    reps, t = 0, 0.0
+    while t < min_seconds or reps == 0:
+        t0 = time.perf_counter()
+        O.triangulate(side, side, seeds, None, assign_method="raster")
+        t += time.perf_counter() - t0
+        reps += 1
+    O.set_threads(0)
+    return side * side * reps / t / 1e6, used, reps, t
+
+
+def cpu_baselines():
+    """The C oracle (the CPU restatement of the reference path) on this host
+    at the 4096^2 / 200k workload and at the 256^2 / 1,000-seed one, with one
+    thread and with every host core (OpenMP: flood-fill passes split by rows /
+    column strips, raster by row bands; detection stays sequential, as the
+    registration order requires), plus the reference NumPy path's own time on
+    the 256^2 workload, measured in the build container (the reference does
+    not travel to this host)."""
+    import json as _json
+
+    out = []
+    for side, n, rng, label in ((4096, 200_000, 2, "4096x4096, 200000 uniform seeds"),
+                                (256, 1000, 1, "256x256, 1000 uniform seeds")):
+        for threads in (0, 1):
+            rate, used, reps, t = _oracle_rate(side, n, rng, threads, 3.0 if side > 256 else 2.0)
+            out.append({"value": rate, "unit": "Mpixel/s", "cores": used, "kind": "port",
+                        "sample": f"{label} (default_rng({rng})), full pipeline in the C oracle: "
+                                  f"unpadded + padded flood fill, detection, exact raster; "
+                                  f"{reps} run(s), {t:.2f} s",
+                        "seconds": t, "host_cpu_count": os.cpu_count()})
+    path = os.path.join(ROOT, "tests", "golden", "c1_timing.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            c1 = _json.load(f)
+        out.append({"value": c1["mpixel_per_s"], "unit": "Mpixel/s", "cores": c1["cores"],
+                    "kind": "reference",
+                    "sample": f"the reference NumPy path (delauney.reference Voronoi + "
+                              f"GridTriangulation) on {c1['workload']}, {c1['total_s']:.1f} s, "
+                              f"timed in the build container ({c1['cpu']}, NumPy {c1['numpy']}), "
+                              f"not on this host",
+                    "seconds": c1["total_s"]})
+    return out
 
 
 def bench_batch(lib, ctx, n_images: int, steps: int):
@@ -519,7 +554,6 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--bfs-config", type=str, default="", help="waves,rows,iters")
-    ap.add_argument("--cpu-sample-side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--e2e-steps", type=int, default=3)
     ap.add_argument("--batch-images", type=int, default=512,
@@ -566,50 +600,70 @@ def main():
     W = H = 4096
     N = 200_000
     P = 16
+    SETS = (2, 12, 22, 32)   # default_rng seeds of the rotated seed sets
     log("generating seeds")
-    seeds = uniform_seeds(W, H, N, 2)
+    seed_sets = [uniform_seeds(W, H, N, r) for r in SETS]
+    seeds = seed_sets[0]
     lib = _lib.load()
     ctx = _lib.context()
     _lib.check(lib.dl_set_autotune(ctx.handle, 1 if args.autotune else 0))
     if args.bfs_config:
         w, r, k = (int(v) for v in args.bfs_config.split(","))
         _lib.check(lib.dl_set_bfs_config(ctx.handle, w, r, k))
-    d_seeds = torch.from_numpy(seeds).to("cuda")
+    # Four distinct seed sets, resident in HBM before timing; step i uses set
+    # i mod 4, so the fill's launch-count prediction meets inputs it has not
+    # seen in the previous call.
+    d_sets = [torch.from_numpy(x).to("cuda") for x in seed_sets]
     d_out = torch.empty((H, W, 3), dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
     n_tri = ctypes.c_int64(0)
+    tri_counts = {}
 
-    def step():
-        _lib.check(lib.dl_triangulate_device(ctx.handle, W, H, ctypes.c_void_p(d_seeds.data_ptr()),
+    def step(i, d_xy=None):
+        d = d_sets[i % len(d_sets)] if d_xy is None else d_xy
+        _lib.check(lib.dl_triangulate_device(ctx.handle, W, H, ctypes.c_void_p(d.data_ptr()),
                                              N, P, ctypes.c_void_p(d_out.data_ptr()), None,
                                              ctypes.byref(n_tri)))
+        tri_counts[i % len(d_sets)] = int(n_tri.value)
+
+    def shortfalls():
+        sf = ctypes.c_int64(0)
+        _lib.check(lib.dl_get_fill_stats(ctx.handle, None, None, None, None, ctypes.byref(sf)))
+        return int(sf.value)
 
     for i in range(args.warmup):
-        step()
+        step(i)
         log(f"warmup {i + 1}/{args.warmup} done")
     # Timed region: the pipeline alone, in the library's default graph mode
     # (auto: at this size every launch is issued from the host).
     g_main = graph_stats(lib, ctx)
+    sf0 = shortfalls()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step()
+        step(i)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     log(f"timed {args.steps} steps")
+    timed_shortfalls = shortfalls() - sf0
     default_replays = graph_stats(lib, ctx)[1] - g_main[1]
     ms_per_step = (t1 - t0) / args.steps * 1e3
     value = W * H / (ms_per_step / 1e3) / 1e6
     # The same steps as hipGraph replays (graphs on: the third call of the
-    # signature is captured, the timed ones replay it).
+    # signature is captured, the timed ones replay it).  A graph holds the
+    # seed buffer's address, so here each step first copies its seed set
+    # into one fixed buffer (a 1.6 MB device copy, inside the timed loop).
+    d_fixed = torch.empty_like(d_sets[0])
     _lib.check(lib.dl_set_graphs(ctx.handle, 1))
-    for _ in range(3):
-        step()
+    for i in range(3):
+        d_fixed.copy_(d_sets[i % len(d_sets)])
+        step(i, d_fixed)
     g_before = graph_stats(lib, ctx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step()
+        d_fixed.copy_(d_sets[i % len(d_sets)])
+        step(i, d_fixed)
     torch.cuda.synchronize()
     replayed_ms = (time.perf_counter() - t0) / args.steps * 1e3
     g_after = graph_stats(lib, ctx)
@@ -626,17 +680,21 @@ def main():
     _lib.check(lib.dl_reset_kernel_stats(ctx.handle))
     _lib.check(lib.dl_set_kernel_profiling(ctx.handle, 1))
     torch.cuda.synchronize()
+    fill_launches = fill_visits = fill_converged = 0
     t2 = time.perf_counter()
     for i in range(args.steps):
-        step()
+        step(i)
+        la, cv, vi = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.check(lib.dl_get_fill_stats(ctx.handle, ctypes.byref(la), ctypes.byref(cv),
+                                         ctypes.byref(vi), None, None))
+        fill_launches += la.value
+        fill_converged += cv.value
+        fill_visits += vi.value
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     _lib.check(lib.dl_set_kernel_profiling(ctx.handle, 0))
     profiled_ms_per_step = (t3 - t2) / args.steps * 1e3
     stats = kernel_stats(lib, ctx)
-    timings = np.zeros(9)
-    lib.dl_get_timings(ctx.handle, timings.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    fill_launches, fill_visits = float(timings[7]), float(timings[8])
 
     # Algorithmic bytes per launch of each kernel kind.
     Wp, Hp = W + 2 * P, H + 2 * P
@@ -646,8 +704,9 @@ def main():
     cfg_w, cfg_r, cfg_k = cw.value, cr.value, ck.value
     tile_cells = (128 - 2 * cfg_k) * (cfg_w * cfg_r - 2 * cfg_k)
     alg = {
-        # 4 B read + 4 B written per cell of every visited tile
-        "fill": 8.0 * tile_cells * fill_visits / max(fill_launches, 1.0),
+        # 4 B read + 4 B written per interior cell of every visited tile,
+        # per launch (the kernel events count every launch issued)
+        "fill": 8.0 * tile_cells * fill_visits / max(stats["fill"][0], 1),
         # labels read once (4 B/cell) + candidate records written/read (20 B each)
         "detect": 4.0 * Wp * Hp,
         # fused assignment + output: (H, W, 3) int32 written once, the label
@@ -675,6 +734,23 @@ def main():
         if ent and (dominant != "fill" or ent.get("config") == [cfg_w, cfg_r, cfg_k]):
             traffic = ent["hbm_bytes_per_launch"]
             traffic_src = tj.get("source")
+    # VALU issue of the dominant kernel: wave instructions per launch from
+    # the SQ counter runs summarised under profiles/, over the live launch
+    # time; peak = 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU
+    # instruction (16 lanes per SIMD) = 614.4 G instructions/s.
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            vj = json.load(f)
+        ent = vj.get(dominant)
+        if ent and (dominant != "fill" or ent.get("config") == [cfg_w, cfg_r, cfg_k]):
+            ach = ent["valu_insts_per_launch"] / (dom["avg_launch_us"] * 1e-6) / 1e9
+            valu = {"kernel": dominant, "bound": "valu", "achieved": ach,
+                    "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                    "frac": ach / VALU_PEAK_GINST,
+                    "valu_insts_per_launch": ent["valu_insts_per_launch"],
+                    "source": vj.get("source")}
     b_alg_image = 12.0 * W * H + 8.0 * Wp * Hp + 8.0 * N + 32.0 * T
 
     # End to end through the host API: H2D of the seeds, D2H of the grid.
@@ -683,8 +759,8 @@ def main():
         from integer_delauney_amd import triangulate
         triangulate(W, H, seeds, P, as_arrays=True)
         t2 = time.perf_counter()
-        for _ in range(args.e2e_steps):
-            triangulate(W, H, seeds, P, as_arrays=True)
+        for i in range(args.e2e_steps):
+            triangulate(W, H, seed_sets[i % len(seed_sets)], P, as_arrays=True)
         t3 = time.perf_counter()
         e2e = W * H / ((t3 - t2) / args.e2e_steps) / 1e6
         log("end-to-end steps done")
@@ -708,10 +784,11 @@ def main():
     if args.query_points > 0:
         queries = bench_queries(W, N, args.query_points, 5)
 
-    cpu = None
+    cpu = cpus = None
     if not args.no_cpu_baseline:
-        log("timing the CPU oracle sample")
-        cpu = cpu_baseline(args.cpu_sample_side)
+        log("timing the CPU oracle samples")
+        cpus = cpu_baselines()
+        cpu = cpus[0]   # the 4096^2 workload on every host core
 
     line = {
         "metric": "Mpixel/s full pipeline (bit-exact) + % HBM roofline; p50/p99 latency served",
@@ -726,24 +803,31 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "single image 4096x4096, 200000 distinct uniform seeds "
-                               "(default_rng(2)), border padding 16, full pipeline "
-                               "(Voronoi + Delaunay extraction + pixel-to-triangle assignment), "
-                               "inputs and outputs resident in HBM",
+        "config": {"workload": "single image 4096x4096, 200000 distinct uniform seeds, "
+                               "border padding 16, full pipeline (Voronoi + Delaunay "
+                               "extraction + pixel-to-triangle assignment), inputs and outputs "
+                               "resident in HBM; four seed sets (default_rng 2, 12, 22, 32) "
+                               "rotated across the steps",
                    "width": W, "height": H, "seeds": N, "border_padding": P,
-                   "triangles": T, "parallelism": "single GPU"},
+                   "seed_sets": list(SETS), "triangles": T,
+                   "triangles_per_set": [tri_counts.get(k) for k in range(len(SETS))],
+                   "parallelism": "single GPU"},
+        "fill_prediction_shortfalls_in_timed_steps": timed_shortfalls,
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "traffic_source": traffic_src,
                      "alg_bytes_per_launch": alg.get(dominant),
                      "avg_launch_us": dom["avg_launch_us"]},
+        "roofline_valu": valu,
         "graphs": graphs,
         "kernel_timing_region_ms_per_step": profiled_ms_per_step,
         "pipeline_alg_bytes_per_image": b_alg_image,
         "pipeline_hbm_frac": b_alg_image / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "kernels": per_kernel,
-        "fill": {"launches": fill_launches, "tile_visits": fill_visits,
+        "fill": {"launches_per_step": fill_launches / args.steps,
+                 "launches_to_convergence_per_step": fill_converged / args.steps,
+                 "tile_visits_per_step": fill_visits / args.steps,
                  "config": [cfg_w, cfg_r, cfg_k]},
         "end_to_end_mpixel_per_s": e2e,
         "small_images": small,
@@ -752,6 +836,7 @@ def main():
         "serving": serving,
         "queries": queries,
         "cpu_baseline": cpu,
+        "cpu_baselines": cpus,
     }
     print(json.dumps(line), flush=True)
 

@@ -157,12 +157,16 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
     const uint32_t px1 = static_cast<uint32_t>(c0 + 2) << 16;
 #define DL_P0(r) (px0 | (static_cast<uint32_t>(y0 + (r) + 1) & 0xFFFFu))
 #define DL_P1(r) (px1 | (static_cast<uint32_t>(y0 + (r) + 1) & 0xFFFFu))
+    // The watch records, per pass, which rows had a watched update (bit r + 1
+    // of a 32-bit word: a constant select for every row, row 0 included) and
+    // applies the row condition once per pass, so no per-row lane masks are
+    // kept live across the loop.
     bool chx = true, chy = true;   // changes of the previous iteration (forced on at entry)
     for (int k = 0; k < K; ++k) {
         bool cx = false;
         if (chx || chy) {   // wave-uniform
             bool ch = false;
-            uint64_t hit = 0ull;
+            uint32_t hr = 0u;   // rows with a watched update in pass 1 or 2
             // pass 1: source x-1
             uint32_t left = kFarInv;
 #pragma unroll
@@ -172,8 +176,7 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t1 = relax_key(K1[r], own0, DL_P1(r));
                 const bool t0 = relax_key(K0[r], left, DL_P0(r));
                 ch |= t0 | t1;
-                const uint64_t rb = ((wt.rows_in >> r) & 1u) ? ~0ull : 0ull;
-                hit |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) & rb;
+                hr |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) ? (2u << r) : 0u;
             }
             // pass 2: source x+1
             uint32_t right = kFarInv;
@@ -184,10 +187,9 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t0 = relax_key(K0[r], own1, DL_P0(r));
                 const bool t1 = relax_key(K1[r], right, DL_P1(r));
                 ch |= t0 | t1;
-                const uint64_t rb = ((wt.rows_in >> r) & 1u) ? ~0ull : 0ull;
-                hit |= ((__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1)) & rb;
+                hr |= ((__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1)) ? (2u << r) : 0u;
             }
-            esc |= hit != 0ull;
+            esc |= (hr & (wt.rows_in << 1)) != 0u;
             cx = __ballot(ch) != 0ull;
         }
         // pass 3: source y-1.  Publish this wave's last row and its q flag.
@@ -202,7 +204,7 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
         const bool run_y = lds_q[k & 1][wave] || (wave > 0 && lds_q[k & 1][wave - 1]) ||
                            (wave < NW - 1 && lds_q[k & 1][wave + 1]);
         bool ch = false;
-        uint64_t hit = 0ull;
+        uint32_t h3 = 0u, h4 = 0u;   // rows with an update of a watched column, pass 3 / 4
         if (run_y) {
             const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : kFarInv;
             const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : kFarInv;
@@ -213,8 +215,7 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t0 = relax_key(K0[r], a0, DL_P0(r));
                 const bool t1 = relax_key(K1[r], a1, DL_P1(r));
                 ch |= t0 | t1;
-                const uint64_t rb = r == wt.r_top ? ~0ull : 0ull;
-                hit |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) & rb;
+                h3 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
             }
         }
         // pass 4: source y+1
@@ -231,11 +232,12 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t0 = relax_key(K0[r], b0, DL_P0(r));
                 const bool t1 = relax_key(K1[r], b1, DL_P1(r));
                 ch |= t0 | t1;
-                const uint64_t rb = r == wt.r_bot ? ~0ull : 0ull;
-                hit |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) & rb;
+                h4 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
             }
         }
-        esc |= hit != 0ull;
+        // only the interior's first row (pass 3) and last row (pass 4) are watched
+        esc |= (wt.r_top >= 0 && wt.r_top < R && ((h3 >> (wt.r_top + 1)) & 1u)) ||
+               (wt.r_bot >= 0 && wt.r_bot < R && ((h4 >> (wt.r_bot + 1)) & 1u));
         chx = cx;
         chy = __ballot(ch) != 0ull;
     }
@@ -436,12 +438,12 @@ __device__ __forceinline__ void tile_body(
 // go round-robin over the 8 XCDs, so give each XCD a contiguous slice of tiles
 // (neighbouring tiles share halo lines in the same L2).  The grid is launched
 // as a multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
-// Tile-kernel shapes of at most 8 waves x 8 rows fit 64 VGPRs without
-// spilling (the compiler's own choice is 67): 8 waves per SIMD instead of 7,
-// i.e. four resident blocks per CU instead of three, to hide each tile's
-// loads.  (The list kernel would spill under the same bound.)
+// Tile-kernel shapes of at most 16 waves x 8 rows are compiled for 64 VGPRs
+// (8 waves per SIMD): four resident 8-wave blocks per CU instead of three, or
+// two 16-wave blocks instead of one.  (The list kernel would spill under the
+// same bound.)
 template <int NW, int R>
-constexpr int kFillWavesPerEu = (NW <= 8 && R <= 8) ? 8 : 1;
+constexpr int kFillWavesPerEu = (NW <= 16 && R <= 8) ? 8 : 1;
 
 template <int NW, int R, int K>
 //
@@ -646,9 +648,10 @@ struct Variant {
 
 // Instantiated configurations.  The tile is 128 wide; height waves*rows.
 const Variant kVariants[] = {
-    DL_VARIANT(16, 8, 8), DL_VARIANT(16, 8, 4),  DL_VARIANT(16, 8, 12), DL_VARIANT(16, 8, 16),
-    DL_VARIANT(8, 8, 4),  DL_VARIANT(8, 8, 8),   DL_VARIANT(8, 16, 8),  DL_VARIANT(16, 4, 4),
-    DL_VARIANT(16, 4, 8),
+    DL_VARIANT(16, 8, 8), DL_VARIANT(16, 8, 7),  DL_VARIANT(16, 8, 4),  DL_VARIANT(16, 8, 12),
+    DL_VARIANT(16, 8, 16),
+    DL_VARIANT(8, 8, 4),  DL_VARIANT(8, 8, 5),   DL_VARIANT(8, 8, 6),   DL_VARIANT(8, 8, 7),
+    DL_VARIANT(8, 8, 8),  DL_VARIANT(8, 16, 8),  DL_VARIANT(16, 4, 4),  DL_VARIANT(16, 4, 8),
 };
 
 const Variant* find_variant(const BfsConfig& c)

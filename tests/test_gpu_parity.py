@@ -94,8 +94,10 @@ def test_random_against_oracle(seed):
         np.testing.assert_array_equal(_rows(m), rows)
 
 
-@pytest.mark.parametrize("cfg", [(16, 4, 4), (16, 4, 8), (8, 8, 4), (8, 8, 8), (16, 8, 4),
-                                 (16, 8, 8), (16, 8, 12), (16, 8, 16), (8, 16, 8)])
+@pytest.mark.parametrize("cfg", [(16, 4, 4), (16, 4, 8), (8, 8, 4), (8, 8, 5), (8, 8, 6),
+                                 (8, 8, 7), (8, 8, 8), (16, 8, 4), (16, 8, 7), (16, 8, 8),
+                                 (16, 8, 12),
+                                 (16, 8, 16), (8, 16, 8)])
 def test_every_tile_configuration_is_exact(cfg):
     ctx = _lib.context()
     lib = _lib.load()
@@ -326,7 +328,7 @@ def test_fill_watch_never_misses_a_crossing(seed):
     assert seen_cross > 0
 
 
-@pytest.mark.parametrize("cfg", [(16, 4, 8), (8, 8, 4), (16, 8, 16)])
+@pytest.mark.parametrize("cfg", [(16, 4, 8), (8, 8, 4), (16, 8, 16), (8, 8, 7), (16, 8, 8)])
 def test_fill_watch_on_padded_canvases(cfg):
     """On the pipeline's own padded canvas (seeds only inside the image) the
     oracle sees no crossing in these cases; the GPU flag must stay clear for
