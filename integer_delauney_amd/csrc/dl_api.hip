@@ -1225,6 +1225,7 @@ void for_each_devbuf(dl_context* c, F&& f)
     TriWork& t = c->tri;
     for (DevBuf* d : {&t.seed_bm, &t.seed_idx, &t.bitmap, &t.scan, &t.scan_tmp, &t.flags,
                       &t.table_tag, &t.table_tri, &t.cand, &t.counts, &t.offsets, &t.positions,
+                      &t.route_ent, &t.route_idx, &t.route_aux,
                       &t.rows, &t.tidgrid, &t.bin_counts, &t.bin_offsets, &t.bin_list, &t.desc,
                       &t.status_dev})
         f(*d);

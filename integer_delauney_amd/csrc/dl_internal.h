@@ -265,7 +265,8 @@ struct ImageDesc {
     int64_t det_off;      // first detection thread block
     int bx_per_row;       // detection thread blocks per row of 2x2 blocks
     int tiles_x;          // 64x64 assignment tiles per row
-    int64_t key_off;      // first word of the registration bitmap (4*nb bits)
+    int64_t key_off;      // first word of the registration bitmap (4 scans x rows of
+                          // blocks, each row padded to whole detection strips)
     int64_t tile_off;     // first assignment tile
     // Source of output channels 0/1: cell (x + out_off, y + out_off) of a
     // plane out_W wide; an owner's image position is its label's position
@@ -328,6 +329,8 @@ struct TriWork {
     DevBuf counts;      // per-thread / per-block candidate counts
     DevBuf offsets;     // their prefix sums
     DevBuf positions;   // compact list of block positions holding candidates
+    DevBuf route_ent, route_idx;   // owner-bucket registration: entries, their slots
+    DevBuf route_aux;   // bucket counters, per-slot first flags, per-position scans
     DevBuf rows;        // (T, 5) triangle rows of all images, image after image
     DevBuf tidgrid;     // W x H tid raster
     DevBuf bin_counts, bin_offsets, bin_list;   // triangle -> tile bins

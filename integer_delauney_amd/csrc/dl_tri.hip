@@ -247,6 +247,7 @@ constexpr int kDetItems = 1;
 constexpr int kDetChunk = kDetThreads * kDetItems;
 constexpr int kDetRows = 32;
 static_assert(kDetItems * kDetRows <= 32, "one mask bit per 2x2 block of a thread");
+static_assert(kDetChunk % 32 == 0, "a detection strip is whole registration bitmap words");
 constexpr int kTileSide = 64;   // assignment tile side in pixels
 
 // Largest i in [0, n) with off[i] <= g (off is non-decreasing, off[0] = 0).
@@ -294,8 +295,10 @@ void plan_batch(BatchPlan& plan)
         d.bx_per_row = d.nb ? (bw + kDetChunk - 1) / kDetChunk : 0;
         d.det_off = det;
         det += d.nb ? static_cast<int64_t>(d.bx_per_row) * ((bh + kDetRows - 1) / kDetRows) : 0;
+        // registration bitmap: 4 scans x bh rows x whole detection strips
+        // (8 words of 32 blocks each per strip and row; see reg_word)
         d.key_off = keyw;
-        keyw += (4 * d.nb + 31) / 32;
+        keyw += d.nb ? 4 * static_cast<int64_t>(bh) * d.bx_per_row * (kDetChunk / 32) : 0;
         d.tiles_x = (d.W + kTileSide - 1) / kTileSide;
         d.tile_off = tiles;
         tiles += static_cast<int64_t>(d.tiles_x) * ((d.H + kTileSide - 1) / kTileSide);
@@ -723,6 +726,15 @@ struct DetectArgs {
     uint64_t table_cap;
     uint32_t pos_cap;             // capacity of positions (cand holds 2 per position)
     uint32_t rows_cap;            // capacity of rows
+    // Registration by owner buckets (ent null: the hash table registers).
+    uint4* ent;                   // own_cap entries per bucket: (a, b, c) sorted
+                                  // batch-wide ids, key (kBlockedKey: blocked)
+    uint32_t* eidx;               // each entry's candidate slot
+    uint32_t* fill;               // entries routed to each bucket
+    uint8_t* first;               // per candidate slot: 1 = a first registration
+    uint8_t* scode;               // per position: scans of its two candidates
+    uint32_t n_buckets;
+    uint32_t own_cap;
 };
 
 __device__ __forceinline__ int32_t to_id(const DetectArgs& A, const ImageDesc& d, uint32_t v)
@@ -922,6 +934,10 @@ __global__ __launch_bounds__(kDetThreads) void detect_write_kernel(DetectArgs A)
         if (n > A.pos_cap || 4 * (static_cast<uint64_t>(n) + 2ull * q) > 3 * A.table_cap)
             atomicOr(A.flags, kFlagOverflow);
     }
+    if (A.ent)   // the owner buckets' counters, for route_kernel
+        for (uint64_t k = static_cast<uint64_t>(gb) * kDetThreads + threadIdx.x; k < A.n_buckets;
+             k += static_cast<uint64_t>(gridDim.x) * kDetThreads)
+            A.fill[k] = 0u;
     int img, ry0, rx0;
     det_block_origin(A, gb, img, ry0, rx0);
     const ImageDesc& d = A.D[img];
@@ -1043,7 +1059,38 @@ __device__ __forceinline__ bool is_first(const Cand& cd, const Table& t)
     return sl.minkey == cd.key && sl.blk == 0u;
 }
 
-// Bit key_off * 32 + key of the registration bitmap marks a first registration.
+// The registration bitmap holds one bit per (scan, row of blocks, column):
+// word key_off + (scan * bh + ry) * (8 * bx_per_row) + rx / 32, bit rx % 32.
+// Rows are padded to whole detection strips of kDetChunk columns, so every
+// word belongs to the blocks of one detection thread block, and word order is
+// key order (scan, then row-major position): the reference's registration
+// order.
+struct RegBit {
+    int64_t word;
+    uint32_t bit;
+};
+
+__device__ __forceinline__ RegBit reg_bit(const ImageDesc& d, uint32_t scan, int ry, int rx)
+{
+    const int64_t row = static_cast<int64_t>(scan) * (d.Hc - 1) + ry;
+    return RegBit{d.key_off + row * (d.bx_per_row * (kDetChunk / 32)) + (rx >> 5),
+                  static_cast<uint32_t>(rx & 31)};
+}
+
+// Scan and block position of a registration key scan * nb + ry * bw + rx.
+__device__ __forceinline__ RegBit key_bit(const ImageDesc& d, uint32_t key, uint32_t& scan,
+                                          int& ry, int& rx)
+{
+    const uint32_t nb = static_cast<uint32_t>(d.nb);
+    scan = key / nb;
+    const uint32_t rem = key - scan * nb;
+    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
+    ry = static_cast<int>(rem / bw);
+    rx = static_cast<int>(rem - static_cast<uint32_t>(ry) * bw);
+    return reg_bit(d, scan, ry, rx);
+}
+
+// A set bit of the registration bitmap marks a first registration.
 __device__ void mark_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
                          const ImageDesc* __restrict__ D, uint32_t* __restrict__ bm,
                          uint32_t* __restrict__ flags);
@@ -1074,8 +1121,10 @@ __device__ void mark_one(const Cand* __restrict__ cand, uint32_t i, const Table&
         }
     }
     if (is_first(cd, t)) {
-        const int64_t w = D[cd.img].key_off + (cd.key >> 5);
-        atomicOr(&bm[w], 1u << (cd.key & 31));
+        uint32_t scan;
+        int ry, rx;
+        const RegBit rb = key_bit(D[cd.img], cd.key, scan, ry, rx);
+        atomicOr(&bm[rb.word], 1u << rb.bit);
     }
 }
 
@@ -1090,12 +1139,15 @@ __global__ void tri_base_kernel(const ImageDesc* __restrict__ D, int n_img,
 }
 
 __device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
-                         const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
-                         const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
-                         uint32_t rows_cap, uint32_t* __restrict__ flags);
+                         const uint8_t* __restrict__ first, const ImageDesc* __restrict__ D,
+                         const uint32_t* __restrict__ bm, const uint32_t* __restrict__ pre,
+                         int32_t* __restrict__ rows, uint32_t rows_cap,
+                         uint32_t* __restrict__ flags);
 
+// first: the owner buckets' per-slot decisions; null: the hash table's.
 __global__ void emit_kernel(const Cand* __restrict__ cand, const uint32_t* __restrict__ npos,
-                            Table t, const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
+                            Table t, const uint8_t* __restrict__ first,
+                            const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
                             const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
                             uint32_t rows_cap, uint32_t* __restrict__ flags, int n_img,
                             int64_t n_words, int64_t* __restrict__ tri_base)
@@ -1108,32 +1160,36 @@ __global__ void emit_kernel(const Cand* __restrict__ cand, const uint32_t* __res
     if (*flags & kFlagOverflow) return;   // rerun with exact sizes follows
     const XcdRange r = xcd_range(2 * *npos);
     for (uint32_t i = r.first; i < r.end; i += r.step)
-        emit_one(cand, i, t, D, bm, pre, rows, rows_cap, flags);
+        emit_one(cand, i, t, first, D, bm, pre, rows, rows_cap, flags);
 }
 
 __device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table& t,
-                         const ImageDesc* __restrict__ D, const uint32_t* __restrict__ bm,
-                         const uint32_t* __restrict__ pre, int32_t* __restrict__ rows,
-                         uint32_t rows_cap, uint32_t* __restrict__ flags)
+                         const uint8_t* __restrict__ first, const ImageDesc* __restrict__ D,
+                         const uint32_t* __restrict__ bm, const uint32_t* __restrict__ pre,
+                         int32_t* __restrict__ rows, uint32_t rows_cap,
+                         uint32_t* __restrict__ flags)
 {
-    // Most positions fill one of their two slots: read the key alone first.
-    if (cand[i].key == 0xFFFFFFFFu) return;
-    const Cand cd = cand[i];
-    if (!is_first(cd, t)) return;
+    Cand cd;
+    if (first) {
+        if (!first[i]) return;
+        cd = cand[i];
+    } else {
+        // Most positions fill one of their two slots: read the key alone first.
+        if (cand[i].key == 0xFFFFFFFFu) return;
+        cd = cand[i];
+        if (!is_first(cd, t)) return;
+    }
     const ImageDesc& d = D[cd.img];
-    const int64_t w = d.key_off + (cd.key >> 5);
+    uint32_t scan;
+    int ry, rx;
+    const RegBit rb = key_bit(d, cd.key, scan, ry, rx);
     // global row = first registrations of earlier images and earlier keys
-    const uint32_t row = pre[w] + static_cast<uint32_t>(__popc(bm[w] & ((1u << (cd.key & 31)) - 1u)));
+    const uint32_t row = pre[rb.word] +
+                         static_cast<uint32_t>(__popc(bm[rb.word] & ((1u << rb.bit) - 1u)));
     if (row >= rows_cap) {
         atomicOr(flags, kFlagOverflow);
         return;
     }
-    const uint32_t nb = static_cast<uint32_t>(d.nb);
-    const uint32_t scan = cd.key / nb;
-    const uint32_t rem = cd.key - scan * nb;
-    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
-    const int ry = static_cast<int>(rem / bw);
-    const int rx = static_cast<int>(rem - static_cast<uint32_t>(ry) * bw);
     // detection pixel offsets of the four scans (triangulation.py:174-183)
     const int gx = rx + ((scan == 0 || scan == 3) ? 1 : 0);
     const int gy = ry + ((scan >= 2) ? 1 : 0);
@@ -1143,6 +1199,278 @@ __device__ void emit_one(const Cand* __restrict__ cand, uint32_t i, const Table&
     r[2] = cd.a;
     r[3] = cd.b;
     r[4] = cd.c;
+}
+
+// ---------------------------------------------------------------------------
+// Registration by owner buckets (speculative runs).  Every occurrence of a
+// triplet -- a candidate, or a triplet that a four-region block blocks -- has
+// the same smallest seed id, so routing each occurrence to the bucket of its
+// smallest batch-wide id brings all occurrences of a triplet to one bucket.
+// There one workgroup decides in LDS which candidate is the first
+// registration (triangulation.py:146-183): the one with the smallest key
+// scan * nb + block whose triplet no occurrence blocks.  No global atomic is
+// made per candidate: a routing workgroup reserves each bucket's slots with
+// one atomicAdd, and each detection thread block writes its own words of the
+// registration bitmap with plain stores.  A bucket whose fixed capacity
+// overflows raises kFlagOverflow, and the exact rerun uses the hash table.
+constexpr int kOwnShift = 8;                    // 256 consecutive seed ids per bucket
+constexpr uint32_t kOwnCap = 1536;              // entries per bucket: 6 per seed
+constexpr uint32_t kBlockedKey = 0xFFFFFFFFu;   // key of a blocked-triplet entry
+constexpr int kRouteSpan = 512;                 // buckets a routing block counts in LDS
+
+// The entries of listed position i: bit k of the result marks entry ev[k]
+// valid; entries 0 and 1 are the candidates in slots 2i and 2i+1, entries 2
+// and 3 the triplets a four-region block blocks.  Also writes the candidate
+// records, clears the slots' first flags and records their scans.
+__device__ uint32_t route_position(const DetectArgs& A, uint32_t i, uint4 ev[4])
+{
+    const uint2 ip = A.positions[i];
+    const uint32_t img = ip.x, pos = ip.y;
+    const ImageDesc& d = A.D[img];
+    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
+    const int ry = static_cast<int>(pos / bw);
+    const int rx = static_cast<int>(pos - static_cast<uint32_t>(ry) * bw);
+    const BlockCorners c = block_corners(A.grid + d.cv_off, d.Wc, rx, ry);
+    Cand* out = A.cand + 2 * static_cast<size_t>(i);
+    *reinterpret_cast<uint16_t*>(A.first + 2 * static_cast<size_t>(i)) = 0;
+    const int32_t tl = to_id(A, d, c.tl), tr = to_id(A, d, c.tr);
+    const int32_t bl = to_id(A, d, c.bl), br = to_id(A, d, c.br);
+    Cand cd;
+    cd.slot = 0;
+    cd.img = img;
+    if (tl < 0 || tr < 0 || bl < 0 || br < 0) {   // flagged: the host raises
+        cd.key = 0xFFFFFFFFu;
+        out[0] = cd;
+        out[1] = cd;
+        A.scode[i] = 0;
+        return 0u;
+    }
+    const uint32_t off = static_cast<uint32_t>(d.seed_off);
+    const uint32_t nb = static_cast<uint32_t>(d.nb);
+    const int32_t tri[4][3] = {{tl, tr, br}, {tl, tr, bl}, {bl, br, tl}, {bl, br, tr}};
+    auto entry = [&](int32_t a, int32_t b, int32_t cc, uint32_t key) {
+        sort3(a, b, cc);
+        return make_uint4(off + static_cast<uint32_t>(a), off + static_cast<uint32_t>(b),
+                          off + static_cast<uint32_t>(cc), key);
+    };
+    if (!c.quad) {
+        int k = 0;
+        while (!c.v[k]) ++k;
+        cd.key = static_cast<uint32_t>(k) * nb + pos;
+        cd.a = tri[k][0];
+        cd.b = tri[k][1];
+        cd.c = tri[k][2];
+        out[0] = cd;
+        ev[0] = entry(cd.a, cd.b, cd.c, cd.key);
+        cd.key = 0xFFFFFFFFu;
+        out[1] = cd;
+        A.scode[i] = static_cast<uint8_t>(k);
+        return 1u;
+    }
+    const int32_t ids[4] = {tl, tr, bl, br};
+    const uint32_t* srt = A.sorted + d.seed_off;
+    int64_t px[4], py[4];
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t l = srt[ids[k]];
+        px[k] = static_cast<int64_t>(l >> 16);
+        py[k] = static_cast<int64_t>(l & 0xFFFFu);
+    }
+    int32_t blk[2][3];
+    quad_rule(ids, px, py, blk);
+    ev[2] = entry(blk[0][0], blk[0][1], blk[0][2], kBlockedKey);
+    ev[3] = entry(blk[1][0], blk[1][1], blk[1][2], kBlockedKey);
+    // the two triplets of the chosen diagonal, in scan order, to slots 0 and 1
+    int slot = 0;
+    uint32_t code = 0;
+    for (int k = 0; k < 4; ++k) {
+        const uint4 e = entry(tri[k][0], tri[k][1], tri[k][2], 0u);
+        const bool blocked = (e.x == ev[2].x && e.y == ev[2].y && e.z == ev[2].z) ||
+                             (e.x == ev[3].x && e.y == ev[3].y && e.z == ev[3].z);
+        if (blocked || slot == 2) continue;
+        cd.key = static_cast<uint32_t>(k) * nb + pos;
+        cd.a = tri[k][0];
+        cd.b = tri[k][1];
+        cd.c = tri[k][2];
+        out[slot] = cd;
+        code |= static_cast<uint32_t>(k) << (2 * slot);
+        if (slot == 0) ev[0] = make_uint4(e.x, e.y, e.z, cd.key);
+        else ev[1] = make_uint4(e.x, e.y, e.z, cd.key);
+        ++slot;
+    }
+    A.scode[i] = static_cast<uint8_t>(code);
+    // (a four-region block leaves exactly two; the rest is defensive)
+    const uint32_t m = slot == 2 ? 0xFu : (slot == 1 ? 0xDu : 0xCu);
+    for (cd.key = 0xFFFFFFFFu; slot < 2; ++slot) out[slot] = cd;
+    return m;
+}
+
+// One thread per listed position.  The block's entries are counted per
+// bucket in LDS (buckets within kRouteSpan of the block's smallest; a block's
+// positions are neighbours in x, so its buckets are too), each bucket's slots
+// are reserved with one global atomicAdd, and the entries are written there.
+__global__ __launch_bounds__(kDetThreads) void route_kernel(DetectArgs A)
+{
+    __shared__ uint32_t s_cnt[kRouteSpan];
+    __shared__ uint32_t s_base[kRouteSpan];
+    __shared__ uint32_t s_min[kDetThreads / 64];
+    __shared__ uint32_t s_skip;
+    const uint32_t npos = *A.npos;
+    const uint32_t p0 = blockIdx.x * kDetThreads;
+    if (p0 >= npos) return;   // block-uniform
+    if (threadIdx.x == 0) s_skip = *A.flags & kFlagOverflow;
+    for (int t = threadIdx.x; t < kRouteSpan; t += kDetThreads) s_cnt[t] = 0u;
+    __syncthreads();
+    // Sizes already known to be too small: the host reruns with exact ones.
+    if (s_skip) return;       // block-uniform (read once above)
+    const uint32_t i = p0 + threadIdx.x;
+    uint4 ev[4];
+    const uint32_t m = i < npos ? route_position(A, i, ev) : 0u;
+    uint32_t bmin = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (m & (1u << k)) bmin = min(bmin, ev[k].x >> kOwnShift);
+    for (int o = 32; o > 0; o >>= 1) bmin = min(bmin, static_cast<uint32_t>(__shfl_xor(bmin, o, 64)));
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = bmin;
+    __syncthreads();
+    uint32_t b0 = s_min[0];
+#pragma unroll
+    for (int w = 1; w < kDetThreads / 64; ++w) b0 = min(b0, s_min[w]);
+    uint32_t rank[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        rank[k] = 0xFFFFFFFFu;
+        if (m & (1u << k)) {
+            const uint32_t o = (ev[k].x >> kOwnShift) - b0;
+            if (o < static_cast<uint32_t>(kRouteSpan)) rank[k] = atomicAdd(&s_cnt[o], 1u);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kRouteSpan; t += kDetThreads) {
+        const uint32_t n = s_cnt[t];
+        if (n) s_base[t] = atomicAdd(&A.fill[b0 + t], n);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!(m & (1u << k))) continue;
+        const uint32_t b = ev[k].x >> kOwnShift;
+        const uint32_t sl = rank[k] != 0xFFFFFFFFu ? s_base[b - b0] + rank[k]
+                                                   : atomicAdd(&A.fill[b], 1u);
+        if (sl >= A.own_cap) {
+            atomicOr(A.flags, kFlagOverflow);
+            continue;
+        }
+        const size_t e = static_cast<size_t>(b) * A.own_cap + sl;
+        A.ent[e] = ev[k];
+        A.eidx[e] = k < 2 ? 2 * i + static_cast<uint32_t>(k) : 0xFFFFFFFFu;
+    }
+}
+
+// One workgroup per bucket: its entries are grouped by smallest id in LDS (a
+// counting sort over the bucket's 256 ids), and each candidate compares
+// itself with the entries of its group.  It is a first registration unless an
+// entry of the same triplet has a smaller key or blocks the triplet.
+__global__ __launch_bounds__(256) void owner_kernel(DetectArgs A)
+{
+    __shared__ uint4 E[kOwnCap];
+    __shared__ uint16_t ord[kOwnCap];
+    __shared__ uint32_t start[257];
+    __shared__ uint32_t cur[256];
+    __shared__ uint32_t wsum[4];
+    if (*A.flags & kFlagOverflow) return;   // set before this launch: uniform
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = min(A.fill[b], A.own_cap);
+    const uint32_t base = b << kOwnShift;
+    const int t = threadIdx.x;
+    start[t] = 0u;
+    cur[t] = 0u;
+    __syncthreads();
+    const uint4* ent = A.ent + static_cast<size_t>(b) * A.own_cap;
+    for (uint32_t j = t; j < n; j += 256) {
+        const uint4 e = ent[j];
+        E[j] = e;
+        atomicAdd(&start[e.x - base], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the 256 group sizes
+    const uint32_t v = start[t];
+    uint32_t incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if ((t & 63) >= o) incl += u;
+    }
+    if ((t & 63) == 63) wsum[t >> 6] = incl;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < (t >> 6); ++w) add += wsum[w];
+    __syncthreads();
+    start[t] = add + incl - v;
+    if (t == 255) start[256] = add + incl;
+    __syncthreads();
+    for (uint32_t j = t; j < n; j += 256) {
+        const uint32_t g = E[j].x - base;
+        ord[start[g] + atomicAdd(&cur[g], 1u)] = static_cast<uint16_t>(j);
+    }
+    __syncthreads();
+    for (uint32_t j = t; j < n; j += 256) {
+        const uint4 e = E[j];
+        if (e.w == kBlockedKey) continue;
+        const uint32_t g = e.x - base;
+        bool first = true;
+        for (uint32_t q = start[g]; q < start[g + 1]; ++q) {
+            const uint4 f = E[ord[q]];
+            if (f.y == e.y && f.z == e.z && (f.w < e.w || f.w == kBlockedKey)) {
+                first = false;
+                break;
+            }
+        }
+        if (first) A.first[A.eidx[static_cast<size_t>(b) * A.own_cap + j]] = 1u;
+    }
+}
+
+// One workgroup per detection thread block (its 256 x 32 window of blocks):
+// the first registrations among its listed positions set bits of its LDS copy
+// of its registration bitmap words, which it then stores, zeros included.
+__global__ __launch_bounds__(256) void mark_blocks_kernel(DetectArgs A, uint32_t* __restrict__ bm)
+{
+    constexpr int kWords = kDetChunk / 32;          // words per strip row
+    __shared__ uint32_t W[4 * kDetRows * kWords];   // [scan][row][word]
+    if (*A.flags & kFlagOverflow) return;   // set before this launch: uniform
+    const int64_t gb = blockIdx.x;
+    const int img = A.n_img > 1 ? find_seg(A.det_seg, A.n_img, gb) : 0;
+    const ImageDesc& d = A.D[img];
+    const int64_t lb = gb - A.det_seg[img];
+    const int bh = d.Hc - 1;
+    const int n_bands = (bh + kDetRows - 1) / kDetRows;
+    const int bx = static_cast<int>(lb / n_bands);
+    const int ry0 = static_cast<int>(lb - static_cast<int64_t>(bx) * n_bands) * kDetRows;
+    const int rx0 = bx * kDetChunk;
+    for (int t = threadIdx.x; t < 4 * kDetRows * kWords; t += 256) W[t] = 0u;
+    __syncthreads();
+    const uint32_t npos = *A.npos;
+    const uint32_t lo = min(A.block_offsets[gb], npos), hi = min(A.block_offsets[gb + 1], npos);
+    const uint32_t bw = static_cast<uint32_t>(d.Wc - 1);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+        const uint32_t f = *reinterpret_cast<const uint16_t*>(A.first + 2 * static_cast<size_t>(i));
+        if (!f) continue;
+        const uint32_t pos = A.positions[i].y;
+        const int ry = static_cast<int>(pos / bw);
+        const int rx = static_cast<int>(pos - static_cast<uint32_t>(ry) * bw);
+        const int lr = ry - ry0, lc = rx - rx0;
+        const uint32_t code = A.scode[i];
+        const uint32_t bit = 1u << (lc & 31);
+        if (f & 0xFFu) atomicOr(&W[((code & 3u) * kDetRows + lr) * kWords + (lc >> 5)], bit);
+        if (f >> 8) atomicOr(&W[(((code >> 2) & 3u) * kDetRows + lr) * kWords + (lc >> 5)], bit);
+    }
+    __syncthreads();
+    const int rows = min(kDetRows, bh - ry0);
+    for (int t = threadIdx.x; t < 4 * kDetRows * kWords; t += 256) {
+        const int sc = t / (kDetRows * kWords), r = (t / kWords) % kDetRows, w = t % kWords;
+        if (r < rows) {
+            const RegBit rb = reg_bit(d, static_cast<uint32_t>(sc), ry0 + r, rx0);
+            bm[rb.word + w] = W[t];
+        }
+    }
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, int n)
@@ -1210,7 +1538,10 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         ProfScope ps(kKDetect, s);
         detect_count_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
         DL_LAUNCH_CHECK();
-        if (nblocks <= 1024 * 256) {
+        // One 1024-thread block walks its chunks serially (≈ 5 us for the
+        // 2,200 blocks of a 4096^2 image, 440 us for 77,000); beyond 8 per
+        // thread the multi-block scans are faster.
+        if (nblocks <= 1024 * 8) {
             scan_two_small_kernel<<<1, 1024, 0, s>>>(A.block_counts, A.block_quads, nblocks,
                                                      block_offsets, quad_scan);
             DL_LAUNCH_CHECK();
@@ -1251,8 +1582,26 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     A.rows_cap = 2 * A.pos_cap;
     work.rows_cap = A.rows_cap;
     work.cand.reserve(sizeof(Cand) * 2 * static_cast<size_t>(A.pos_cap));
-    work.table_tag.reserve(sizeof(Slot) * table_cap);
+    A.cand = work.cand.as<Cand>();
     work.rows.reserve(sizeof(int32_t) * 5 * static_cast<size_t>(A.rows_cap));
+    // Speculative runs register by owner buckets (batch-wide seed ids must
+    // fit 31 bits); the exact rerun after any overflow uses the hash table.
+    const bool route = speculative && n_seeds < (1ull << 31);
+    if (route) {
+        const uint64_t nbk = std::max<uint64_t>(1, (n_seeds + (1u << kOwnShift) - 1) >> kOwnShift);
+        A.n_buckets = static_cast<uint32_t>(nbk);
+        A.own_cap = work.spec_tiny ? 1u : kOwnCap;
+        work.route_ent.reserve(sizeof(uint4) * nbk * kOwnCap);
+        work.route_idx.reserve(sizeof(uint32_t) * nbk * kOwnCap);
+        work.route_aux.reserve(sizeof(uint32_t) * nbk + 3 * static_cast<size_t>(A.pos_cap) + 16);
+        A.ent = work.route_ent.as<uint4>();
+        A.eidx = work.route_idx.as<uint32_t>();
+        A.fill = work.route_aux.as<uint32_t>();
+        A.first = reinterpret_cast<uint8_t*>(A.fill + nbk);
+        A.scode = A.first + 2 * static_cast<size_t>(A.pos_cap);
+    } else {
+        work.table_tag.reserve(sizeof(Slot) * table_cap);
+    }
     {
         ProfScope ps(kKDetect, s);
         detect_write_kernel<<<static_cast<unsigned>(nblocks), kDetThreads, 0, s>>>(A);
@@ -1264,15 +1613,36 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     work.scan.reserve(sizeof(uint32_t) * static_cast<size_t>(nwords + 1));
     uint32_t* bm = work.bitmap.as<uint32_t>();
     uint32_t* pre = work.scan.as<uint32_t>();
+    // grid-stride launches sized for the capacity, at most 16 blocks per CU
+    // (multiples of 8: the kernels split their range over the XCDs)
+    const unsigned cblocks = static_cast<unsigned>(
+        (std::min<uint64_t>((2ull * A.pos_cap + 255) / 256, 4096) + 7) / 8 * 8);
+    if (route) {
+        {
+            ProfScope ps(kKDetect, s);
+            route_kernel<<<static_cast<unsigned>((A.pos_cap + kDetThreads - 1) / kDetThreads),
+                           kDetThreads, 0, s>>>(A);
+            DL_LAUNCH_CHECK();
+        }
+        ProfScope ps(kKRegister, s);
+        owner_kernel<<<A.n_buckets, 256, 0, s>>>(A);
+        DL_LAUNCH_CHECK();
+        // (every word of the bitmap is stored by its detection thread block)
+        mark_blocks_kernel<<<static_cast<unsigned>(nblocks), 256, 0, s>>>(A, bm);
+        DL_LAUNCH_CHECK();
+        popcount_scan(bm, nwords, pre, work.scan_tmp, s);
+        emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.npos, Table{}, A.first, A.D, bm, pre,
+                                            work.rows.as<int32_t>(), A.rows_cap, dflags, n_img,
+                                            nwords, tri_base);
+        DL_LAUNCH_CHECK();
+        res.n_tri = -1;   // known on the device only (tri_base[n_img])
+        return res;
+    }
     uint64_t salt = 0x243F6A8885A308D3ull;
     const int img_bits = n_img > 1 ? bits_for(n_img - 1) : 0;
     const int exact = 3 * plan.max_seed_bits + img_bits <= 63 ? 1 : 0;
-    // grid-stride launches sized for the capacity, at most 16 blocks per CU
-    // (multiples of 8: the kernels split their range over the XCDs)
     const unsigned pblocks = static_cast<unsigned>(
         (std::min<uint64_t>((A.pos_cap + kDetThreads - 1) / kDetThreads, 4096) + 7) / 8 * 8);
-    const unsigned cblocks = static_cast<unsigned>(
-        (std::min<uint64_t>((2ull * A.pos_cap + 255) / 256, 4096) + 7) / 8 * 8);
     if (!exact) work.table_tri.reserve(sizeof(SlotTri) * table_cap);
     for (int attempt = 0; attempt < 4; ++attempt) {
         Table t;
@@ -1331,7 +1701,7 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
         }
         {
             ProfScope ps(kKRegister, s);
-            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.npos, t, A.D, bm, pre,
+            emit_kernel<<<cblocks, 256, 0, s>>>(A.cand, A.npos, t, nullptr, A.D, bm, pre,
                                                 work.rows.as<int32_t>(), A.rows_cap, dflags,
                                                 n_img, nwords, speculative ? tri_base : nullptr);
             DL_LAUNCH_CHECK();

@@ -63,6 +63,30 @@ def test_batch_of_uneven_larger_images():
         np.testing.assert_array_equal(verts, v1)
 
 
+def test_batch_with_many_detection_blocks():
+    """A batch with more than 8,192 detection thread blocks (each covers 256 x
+    32 2x2 blocks) scans its per-block candidate counts with the multi-block
+    scan: every image equals its single-image result, and a sample the
+    oracle's."""
+    from integer_delauney_amd.workloads import uniform_seeds
+    rng = np.random.default_rng(41)
+    imgs = []
+    for i in range(44):
+        W, H = int(rng.integers(900, 1300)), int(rng.integers(1000, 1400))
+        imgs.append((W, H, uniform_seeds(W, H, int(W * H * rng.uniform(0.003, 0.02)), 900 + i)))
+    det_blocks = sum(-(-(W + 31) // 256) * -(-(H + 31) // 32) for W, H, _ in imgs)
+    assert det_blocks > 8192
+    res = dl.triangulate_batch(imgs, as_arrays=True)
+    for i, ((W, H, s), (verts, grid)) in enumerate(zip(imgs, res)):
+        v1, g1 = dl.triangulate(W, H, s, as_arrays=True)
+        np.testing.assert_array_equal(grid, g1, err_msg=f"image {i}")
+        np.testing.assert_array_equal(verts, v1, err_msg=f"image {i}")
+        if i % 11 == 0:
+            rows, ref = O.triangulate(W, H, s, 16, return_rows=True)
+            np.testing.assert_array_equal(grid, ref, err_msg=f"image {i}")
+            np.testing.assert_array_equal(verts, rows[:, 2:5], err_msg=f"image {i}")
+
+
 def test_batch_with_forced_second_fill():
     ctx = _lib.context()
     lib = _lib.load()
