@@ -307,12 +307,17 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
         _lib.check(lib.dl_session_finalise(ctx.handle, h, ctypes.byref(nt)))
         t_build = time.perf_counter() - t0
         per_round = []
+        fill_rounds = []   # per round: fill launches, launches to convergence, tile visits
         for ins, dele in plan:
             t1 = time.perf_counter()
             _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(ins), None, ins.shape[0]))
             _lib.check(lib.dl_session_remove(ctx.handle, h, _lib.i32p(dele), dele.shape[0]))
             _lib.check(lib.dl_session_finalise(ctx.handle, h, ctypes.byref(nt)))
             per_round.append(time.perf_counter() - t1)
+            la, cv, vi = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+            _lib.check(lib.dl_get_fill_stats(ctx.handle, ctypes.byref(la), ctypes.byref(cv),
+                                             ctypes.byref(vi), None, None))
+            fill_rounds.append((la.value, cv.value, vi.value))
         log("update rounds done")
         # bit-exact check against a fresh batch run on the final seed set
         n_live = ctypes.c_int64(0)
@@ -320,10 +325,18 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
         d_xy = torch.from_numpy(live).to("cuda")
         d_ref = torch.empty((side, side, 3), dtype=torch.int32, device="cuda")
         nt2 = ctypes.c_int64(0)
+        # (with stage timing on, which also counts the fill's tile visits; every
+        # round's fill is this same full fill of its live set)
+        _lib.check(lib.dl_set_timing(ctx.handle, 1))
         _lib.check(lib.dl_triangulate_device(ctx.handle, side, side, ctypes.c_void_p(d_xy.data_ptr()),
                                              live.shape[0], 16, ctypes.c_void_p(d_ref.data_ptr()),
                                              None, ctypes.byref(nt2)))
         torch.cuda.synchronize()
+        la, cv, vi = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.check(lib.dl_get_fill_stats(ctx.handle, ctypes.byref(la), ctypes.byref(cv),
+                                         ctypes.byref(vi), None, None))
+        _lib.check(lib.dl_set_timing(ctx.handle, 0))
+        batch_fill = (la.value, cv.value, vi.value)
         # the session's grid through the host API (12 B/px device-to-host)
         host = np.empty((side, side, 3), dtype=np.int32)
         _lib.check(lib.dl_session_copy_result(h, _lib.i32p(host), None))
@@ -340,6 +353,10 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             "round_ms_p99": float(np.percentile(pr, 99) * 1e3),
             "effective_mpixel_per_s": side * side / pr.mean() / 1e6,
             "final_seeds": int(live.shape[0]), "final_triangles": int(nt.value),
+            "fill_launches_per_round": float(np.mean([f[0] for f in fill_rounds])),
+            "fill_launches_to_convergence_per_round": float(np.mean([f[1] for f in fill_rounds])),
+            "fill_tile_visits_final_batch": batch_fill[2],
+            "fill_launches_final_batch": batch_fill[0],
             "final_equals_batch_bitwise": same}
 
 

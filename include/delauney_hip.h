@@ -328,7 +328,7 @@ int dl_triangulate_device(dl_context* ctx, int width, int height, const int32_t*
  * device array of DL_NOALLOC_SCRATCH_BYTES per GPU (held from the library's
  * load on that GPU until the process ends; its address is looked up once per
  * device) and a static host block.  Limit: images for which
- * 10 * (W + 2P)(H + 2P) + 13 * W * H + 600 * n + 2 MiB <= DL_NOALLOC_SCRATCH_BYTES
+ * 10 * (W + 2P)(H + 2P) + 13 * W * H + 740 * n + 2 MiB <= DL_NOALLOC_SCRATCH_BYTES
  * (e.g. 1024 x 1024 at padding 16 with up to 60,000 seeds); larger inputs are
  * DL_EVALUE before anything runs.  The grid goes to d_out_hw3 and the
  * (n_tri, 5) rows to d_rows (capacity rows_cap rows; NULL: not copied;

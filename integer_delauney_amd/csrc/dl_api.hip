@@ -1285,7 +1285,7 @@ size_t noalloc_bound(int W, int H, int P, int64_t n)
 {
     const size_t C = static_cast<size_t>(W + 2 * P) * static_cast<size_t>(H + 2 * P);
     const size_t px = static_cast<size_t>(W) * static_cast<size_t>(H);
-    return 10 * C + 13 * px + 600 * static_cast<size_t>(n) + (size_t(2) << 20);
+    return 10 * C + 13 * px + 740 * static_cast<size_t>(n) + (size_t(2) << 20);
 }
 
 // The device address of `scratch` on the current device (set up on first use).
@@ -1692,7 +1692,7 @@ int dl_triangulate_device_noalloc(dl_context* ctx, int width, int height,
                               std::to_string(P) + " may need " + std::to_string(need) +
                               " bytes of scratch; the no-allocation entry point has " +
                               std::to_string(kNoAllocBytes) +
-                              " (10 per canvas cell + 13 per pixel + 600 per seed + 2 MiB)");
+                              " (10 per canvas cell + 13 per pixel + 740 per seed + 2 MiB)");
         DL_HIP(hipSetDevice(ctx->device));
         char* dev = scratch_on(ctx->device);
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;

@@ -440,8 +440,10 @@ __device__ __forceinline__ void tile_body(
 // as a multiple of 8 blocks, which makes this a bijection on [0, gridDim.x).
 // Tile-kernel shapes of at most 16 waves x 8 rows are compiled for 64 VGPRs
 // (8 waves per SIMD): four resident 8-wave blocks per CU instead of three, or
-// two 16-wave blocks instead of one.  (The list kernel would spill under the
-// same bound.)
+// two 16-wave blocks instead of one.  The list kernel takes the same bound:
+// its spills (a few words kept across its loop over tiles) sit outside the
+// iteration loop, and the extra resident block is worth more (16384^2
+// clustered round 48.9 -> 47.6 ms).
 template <int NW, int R>
 constexpr int kFillWavesPerEu = (NW <= 16 && R <= 8) ? 8 : 1;
 
@@ -470,7 +472,7 @@ __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_tile_ke
 // On large canvases late launches have few active tiles, and a block per tile
 // would spend most of the launch starting and retiring idle blocks.
 template <int NW, int R, int K>
-__global__ __launch_bounds__(NW * 64) void bfs_list_kernel(
+__global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_list_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, const int* __restrict__ list,
     const int* __restrict__ count, int src_plane, const int* __restrict__ flags_prev,
     int* __restrict__ flags_cur, int* __restrict__ changed_any, int* __restrict__ visits)
@@ -493,14 +495,19 @@ __global__ __launch_bounds__(NW * 64) void bfs_list_kernel(
 __global__ __launch_bounds__(256) void bfs_active_list_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles,
     const int* __restrict__ flags_prev, int* __restrict__ flags_next, int* __restrict__ list,
-    int* __restrict__ count)
+    int* __restrict__ count, const int* __restrict__ prev_changed)
 {
     __shared__ int wave_n[4];
     __shared__ int base;
     const int tile = blockIdx.x * 256 + threadIdx.x;
+    // After a launch that changed nothing the fill has converged: list no
+    // tile (the flags are still cleared, for the launch after this one).
+    const bool live = !prev_changed || *prev_changed != 0;
     bool act = false;
     if (tile < total_tiles) {
         flags_next[tile] = 0;
+    }
+    if (tile < total_tiles && live) {
         int ci = 0;
         if (n_canvas > 1) {
             int lo = 0, hi = n_canvas - 1;
@@ -663,6 +670,9 @@ const Variant* find_variant(const BfsConfig& c)
 
 // Launches after the first go through the active-tile list on fills of at
 // least BfsWork::list_min_tiles tiles; the list launch uses a fixed grid.
+// (Smaller fills launch a block per tile throughout: switching them to the
+// list for their sparse late launches measured slower at 4096^2, since each
+// list launch needs the listing kernel first.)
 constexpr int kListGrid = 2048;
 
 // One tile launch of the fill: launch 0 (and every launch of a small fill)
@@ -683,7 +693,7 @@ void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_c
     if (launch > 0 && total >= work.list_min_tiles) {
         int* list = work.list.as<int>();
         bfs_active_list_kernel<<<(total + 255) / 256, 256, 0, s>>>(cv, n_canvas, total, fprev, fcur,
-                                                                   list, listcnt);
+                                                                   list, listcnt, prev_changed);
         DL_LAUNCH_CHECK();
         v->list_fn<<<kListGrid, cfg.waves * 64, 0, s>>>(cv, n_canvas, list, listcnt, src_plane,
                                                         fprev, fcur, changed, visits);

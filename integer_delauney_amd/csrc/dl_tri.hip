@@ -1315,7 +1315,13 @@ __global__ __launch_bounds__(kDetThreads) void route_kernel(DetectArgs A)
     __shared__ uint32_t s_min[kDetThreads / 64];
     __shared__ uint32_t s_skip;
     const uint32_t npos = *A.npos;
-    const uint32_t p0 = blockIdx.x * kDetThreads;
+    // Block b runs on XCD b % 8 (round-robin dispatch; the grid is a multiple
+    // of 8, sized for the capacity): each XCD takes a contiguous eighth of the
+    // listed positions, so the label lines and seed-index words of its x
+    // range stay in its L2.
+    const uint32_t per = ((npos + kDetThreads - 1) / kDetThreads + 7) / 8;
+    if ((blockIdx.x >> 3) >= per) return;   // block-uniform
+    const uint32_t p0 = ((blockIdx.x & 7u) * per + (blockIdx.x >> 3)) * kDetThreads;
     if (p0 >= npos) return;   // block-uniform
     if (threadIdx.x == 0) s_skip = *A.flags & kFlagOverflow;
     for (int t = threadIdx.x; t < kRouteSpan; t += kDetThreads) s_cnt[t] = 0u;
@@ -1620,7 +1626,8 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     if (route) {
         {
             ProfScope ps(kKDetect, s);
-            route_kernel<<<static_cast<unsigned>((A.pos_cap + kDetThreads - 1) / kDetThreads),
+            route_kernel<<<static_cast<unsigned>(
+                               ((A.pos_cap + kDetThreads - 1) / kDetThreads + 7) / 8 * 8),
                            kDetThreads, 0, s>>>(A);
             DL_LAUNCH_CHECK();
         }

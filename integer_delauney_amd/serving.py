@@ -82,11 +82,11 @@ def footprint(width: int, height: int, n_seeds: int, padding: int = DEFAULT_BORD
     registration bitmap and scan, 1/8 B of detection counts; per image pixel
     12 B of staged output and 8 B of label planes for the unpadded fill (run
     only after a padding crossing), plus the seed index; per seed about
-    580 B of candidates, hash table slots and triplets, triangle rows and bin
-    entries (two triangles per seed); and a constant for the speculative
+    720 B of candidates, registration buckets, hash table slots and triplets
+    (the exact rerun's), triangle rows and bin entries (two triangles per seed); and a constant for the speculative
     capacities' fixed terms and allocation rounding."""
     wc, hc = width + 2 * padding, height + 2 * padding
-    return int(10 * wc * hc + 21 * width * height + 640 * n_seeds + (4 << 20))
+    return int(10 * wc * hc + 21 * width * height + 780 * n_seeds + (4 << 20))
 
 
 def _resolve_padding(p) -> int:
