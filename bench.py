@@ -360,6 +360,65 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             "final_equals_batch_bitwise": same}
 
 
+def bench_published(reps: int):
+    """The inputs of the reference's own published measurements (its CUDA path
+    on an unspecified NVIDIA GPU, API_REFERENCE.md:36 and 294-300), timed here
+    through the same Python API with host arrays in and out: triangulate() and
+    the Voronoi + GridTriangulation pair at 1510 x 1018 with 30,000 seeds, and
+    the incremental workload on 1240 x 1240 (22,000 initial seeds, then five
+    scattered batches of 1,800): insert() per batch (six calls), six
+    insert_deferred() calls and one finalise(), and one stateless rebuild of
+    the final set.  Seeds are uniform (the reference does not publish its
+    draws); padding is the default."""
+    import numpy as np
+
+    import integer_delauney_amd as dl
+    from integer_delauney_amd.workloads import uniform_seeds
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts) * 1e3)
+
+    W, H = 1510, 1018
+    s = uniform_seeds(W, H, 30_000, 5)
+    tri_ms = best(lambda: dl.triangulate(W, H, s))
+    tri_arrays_ms = best(lambda: dl.triangulate(W, H, s, as_arrays=True))
+    pair_ms = best(lambda: dl.GridTriangulation().compute(dl.Voronoi().compute(W, H, s), s))
+    S = 1240
+    allp = uniform_seeds(S, S, 22_000 + 5 * 1_800, 6)
+    batches = [allp[:22_000]] + [allp[22_000 + 1_800 * k: 22_000 + 1_800 * (k + 1)] for k in range(5)]
+
+    def per_round():
+        m = dl.Delaunay(S, S, allp.shape[0])
+        for b in batches:
+            m.insert(b, as_arrays=True)
+
+    def deferred():
+        m = dl.Delaunay(S, S, allp.shape[0])
+        for b in batches:
+            m.insert_deferred(b)
+        m.finalise(as_arrays=True)
+
+    ins_ms = best(per_round)
+    def_ms = best(deferred)
+    rebuild_ms = best(lambda: dl.triangulate(S, S, allp, as_arrays=True))
+    log("published-workload timings done")
+    return {"note": "inputs of the reference's published CUDA measurements (unspecified NVIDIA "
+                    "GPU, API_REFERENCE.md:36, 294-300); here on one MI355X through the Python "
+                    "API, host arrays in and out, median of %d" % reps,
+            "triangulate_1510x1018_30k_ms": tri_ms, "reference_ms": 116.8,
+            "triangulate_1510x1018_30k_as_arrays_ms": tri_arrays_ms,
+            "voronoi_plus_grid_triangulation_ms": pair_ms, "reference_pair_ms": 157.0,
+            "incremental_1240_insert_x6_ms": ins_ms, "reference_insert_x6_ms": 181.0,
+            "incremental_1240_deferred_x6_finalise_ms": def_ms, "reference_deferred_ms": 76.0,
+            "incremental_1240_stateless_rebuild_ms": rebuild_ms, "reference_rebuild_ms": 103.0}
+
+
 def bench_queries(side: int, n_seeds: int, n_points: int, reps: int):
     """Session queries on one image: locate / in_circumsphere over n_points
     host points, the edge list, edge scores, best-k midpoint selection and
@@ -588,6 +647,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--autotune", type=int, default=1,
                     help="0: fixed default flood-fill shape (no first-use timing of the candidates)")
+    ap.add_argument("--published-reps", type=int, default=5,
+                    help="repetitions of the reference's published workloads (0: skip)")
     ap.add_argument("--query-points", type=int, default=1_000_000,
                     help="points per locate / in_circumsphere call in the query section (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -797,6 +858,7 @@ def main():
     if args.serve_requests > 0:
         serving = bench_serving(args.serve_requests, args.serve_sessions, args.serve_slots)
 
+    published = bench_published(args.published_reps) if args.published_reps > 0 else None
     queries = None
     if args.query_points > 0:
         queries = bench_queries(W, N, args.query_points, 5)
@@ -852,6 +914,7 @@ def main():
         "incremental": incremental,
         "serving": serving,
         "queries": queries,
+        "published_workloads": published,
         "cpu_baseline": cpu,
         "cpu_baselines": cpus,
     }
