@@ -72,8 +72,14 @@ def resolve_padding(border_padding) -> int:
 
 def rows_to_map(rows: np.ndarray) -> dict:
     """{tid: (x, y, id_a, id_b, id_c)} with Python ints
-    (src/cuda/bindings/binding_helpers.hpp:61-87)."""
-    return {i: tuple(r) for i, r in enumerate(rows.tolist())}
+    (src/cuda/bindings/binding_helpers.hpp:61-87), built in C by the
+    _rowsmap extension (built with libdelauney_hip.so)."""
+    try:
+        from . import _rowsmap
+    except ImportError as e:
+        raise ImportError("integer_delauney_amd/_rowsmap is not built; run "
+                          "`python -c 'import __graft_entry__ as g; g.build()'`") from e
+    return _rowsmap.rows_to_map(np.ascontiguousarray(rows, dtype=np.int32).reshape(-1, 5))
 
 
 def _pack(rows: np.ndarray, as_arrays: bool):
