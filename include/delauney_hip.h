@@ -384,9 +384,11 @@ int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* s
  * tiles; 0 forces that path on every fill.  Results are the same either way. */
 int dl_set_fill_list_threshold(dl_context* ctx, int min_tiles);
 
-/* Cross-checking only: makes the speculative sizes of detection and
- * assignment too small on purpose, so that every call takes the exact-size
- * rerun.  Results are unchanged.  No reference counterpart. */
+/* Cross-checking only: makes speculative sizes too small on purpose, so that
+ * every call takes the exact-size rerun.  enabled: 0 off; 1 every speculative
+ * size of detection and assignment; 2 only the capacity of the registration
+ * buckets (the rerun then registers through the hash table).  Results are
+ * unchanged.  No reference counterpart. */
 int dl_set_tiny_capacities(dl_context* ctx, int enabled);
 
 #ifdef __cplusplus

@@ -932,7 +932,7 @@ std::vector<int64_t> pipeline_sig(const dl_context* ctx, const int32_t* d_xy, co
                            cfg.iters,
                            pred,
                            ctx->bfs.list_min_tiles,
-                           ctx->tri.spec_tiny ? 1 : 0,
+                           ctx->tri.spec_tiny,
                            ctx->force_unpadded ? 1 : 0,
                            ctx->bins_exact.count(tune_key(ctx->device, ctx->plan.n_cells,
                                                           ctx->plan.n_seeds))
@@ -1534,7 +1534,8 @@ int dl_set_tiny_capacities(dl_context* ctx, int enabled)
 {
     return guarded([&] {
         if (!ctx) throw value_error("null context");
-        ctx->tri.spec_tiny = enabled != 0;
+        if (enabled < 0 || enabled > 2) throw value_error("tiny-capacity mode must be 0, 1 or 2");
+        ctx->tri.spec_tiny = enabled;
     });
 }
 

@@ -344,7 +344,8 @@ struct TriWork {
     std::vector<int64_t> tri_base;   // host copy: first row of each image, then the total
     uint32_t rows_cap = 0;           // capacity of rows (detection)
     int64_t list_cap = 0;            // capacity of bin_list (assignment)
-    bool spec_tiny = false;          // test hook: speculative capacities of one
+    int spec_tiny = 0;               // test hook: 1 every speculative capacity of one,
+                                     // 2 only the registration buckets'
 };
 
 // Uploads the plan's descriptors and offset tables (the output sources may be

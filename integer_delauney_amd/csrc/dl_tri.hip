@@ -1564,7 +1564,7 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     uint64_t pos_cap = 3 * n_seeds + 4096;
     uint64_t table_cap = next_pow2(4 * n_seeds + 4096);
     if (speculative) {
-        if (work.spec_tiny) {   // test hook: capacities certain to overflow
+        if (work.spec_tiny == 1) {   // test hook: capacities certain to overflow
             pos_cap = 1;
             table_cap = 1024;
         }
@@ -1596,7 +1596,7 @@ DetectResult detect_triangles(const BatchPlan& plan, const uint32_t* grid_base, 
     if (route) {
         const uint64_t nbk = std::max<uint64_t>(1, (n_seeds + (1u << kOwnShift) - 1) >> kOwnShift);
         A.n_buckets = static_cast<uint32_t>(nbk);
-        A.own_cap = work.spec_tiny ? 1u : kOwnCap;
+        A.own_cap = work.spec_tiny ? 1u : kOwnCap;   // (test hook: buckets of one entry)
         work.route_ent.reserve(sizeof(uint4) * nbk * kOwnCap);
         work.route_idx.reserve(sizeof(uint32_t) * nbk * kOwnCap);
         work.route_aux.reserve(sizeof(uint32_t) * nbk + 3 * static_cast<size_t>(A.pos_cap) + 16);
@@ -2391,7 +2391,7 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
     // count, no scan); an overflowing tile raises kFlagOverflow and the host
     // reruns exactly.
     const uint32_t fixed = speculative && allow_fixed
-                               ? (work.spec_tiny ? 1u : fixed_bin_capacity(plan, t_bound))
+                               ? (work.spec_tiny == 1 ? 1u : fixed_bin_capacity(plan, t_bound))
                                : 0u;
     int64_t cap;
     if (fixed) {
@@ -2419,7 +2419,7 @@ void assign_tiled(const BatchPlan& plan, const int32_t* rows, int64_t n_tri,
         }
         if (speculative) {
             // a small triangle's box covers up to four tiles
-            cap = work.spec_tiny ? 1 : 4 * t_bound + ntiles + 1024;
+            cap = work.spec_tiny == 1 ? 1 : 4 * t_bound + ntiles + 1024;
         } else {
             DL_HIP(hipMemcpyAsync(work.host_flags.ptr, offs + ntiles, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, s));

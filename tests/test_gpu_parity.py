@@ -356,10 +356,14 @@ def test_fill_watch_on_padded_canvases(cfg):
         _lib.check(lib.dl_set_bfs_config(ctx.handle, 0, 0, 0))
 
 
-def test_speculative_sizes_overflowing_fall_back_exactly():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_speculative_sizes_overflowing_fall_back_exactly(mode):
     """Detection and assignment are sized from the seed count without a host
     round trip; when the sizes are too small the pipeline reruns with exact
-    ones.  Forcing that path must not change any result."""
+    ones.  Forcing that path must not change any result.  Mode 1 makes every
+    speculative size too small; mode 2 only the registration buckets', so the
+    candidate list is complete and only the bucket overflow sends the rerun
+    through the hash table."""
     ctx = _lib.context()
     lib = _lib.load()
     rng = np.random.default_rng(44)
@@ -369,7 +373,7 @@ def test_speculative_sizes_overflowing_fall_back_exactly():
         cases.append((W, H, np.stack([idx % W, idx // W], 1).astype(np.int32)))
     base = [dl.triangulate(W, H, s) for W, H, s in cases]
     batch = dl.triangulate_batch(cases)
-    _lib.check(lib.dl_set_tiny_capacities(ctx.handle, 1))
+    _lib.check(lib.dl_set_tiny_capacities(ctx.handle, mode))
     try:
         tiny = [dl.triangulate(W, H, s) for W, H, s in cases]
         tiny_batch = dl.triangulate_batch(cases)
