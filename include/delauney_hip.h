@@ -384,6 +384,12 @@ int dl_debug_fill_watch(dl_context* ctx, int width, int height, const int32_t* s
  * tiles; 0 forces that path on every fill.  Results are the same either way. */
 int dl_set_fill_list_threshold(dl_context* ctx, int min_tiles);
 
+/* Test hook: enabled = 0 keeps every flood fill on the compare-and-select
+ * key update that canvases with (W-1)^2 + (H-1)^2 > 32767^2 use; 1 (the
+ * default) lets smaller canvases use the min-based update.  Results are the
+ * same either way.  No reference counterpart. */
+int dl_set_fill_min_encoding(dl_context* ctx, int enabled);
+
 /* Cross-checking only: makes speculative sizes too small on purpose, so that
  * every call takes the exact-size rerun.  enabled: 0 off; 1 every speculative
  * size of detection and assignment; 2 only the capacity of the registration

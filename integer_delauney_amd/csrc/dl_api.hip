@@ -1461,6 +1461,14 @@ int dl_set_autotune(dl_context* ctx, int enabled)
     });
 }
 
+int dl_set_fill_min_encoding(dl_context* ctx, int enabled)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->bfs.allow_min_encoding = enabled != 0;
+    });
+}
+
 int dl_set_fill_prediction(dl_context* ctx, int enabled)
 {
     return guarded([&] {

@@ -60,32 +60,69 @@ __device__ __forceinline__ bool relax(uint32_t& lab, int& d, uint32_t nl, int cx
 // (cx - x, cy - y), exact as int16 because canvas sides (with a tile halo) stay
 // below 32767, and one dot2 gives dx^2 + dy^2 (below 2^31).
 //
-// Undefined and outside cells carry inv = kFarInv = 0x7FFF7FFF, which no real
-// label has (it would be x = 32768).  The half-wise sum saturates, so at any
-// cell with cx, cy >= 0 that label is at (32767, 32767) relative offset:
-// distance kFarD = 2 * 32767^2, above every real distance.  An undefined cell
-// is the key (kFarD, kFarInv): a real candidate is smaller, and the far label
-// itself ties, so it never updates anything.  A cell outside the canvas is
-// (0, kFarInv): no candidate is smaller (d = 0 would need a seed there), so it
-// never changes and only ever offers the far label.  No per-update validity
-// test is needed.
+// Undefined and outside cells carry the far label inv = kFarInvWide =
+// 0x7FFF7FFF, which no real label has (it would be x = 32768).  The half-wise
+// sum saturates, so at any cell with cx, cy >= 0 that label is at
+// (32767, 32767) relative offset: distance kFarD = 2 * 32767^2, above every
+// real distance.  An undefined cell is the key (kFarD, kFarInvWide): a real
+// candidate is smaller, and the far label itself ties, so it never updates
+// anything.  A cell outside the canvas is (0, far label): no candidate is
+// smaller (d = 0 would need a seed there), so it never changes and only ever
+// offers the far label.  No per-update validity test is needed.
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
-constexpr uint32_t kFarInv = 0x7FFF7FFFu;
-constexpr uint32_t kFarD = 2u * 32767u * 32767u;
+constexpr uint32_t kFarInvWide = 0x7FFF7FFFu;
 
-// A cell's key lives in one 64-bit register pair, so the compare needs no
-// operand assembly; only the candidate pair (ninv, c) is formed.
-__device__ __forceinline__ bool relax_key(uint64_t& key, uint32_t ninv, uint32_t cxy1)
+// Second encoding, for canvases with (W-1)^2 + (H-1)^2 <= 32767^2 (sides up
+// to 23,170): the key's smaller-of-two is one v_min_f64.  A key (d, inv) with
+// d < 0x7FF00000 read as an IEEE double is a positive finite number (or a
+// subnormal, which gfx950 keeps for f64), and positive finite doubles order as
+// their bit patterns, so min_f64 of two keys is the smaller key, bit for bit.
+// Every key and candidate must then stay finite: the far label becomes
+// kFarInvNarrow = 0x7FFF0000 (x half saturates as before, y half adds cy + 1
+// without saturating), whose distance at a canvas cell is
+// 32767^2 + (cy + 1)^2: finite (below 0x7FF00000 for cy < 32752) and above
+// every real distance on such canvases.  An undefined cell holds exactly
+// that far candidate's key, so the far label still ties and never updates.
+constexpr uint32_t kFarInvNarrow = 0x7FFF0000u;
+
+template <bool F>
+__device__ __forceinline__ constexpr uint32_t far_inv() { return F ? kFarInvNarrow : kFarInvWide; }
+
+// Largest canvas side pair for the min encoding.
+inline bool bfs_min_encoding_ok(int W, int H)
+{
+    const int64_t w = W - 1, h = H - 1;
+    return w * w + h * h <= 32767LL * 32767LL;
+}
+
+__device__ __forceinline__ uint32_t key_dist(uint32_t ninv, uint32_t cxy1)
 {
     const s16x2 dd = __builtin_elementwise_add_sat(__builtin_bit_cast(s16x2, ninv),
                                                    __builtin_bit_cast(s16x2, cxy1));
     // Clamping cannot change the sum (it stays below 2^31); it selects the
     // three-operand dot2 with an inline zero accumulator.
-    const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_sdot2(dd, dd, 0, true));
+    return static_cast<uint32_t>(__builtin_amdgcn_sdot2(dd, dd, 0, true));
+}
+
+// A cell's key lives in one 64-bit register pair, so the compare needs no
+// operand assembly; only the candidate pair (ninv, c) is formed.  F: the min
+// encoding (compare, then one v_min_f64 instead of two selects).
+template <bool F>
+__device__ __forceinline__ bool relax_key(uint64_t& key, uint32_t ninv, uint32_t cxy1)
+{
+    const uint32_t c = key_dist(ninv, cxy1);
     const uint64_t ck = (static_cast<uint64_t>(c) << 32) | ninv;
     const bool take = ck < key;
-    key = take ? ck : key;
+    if constexpr (F) {
+        double r;
+        asm("v_min_f64 %0, %1, %2"
+            : "=v"(r)
+            : "v"(__builtin_bit_cast(double, key)), "v"(__builtin_bit_cast(double, ck)));
+        key = __builtin_bit_cast(uint64_t, r);
+    } else {
+        key = take ? ck : key;
+    }
     return take;
 }
 
@@ -93,8 +130,8 @@ __device__ __forceinline__ uint32_t inv_of(uint64_t key) { return static_cast<ui
 
 // Whole-wave shifts.  The lane without a source (0 for a right shift, 63 for
 // a left shift) keeps `old`.  Callers chain the shifts of a pass through one
-// variable that starts at kFarInv: that lane never receives data, so it holds
-// kFarInv for every row and no per-row re-initialisation is needed.
+// variable that starts at the far label: that lane never receives data, so it
+// holds the far label for every row and no per-row re-initialisation is needed.
 __device__ __forceinline__ uint32_t from_left(uint32_t old, uint32_t v)   // lane l <- lane l-1
 {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
@@ -144,7 +181,7 @@ struct Watch {
 //    change (q = chy of the previous iteration or chx of this one, all false).
 // A tile where no wave has q set is a fixed point: the remaining iterations
 // are skipped.
-template <int NW, int R, int K>
+template <int NW, int R, int K, bool F>
 __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], int c0, int y0,
                                         const Watch& wt,
                                         uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
@@ -168,24 +205,24 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
             bool ch = false;
             uint32_t hr = 0u;   // rows with a watched update in pass 1 or 2
             // pass 1: source x-1
-            uint32_t left = kFarInv;
+            uint32_t left = far_inv<F>();
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 left = from_left(left, inv_of(K1[r]));
                 const uint32_t own0 = inv_of(K0[r]);
-                const bool t1 = relax_key(K1[r], own0, DL_P1(r));
-                const bool t0 = relax_key(K0[r], left, DL_P0(r));
+                const bool t1 = relax_key<F>(K1[r], own0, DL_P1(r));
+                const bool t0 = relax_key<F>(K0[r], left, DL_P0(r));
                 ch |= t0 | t1;
                 hr |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) ? (2u << r) : 0u;
             }
             // pass 2: source x+1
-            uint32_t right = kFarInv;
+            uint32_t right = far_inv<F>();
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 right = from_right(right, inv_of(K0[r]));
                 const uint32_t own1 = inv_of(K1[r]);
-                const bool t0 = relax_key(K0[r], own1, DL_P0(r));
-                const bool t1 = relax_key(K1[r], right, DL_P1(r));
+                const bool t0 = relax_key<F>(K0[r], own1, DL_P0(r));
+                const bool t1 = relax_key<F>(K1[r], right, DL_P1(r));
                 ch |= t0 | t1;
                 hr |= ((__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1)) ? (2u << r) : 0u;
             }
@@ -206,14 +243,14 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
         bool ch = false;
         uint32_t h3 = 0u, h4 = 0u;   // rows with an update of a watched column, pass 3 / 4
         if (run_y) {
-            const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : kFarInv;
-            const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : kFarInv;
+            const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : far_inv<F>();
+            const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : far_inv<F>();
 #pragma unroll
             for (int r = R - 1; r >= 0; --r) {
                 const uint32_t a0 = r > 0 ? inv_of(K0[r > 0 ? r - 1 : 0]) : top0;
                 const uint32_t a1 = r > 0 ? inv_of(K1[r > 0 ? r - 1 : 0]) : top1;
-                const bool t0 = relax_key(K0[r], a0, DL_P0(r));
-                const bool t1 = relax_key(K1[r], a1, DL_P1(r));
+                const bool t0 = relax_key<F>(K0[r], a0, DL_P0(r));
+                const bool t1 = relax_key<F>(K1[r], a1, DL_P1(r));
                 ch |= t0 | t1;
                 h3 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
             }
@@ -223,14 +260,14 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
         ldsB[wave][lane][1] = inv_of(K1[0]);
         __syncthreads();
         if (run_y) {
-            const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : kFarInv;
-            const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : kFarInv;
+            const uint32_t bot0 = wave < NW - 1 ? ldsB[wave + 1][lane][0] : far_inv<F>();
+            const uint32_t bot1 = wave < NW - 1 ? ldsB[wave + 1][lane][1] : far_inv<F>();
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t b0 = r < R - 1 ? inv_of(K0[r < R - 1 ? r + 1 : r]) : bot0;
                 const uint32_t b1 = r < R - 1 ? inv_of(K1[r < R - 1 ? r + 1 : r]) : bot1;
-                const bool t0 = relax_key(K0[r], b0, DL_P0(r));
-                const bool t1 = relax_key(K1[r], b1, DL_P1(r));
+                const bool t0 = relax_key<F>(K0[r], b0, DL_P0(r));
+                const bool t1 = relax_key<F>(K1[r], b1, DL_P1(r));
                 ch |= t0 | t1;
                 h4 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
             }
@@ -257,7 +294,8 @@ __host__ __device__ inline bool tile_watched(int P, int W, int H, int tx, int ty
            (oy - K <= P && P < oy + IH + K) || (oy - K <= H - P - 1 && H - P - 1 < oy + IH + K);
 }
 
-template <int NW, int R, int K>
+// One launch of one tile.
+template <int NW, int R, int K, bool F>
 __device__ __forceinline__ void tile_body(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int tile, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
@@ -323,22 +361,37 @@ __device__ __forceinline__ void tile_body(
     const bool in1 = c0 + 1 >= 0 && c0 + 1 < W;
     const int cc0 = min(max(c0, 0), W - 1);
     const int cc1 = min(max(c0 + 1, 0), W - 1);
+    // All loads first, then the keys.
+    uint32_t la[R], lc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = y0 + r;
+        la[r] = lc[r] = kInvalid;
+        if (y >= 0 && y < H) {   // wave-uniform
+            const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
+            la[r] = src[rowoff + cc0];
+            lc[r] = src[rowoff + cc1];
+        }
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int y = y0 + r;
         if (y >= 0 && y < H) {   // wave-uniform
-            const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
-            const uint32_t a = src[rowoff + cc0];
-            const uint32_t c = src[rowoff + cc1];
-            const uint32_t ia = (in0 && a != kInvalid) ? ~a : kFarInv;
-            const uint32_t ic = (in1 && c != kInvalid) ? ~c : kFarInv;
-            const uint32_t da = a == kInvalid ? kFarD : static_cast<uint32_t>(dist2_lab(c0, y, a));
-            const uint32_t dc = c == kInvalid ? kFarD : static_cast<uint32_t>(dist2_lab(c0 + 1, y, c));
+            const uint32_t a = la[r];
+            const uint32_t c = lc[r];
+            const uint32_t ia = (in0 && a != kInvalid) ? ~a : far_inv<F>();
+            const uint32_t ic = (in1 && c != kInvalid) ? ~c : far_inv<F>();
+            // A defined cell's distance, or at an undefined one the far
+            // label's (the same packed arithmetic as the update; on a canvas
+            // cell the wide far label saturates to kFarD = 2 * 32767^2).
+            const uint32_t yy = static_cast<uint32_t>(y + 1) & 0xFFFFu;
+            const uint32_t da = key_dist(ia, (static_cast<uint32_t>(c0 + 1) << 16) | yy);
+            const uint32_t dc = key_dist(ic, (static_cast<uint32_t>(c0 + 2) << 16) | yy);
             K0[r] = (static_cast<uint64_t>(in0 ? da : 0u) << 32) | ia;
             K1[r] = (static_cast<uint64_t>(in1 ? dc : 0u) << 32) | ic;
         } else {
-            K0[r] = kFarInv;   // outside: distance 0
-            K1[r] = kFarInv;
+            K0[r] = far_inv<F>();   // outside: distance 0
+            K1[r] = far_inv<F>();
         }
     }
 
@@ -359,7 +412,7 @@ __device__ __forceinline__ void tile_body(
 #pragma unroll
     for (int r = 0; r < R; ++r)
         if (on && y0 + r >= P && y0 + r < H - P) wt.rows_in |= 1u << r;
-    iterate<NW, R, K>(K0, K1, c0, y0, wt, ldsA, ldsB, lds_any, wave, lane, esc);
+    iterate<NW, R, K, F>(K0, K1, c0, y0, wt, ldsA, ldsB, lds_any, wave, lane, esc);
 
     // Write the interior back and note whether it changed.  The row origin is
     // passed through an empty asm statement so that the compiler recomputes
@@ -376,25 +429,34 @@ __device__ __forceinline__ void tile_body(
     bool ch0 = false, ch1 = false;      // column 0 / 1 changed in some row
     bool top0 = false, top1 = false;    // ... in a row of the top band
     bool bot0 = false, bot1 = false;    // ... in a row of the bottom band
+    // The rows' launch-start labels, all loaded before the first store.
+    uint32_t old0[R], old1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ly = wave * R + r - K;
+        const int y = yb + r;
+        old0[r] = old1[r] = 0u;
+        if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
+        const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
+        if (w0) old0[r] = src[rowoff + c0];
+        if (w1) old1[r] = src[rowoff + c0 + 1];
+    }
+    // Labels and change flags, then the stores: nothing waits for a load
+    // after the first store (on this GPU a wait for a load also waits for the
+    // stores issued before it).
+    uint32_t nv0[R], nv1[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        nv0[r] = inv_of(K0[r]) == far_inv<F>() ? kInvalid : ~inv_of(K0[r]);
+        nv1[r] = inv_of(K1[r]) == far_inv<F>() ? kInvalid : ~inv_of(K1[r]);
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ly = wave * R + r - K;
         const int y = yb + r;
         if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
-        const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
-        bool c0r = false, c1r = false;
-        if (w0) {
-            const size_t i = rowoff + c0;
-            const uint32_t v = inv_of(K0[r]) == kFarInv ? kInvalid : ~inv_of(K0[r]);
-            c0r = src[i] != v;
-            dst[i] = v;
-        }
-        if (w1) {
-            const size_t i = rowoff + c0 + 1;
-            const uint32_t v = inv_of(K1[r]) == kFarInv ? kInvalid : ~inv_of(K1[r]);
-            c1r = src[i] != v;
-            dst[i] = v;
-        }
+        const bool c0r = w0 && old0[r] != nv0[r];
+        const bool c1r = w1 && old1[r] != nv1[r];
         ch0 |= c0r;
         ch1 |= c1r;
         if (ly < K) {   // wave-uniform
@@ -405,6 +467,15 @@ __device__ __forceinline__ void tile_body(
             bot0 |= c0r;
             bot1 |= c1r;
         }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ly = wave * R + r - K;
+        const int y = yb + r;
+        if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
+        const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
+        if (w0) dst[rowoff + c0] = nv0[r];
+        if (w1) dst[rowoff + c0 + 1] = nv1[r];
     }
     if (esc && lane == 0 && cv.escape) atomicOr(cv.escape, 1);   // esc is wave-uniform
     const bool l0 = lx0 < K, l1 = lx0 + 1 < K;                  // left band columns
@@ -447,7 +518,7 @@ __device__ __forceinline__ void tile_body(
 template <int NW, int R>
 constexpr int kFillWavesPerEu = (NW <= 16 && R <= 8) ? 8 : 1;
 
-template <int NW, int R, int K>
+template <int NW, int R, int K, bool F>
 //
 // prev_changed (nullable): the previous launch's change counter.  When it is
 // 0 the fill has converged (no cell changed anywhere, so both planes hold the
@@ -463,15 +534,15 @@ __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_tile_ke
     const int b = blockIdx.x;
     const int tile = (b % 8) * per_xcd + b / 8;
     if (tile >= total_tiles) return;
-    tile_body<NW, R, K>(cvs, n_canvas, tile, src_plane, flags_prev, flags_cur, first_launch,
-                        changed_any, visits);
+    tile_body<NW, R, K, F>(cvs, n_canvas, tile, src_plane, flags_prev, flags_cur, first_launch,
+                           changed_any, visits);
 }
 
 // The same launch over a list of the active tiles (bfs_active_list_kernel):
 // a fixed grid strides over the list, each XCD over a contiguous eighth of it.
 // On large canvases late launches have few active tiles, and a block per tile
 // would spend most of the launch starting and retiring idle blocks.
-template <int NW, int R, int K>
+template <int NW, int R, int K, bool F>
 __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_list_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, const int* __restrict__ list,
     const int* __restrict__ count, int src_plane, const int* __restrict__ flags_prev,
@@ -482,8 +553,8 @@ __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_list_ke
     const int lo = static_cast<int>(static_cast<int64_t>(n) * xcd / 8);
     const int hi = static_cast<int>(static_cast<int64_t>(n) * (xcd + 1) / 8);
     for (int i = lo + rank; i < hi; i += per) {
-        tile_body<NW, R, K>(cvs, n_canvas, list[i], src_plane, flags_prev, flags_cur, 0,
-                            changed_any, visits);
+        tile_body<NW, R, K, F>(cvs, n_canvas, list[i], src_plane, flags_prev, flags_cur, 0,
+                               changed_any, visits);
         __syncthreads();   // the next tile reuses the LDS exchange buffers
     }
 }
@@ -644,14 +715,17 @@ using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, i
 using ListKernel = void (*)(const BfsCanvas*, int, const int*, const int*, int, const int*, int*,
                             int*, int*);
 
+// fn / list_fn[e]: e = 1 for the min encoding (canvases within
+// bfs_min_encoding_ok), 0 for the compare-and-select one.
 struct Variant {
     int waves, rows, iters;
-    TileKernel fn;
-    ListKernel list_fn;
+    TileKernel fn[2];
+    ListKernel list_fn[2];
 };
 
-#define DL_VARIANT(W_, R_, K_) \
-    {W_, R_, K_, bfs_tile_kernel<W_, R_, K_>, bfs_list_kernel<W_, R_, K_>}
+#define DL_VARIANT(W_, R_, K_)                                                       \
+    {W_, R_, K_, {bfs_tile_kernel<W_, R_, K_, false>, bfs_tile_kernel<W_, R_, K_, true>}, \
+     {bfs_list_kernel<W_, R_, K_, false>, bfs_list_kernel<W_, R_, K_, true>}}
 
 // Instantiated configurations.  The tile is 128 wide; height waves*rows.
 const Variant kVariants[] = {
@@ -695,13 +769,13 @@ void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_c
         bfs_active_list_kernel<<<(total + 255) / 256, 256, 0, s>>>(cv, n_canvas, total, fprev, fcur,
                                                                    list, listcnt, prev_changed);
         DL_LAUNCH_CHECK();
-        v->list_fn<<<kListGrid, cfg.waves * 64, 0, s>>>(cv, n_canvas, list, listcnt, src_plane,
-                                                        fprev, fcur, changed, visits);
+        v->list_fn[work.enc]<<<kListGrid, v->waves * 64, 0, s>>>(
+            cv, n_canvas, list, listcnt, src_plane, fprev, fcur, changed, visits);
     } else {
         const int grid_blocks = (total + 7) / 8 * 8;
-        v->fn<<<grid_blocks, cfg.waves * 64, 0, s>>>(cv, n_canvas, total, src_plane, fprev, fcur,
-                                                    launch == 0 ? 1 : 0, changed, visits,
-                                                    launch > 0 ? prev_changed : nullptr);
+        v->fn[work.enc]<<<grid_blocks, v->waves * 64, 0, s>>>(
+            cv, n_canvas, total, src_plane, fprev, fcur, launch == 0 ? 1 : 0, changed, visits,
+            launch > 0 ? prev_changed : nullptr);
     }
     DL_LAUNCH_CHECK();
 }
@@ -757,7 +831,11 @@ static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     const int iw = bfs_tile_iw(cfg), ih = bfs_tile_ih(cfg);
     int total = 0;
     int64_t max_iters = 64;
+    // The planes hold labels, not keys, so the key encoding may differ from
+    // one fill (or launch) to the next.
+    bool min_ok = work.allow_min_encoding;
     for (auto& c : canvases) {
+        min_ok = min_ok && bfs_min_encoding_ok(c.W, c.H);
         c.tiles_x = (c.W + iw - 1) / iw;
         c.tiles_y = (c.H + ih - 1) / ih;
         c.tile_base = total;
@@ -765,6 +843,7 @@ static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
         max_iters = std::max<int64_t>(max_iters, 8LL * (c.W + c.H) + 64);
     }
     if (total == 0) return 0;
+    work.enc = min_ok ? 1 : 0;
     const int n_canvas = static_cast<int>(canvases.size());
     work.canvases.reserve(sizeof(BfsCanvas) * n_canvas);
     work.host_canvases.reserve(sizeof(BfsCanvas) * n_canvas);

@@ -197,6 +197,11 @@ struct BfsWork {
     // bfs_run waits for a chunk's counters with an event; false: with a stream
     // synchronisation (no event objects are created)
     bool use_events = true;
+    // Key encoding of the fill's tile kernels (set per fill from the canvas
+    // sizes): 1 = min encoding, 0 = compare and select.  allow_min_encoding
+    // false (a test hook) keeps every fill on the compare-and-select one.
+    int enc = 0;
+    bool allow_min_encoding = true;
 };
 
 // Interior tile size for a configuration.

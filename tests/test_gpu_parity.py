@@ -448,6 +448,49 @@ def test_active_tile_list_fill_equals_oracle(case):
         _lib.check(lib.dl_set_fill_list_threshold(ctx.handle, -1))
 
 
+@pytest.mark.parametrize("case", range(10))
+def test_wide_key_encoding_equals_oracle(case):
+    """Canvases with sides beyond about 23,170 use the compare-and-select key
+    update; smaller ones the min-based one.  Forced onto the former here
+    (dl_set_fill_min_encoding(0)) on small cases against the oracle, with every
+    padding rule, the active-tile list on odd cases, and the goldens on the
+    first case; then the same input with both updates, bit for bit."""
+    from integer_delauney_amd import _lib
+    from integer_delauney_amd.workloads import clustered_seeds
+    lib = _lib.load()
+    ctx = _lib.context()
+    rng = np.random.default_rng(1300 + case)
+    W, H = int(rng.integers(30, 600)), int(rng.integers(3, 450))
+    n = int(rng.integers(1, max(2, W * H // 30)))
+    if case % 3 == 1 and W * H > 2000:
+        s = clustered_seeds(W, H, min(n, W * H // 4), 1300 + case)
+    else:
+        idx = rng.choice(W * H, size=min(n, W * H), replace=False)
+        s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    P = [None, 0, 5, 16, -3][case % 5]
+    _lib.check(lib.dl_set_fill_min_encoding(ctx.handle, 0))
+    if case % 2:
+        _lib.check(lib.dl_set_fill_list_threshold(ctx.handle, 0))
+    try:
+        rows, ref = O.triangulate(W, H, s, P, return_rows=True)
+        m, got = dl.triangulate(W, H, s, -1 if P is None else P)
+        np.testing.assert_array_equal(got, ref, err_msg=f"W={W} H={H} n={len(s)} P={P}")
+        np.testing.assert_array_equal(_rows(m), rows)
+        if case == 0:
+            for name in FULL:
+                g = goldens.load(name)
+                mg, tg = dl.triangulate(g["W"], g["H"], g["seeds"], _pad_arg(g["P"]))
+                np.testing.assert_array_equal(tg[:, :, :2], g["vgrid"])
+                np.testing.assert_array_equal(_rows(mg), g["rows"])
+                np.testing.assert_array_equal(tg[:, :, 2], g["tid"])
+    finally:
+        _lib.check(lib.dl_set_fill_min_encoding(ctx.handle, 1))
+        _lib.check(lib.dl_set_fill_list_threshold(ctx.handle, -1))
+    m2, got2 = dl.triangulate(W, H, s, -1 if P is None else P)
+    np.testing.assert_array_equal(got2, got)
+    np.testing.assert_array_equal(_rows(m2), _rows(m))
+
+
 def test_dense_cluster_bins_fall_back_exactly():
     """Uniform seeds plus one dense cluster: the cluster's tiles hold far more
     triangles than the one-pass binning's fixed per-tile slices (sized from
