@@ -251,7 +251,8 @@ def bench_small_images(lib, ctx, steps: int):
     return res
 
 
-def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_del: int):
+def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_del: int,
+                      record: bool = True):
     """Clustered seeds + incremental updates: a side x side session with n0
     seeds from a 64-component Gaussian mixture (sigma 200 px), then `rounds`
     rounds of n_ins new mixture samples and n_del deletions drawn uniformly
@@ -300,6 +301,10 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
     h = ctypes.c_void_p()
     _lib.check(lib.dl_session_create(ctx.handle, side, side, n0 + rounds * n_ins, 16,
                                      ctypes.byref(h)))
+    if record:
+        # keep each finalise's fill trajectory: the next finalise recomputes
+        # only the tiles the round's inserts and deletes reach (exact)
+        _lib.check(lib.dl_session_set_fill_record(h, 1))
     nt = ctypes.c_int64(0)
     try:
         t0 = time.perf_counter()
@@ -317,7 +322,10 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             la, cv, vi = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
             _lib.check(lib.dl_get_fill_stats(ctx.handle, ctypes.byref(la), ctypes.byref(cv),
                                              ctypes.byref(vi), None, None))
-            fill_rounds.append((la.value, cv.value, vi.value))
+            rv, re_, rc = ctypes.c_int(0), ctypes.c_int64(0), ctypes.c_int64(0)
+            _lib.check(lib.dl_session_fill_record_stats(h, ctypes.byref(rv), ctypes.byref(re_),
+                                                        ctypes.byref(rc)))
+            fill_rounds.append((la.value, cv.value, vi.value, rc.value, rv.value))
         log("update rounds done")
         # bit-exact check against a fresh batch run on the final seed set
         n_live = ctypes.c_int64(0)
@@ -355,6 +363,10 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             "final_seeds": int(live.shape[0]), "final_triangles": int(nt.value),
             "fill_launches_per_round": float(np.mean([f[0] for f in fill_rounds])),
             "fill_launches_to_convergence_per_round": float(np.mean([f[1] for f in fill_rounds])),
+            "fill_record": bool(record),
+            "fill_tile_computations_per_round": float(np.mean([f[3] for f in fill_rounds]))
+            if record else None,
+            "rounds_with_a_record": int(sum(f[4] for f in fill_rounds)) if record else None,
             "fill_tile_visits_final_batch": batch_fill[2],
             "fill_launches_final_batch": batch_fill[0],
             "final_equals_batch_bitwise": same}
@@ -637,6 +649,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--small-steps", type=int, default=200,
                     help="calls per small-image case, graphs on and off (0: skip)")
+    ap.add_argument("--inc-record", type=int, default=1,
+                    help="1: the incremental session keeps its fill record (replays only what "
+                         "each round's changes reach); 0: every round recomputes the fill")
     ap.add_argument("--inc-rounds", type=int, default=100,
                     help="update rounds of the incremental workload (0: skip it)")
     ap.add_argument("--inc-side", type=int, default=16384)
@@ -852,7 +867,7 @@ def main():
     incremental = None
     if args.inc_rounds > 0:
         incremental = bench_incremental(lib, ctx, args.inc_side, args.inc_seeds, args.inc_rounds,
-                                        10_000, 5_000)
+                                        10_000, 5_000, record=args.inc_record != 0)
 
     serving = None
     if args.serve_requests > 0:

@@ -145,6 +145,22 @@ int dl_session_copy_seeds(dl_session* s, int32_t* out_xy);
 /* Device pointer of the last finalise's (H, W, 3) grid (NULL while pending). */
 const int32_t* dl_session_device_grid(dl_session* s);
 
+/* Fill records (no reference counterpart: the reference recomputes, or
+ * patches inexactly).  enabled != 0: the session keeps its last flood fill's
+ * trajectory on the device (per tile and launch, the tile's labels after
+ * every launch that changed it), and the next finalise recomputes only the
+ * tiles whose neighbourhood the inserted and removed seeds reach, launch by
+ * launch, taking every other tile from the record; the result is the same as
+ * recomputing from scratch, bit for bit.  The record costs device memory
+ * (a pool of tile states, grown on demand; dl_session_device_bytes counts
+ * it); enabled = 0 frees it.  Applies to canvases of the default tile shape
+ * with sides up to about 23,170 (others always recompute).
+ * dl_session_fill_record_stats: whether a record is held, its entries, and
+ * the tile computations of the last finalise's fill. */
+int dl_session_set_fill_record(dl_session* s, int enabled);
+int dl_session_fill_record_stats(dl_session* s, int* valid, int64_t* entries,
+                                 int64_t* tiles_computed);
+
 /* The scalar field: whether any insert carried values (edge_scores and
  * select_midpoints need it, delaunay_scalar_field.cu:218-220), and the values
  * of the live seeds in insertion order (get_values, delaunay_output.cu:221-224). */

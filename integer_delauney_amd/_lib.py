@@ -95,6 +95,8 @@ _SIGNATURES = {
                                          ctypes.c_int64]),
     "dl_session_seed_count": (ctypes.c_int, [_VP, _I64P]),
     "dl_session_finalise": (ctypes.c_int, [_VP, _VP, _I64P]),
+    "dl_session_set_fill_record": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_session_fill_record_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), _I64P, _I64P]),
     "dl_session_copy_result": (ctypes.c_int, [_VP, _I32P, _I32P]),
     "dl_session_copy_seeds": (ctypes.c_int, [_VP, _I32P]),
     "dl_session_device_grid": (_VP, [_VP]),

@@ -298,6 +298,15 @@ struct dl_context {
     int device = 0;
     hipStream_t stream = nullptr;
     BfsWork bfs;
+    // Set by a session's finalise that keeps a fill record (see FillRecord):
+    // the record and the session's seed bitmap; the pipeline's padded fill
+    // then records, or replays from the record.  last_escaped: the padding
+    // watch of the last pipeline fired for some image.
+    FillRecord* rec = nullptr;
+    const uint32_t* rec_bitmap = nullptr;
+    int rec_w = 0, rec_h = 0;
+    bool last_escaped = false;
+    int64_t record_fills = 0, replay_fills = 0;
     TriWork tri;
     BatchPlan plan;   // images of the last call
     DevBuf xy, sorted;
@@ -363,6 +372,10 @@ struct dl_session {
     DevBuf pix_tids, pix_sids, pix_mask;   // per-pixel arrays of dl_session_finalise_device
     DevBuf q_xy, q_t, q_tid, q_mask, q_out;   // query staging
     QueryWork qw;
+    // dl_session_set_fill_record: keep the fill's trajectory between
+    // finalises, so that the next one computes only what the changes reach
+    bool record = false;
+    FillRecord rec;
 };
 
 namespace {
@@ -486,7 +499,9 @@ BfsConfig choose_config(const dl_context* ctx, int64_t cells, int64_t seeds, boo
         const auto it = g_tuned.find(tune_key(ctx->device, cells, seeds));
         if (it != g_tuned.end()) return it->second;
     }
-    *tune = ctx->autotune && cells >= kTuneMinCells;
+    // (a session keeping a fill record fills with the default shape: the
+    // shape is part of the record)
+    *tune = ctx->autotune && cells >= kTuneMinCells && !ctx->rec;
     return kDefaultCfg;
 }
 
@@ -560,7 +575,7 @@ void prepare(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, hipStr
 // convergence with waits).
 int fill_prediction(const dl_context* ctx, const BfsConfig& cfg, bool tune)
 {
-    if (tune || !ctx->predict) return 0;
+    if (tune || !ctx->predict || ctx->rec) return 0;
     const auto it =
         ctx->fill_pred.find(pred_key(ctx->device, ctx->plan.n_cells, ctx->plan.n_seeds, cfg));
     return it != ctx->fill_pred.end() && it->second <= kMaxPredicted ? it->second : 0;
@@ -613,7 +628,24 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         cv.P = watch[i] >= 0 ? watch[i] : 0;
         cv.escape = watch[i] >= 0 ? ctx->esc.as<int>() + i : nullptr;
     }
-    if (pred > 0) {
+    const bool recorded = ctx->rec && !tune && n_img == 1 && bfs_record_supported(cfg, cvs[0]);
+    if (recorded) {
+        // A session keeping its fill record: replay from it when it describes
+        // this canvas, otherwise record this fill (host waits, no prediction).
+        FillRecord& rec = *ctx->rec;
+        if (bfs_record_usable(rec, cvs[0], cfg) && rec.prev_bitmap.ptr) {
+            plane = bfs_run_incremental(cvs, cfg, ctx->bfs, s, &st, rec,
+                                        rec.prev_bitmap.as<uint32_t>(), ctx->rec_bitmap,
+                                        ctx->rec_w, ctx->rec_h, plan.img[0].P);
+            ++ctx->replay_fills;
+        } else {
+            plane = bfs_run_recorded(cvs, cfg, ctx->bfs, s, &st, rec);
+            ++ctx->record_fills;
+        }
+        ctx->fill_converged += st.converged + 1;
+        if (visits_async) *visits_async = st.tile_visits;
+        st.tile_visits = 0;
+    } else if (pred > 0) {
         // change flags and visit total stay on the device for the final status
         // read; visits are counted only for statistics
         ctx->bfs.count_visits = ctx->timing || ctx->profiling;
@@ -896,6 +928,8 @@ void finish_pipeline(dl_context* ctx, const Issued& iss, hipStream_t s, StageClo
         assign_tiled(plan, rows, st.n_tri, ctx->sorted.as<uint32_t>(), C, ctx->tri, s, false);
         clk.mark(4);
     }
+    ctx->last_escaped = false;
+    for (size_t i = 0; i < st.escaped.size(); ++i) ctx->last_escaped |= st.escaped[i] != 0;
     if (C == 3) {
         // Did a padding cell ever update an image cell?  If not, the image's
         // part of the padded canvas evolved exactly as the unpadded fill.
@@ -1042,7 +1076,7 @@ void run_pipeline(dl_context* ctx, const int32_t* d_xy, int32_t* d_out, int C, h
     plan_batch(ctx->plan);
     const bool graphs = ctx->graph_mode == 1 ||
                         (ctx->graph_mode == 2 && ctx->plan.n_cells < kGraphAutoMaxCells);
-    if (!graphs || ctx->timing || ctx->profiling) {
+    if (!graphs || ctx->timing || ctx->profiling || ctx->rec) {
         const Issued iss = issue_pipeline(ctx, d_xy, d_out, C, s, clk);
         finish_pipeline(ctx, iss, s, clk);
         return;
@@ -1177,7 +1211,30 @@ void session_finalise(dl_context* ctx, dl_session* s)
         ProfInstall pi(ctx);
         StageClock clk(ctx);
         plan_single(ctx, s->W, s->H, s->P, s->st.n_live);
+        // A session keeping its fill record hands it to the pipeline's fill
+        // for this call; the bitmap then becomes the record's.
+        struct RecScope {
+            dl_context* c;
+            ~RecScope() { c->rec = nullptr; c->rec_bitmap = nullptr; }
+        } rs{ctx};
+        if (s->record) {
+            ctx->rec = &s->rec;
+            ctx->rec_bitmap = s->st.bitmap.as<uint32_t>();
+            ctx->rec_w = s->W;
+            ctx->rec_h = s->H;
+        }
         run_pipeline(ctx, s->st.list.as<int32_t>(), s->out.as<int32_t>(), 3, st, clk);
+        if (s->record) {
+            FillRecord& r = s->rec;
+            r.prev_bitmap.reserve(sizeof(uint32_t) * static_cast<size_t>(s->st.words));
+            DL_HIP(hipMemcpyAsync(r.prev_bitmap.ptr, s->st.bitmap.ptr,
+                                  sizeof(uint32_t) * static_cast<size_t>(s->st.words),
+                                  hipMemcpyDeviceToDevice, st));
+            // a padding crossing anywhere: the record cannot vouch for the
+            // tiles it did not recompute next time
+            r.escaped = ctx->last_escaped;
+            r.synced = true;
+        }
         s->n_tri = ctx->n_tri;
         keep_session_result(ctx, s, 0, 0, st);
         clk.mark(5);
@@ -1245,6 +1302,10 @@ void for_each_devbuf(dl_session* s, F&& f)
                       &s->pix_tids, &s->pix_sids, &s->pix_mask, &s->q_xy, &s->q_t, &s->q_tid,
                       &s->q_mask, &s->q_out, &s->qw.a, &s->qw.b, &s->qw.c, &s->qw.d, &s->qw.e,
                       &s->qw.sort_tmp, &s->qw.scan_tmp})
+        f(*d);
+    FillRecord& r = s->rec;
+    for (DevBuf* d : {&r.pool, &r.csr[0], &r.csr[1], &r.tile_off[0], &r.tile_off[1], &r.app,
+                      &r.rank, &r.tstate, &r.ctl, &r.prev_bitmap, &r.list})
         f(*d);
 }
 
@@ -2116,6 +2177,36 @@ int dl_session_seed_count(dl_session* s, int64_t* n)
     return guarded([&] {
         if (!s || !n) throw value_error("null argument");
         *n = s->st.n_live;
+    });
+}
+
+int dl_session_set_fill_record(dl_session* s, int enabled)
+{
+    return guarded([&] {
+        if (!s) throw value_error("null session");
+        DL_HIP(hipSetDevice(s->device));
+        s->record = enabled != 0;
+        if (!s->record) {
+            FillRecord& r = s->rec;
+            for (DevBuf* d : {&r.pool, &r.csr[0], &r.csr[1], &r.tile_off[0], &r.tile_off[1],
+                              &r.app, &r.rank, &r.tstate, &r.ctl, &r.prev_bitmap, &r.list})
+                d->release();
+            r.valid = r.synced = r.escaped = false;
+            r.pool_cap = r.ent_cap = r.entries = r.last_computed = 0;
+            r.states = 0;
+        }
+    });
+}
+
+int dl_session_fill_record_stats(dl_session* s, int* valid, int64_t* entries,
+                                 int64_t* tiles_computed)
+{
+    return guarded([&] {
+        if (!s) throw value_error("null session");
+        const FillRecord& r = s->rec;
+        if (valid) *valid = (r.valid && r.synced && !r.escaped) ? 1 : 0;
+        if (entries) *entries = r.entries;
+        if (tiles_computed) *tiles_computed = r.last_computed;
     });
 }
 

@@ -159,6 +159,8 @@ using GU32 = __attribute__((address_space(1))) uint32_t;
 // right, top or bottom band / in one of its K x K corners.
 constexpr int kChAny = 1, kChLeft = 2, kChRight = 4, kChTop = 8, kChBottom = 16;
 constexpr int kChTL = 32, kChTR = 64, kChBL = 128, kChBR = 256;
+constexpr int kChMask = 511;
+constexpr int kDiffers = 512;   // (replay) the result differs from the record
 
 struct Watch {
     uint64_t l0, l1;     // lanes whose column 0 / 1 is the interior's first column
@@ -294,18 +296,50 @@ __host__ __device__ inline bool tile_watched(int P, int W, int H, int tx, int ty
            (oy - K <= P && P < oy + IH + K) || (oy - K <= H - P - 1 && H - P - 1 < oy + IH + K);
 }
 
-// One launch of one tile.
-template <int NW, int R, int K, bool F>
+// What a recorded (MODE 1) or replayed (MODE 2) launch needs beyond the plain
+// one (see FillRecord).  launch: the launch's index, i.e. the state it reads.
+struct TileExtra {
+    RecArgs rec;
+    int launch;
+    // MODE 2: the record being replayed and the per-tile replay state
+    const int4* csr;              // entries by tile: (state, mask, slot, 0)
+    const int* tile_off;
+    const int* next_ent;          // per tile: first entry of state >= launch + 1
+    const int* dirty_cur;         // per tile: differs from the record at state launch
+    int* dirty_next;              // ... at state launch + 1 (written for this tile)
+    int* mask_next;               // this tile's change mask of this launch
+};
+
+// Pool slot of tile u at state `launch` in the record (its latest entry of a
+// state <= launch; every tile has an entry of state 1).
+__device__ __forceinline__ int rec_slot_at(const TileExtra& ex, int u)
+{
+    const int e = ex.next_ent[u];
+    return e > ex.tile_off[u] ? ex.csr[e - 1].z : 0;
+}
+
+// One launch of one tile.  MODE 0: the plain fill.  MODE 1: also records the
+// tile's interior after the launch (when it changed, and always in launch 0).
+// MODE 2: replays from a record: the 3 x 3 tiles' states come from the label
+// plane where they differ from the record and from the record's pool
+// elsewhere, the result is compared with the record's next state, and the
+// tile is recorded as in MODE 1.
+template <int NW, int R, int K, bool F, int MODE = 0>
 __device__ __forceinline__ void tile_body(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int tile, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
-    int* __restrict__ changed_any, int* __restrict__ visits)
+    int* __restrict__ changed_any, int* __restrict__ visits, const TileExtra& ex)
 {
     constexpr int IW = 128 - 2 * K;
     constexpr int IH = NW * R - 2 * K;
     __shared__ uint32_t ldsA[NW][64][2];
     __shared__ uint32_t ldsB[NW][64][2];
     __shared__ int lds_any[2][NW];   // per-wave flags (see iterate)
+    // MODE 2: source of each of the 3 x 3 tiles (dy * 3 + dx): base address,
+    // row pitch and the canvas origin of that base; MODE 1, 2: record slot
+    __shared__ const uint32_t* lds_src[9];
+    __shared__ int lds_pitch[9], lds_col[9], lds_row[9];
+    __shared__ int lds_slot;
 
     int ci = 0;
     if (n_canvas > 1) {
@@ -332,7 +366,7 @@ __device__ __forceinline__ void tile_body(
     // or of that halo -- the band (or corner) of a neighbour's interior that
     // faces this tile.  Tile flags are change masks (see the write-back).
     // Every wave reads the nine flags itself, so no barrier is needed.
-    if (!first_launch) {
+    if (MODE != 2 && !first_launch) {
         int f = 0;
         if (lane < 9) {
             const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
@@ -353,6 +387,30 @@ __device__ __forceinline__ void tile_body(
     const int y0 = oy - K + wave * R;       // canvas row of this thread's first row
     const GU32* src = (const GU32*)(src_plane ? cv.lab1 : cv.lab0);
     GU32* dst = (GU32*)(src_plane ? cv.lab0 : cv.lab1);
+    if constexpr (MODE == 2) {
+        // Sources: the plane for tiles that differ from the record (and for
+        // every tile in launch 0, whose input is the seeds), the record's
+        // pool slot of state `launch` otherwise.
+        if (tid < 9) {
+            const int nx = tx + tid % 3 - 1, ny = ty + tid / 3 - 1;
+            const uint32_t* base = src_plane ? cv.lab1 : cv.lab0;
+            int pitch = W, col = 0, row = 0;
+            if (nx >= 0 && ny >= 0 && nx < tiles_x && ny < tiles_y) {
+                const int u = tile_base + ny * tiles_x + nx;
+                if (ex.launch > 0 && !ex.dirty_cur[u]) {
+                    base = ex.rec.pool + static_cast<size_t>(rec_slot_at(ex, u)) * (IW * IH);
+                    pitch = IW;
+                    col = nx * IW;
+                    row = ny * IH;
+                }
+            }
+            lds_src[tid] = base;
+            lds_pitch[tid] = pitch;
+            lds_col[tid] = col;
+            lds_row[tid] = row;
+        }
+        __syncthreads();
+    }
 
     // Per-cell state: ~label and squared distance (see relax_key).  Loads use
     // clamped addresses and selects, so no per-row lane masks stay live.
@@ -363,14 +421,26 @@ __device__ __forceinline__ void tile_body(
     const int cc1 = min(max(c0 + 1, 0), W - 1);
     // All loads first, then the keys.
     uint32_t la[R], lc[R];
+    // (MODE 2: the source tile of each loaded column: 0, 1, 2 = left, own, right)
+    const int sx0 = cc0 < ox ? 0 : cc0 >= ox + IW ? 2 : 1;
+    const int sx1 = cc1 < ox ? 0 : cc1 >= ox + IW ? 2 : 1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int y = y0 + r;
         la[r] = lc[r] = kInvalid;
         if (y >= 0 && y < H) {   // wave-uniform
-            const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
-            la[r] = src[rowoff + cc0];
-            lc[r] = src[rowoff + cc1];
+            if constexpr (MODE == 2) {
+                const int sy = y < oy ? 0 : y >= oy + IH ? 6 : 3;   // row of sources, wave-uniform
+                const int k0 = sy + sx0, k1 = sy + sx1;
+                la[r] = lds_src[k0][static_cast<size_t>(y - lds_row[k0]) * lds_pitch[k0] +
+                                    (cc0 - lds_col[k0])];
+                lc[r] = lds_src[k1][static_cast<size_t>(y - lds_row[k1]) * lds_pitch[k1] +
+                                    (cc1 - lds_col[k1])];
+            } else {
+                const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
+                la[r] = src[rowoff + cc0];
+                lc[r] = src[rowoff + cc1];
+            }
         }
     }
 #pragma unroll
@@ -438,8 +508,16 @@ __device__ __forceinline__ void tile_body(
         old0[r] = old1[r] = 0u;
         if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
         const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
-        if (w0) old0[r] = src[rowoff + c0];
-        if (w1) old1[r] = src[rowoff + c0 + 1];
+        if constexpr (MODE == 2) {
+            // the tile's own source (the plane or its record slot)
+            const uint32_t* own = lds_src[4] + static_cast<size_t>(y - lds_row[4]) * lds_pitch[4] -
+                                  lds_col[4];
+            if (w0) old0[r] = own[c0];
+            if (w1) old1[r] = own[c0 + 1];
+        } else {
+            if (w0) old0[r] = src[rowoff + c0];
+            if (w1) old1[r] = src[rowoff + c0 + 1];
+        }
     }
     // Labels and change flags, then the stores: nothing waits for a load
     // after the first store (on this GPU a wait for a load also waits for the
@@ -490,18 +568,83 @@ __device__ __forceinline__ void tile_body(
     if (__ballot((top0 && r0) || (top1 && r1))) m |= kChTR;
     if (__ballot((bot0 && l0) || (bot1 && l1))) m |= kChBL;
     if (__ballot((bot0 && r0) || (bot1 && r1))) m |= kChBR;
+    if constexpr (MODE == 2) {
+        // Compare with the record's state launch + 1 (its entry of that state,
+        // or the state before when the recorded launch changed nothing).
+        const int e = ex.next_ent[tile];
+        const bool has1 = e < ex.tile_off[tile + 1] && ex.csr[e].x == ex.launch + 1;
+        const int slot = has1 ? ex.csr[e].z : (e > ex.tile_off[tile] ? ex.csr[e - 1].z : 0);
+        const uint32_t* cmp = ex.rec.pool + static_cast<size_t>(slot) * (IW * IH);
+        bool diff = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int ly = wave * R + r - K;
+            if (ly < 0 || ly >= IH || yb + r >= H) continue;   // wave-uniform
+            if (w0) diff |= cmp[ly * IW + lx0] != nv0[r];
+            if (w1) diff |= cmp[ly * IW + lx0 + 1] != nv1[r];
+        }
+        if (__ballot(diff)) m |= kDiffers;
+    }
     __syncthreads();   // every wave is past its last read of lds_any
     if (lane == 0) lds_any[0][wave] = m;
     __syncthreads();
-    if (tid == 0) {
+    if constexpr (MODE < 2) {
+        if (tid == 0) {
+            int any = 0;
+            for (int w = 0; w < NW; ++w) any |= lds_any[0][w];
+            flags_cur[tile] = any;
+            // No atomics here: thousands of blocks would contend for one
+            // address.  Only this block owns the tile's visit count in this
+            // launch, and the change flag is an idempotent store.
+            visits[tile] += 1;
+            if (any & kChAny) *changed_any = 1;
+        }
+    }
+    if constexpr (MODE >= 1) {
+        // Record the interior after this launch when it changed (and every
+        // tile in launch 0, so that a record has every tile's state 1).
         int any = 0;
+#pragma unroll
         for (int w = 0; w < NW; ++w) any |= lds_any[0][w];
-        flags_cur[tile] = any;
-        // No atomics here: thousands of blocks would contend for one address.
-        // Only this block owns the tile's visit count in this launch, and the
-        // change flag is an idempotent store.
-        visits[tile] += 1;
-        if (any & kChAny) *changed_any = 1;
+        const int mask = any & kChMask;
+        if (tid == 0) {
+            if constexpr (MODE == 2) {
+                ex.dirty_next[tile] = (any & kDiffers) ? 1 : 0;
+                ex.mask_next[tile] = mask;
+                visits[tile] += 1;
+                if (mask & kChAny) *changed_any = 1;
+            }
+            int slot = -1;
+            if ((mask & kChAny) || ex.launch == 0) {
+                const unsigned long long j = atomicAdd(ex.rec.used, 1ull);
+                const unsigned long long id = atomicAdd(ex.rec.used + 1, 1ull);
+                const unsigned long long fn = static_cast<unsigned long long>(ex.rec.free_n);
+                const unsigned long long sl =
+                    j < fn ? static_cast<unsigned long long>(ex.rec.freelist[j])
+                           : static_cast<unsigned long long>(ex.rec.fresh) + (j - fn);
+                if (sl < static_cast<unsigned long long>(ex.rec.pool_cap) &&
+                    id < static_cast<unsigned long long>(ex.rec.ent_cap)) {
+                    slot = static_cast<int>(sl);
+                    ex.rec.app[id] = make_int4(tile, ex.launch + 1, mask, slot);
+                    ex.rec.rank[id] = ex.rec.nent[tile]++;
+                } else {
+                    *ex.rec.overflow = 1;
+                }
+            }
+            lds_slot = slot;
+        }
+        __syncthreads();
+        const int slot = lds_slot;
+        if (slot >= 0) {
+            uint32_t* out = ex.rec.pool + static_cast<size_t>(slot) * (IW * IH);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int ly = wave * R + r - K;
+                if (ly < 0 || ly >= IH || yb + r >= H) continue;   // wave-uniform
+                if (w0) out[ly * IW + lx0] = nv0[r];
+                if (w1) out[ly * IW + lx0 + 1] = nv1[r];
+            }
+        }
     }
 }
 
@@ -518,43 +661,47 @@ __device__ __forceinline__ void tile_body(
 template <int NW, int R>
 constexpr int kFillWavesPerEu = (NW <= 16 && R <= 8) ? 8 : 1;
 
-template <int NW, int R, int K, bool F>
+template <int NW, int R, int K, bool F, int MODE = 0>
 //
 // prev_changed (nullable): the previous launch's change counter.  When it is
 // 0 the fill has converged (no cell changed anywhere, so both planes hold the
 // final labels) and every block returns at once, writing nothing: the next
 // launch sees this launch's counter still 0 and returns too.
+// ex: MODE 1 (recorded fill) only.
 __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_tile_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int total_tiles, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
-    int* __restrict__ changed_any, int* __restrict__ visits, const int* __restrict__ prev_changed)
+    int* __restrict__ changed_any, int* __restrict__ visits, const int* __restrict__ prev_changed,
+    TileExtra ex)
 {
     if (prev_changed && *prev_changed == 0) return;   // uniform across the grid
     const int per_xcd = gridDim.x / 8;
     const int b = blockIdx.x;
     const int tile = (b % 8) * per_xcd + b / 8;
     if (tile >= total_tiles) return;
-    tile_body<NW, R, K, F>(cvs, n_canvas, tile, src_plane, flags_prev, flags_cur, first_launch,
-                           changed_any, visits);
+    tile_body<NW, R, K, F, MODE>(cvs, n_canvas, tile, src_plane, flags_prev, flags_cur,
+                                 first_launch, changed_any, visits, ex);
 }
 
 // The same launch over a list of the active tiles (bfs_active_list_kernel):
 // a fixed grid strides over the list, each XCD over a contiguous eighth of it.
 // On large canvases late launches have few active tiles, and a block per tile
 // would spend most of the launch starting and retiring idle blocks.
-template <int NW, int R, int K, bool F>
+// MODE 2 (replay): the list is the plan kernel's and ex describes the record.
+template <int NW, int R, int K, bool F, int MODE = 0>
 __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_list_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, const int* __restrict__ list,
     const int* __restrict__ count, int src_plane, const int* __restrict__ flags_prev,
-    int* __restrict__ flags_cur, int* __restrict__ changed_any, int* __restrict__ visits)
+    int* __restrict__ flags_cur, int* __restrict__ changed_any, int* __restrict__ visits,
+    TileExtra ex)
 {
     const int n = *count;
     const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, rank = blockIdx.x >> 3;
     const int lo = static_cast<int>(static_cast<int64_t>(n) * xcd / 8);
     const int hi = static_cast<int>(static_cast<int64_t>(n) * (xcd + 1) / 8);
     for (int i = lo + rank; i < hi; i += per) {
-        tile_body<NW, R, K, F>(cvs, n_canvas, list[i], src_plane, flags_prev, flags_cur, 0,
-                               changed_any, visits);
+        tile_body<NW, R, K, F, MODE>(cvs, n_canvas, list[i], src_plane, flags_prev, flags_cur, 0,
+                                     changed_any, visits, ex);
         __syncthreads();   // the next tile reuses the LDS exchange buffers
     }
 }
@@ -711,9 +858,9 @@ __global__ void zero_ints_kernel(int* p, int n)
 }
 
 using TileKernel = void (*)(const BfsCanvas*, int, int, int, const int*, int*, int, int*, int*,
-                           const int*);
+                           const int*, TileExtra);
 using ListKernel = void (*)(const BfsCanvas*, int, const int*, const int*, int, const int*, int*,
-                            int*, int*);
+                            int*, int*, TileExtra);
 
 // fn / list_fn[e]: e = 1 for the min encoding (canvases within
 // bfs_min_encoding_ok), 0 for the compare-and-select one.
@@ -753,10 +900,25 @@ constexpr int kListGrid = 2048;
 // visits every tile; later launches of a large fill list the active tiles
 // first.  changed / listcnt: this launch's zeroed counters.
 // prev_changed: see bfs_tile_kernel (nullptr: always run).
+// The recording (MODE 1) kernels: built for the default shape and the min
+// encoding only (the shapes and canvases a session records; see
+// bfs_record_supported).
+struct RecKernels {
+    TileKernel fn;
+    ListKernel list_fn;
+};
+const RecKernels kRecKernels = {bfs_tile_kernel<8, 8, 8, true, 1>, bfs_list_kernel<8, 8, 8, true, 1>};
+
+// rec (nullable): record this launch into the record ex describes.
 void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_canvas, int total,
                   int launch, int* changed, int* listcnt, hipStream_t s,
-                  const int* prev_changed = nullptr)
+                  const int* prev_changed = nullptr, const TileExtra* rec = nullptr)
 {
+    TileExtra ex{};
+    if (rec) {
+        ex = *rec;
+        ex.launch = launch;
+    }
     int* flags = work.flags.as<int>();
     const int src_plane = launch & 1;
     int* fprev = flags + ((launch + 1) & 1) * total;
@@ -769,13 +931,13 @@ void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_c
         bfs_active_list_kernel<<<(total + 255) / 256, 256, 0, s>>>(cv, n_canvas, total, fprev, fcur,
                                                                    list, listcnt, prev_changed);
         DL_LAUNCH_CHECK();
-        v->list_fn[work.enc]<<<kListGrid, v->waves * 64, 0, s>>>(
-            cv, n_canvas, list, listcnt, src_plane, fprev, fcur, changed, visits);
+        (rec ? kRecKernels.list_fn : v->list_fn[work.enc])<<<kListGrid, v->waves * 64, 0, s>>>(
+            cv, n_canvas, list, listcnt, src_plane, fprev, fcur, changed, visits, ex);
     } else {
         const int grid_blocks = (total + 7) / 8 * 8;
-        v->fn[work.enc]<<<grid_blocks, v->waves * 64, 0, s>>>(
+        (rec ? kRecKernels.fn : v->fn[work.enc])<<<grid_blocks, v->waves * 64, 0, s>>>(
             cv, n_canvas, total, src_plane, fprev, fcur, launch == 0 ? 1 : 0, changed, visits,
-            launch > 0 ? prev_changed : nullptr);
+            launch > 0 ? prev_changed : nullptr, ex);
     }
     DL_LAUNCH_CHECK();
 }
@@ -891,8 +1053,23 @@ int bfs_launch_fixed(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     return n_launches & 1;
 }
 
+namespace {
+int run_impl(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+             hipStream_t s, BfsStats* stats, int* visits_async, int start_launch,
+             const TileExtra* rec);
+}  // namespace
+
 int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
             hipStream_t s, BfsStats* stats, int* visits_async, int start_launch)
+{
+    return run_impl(canvases, cfg, work, s, stats, visits_async, start_launch, nullptr);
+}
+
+namespace {
+// bfs_run, recording every launch into rec when given (see launch_tiles).
+int run_impl(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+             hipStream_t s, BfsStats* stats, int* visits_async, int start_launch,
+             const TileExtra* rec)
 {
     const Variant* v = find_variant(cfg);
     if (!v) throw runtime_error("unsupported flood-fill tile configuration");
@@ -937,7 +1114,7 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
         zero_ints_kernel<<<1, 64, 0, s>>>(dcnt + 2 * slot0, 2 * kChunk);
         for (int j = 0; j < kChunk; ++j) {
             launch_tiles(v, cfg, work, n_canvas, total, launch, dcnt + 2 * (slot0 + j),
-                         dcnt + 2 * (slot0 + j) + 1, s);
+                         dcnt + 2 * (slot0 + j) + 1, s, nullptr, rec);
             ++launch;
         }
         DL_HIP(hipMemcpyAsync(hcnt + 2 * slot0, dcnt + 2 * slot0, sizeof(int) * 2 * kChunk,
@@ -993,6 +1170,7 @@ int bfs_run(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& wor
     if (stats) *stats = st;
     return final_plane;
 }
+}  // namespace
 
 int bfs_run_simple(const BfsCanvas& cv, BfsWork& work, hipStream_t s, int64_t* iters_out)
 {
@@ -1021,6 +1199,509 @@ int bfs_run_simple(const BfsCanvas& cv, BfsWork& work, hipStream_t s, int64_t* i
     }
     if (iters_out) *iters_out = iters;
     return plane;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Fill records (see FillRecord): recording, and replaying only what changed.
+//
+// Replay of launch L (computing state L + 1), exactly as the launch-by-launch
+// fill would on the new seeds:
+//  * dirty[t]: the tile's state differs from the record's at state L.  Seeded
+//    from the seed-presence bitmaps (tiles holding an inserted or removed
+//    seed differ at state 0).
+//  * A tile outside the 3 x 3 neighbourhood of every dirty tile reads exactly
+//    what the recorded launch read, so its state L + 1 is the record's.
+//  * Inside it, a tile is computed when a neighbour's change of the last
+//    launch faces it in either fill (otherwise neither fill changes it, and
+//    its difference from the record stays what it was), from the plane for
+//    dirty tiles and from the record for the others; its result is compared
+//    with the record's state L + 1.
+//  * A dirty tile holds its state in both label planes whenever it is not
+//    computed: it becomes dirty only as the result of a launch that changed
+//    it (or whose recorded counterpart did), so it is computed in the next
+//    launch too, and a launch that changes nothing writes what it read.
+// The fill has converged after the first launch that changes nothing in the
+// new fill (computed tiles, and record changes of the tiles that follow it).
+// Clean tiles then take their record state, and the new fill is recorded in
+// turn: computed tiles store their results, the others reuse record slots.
+
+// Per-tile replay state words (FillRecord::tstate), by parity where noted.
+constexpr int kTsNext = 0;    // [2][tiles] first record entry of state >= L (parity L)
+constexpr int kTsDirty = 2;   // [2][tiles]
+constexpr int kTsMask = 4;    // [2][tiles] the new fill's change mask of the last launch
+constexpr int kTsNent = 6;    // [tiles] entries of the fill being recorded
+constexpr int kTsWords = 7;
+
+// Control words (FillRecord::ctl): 64-bit slot / entry counters, overflow,
+// list count, then one change flag per launch.
+constexpr int kCtlUsed = 0;     // 2 x uint64
+constexpr int kCtlOverflow = 4;
+constexpr int kCtlCount = 5;
+constexpr int kCtlChanged = 8;
+
+__global__ void rec_reset_kernel(int* nent, int tiles, uint32_t* pool0, int slot_cells,
+                                 unsigned long long* used, int* overflow)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < tiles) nent[i] = 0;
+    if (i < slot_cells) pool0[i] = kInvalid;   // slot 0: every cell undefined
+    if (i == 0) {
+        used[0] = 0ull;
+        used[1] = 0ull;
+        *overflow = 0;
+    }
+}
+
+// Entries (append order) -> by tile: csr[tile_off[tile] + rank] = (state, mask, slot, 0).
+__global__ void rec_scatter_kernel(const int4* __restrict__ app, const int* __restrict__ rank,
+                                   int64_t n, const uint32_t* __restrict__ tile_off,
+                                   int4* __restrict__ csr)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 a = app[i];
+    csr[tile_off[a.x] + rank[i]] = make_int4(a.y, a.z, a.w, 0);
+}
+
+// Slots used by a record.
+__global__ void rec_mark_kernel(const int4* __restrict__ csr, int64_t n, uint32_t* mark)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int slot = csr[i].z;
+    atomicOr(mark + (slot >> 5), 1u << (slot & 31));
+}
+
+// Slots in [1, hw) the record does not use, appended to freelist (any order).
+__global__ __launch_bounds__(256) void rec_free_kernel(const uint32_t* __restrict__ mark,
+                                                       int64_t hw, int* __restrict__ freelist,
+                                                       int* count)
+{
+    __shared__ int wave_n[4], base;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const bool fr = i >= 1 && i < hw && !((mark[i >> 5] >> (i & 31)) & 1u);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(fr);
+    if (lane == 0) wave_n[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+        base = t ? atomicAdd(count, t) : 0;
+    }
+    __syncthreads();
+    if (fr) {
+        int off = base;
+        for (int k = 0; k < w; ++k) off += wave_n[k];
+        freelist[off + __popcll(m & ((1ull << lane) - 1ull))] = static_cast<int>(i);
+    }
+}
+
+// Replay set-up: cursors at the first entries, nothing dirty or changed yet.
+__global__ void incr_init_kernel(const int* __restrict__ tile_off, int* ts, int tiles,
+                                 int* ctl, int n_flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < tiles) {
+        ts[kTsNext * tiles + i] = tile_off[i];
+        ts[kTsDirty * tiles + i] = 0;
+        ts[kTsMask * tiles + i] = 0;
+        ts[kTsNent * tiles + i] = 0;
+    }
+    if (i < n_flags) ctl[kCtlChanged + i] = 0;
+}
+
+// Tiles holding a seed that only one of the two bitmaps has differ at state 0.
+__global__ void incr_seed_diff_kernel(const uint32_t* __restrict__ prev,
+                                      const uint32_t* __restrict__ cur, int64_t words, int img_w,
+                                      int img_h,
+                                      int P, int IW, int IH, int tiles_x, int* dirty)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= words) return;
+    uint32_t d = prev[i] ^ cur[i];
+    while (d) {
+        const int b = __builtin_ctz(d);
+        d &= d - 1u;
+        const int64_t bit = i * 32 + b;   // bit x * img_h + y
+        const int x = static_cast<int>(bit / img_h), y = static_cast<int>(bit % img_h);
+        if (x >= img_w) continue;         // (bits past the image are never set)
+        dirty[((y + P) / IH) * tiles_x + (x + P) / IW] = 1;
+    }
+}
+
+// Launch L's plan: which tiles to compute; the others' next dirty flag and
+// change mask; record entries of the tiles that follow the record.  Nothing
+// after convergence (the previous launch changed nothing).
+__global__ __launch_bounds__(256) void incr_plan_kernel(
+    int tiles_x, int tiles_y, int L, const int4* __restrict__ csr, const int* __restrict__ tile_off,
+    const int* __restrict__ next_cur, int* __restrict__ next_nxt, const int* __restrict__ dirty_cur,
+    int* __restrict__ dirty_nxt, const int* __restrict__ mask_cur, int* __restrict__ mask_nxt,
+    int* __restrict__ list, int* ctl, RecArgs rec)
+{
+    __shared__ int wave_n[4], base;
+    int* changed = ctl + kCtlChanged;
+    if (L > 0 && changed[L - 1] == 0) return;   // uniform: converged
+    const int tiles = tiles_x * tiles_y;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    bool comp = false;
+    bool follow = false;   // a record entry of state L + 1 to reuse
+    int4 follow_e{};
+    if (t < tiles) {
+        const int e = next_cur[t], end = tile_off[t + 1];
+        const int e1 = e + ((e < end && csr[e].x == L) ? 1 : 0);
+        next_nxt[t] = e1;
+        const bool has1 = e1 < end && csr[e1].x == L + 1;
+        const int tx = t % tiles_x, ty = t / tiles_x;
+        // bit of neighbour (dx, dy) = (k % 3 - 1, k / 3 - 1) that faces this tile
+        constexpr int kFacing[9] = {kChBR, kChBottom, kChBL, kChRight, kChAny,
+                                    kChLeft, kChTR, kChTop, kChTL};
+        int nd = 0, act = L == 0 ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
+            if (nx < 0 || ny < 0 || nx >= tiles_x || ny >= tiles_y) continue;
+            const int u = ny * tiles_x + nx;
+            nd |= dirty_cur[u];
+            if (L > 0) {
+                const int eu = next_cur[u];
+                const int old_m = (eu < tile_off[u + 1] && csr[eu].x == L) ? csr[eu].y : 0;
+                act |= ((mask_cur[u] | old_m) & kFacing[k]) ? 1 : 0;
+            }
+        }
+        comp = nd && act;
+        if (!comp) {
+            const int dc = dirty_cur[t];
+            dirty_nxt[t] = dc;
+            int m = 0;
+            if (!dc && has1) {
+                follow_e = csr[e1];
+                m = follow_e.y;
+                follow = true;
+            }
+            mask_nxt[t] = m;
+            if (m & kChAny) changed[L] = 1;
+        }
+    }
+    // list of tiles to compute (one atomic per block)
+    const uint64_t mc = __ballot(comp);
+    if (lane == 0) wave_n[w] = __popcll(mc);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int n = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+        base = n ? atomicAdd(ctl + kCtlCount, n) : 0;
+    }
+    __syncthreads();
+    if (comp) {
+        int off = base;
+        for (int k = 0; k < w; ++k) off += wave_n[k];
+        list[off + __popcll(mc & ((1ull << lane) - 1ull))] = t;
+    }
+    // record entries reused from the record (one atomic per wave)
+    const uint64_t mf = __ballot(follow);
+    if (mf) {
+        unsigned long long b0 = 0ull;
+        if (lane == 0) b0 = atomicAdd(rec.used + 1, static_cast<unsigned long long>(__popcll(mf)));
+        b0 = __shfl(b0, 0, 64);
+        if (follow) {
+            const unsigned long long id = b0 + __popcll(mf & ((1ull << lane) - 1ull));
+            if (id < static_cast<unsigned long long>(rec.ent_cap)) {
+                rec.app[id] = make_int4(t, L + 1, follow_e.y, follow_e.z);
+                rec.rank[id] = rec.nent[t]++;
+            } else {
+                *rec.overflow = 1;
+            }
+        }
+    }
+}
+
+// Clean tiles take the record's state at state `st` (into `plane`).
+__global__ __launch_bounds__(256) void incr_materialise_kernel(
+    int W, int H, int tiles_x, int IW, int IH, int st, const int4* __restrict__ csr,
+    const int* __restrict__ tile_off, const int* __restrict__ next_ent,
+    const int* __restrict__ dirty, const uint32_t* __restrict__ pool, uint32_t* __restrict__ plane)
+{
+    const int t = blockIdx.x;
+    if (dirty[t]) return;
+    const int e = next_ent[t], beg = tile_off[t], end = tile_off[t + 1];
+    const int slot = (e < end && csr[e].x == st) ? csr[e].z : (e > beg ? csr[e - 1].z : 0);
+    const uint32_t* src = pool + static_cast<size_t>(slot) * IW * IH;
+    const int ox = (t % tiles_x) * IW, oy = (t / tiles_x) * IH;
+    for (int i = threadIdx.x; i < IW * IH; i += 256) {
+        const int x = ox + i % IW, y = oy + i / IW;
+        if (x < W && y < H) plane[static_cast<size_t>(y) * W + x] = src[i];
+    }
+}
+
+const ListKernel kIncrKernel = bfs_list_kernel<8, 8, 8, true, 2>;
+
+int tiles_of(const BfsCanvas& cv, const BfsConfig& cfg, int* tx, int* ty)
+{
+    *tx = (cv.W + bfs_tile_iw(cfg) - 1) / bfs_tile_iw(cfg);
+    *ty = (cv.H + bfs_tile_ih(cfg) - 1) / bfs_tile_ih(cfg);
+    return *tx * *ty;
+}
+
+// Capacities and reset of the recording buffers for a fill of `tiles` tiles.
+// reset_slots: a new record from scratch (the pool is reused from slot 1).
+RecArgs rec_prepare(FillRecord& rec, const BfsConfig& cfg, int tiles, int max_launches,
+                    bool reset_slots, hipStream_t s)
+{
+    const int cells = bfs_tile_iw(cfg) * bfs_tile_ih(cfg);
+    if (rec.pool_cap == 0) rec.pool_cap = std::max<int64_t>(4096, 24LL * tiles + 1);
+    rec.pool.reserve(sizeof(uint32_t) * static_cast<size_t>(cells) *
+                     static_cast<size_t>(rec.pool_cap));
+    rec.ent_cap = 2 * rec.pool_cap;
+    rec.app.reserve(sizeof(int4) * static_cast<size_t>(rec.ent_cap));
+    rec.rank.reserve(sizeof(int) * static_cast<size_t>(rec.ent_cap));
+    rec.tstate.reserve(sizeof(int) * kTsWords * static_cast<size_t>(tiles));
+    rec.ctl.reserve(sizeof(int) * (kCtlChanged + static_cast<size_t>(max_launches) + 8));
+    rec.list.reserve(sizeof(int) * static_cast<size_t>(tiles));
+    rec.host.reserve(sizeof(int) * 64);
+    RecArgs a{};
+    a.pool = rec.pool.as<uint32_t>();
+    a.pool_cap = rec.pool_cap;
+    a.app = rec.app.as<int4>();
+    a.rank = rec.rank.as<int>();
+    a.nent = rec.tstate.as<int>() + kTsNent * tiles;
+    a.used = reinterpret_cast<unsigned long long*>(rec.ctl.as<int>() + kCtlUsed);
+    a.overflow = rec.ctl.as<int>() + kCtlOverflow;
+    a.ent_cap = rec.ent_cap;
+    if (reset_slots) {
+        rec.hw = 1;
+        rec.free_n = 0;
+    }
+    a.freelist = rec.freelist.as<int>();
+    a.free_n = rec.free_n;
+    a.fresh = rec.hw;
+    const int n = std::max(tiles, cells);
+    rec_reset_kernel<<<(n + 255) / 256, 256, 0, s>>>(a.nent, tiles, a.pool, cells, a.used,
+                                                     a.overflow);
+    DL_LAUNCH_CHECK();
+    return a;
+}
+
+// Entries recorded by the fill just run -> the record (by tile).  Waits for
+// the stream.  False when something overflowed (no valid record then).
+bool rec_finish(FillRecord& rec, const RecArgs& a, int tiles, DevBuf& tmp, hipStream_t s)
+{
+    int* h = rec.host.as<int>();
+    DL_HIP(hipMemcpyAsync(h, rec.ctl.ptr, sizeof(int) * (kCtlOverflow + 1), hipMemcpyDeviceToHost,
+                          s));
+    DL_HIP(hipStreamSynchronize(s));
+    const unsigned long long* used = reinterpret_cast<const unsigned long long*>(h + kCtlUsed);
+    if (h[kCtlOverflow] || used[1] > static_cast<unsigned long long>(rec.ent_cap)) {
+        rec.valid = false;
+        // the next record gets twice the room
+        rec.pool_cap *= 2;
+        return false;
+    }
+    const int nx = rec.cur ^ 1;
+    const int64_t n = static_cast<int64_t>(used[1]);
+    rec.tile_off[nx].reserve(sizeof(uint32_t) * (static_cast<size_t>(tiles) + 1));
+    rec.csr[nx].reserve(sizeof(int4) * static_cast<size_t>(std::max<int64_t>(n, 1)));
+    exclusive_scan_u32(reinterpret_cast<const uint32_t*>(a.nent), tiles,
+                       rec.tile_off[nx].as<uint32_t>(), tmp, s);
+    if (n > 0) {
+        rec_scatter_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(
+            a.app, a.rank, n, rec.tile_off[nx].as<uint32_t>(), rec.csr[nx].as<int4>());
+        DL_LAUNCH_CHECK();
+    }
+    rec.cur = nx;
+    rec.entries = n;
+    rec.valid = true;
+    // Slots this fill took beyond the free list extend the pool; every slot
+    // below the new high-water mark that the new record does not use is
+    // free for the next fill.
+    const int64_t taken = static_cast<int64_t>(used[0]);
+    rec.hw = a.fresh + std::max<int64_t>(0, taken - a.free_n);
+    const size_t words = static_cast<size_t>((rec.hw + 31) / 32);
+    rec.mark.reserve(sizeof(uint32_t) * words);
+    rec.freelist.reserve(sizeof(int) * static_cast<size_t>(rec.pool_cap));
+    fill_u32(rec.mark.as<uint32_t>(), 0u, words, s);
+    if (n > 0) {
+        rec_mark_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(
+            rec.csr[nx].as<int4>(), n, rec.mark.as<uint32_t>());
+        DL_LAUNCH_CHECK();
+    }
+    int* cnt = rec.ctl.as<int>() + kCtlCount + 2;
+    zero_ints_kernel<<<1, 64, 0, s>>>(cnt, 1);
+    DL_LAUNCH_CHECK();
+    rec_free_kernel<<<static_cast<unsigned>((rec.hw + 255) / 256), 256, 0, s>>>(
+        rec.mark.as<uint32_t>(), rec.hw, rec.freelist.as<int>(), cnt);
+    DL_LAUNCH_CHECK();
+    DL_HIP(hipMemcpyAsync(h + 40, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    rec.free_n = h[40];
+    return true;
+}
+
+}  // namespace
+
+bool bfs_record_supported(const BfsConfig& cfg, const BfsCanvas& cv)
+{
+    return cfg.waves == 8 && cfg.rows == 8 && cfg.iters == 8 && bfs_min_encoding_ok(cv.W, cv.H);
+}
+
+bool bfs_record_usable(const FillRecord& rec, const BfsCanvas& cv, const BfsConfig& cfg)
+{
+    return rec.valid && rec.synced && !rec.escaped && rec.W == cv.W && rec.H == cv.H &&
+           rec.P == cv.P &&
+           rec.cfg.waves == cfg.waves && rec.cfg.rows == cfg.rows && rec.cfg.iters == cfg.iters;
+}
+
+int bfs_run_recorded(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+                     hipStream_t s, BfsStats* stats, FillRecord& rec)
+{
+    if (canvases.size() != 1 || !bfs_record_supported(cfg, canvases[0]))
+        throw runtime_error("fill records need one canvas and the default tile shape");
+    int tx = 0, ty = 0;
+    const int tiles = tiles_of(canvases[0], cfg, &tx, &ty);
+    const int max_launches =
+        static_cast<int>((8LL * (canvases[0].W + canvases[0].H) + 64 + cfg.iters - 1) / cfg.iters) +
+        2;
+    rec.valid = false;
+    rec.synced = false;
+    work.enc = 1;   // (the recording kernels use the min encoding)
+    TileExtra ex{};
+    ex.rec = rec_prepare(rec, cfg, tiles, max_launches, true, s);
+    BfsStats st;
+    const bool allow = work.allow_min_encoding;
+    work.allow_min_encoding = true;
+    int plane = 0;
+    try {
+        plane = run_impl(canvases, cfg, work, s, &st, nullptr, 0, &ex);
+    } catch (...) {
+        work.allow_min_encoding = allow;
+        throw;
+    }
+    work.allow_min_encoding = allow;
+    if (rec_finish(rec, ex.rec, tiles, work.scan_tmp, s)) {
+        rec.W = canvases[0].W;
+        rec.H = canvases[0].H;
+        rec.P = canvases[0].P;
+        rec.tiles = tiles;
+        rec.cfg = cfg;
+        rec.states = st.converged + 2;
+        rec.escaped = false;
+    }
+    rec.last_computed = st.tile_visits;
+    if (stats) *stats = st;
+    return plane;
+}
+
+int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+                        hipStream_t s, BfsStats* stats, FillRecord& rec,
+                        const uint32_t* prev_bitmap, const uint32_t* bitmap, int img_w, int img_h,
+                        int img_pad)
+{
+    if (canvases.size() != 1 || !bfs_record_usable(rec, canvases[0], cfg))
+        throw runtime_error("no usable fill record for this canvas");
+    const Variant* v = find_variant(cfg);
+    int64_t max_iters = 0;
+    const int total = bfs_setup(canvases, cfg, work, s, &max_iters);
+    const int tiles = rec.tiles;
+    if (total != tiles) throw runtime_error("fill record and canvas disagree");
+    const int max_launches = static_cast<int>((max_iters + cfg.iters - 1) / cfg.iters) + 2;
+    const BfsCanvas& cv = canvases[0];
+    const int IW = bfs_tile_iw(cfg), IH = bfs_tile_ih(cfg);
+    const int tiles_x = cv.tiles_x, tiles_y = cv.tiles_y;
+    // The record being replayed stays in csr[cur] / tile_off[cur]; this fill's
+    // entries are appended after its pool slots.  Until the caller has
+    // stored this fill's seed bitmap, no later fill may replay.
+    rec.synced = false;
+    const int cur = rec.cur;
+    RecArgs ra = rec_prepare(rec, cfg, tiles, max_launches, false, s);
+    const int4* csr = rec.csr[cur].as<int4>();
+    const int* toff = rec.tile_off[cur].as<int>();
+    int* ts = rec.tstate.as<int>();
+    int* ctl = rec.ctl.as<int>();
+    int* list = rec.list.as<int>();
+    work.visits.reserve(sizeof(int) * static_cast<size_t>(tiles));
+    fill_u32(work.visits.as<uint32_t>(), 0u, static_cast<size_t>(tiles), s);
+    incr_init_kernel<<<(std::max(tiles, max_launches) + 255) / 256, 256, 0, s>>>(
+        toff, ts, tiles, ctl, max_launches);
+    DL_LAUNCH_CHECK();
+    {
+        const int64_t words = (static_cast<int64_t>(img_w) * img_h + 31) / 32;
+        incr_seed_diff_kernel<<<static_cast<unsigned>((words + 255) / 256), 256, 0, s>>>(
+            prev_bitmap, bitmap, words, img_w, img_h, img_pad, IW, IH, tiles_x,
+            ts + kTsDirty * tiles);
+        DL_LAUNCH_CHECK();
+    }
+    const BfsCanvas* dcv = reinterpret_cast<const BfsCanvas*>(work.canvases.ptr);
+    auto issue = [&](int L) {
+        const int p = L & 1, q = p ^ 1;
+        ProfScope ps(kKFill, s);
+        zero_ints_kernel<<<1, 64, 0, s>>>(ctl + kCtlCount, 1);
+        DL_LAUNCH_CHECK();
+        incr_plan_kernel<<<(tiles + 255) / 256, 256, 0, s>>>(
+            tiles_x, tiles_y, L, csr, toff, ts + (kTsNext + p) * tiles, ts + (kTsNext + q) * tiles,
+            ts + (kTsDirty + p) * tiles, ts + (kTsDirty + q) * tiles, ts + (kTsMask + p) * tiles,
+            ts + (kTsMask + q) * tiles, list, ctl, ra);
+        DL_LAUNCH_CHECK();
+        TileExtra ex{};
+        ex.rec = ra;
+        ex.launch = L;
+        ex.csr = csr;
+        ex.tile_off = toff;
+        ex.next_ent = ts + (kTsNext + q) * tiles;
+        ex.dirty_cur = ts + (kTsDirty + p) * tiles;
+        ex.dirty_next = ts + (kTsDirty + q) * tiles;
+        ex.mask_next = ts + (kTsMask + q) * tiles;
+        kIncrKernel<<<kListGrid, v->waves * 64, 0, s>>>(dcv, 1, list, ctl + kCtlCount, p, nullptr,
+                                                        nullptr, ctl + kCtlChanged + L,
+                                                        work.visits.as<int>(), ex);
+        DL_LAUNCH_CHECK();
+    };
+    // Launches in chunks; the host reads a chunk's change flags while the next
+    // one runs.  Launches after convergence do nothing.
+    constexpr int kChunk = 8;
+    int* h = rec.host.as<int>();
+    int issued = 0, checked = 0, conv = -1;
+    auto issue_chunk = [&]() {
+        for (int j = 0; j < kChunk && issued < max_launches; ++j) issue(issued++);
+    };
+    issue_chunk();
+    while (conv < 0) {
+        const int upto = issued;
+        if (issued < max_launches) issue_chunk();
+        DL_HIP(hipMemcpyAsync(h, ctl + kCtlChanged + checked, sizeof(int) * (upto - checked),
+                              hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        for (int L = checked; L < upto; ++L)
+            if (h[L - checked] == 0) {
+                conv = L;
+                break;
+            }
+        if (conv >= 0) break;
+        checked = upto;
+        if (checked >= max_launches)
+            throw runtime_error("flood fill did not converge within the iteration bound");
+    }
+    // Every launch after `conv` did nothing: the state arrays of parity
+    // (conv + 1) & 1 describe state conv + 1, the final one.
+    const int q = (conv + 1) & 1;
+    uint32_t* out_plane = cv.lab0;
+    incr_materialise_kernel<<<tiles, 256, 0, s>>>(cv.W, cv.H, tiles_x, IW, IH, conv + 1, csr, toff,
+                                                   ts + (kTsNext + q) * tiles,
+                                                   ts + (kTsDirty + q) * tiles, ra.pool, out_plane);
+    DL_LAUNCH_CHECK();
+    // visit total (statistics)
+    BfsStats st;
+    st.converged = conv;
+    st.launches = issued;
+    sum_ints_kernel<<<1, 1024, 0, s>>>(work.visits.as<int>(), tiles, ctl + kCtlCount + 1);
+    DL_LAUNCH_CHECK();
+    DL_HIP(hipMemcpyAsync(h + 32, ctl + kCtlCount + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    st.tile_visits = h[32];
+    if (rec_finish(rec, ra, tiles, work.scan_tmp, s)) rec.states = conv + 2;
+    rec.last_computed = st.tile_visits;
+    if (stats) *stats = st;
+    return 0;
 }
 
 }  // namespace dl

@@ -202,6 +202,7 @@ struct BfsWork {
     // false (a test hook) keeps every fill on the compare-and-select one.
     int enc = 0;
     bool allow_min_encoding = true;
+    DevBuf scan_tmp;  // scratch of the fill record's prefix scan
 };
 
 // Interior tile size for a configuration.
@@ -245,6 +246,84 @@ inline size_t bfs_fixed_counter_words(int n_launches)
 {
     return 2 * static_cast<size_t>(n_launches + 1);
 }
+
+// ---------------------------------------------------------------------------
+// Fill records: replaying only what changed (sessions).
+//
+// A record holds one fill's trajectory: for every tile and every launch that
+// changed the tile's interior, the interior after that launch (a pool slot)
+// and the launch's change mask.  The next fill of the same canvas (the
+// session's next seed set) computes only the tiles whose 3 x 3 neighbourhood
+// differs from the recorded trajectory, launch by launch, and takes every
+// other tile's state from the record: bit for bit the launch-by-launch fill.
+// States are numbered by launches done: state s = after s launches.
+struct FillRecord {
+    bool valid = false;      // csr/tile_off describe the last fill of this canvas
+    bool escaped = false;    // that fill's padding watch fired (no reuse then)
+    bool synced = false;     // prev_bitmap is the seed set of the recorded fill
+    int W = 0, H = 0, P = 0, tiles = 0;
+    BfsConfig cfg{};
+    int states = 0;          // states of the recorded fill (its launches + 1)
+    int64_t entries = 0;     // its entries
+    DevBuf pool;             // slots of IW x IH labels; slot 0 = all undefined
+    int64_t pool_cap = 0;    // slots
+    DevBuf csr[2];           // entries by tile, state order: int4 (state, mask, slot, 0)
+    DevBuf tile_off[2];      // tiles + 1 offsets into csr
+    int cur = 0;             // csr[cur] / tile_off[cur]: the record
+    DevBuf app;              // entries of the fill being recorded, append order:
+                             // int4 (tile, state, mask, slot)
+    DevBuf rank;             // their per-tile ranks
+    int64_t ent_cap = 0;
+    DevBuf tstate;           // per tile: entry cursors, dirty and mask by parity, counts
+    DevBuf ctl;              // counters (slots used, entries used, overflow, list count,
+                             // per-launch changes)
+    DevBuf prev_bitmap;      // seed presence bitmap of the recorded fill
+    DevBuf list;             // tiles computed in the coming launch
+    DevBuf freelist;         // pool slots below hw that the record does not use
+    DevBuf mark;             // scratch: slots the record uses
+    int64_t hw = 1;          // slots [0, hw) have been handed out
+    int64_t free_n = 0;      // entries of freelist
+    PinnedBuf host;          // read-back of the counters
+    int64_t last_computed = 0;   // tile runs of the last fill (statistics)
+};
+
+// Device view of a record's buffers for the fill kernels.
+struct RecArgs {
+    uint32_t* pool;
+    int64_t pool_cap;            // slots
+    int4* app;                   // entries of the fill being recorded
+    int* rank;
+    int* nent;                   // per tile: entries appended so far
+    unsigned long long* used;    // [0] slots handed out by this fill, [1] entries
+    int* overflow;               // set when a slot or an entry did not fit
+    int64_t ent_cap;
+    // the j-th slot handed out is freelist[j] for j < free_n, then
+    // fresh + (j - free_n)
+    const int* freelist;
+    long long free_n;
+    long long fresh;
+};
+
+// The padded fill of a session's canvas, launch by launch, recording its
+// trajectory into rec (from scratch: the pool is reused from slot 1).  Same
+// result and plane convention as bfs_run.  The record is valid afterwards
+// unless it overflowed its capacities.
+int bfs_run_recorded(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+                     hipStream_t s, BfsStats* stats, FillRecord& rec);
+// The same fill computed from rec, the record of the previous fill of this
+// canvas: prev_bitmap and bitmap are the seed-presence bitmaps (bit
+// x * img_h + y of the img_w x img_h image, which sits at offset img_pad in
+// the canvas) of the recorded fill and of this one, and only tiles whose
+// neighbourhood differs from the record are computed.  Records this fill in
+// turn.  Returns the plane holding the labels.
+int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
+                        hipStream_t s, BfsStats* stats, FillRecord& rec,
+                        const uint32_t* prev_bitmap, const uint32_t* bitmap, int img_w,
+                        int img_h, int img_pad);
+// Can rec serve the next fill of this canvas with this shape?
+bool bfs_record_usable(const FillRecord& rec, const BfsCanvas& cv, const BfsConfig& cfg);
+// Shapes a record can be made with.
+bool bfs_record_supported(const BfsConfig& cfg, const BfsCanvas& cv);
 
 // One-thread-per-cell, one-pass-per-launch version of the same schedule.  Kept
 // as an independent second implementation for cross-checking the tiled kernel.

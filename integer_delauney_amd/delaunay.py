@@ -96,6 +96,24 @@ class Delaunay:
             raise RuntimeError("the session is closed")
         return self._h
 
+    # -- fill records (no reference counterpart) ---------------------------
+    def set_fill_record(self, enabled: bool = True):
+        """Keep each finalise's flood-fill trajectory on the device, so that
+        the next finalise recomputes only the tiles the inserted and removed
+        seeds reach (the result is the same, bit for bit; see
+        dl_session_set_fill_record).  Costs device memory; False frees it."""
+        _lib.check(self._lib.dl_session_set_fill_record(self._handle(), 1 if enabled else 0))
+
+    def fill_record_stats(self):
+        """(record held, its entries, tile computations of the last finalise's
+        flood fill)."""
+        v = ctypes.c_int(0)
+        e = ctypes.c_int64(0)
+        t = ctypes.c_int64(0)
+        _lib.check(self._lib.dl_session_fill_record_stats(self._handle(), ctypes.byref(v),
+                                                          ctypes.byref(e), ctypes.byref(t)))
+        return bool(v.value), int(e.value), int(t.value)
+
     # -- properties (bindings.cpp:220-227) --------------------------------
     @property
     def width(self) -> int:

@@ -161,16 +161,22 @@ def test_clustered_16384_2m_full_size():
         plan.append((ins, dele))
     _say(f"seeds generated in {time.perf_counter() - t0:.1f} s")
 
-    # the session: initial build, then two rounds of updates
+    # the session, keeping its fill record: initial build (recorded), then two
+    # rounds of updates, each finalised by replaying from the record
     s = dl.Delaunay(W, H, 2_020_000, P)
+    s.set_fill_record(True)
     s.insert_deferred(seeds0)
     s.finalise(as_arrays=True)
+    full = s.fill_record_stats()[2]
     for ins, dele in plan:
         s.insert_deferred(ins)
         s.remove(dele)
-    t0 = time.perf_counter()
-    verts_s, grid_s = s.finalise(as_arrays=True)
-    _say(f"session round finalised in {time.perf_counter() - t0:.2f} s")
+        t0 = time.perf_counter()
+        verts_s, grid_s = s.finalise(as_arrays=True)
+        held, entries, computed = s.fill_record_stats()
+        _say(f"session round finalised in {time.perf_counter() - t0:.2f} s: record {held} "
+             f"({entries} entries), {computed} tile computations (recorded fill {full})")
+        assert held and computed < full
     assert s.seed_count == live.shape[0]
     s.close()
 
@@ -218,3 +224,88 @@ def test_clustered_16384_2m_full_size():
     np.testing.assert_array_equal(grid[:, :, 2], tid)
     # give back the default context's 16384^2 workspaces
     _lib.check(_lib.load().dl_context_trim(_lib.context().handle))
+
+
+# --- fill records: replaying only what the changes reach ----------------------
+
+def test_fill_record_replay_equals_oracle():
+    """A session keeping its fill record recomputes, per launch, only the tiles
+    whose neighbourhood differs from the recorded trajectory.  Clustered rounds
+    at 1536^2 (as above, fewer rounds), then the edge cases: a finalise with no
+    change, inserts only, removals only, seeds on the image border and in the
+    padding's reach, a change in every tile, and a single seed left.  Every
+    result equals the oracle bit for bit, and from the second finalise on the
+    fill replays (a record is held; fewer tile computations than the first,
+    recorded, fill)."""
+    W = H = 1536
+    comps, sigma = 16, 20.0
+    O.set_threads(0)
+    seeds = clustered_seeds(W, H, 20_000, 5, components=comps, sigma=sigma)
+    rng = np.random.default_rng(31)
+    s = dl.Delaunay(W, H, 60_000, 16)
+    s.set_fill_record(True)
+    s.insert_deferred(seeds)
+    live = seeds
+
+    def check(tag):
+        m, grid = s.finalise()
+        ref_rows, ref_grid = O.triangulate(W, H, live, 16, return_rows=True)
+        held, entries, computed = s.fill_record_stats()
+        _say(f"{tag}: {live.shape[0]} seeds, record {held} ({entries} entries), "
+             f"{computed} tile computations")
+        np.testing.assert_array_equal(grid, ref_grid, err_msg=tag)
+        np.testing.assert_array_equal(_rows(m), ref_rows, err_msg=tag)
+        assert held, tag
+        return computed
+
+    full = check("recorded fill")
+    computed = []
+    for r in range(4):
+        ins, dele, live = _round_plan(W, H, live, 1_000, 500, rng, 200 + r, comps, sigma, 5)
+        s.insert_deferred(ins)
+        s.remove(dele)
+        computed.append(check(f"round {r}"))
+    assert max(computed) < full
+    # no change at all: a finalise with nothing pending returns the result
+    # as it is; force a recompute with an insert and its removal
+    extra = clustered_more(W, H, live, 1, 778, components=comps, sigma=sigma, centres_seed=5)
+    s.insert_deferred(extra)
+    s.remove(extra)
+    check("no net change")
+    # inserts only / removals only
+    ins = clustered_more(W, H, live, 300, 777, components=comps, sigma=sigma, centres_seed=5)
+    s.insert_deferred(ins)
+    live = np.concatenate([live, ins])
+    check("inserts only")
+    drop = rng.choice(live.shape[0], size=300, replace=False)
+    s.remove(np.ascontiguousarray(live[drop]))
+    keep = np.ones(live.shape[0], bool)
+    keep[drop] = False
+    live = np.ascontiguousarray(live[keep])
+    check("removals only")
+    # seeds on the border rows and columns (the padding watch's reach)
+    have = set(map(tuple, live.tolist()))
+    border = [(x, y) for x, y in [(0, 0), (W - 1, 0), (0, H - 1), (W - 1, H - 1), (0, 700),
+                                  (W - 1, 31), (500, 0), (900, H - 1), (1, 1), (W - 2, H - 2)]
+              if (x, y) not in have]
+    border = np.array(border, np.int32)
+    s.insert_deferred(border)
+    live = np.concatenate([live, border])
+    check("border seeds")
+    # a change in every tile: uniform inserts over the whole image
+    have = set(map(tuple, live.tolist()))
+    cand = np.stack([rng.integers(0, W, 6000), rng.integers(0, H, 6000)], 1).astype(np.int32)
+    uni = []
+    for x, y in cand.tolist():
+        if (x, y) not in have:
+            have.add((x, y))
+            uni.append((x, y))
+    uni = np.array(uni, np.int32)
+    s.insert_deferred(uni)
+    live = np.concatenate([live, uni])
+    check("changes everywhere")
+    # down to a single seed
+    s.remove(np.ascontiguousarray(live[1:]))
+    live = np.ascontiguousarray(live[:1])
+    check("single seed")
+    s.close()
