@@ -1235,10 +1235,11 @@ constexpr int kTsNent = 6;    // [tiles] entries of the fill being recorded
 constexpr int kTsWords = 7;
 
 // Control words (FillRecord::ctl): 64-bit slot / entry counters, overflow,
-// list count, then one change flag per launch.
+// two scratch words, then one change flag per launch and one list count per
+// launch (both cleared once per fill).
 constexpr int kCtlUsed = 0;     // 2 x uint64
 constexpr int kCtlOverflow = 4;
-constexpr int kCtlCount = 5;
+constexpr int kCtlCount = 5;    // scratch: visit total, free-slot count
 constexpr int kCtlChanged = 8;
 
 __global__ void rec_reset_kernel(int* nent, int tiles, uint32_t* pool0, int slot_cells,
@@ -1309,7 +1310,7 @@ __global__ void incr_init_kernel(const int* __restrict__ tile_off, int* ts, int 
         ts[kTsMask * tiles + i] = 0;
         ts[kTsNent * tiles + i] = 0;
     }
-    if (i < n_flags) ctl[kCtlChanged + i] = 0;
+    if (i < 2 * n_flags) ctl[kCtlChanged + i] = 0;   // change flags and list counts
 }
 
 // Tiles holding a seed that only one of the two bitmaps has differ at state 0.
@@ -1338,7 +1339,7 @@ __global__ __launch_bounds__(256) void incr_plan_kernel(
     int tiles_x, int tiles_y, int L, const int4* __restrict__ csr, const int* __restrict__ tile_off,
     const int* __restrict__ next_cur, int* __restrict__ next_nxt, const int* __restrict__ dirty_cur,
     int* __restrict__ dirty_nxt, const int* __restrict__ mask_cur, int* __restrict__ mask_nxt,
-    int* __restrict__ list, int* ctl, RecArgs rec)
+    int* __restrict__ list, int* count, int* ctl, RecArgs rec)
 {
     __shared__ int wave_n[4], base;
     int* changed = ctl + kCtlChanged;
@@ -1363,9 +1364,15 @@ __global__ __launch_bounds__(256) void incr_plan_kernel(
         for (int k = 0; k < 9; ++k) {
             const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
             if (nx < 0 || ny < 0 || nx >= tiles_x || ny >= tiles_y) continue;
-            const int u = ny * tiles_x + nx;
-            nd |= dirty_cur[u];
-            if (L > 0) {
+            nd |= dirty_cur[ny * tiles_x + nx];
+        }
+        // (activity only matters next to a tile that differs from the record)
+        if (nd && L > 0) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
+                if (nx < 0 || ny < 0 || nx >= tiles_x || ny >= tiles_y) continue;
+                const int u = ny * tiles_x + nx;
                 const int eu = next_cur[u];
                 const int old_m = (eu < tile_off[u + 1] && csr[eu].x == L) ? csr[eu].y : 0;
                 act |= ((mask_cur[u] | old_m) & kFacing[k]) ? 1 : 0;
@@ -1391,7 +1398,7 @@ __global__ __launch_bounds__(256) void incr_plan_kernel(
     __syncthreads();
     if (threadIdx.x == 0) {
         const int n = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
-        base = n ? atomicAdd(ctl + kCtlCount, n) : 0;
+        base = n ? atomicAdd(count, n) : 0;
     }
     __syncthreads();
     if (comp) {
@@ -1457,7 +1464,7 @@ RecArgs rec_prepare(FillRecord& rec, const BfsConfig& cfg, int tiles, int max_la
     rec.app.reserve(sizeof(int4) * static_cast<size_t>(rec.ent_cap));
     rec.rank.reserve(sizeof(int) * static_cast<size_t>(rec.ent_cap));
     rec.tstate.reserve(sizeof(int) * kTsWords * static_cast<size_t>(tiles));
-    rec.ctl.reserve(sizeof(int) * (kCtlChanged + static_cast<size_t>(max_launches) + 8));
+    rec.ctl.reserve(sizeof(int) * (kCtlChanged + 2 * static_cast<size_t>(max_launches) + 8));
     rec.list.reserve(sizeof(int) * static_cast<size_t>(tiles));
     rec.host.reserve(sizeof(int) * 64);
     RecArgs a{};
@@ -1632,15 +1639,14 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         DL_LAUNCH_CHECK();
     }
     const BfsCanvas* dcv = reinterpret_cast<const BfsCanvas*>(work.canvases.ptr);
+    int* counts = ctl + kCtlChanged + max_launches;   // per launch, cleared above
     auto issue = [&](int L) {
         const int p = L & 1, q = p ^ 1;
         ProfScope ps(kKFill, s);
-        zero_ints_kernel<<<1, 64, 0, s>>>(ctl + kCtlCount, 1);
-        DL_LAUNCH_CHECK();
         incr_plan_kernel<<<(tiles + 255) / 256, 256, 0, s>>>(
             tiles_x, tiles_y, L, csr, toff, ts + (kTsNext + p) * tiles, ts + (kTsNext + q) * tiles,
             ts + (kTsDirty + p) * tiles, ts + (kTsDirty + q) * tiles, ts + (kTsMask + p) * tiles,
-            ts + (kTsMask + q) * tiles, list, ctl, ra);
+            ts + (kTsMask + q) * tiles, list, counts + L, ctl, ra);
         DL_LAUNCH_CHECK();
         TileExtra ex{};
         ex.rec = ra;
@@ -1651,23 +1657,21 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         ex.dirty_cur = ts + (kTsDirty + p) * tiles;
         ex.dirty_next = ts + (kTsDirty + q) * tiles;
         ex.mask_next = ts + (kTsMask + q) * tiles;
-        kIncrKernel<<<kListGrid, v->waves * 64, 0, s>>>(dcv, 1, list, ctl + kCtlCount, p, nullptr,
+        kIncrKernel<<<kListGrid, v->waves * 64, 0, s>>>(dcv, 1, list, counts + L, p, nullptr,
                                                         nullptr, ctl + kCtlChanged + L,
                                                         work.visits.as<int>(), ex);
         DL_LAUNCH_CHECK();
     };
-    // Launches in chunks; the host reads a chunk's change flags while the next
-    // one runs.  Launches after convergence do nothing.
-    constexpr int kChunk = 8;
+    // The recorded fill's launch count plus a margin is issued without a wait
+    // (launches after convergence do nothing); the change flags are read once,
+    // and a fill that needs more continues in chunks.
+    rec.host.reserve(sizeof(int) * (static_cast<size_t>(max_launches) + 64));
     int* h = rec.host.as<int>();
     int issued = 0, checked = 0, conv = -1;
-    auto issue_chunk = [&]() {
-        for (int j = 0; j < kChunk && issued < max_launches; ++j) issue(issued++);
-    };
-    issue_chunk();
+    const int first = std::min(max_launches, rec.states + 2 + rec.states / 16);
+    while (issued < first) issue(issued++);
     while (conv < 0) {
         const int upto = issued;
-        if (issued < max_launches) issue_chunk();
         DL_HIP(hipMemcpyAsync(h, ctl + kCtlChanged + checked, sizeof(int) * (upto - checked),
                               hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
@@ -1680,6 +1684,7 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         checked = upto;
         if (checked >= max_launches)
             throw runtime_error("flood fill did not converge within the iteration bound");
+        for (int j = 0; j < 8 && issued < max_launches; ++j) issue(issued++);
     }
     // Every launch after `conv` did nothing: the state arrays of parity
     // (conv + 1) & 1 describe state conv + 1, the final one.
@@ -1695,9 +1700,9 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
     st.launches = issued;
     sum_ints_kernel<<<1, 1024, 0, s>>>(work.visits.as<int>(), tiles, ctl + kCtlCount + 1);
     DL_LAUNCH_CHECK();
-    DL_HIP(hipMemcpyAsync(h + 32, ctl + kCtlCount + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(h, ctl + kCtlCount + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
-    st.tile_visits = h[32];
+    st.tile_visits = h[0];
     if (rec_finish(rec, ra, tiles, work.scan_tmp, s)) rec.states = conv + 2;
     rec.last_computed = st.tile_visits;
     if (stats) *stats = st;
