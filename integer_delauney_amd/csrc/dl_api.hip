@@ -119,11 +119,21 @@ void DevBuf::reserve(size_t n)
         return;
     }
     if (g_capture_log) throw capture_abort("buffer growth during stream capture");
+    const size_t old = ptr && !arena ? bytes : 0;
     release();
     g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
-    const size_t want = std::max<size_t>(n, 256);
+    size_t want = std::max<size_t>(n, 256);
     void* p = nullptr;
-    const hipError_t e = hipMalloc(&p, want);
+    // A buffer that grows takes half its old size again (at most 1 GiB more),
+    // so that sizes creeping up from call to call do not reallocate each time.
+    const size_t grown = std::max(want, old + std::min<size_t>(old / 2, size_t(1) << 30));
+    hipError_t e = hipMalloc(&p, grown);
+    if (e == hipSuccess) {
+        want = grown;
+    } else if (grown > want) {
+        (void)hipGetLastError();
+        e = hipMalloc(&p, want);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         char buf[256];
@@ -156,10 +166,14 @@ void PinnedBuf::reserve(size_t n)
         return;
     }
     note_realloc();
-    if (ptr) (void)hipHostFree(ptr);
+    const size_t old = ptr && !arena ? bytes : 0;
+    if (ptr && !arena) (void)hipHostFree(ptr);
     ptr = nullptr;
     bytes = 0;
-    const size_t want = std::max<size_t>(n, 256);
+    arena = false;
+    // (grows like DevBuf)
+    const size_t want =
+        std::max(std::max<size_t>(n, 256), old + std::min<size_t>(old / 2, size_t(1) << 30));
     DL_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
     bytes = want;
 }
@@ -2189,11 +2203,14 @@ int dl_session_set_fill_record(dl_session* s, int enabled)
         if (!s->record) {
             FillRecord& r = s->rec;
             for (DevBuf* d : {&r.pool, &r.csr[0], &r.csr[1], &r.tile_off[0], &r.tile_off[1],
-                              &r.app, &r.rank, &r.tstate, &r.ctl, &r.prev_bitmap, &r.list})
+                              &r.app, &r.rank, &r.tstate, &r.ctl, &r.prev_bitmap, &r.list,
+                              &r.freelist, &r.mark})
                 d->release();
             r.valid = r.synced = r.escaped = false;
             r.pool_cap = r.ent_cap = r.entries = r.last_computed = 0;
-            r.states = 0;
+            r.states = r.cur = 0;
+            r.hw = 1;
+            r.free_n = 0;
         }
     });
 }

@@ -1310,7 +1310,7 @@ __global__ void incr_init_kernel(const int* __restrict__ tile_off, int* ts, int 
         ts[kTsMask * tiles + i] = 0;
         ts[kTsNent * tiles + i] = 0;
     }
-    if (i < 2 * n_flags) ctl[kCtlChanged + i] = 0;   // change flags and list counts
+    if (i < 3 * n_flags) ctl[kCtlChanged + i] = 0;   // change flags, list counts, busy flags
 }
 
 // Tiles holding a seed that only one of the two bitmaps has differ at state 0.
@@ -1333,17 +1333,20 @@ __global__ void incr_seed_diff_kernel(const uint32_t* __restrict__ prev,
 }
 
 // Launch L's plan: which tiles to compute; the others' next dirty flag and
-// change mask; record entries of the tiles that follow the record.  Nothing
-// after convergence (the previous launch changed nothing).
+// change mask; record entries of the tiles that follow the record.
+// busy[L] is set unless the fill is quiet: nothing to compute in this launch
+// and no record tile next to a tile near a dirty one changes in a later
+// launch, so that no later launch computes anything either.  Nothing after
+// convergence (the previous launch changed nothing) or a quiet launch.
 __global__ __launch_bounds__(256) void incr_plan_kernel(
     int tiles_x, int tiles_y, int L, const int4* __restrict__ csr, const int* __restrict__ tile_off,
     const int* __restrict__ next_cur, int* __restrict__ next_nxt, const int* __restrict__ dirty_cur,
     int* __restrict__ dirty_nxt, const int* __restrict__ mask_cur, int* __restrict__ mask_nxt,
-    int* __restrict__ list, int* count, int* ctl, RecArgs rec)
+    int* __restrict__ list, int* count, int* ctl, int* busy, RecArgs rec)
 {
     __shared__ int wave_n[4], base;
     int* changed = ctl + kCtlChanged;
-    if (L > 0 && changed[L - 1] == 0) return;   // uniform: converged
+    if (L > 0 && (changed[L - 1] == 0 || busy[L - 1] == 0)) return;   // uniform
     const int tiles = tiles_x * tiles_y;
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1367,18 +1370,21 @@ __global__ __launch_bounds__(256) void incr_plan_kernel(
             nd |= dirty_cur[ny * tiles_x + nx];
         }
         // (activity only matters next to a tile that differs from the record)
+        bool later = false;   // a neighbour's record changes in a later launch
         if (nd && L > 0) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
                 const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
                 if (nx < 0 || ny < 0 || nx >= tiles_x || ny >= tiles_y) continue;
                 const int u = ny * tiles_x + nx;
-                const int eu = next_cur[u];
-                const int old_m = (eu < tile_off[u + 1] && csr[eu].x == L) ? csr[eu].y : 0;
+                const int eu = next_cur[u], end = tile_off[u + 1];
+                const int old_m = (eu < end && csr[eu].x == L) ? csr[eu].y : 0;
                 act |= ((mask_cur[u] | old_m) & kFacing[k]) ? 1 : 0;
+                later |= end > tile_off[u] && csr[end - 1].x > L;
             }
         }
         comp = nd && act;
+        if (comp || later) busy[L] = 1;
         if (!comp) {
             const int dc = dirty_cur[t];
             dirty_nxt[t] = dc;
@@ -1424,7 +1430,52 @@ __global__ __launch_bounds__(256) void incr_plan_kernel(
     }
 }
 
-// Clean tiles take the record's state at state `st` (into `plane`).
+// After a quiet launch (state st), the rest of the fill is the record's
+// where it agrees with the record and the current state elsewhere: each
+// tile's record entries of a later state join the new record.
+__global__ __launch_bounds__(256) void incr_tail_kernel(int tiles, int st,
+                                                        const int4* __restrict__ csr,
+                                                        const int* __restrict__ tile_off,
+                                                        const int* __restrict__ next_ent, RecArgs rec)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int e = 0, n = 0;
+    if (t < tiles) {
+        e = next_ent[t];   // first entry of a state >= st
+        const int end = tile_off[t + 1];
+        if (e < end && csr[e].x == st) ++e;   // (state st joined in the last launch)
+        n = end - e;
+    }
+    int incl = n;   // inclusive prefix over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int tot = __shfl(incl, 63, 64);
+    if (tot == 0) return;
+    unsigned long long b0 = 0ull;
+    if (lane == 63) b0 = atomicAdd(rec.used + 1, static_cast<unsigned long long>(tot));
+    b0 = __shfl(b0, 63, 64);
+    if (n == 0) return;
+    const unsigned long long id0 = b0 + static_cast<unsigned long long>(incl - n);
+    const int r0 = rec.nent[t];
+    for (int k = 0; k < n; ++k) {
+        const unsigned long long id = id0 + k;
+        if (id < static_cast<unsigned long long>(rec.ent_cap)) {
+            const int4 c = csr[e + k];
+            rec.app[id] = make_int4(t, c.x, c.y, c.z);
+            rec.rank[id] = r0 + k;
+        } else {
+            *rec.overflow = 1;
+        }
+    }
+    rec.nent[t] = r0 + n;
+}
+
+// Clean tiles take the record's state at state `st` (into `plane`).  With
+// next_ent = tile_off + 1 they take the record's last (final) state.
 __global__ __launch_bounds__(256) void incr_materialise_kernel(
     int W, int H, int tiles_x, int IW, int IH, int st, const int4* __restrict__ csr,
     const int* __restrict__ tile_off, const int* __restrict__ next_ent,
@@ -1464,7 +1515,7 @@ RecArgs rec_prepare(FillRecord& rec, const BfsConfig& cfg, int tiles, int max_la
     rec.app.reserve(sizeof(int4) * static_cast<size_t>(rec.ent_cap));
     rec.rank.reserve(sizeof(int) * static_cast<size_t>(rec.ent_cap));
     rec.tstate.reserve(sizeof(int) * kTsWords * static_cast<size_t>(tiles));
-    rec.ctl.reserve(sizeof(int) * (kCtlChanged + 2 * static_cast<size_t>(max_launches) + 8));
+    rec.ctl.reserve(sizeof(int) * (kCtlChanged + 3 * static_cast<size_t>(max_launches) + 8));
     rec.list.reserve(sizeof(int) * static_cast<size_t>(tiles));
     rec.host.reserve(sizeof(int) * 64);
     RecArgs a{};
@@ -1628,7 +1679,7 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
     int* list = rec.list.as<int>();
     work.visits.reserve(sizeof(int) * static_cast<size_t>(tiles));
     fill_u32(work.visits.as<uint32_t>(), 0u, static_cast<size_t>(tiles), s);
-    incr_init_kernel<<<(std::max(tiles, max_launches) + 255) / 256, 256, 0, s>>>(
+    incr_init_kernel<<<(std::max(tiles, 3 * max_launches) + 255) / 256, 256, 0, s>>>(
         toff, ts, tiles, ctl, max_launches);
     DL_LAUNCH_CHECK();
     {
@@ -1639,14 +1690,16 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         DL_LAUNCH_CHECK();
     }
     const BfsCanvas* dcv = reinterpret_cast<const BfsCanvas*>(work.canvases.ptr);
-    int* counts = ctl + kCtlChanged + max_launches;   // per launch, cleared above
+    int* changed = ctl + kCtlChanged;                  // per launch, cleared above
+    int* counts = changed + max_launches;
+    int* busy = changed + 2 * max_launches;
     auto issue = [&](int L) {
         const int p = L & 1, q = p ^ 1;
         ProfScope ps(kKFill, s);
         incr_plan_kernel<<<(tiles + 255) / 256, 256, 0, s>>>(
             tiles_x, tiles_y, L, csr, toff, ts + (kTsNext + p) * tiles, ts + (kTsNext + q) * tiles,
             ts + (kTsDirty + p) * tiles, ts + (kTsDirty + q) * tiles, ts + (kTsMask + p) * tiles,
-            ts + (kTsMask + q) * tiles, list, counts + L, ctl, ra);
+            ts + (kTsMask + q) * tiles, list, counts + L, ctl, busy, ra);
         DL_LAUNCH_CHECK();
         TileExtra ex{};
         ex.rec = ra;
@@ -1658,42 +1711,66 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         ex.dirty_next = ts + (kTsDirty + q) * tiles;
         ex.mask_next = ts + (kTsMask + q) * tiles;
         kIncrKernel<<<kListGrid, v->waves * 64, 0, s>>>(dcv, 1, list, counts + L, p, nullptr,
-                                                        nullptr, ctl + kCtlChanged + L,
+                                                        nullptr, changed + L,
                                                         work.visits.as<int>(), ex);
         DL_LAUNCH_CHECK();
     };
-    // The recorded fill's launch count plus a margin is issued without a wait
-    // (launches after convergence do nothing); the change flags are read once,
-    // and a fill that needs more continues in chunks.
-    rec.host.reserve(sizeof(int) * (static_cast<size_t>(max_launches) + 64));
+    // Launches are issued a window ahead of the flags read back: the host
+    // waits for the flags of launches [checked, upto) while the window runs.
+    // The replay ends at the first launch that changed nothing (converged) or
+    // that was quiet; launches issued past that point do nothing.
+    constexpr int kWindow = 16;
+    rec.host.reserve(sizeof(int) * (2 * static_cast<size_t>(max_launches) + 64));
     int* h = rec.host.as<int>();
-    int issued = 0, checked = 0, conv = -1;
-    const int first = std::min(max_launches, rec.states + 2 + rec.states / 16);
-    while (issued < first) issue(issued++);
-    while (conv < 0) {
-        const int upto = issued;
-        DL_HIP(hipMemcpyAsync(h, ctl + kCtlChanged + checked, sizeof(int) * (upto - checked),
+    int* h_busy = h + max_launches;
+    const hipEvent_t polled = rec.polled.get();
+    int issued = 0, checked = 0, stop = -1;
+    bool converged = false;
+    while (issued < std::min(max_launches, kWindow)) issue(issued++);
+    while (stop < 0) {
+        const int upto = issued, n = upto - checked;
+        DL_HIP(hipMemcpyAsync(h + checked, changed + checked, sizeof(int) * n,
                               hipMemcpyDeviceToHost, s));
-        DL_HIP(hipStreamSynchronize(s));
-        for (int L = checked; L < upto; ++L)
-            if (h[L - checked] == 0) {
-                conv = L;
-                break;
+        DL_HIP(hipMemcpyAsync(h_busy + checked, busy + checked, sizeof(int) * n,
+                              hipMemcpyDeviceToHost, s));
+        DL_HIP(hipEventRecord(polled, s));
+        for (int j = 0; j < kWindow && issued < max_launches; ++j) issue(issued++);
+        DL_HIP(hipEventSynchronize(polled));
+        for (int L = checked; L < upto && stop < 0; ++L) {
+            if (h[L] == 0) {
+                stop = L;
+                converged = true;
+            } else if (h_busy[L] == 0) {
+                stop = L;
             }
-        if (conv >= 0) break;
+        }
         checked = upto;
-        if (checked >= max_launches)
+        if (stop < 0 && checked >= max_launches)
             throw runtime_error("flood fill did not converge within the iteration bound");
-        for (int j = 0; j < 8 && issued < max_launches; ++j) issue(issued++);
     }
-    // Every launch after `conv` did nothing: the state arrays of parity
-    // (conv + 1) & 1 describe state conv + 1, the final one.
-    const int q = (conv + 1) & 1;
+    // The state arrays of parity (stop + 1) & 1 describe state stop + 1.
+    const int q = (stop + 1) & 1;
     uint32_t* out_plane = cv.lab0;
-    incr_materialise_kernel<<<tiles, 256, 0, s>>>(cv.W, cv.H, tiles_x, IW, IH, conv + 1, csr, toff,
-                                                   ts + (kTsNext + q) * tiles,
-                                                   ts + (kTsDirty + q) * tiles, ra.pool, out_plane);
-    DL_LAUNCH_CHECK();
+    int conv = stop;
+    if (converged) {
+        incr_materialise_kernel<<<tiles, 256, 0, s>>>(
+            cv.W, cv.H, tiles_x, IW, IH, conv + 1, csr, toff, ts + (kTsNext + q) * tiles,
+            ts + (kTsDirty + q) * tiles, ra.pool, out_plane);
+        DL_LAUNCH_CHECK();
+    } else {
+        // No tile is computed after the quiet launch `stop`: a dirty tile's
+        // state (in both planes) is final, a clean one follows the record,
+        // which changes until its own convergence.  The record's later entries
+        // join the new record and its final states are the clean tiles'.
+        conv = std::max(rec.states - 2, stop + 1);
+        incr_tail_kernel<<<(tiles + 255) / 256, 256, 0, s>>>(tiles, stop + 1, csr, toff,
+                                                              ts + (kTsNext + q) * tiles, ra);
+        DL_LAUNCH_CHECK();
+        incr_materialise_kernel<<<tiles, 256, 0, s>>>(cv.W, cv.H, tiles_x, IW, IH, conv + 1, csr,
+                                                       toff, toff + 1, ts + (kTsDirty + q) * tiles,
+                                                       ra.pool, out_plane);
+        DL_LAUNCH_CHECK();
+    }
     // visit total (statistics)
     BfsStats st;
     st.converged = conv;

@@ -257,6 +257,23 @@ inline size_t bfs_fixed_counter_words(int n_launches)
 // differs from the recorded trajectory, launch by launch, and takes every
 // other tile's state from the record: bit for bit the launch-by-launch fill.
 // States are numbered by launches done: state s = after s launches.
+// An event created on first use and destroyed with its owner.
+struct HipEvent {
+    hipEvent_t e = nullptr;
+    HipEvent() = default;
+    HipEvent(const HipEvent&) = delete;
+    HipEvent& operator=(const HipEvent&) = delete;
+    ~HipEvent()
+    {
+        if (e) (void)hipEventDestroy(e);
+    }
+    hipEvent_t get()
+    {
+        if (!e) DL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    }
+};
+
 struct FillRecord {
     bool valid = false;      // csr/tile_off describe the last fill of this canvas
     bool escaped = false;    // that fill's padding watch fired (no reuse then)
@@ -284,6 +301,7 @@ struct FillRecord {
     int64_t hw = 1;          // slots [0, hw) have been handed out
     int64_t free_n = 0;      // entries of freelist
     PinnedBuf host;          // read-back of the counters
+    HipEvent polled;         // replay: the change flags read back so far have landed
     int64_t last_computed = 0;   // tile runs of the last fill (statistics)
 };
 
