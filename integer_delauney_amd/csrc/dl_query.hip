@@ -15,8 +15,6 @@
 // reference's locate is tested against (tests/test_scalar_field.py:252-258).
 #include "dl_internal.h"
 
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 
 namespace dl {
@@ -367,18 +365,129 @@ __global__ void split_kernel(const int32_t* __restrict__ grid, int64_t n, int32_
     }
 }
 
-// Ascending radix sort of n 64-bit keys over bits [0, end_bit): in -> out.
-void sort_keys(const uint64_t* in, uint64_t* out, int64_t n, int end_bit, DevBuf& tmp,
-               hipStream_t s)
+// ---------------------------------------------------------------------------
+// LSD radix sort of 64-bit keys, 8 bits per pass.  A pass is a digit
+// histogram per tile of kSortTile keys (digit-major, so that one exclusive
+// scan gives every tile's start for every digit), the scan, and a stable
+// scatter: a tile's keys are taken in rounds of 256 in their original order,
+// and within a round a key's place among equal digits is its wave's match
+// rank (eight ballots) plus the counts of the waves before it.
+// ---------------------------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 4;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+constexpr int kSortWaves = kSortThreads / 64;
+
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t* __restrict__ keys,
+                                                                  int64_t n, int shift,
+                                                                  uint32_t dmask, int tiles,
+                                                                  uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
+    h[tid] = 0u;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+#pragma unroll 4
+    for (int k = 0; k < kSortRounds; ++k) {
+        const int64_t i = base + k * kSortThreads + tid;
+        if (i < n) atomicAdd(&h[static_cast<uint32_t>(keys[i] >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+    hist[static_cast<int64_t>(tid) * tiles + blockIdx.x] = h[tid];
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const uint64_t* __restrict__ in, uint64_t* __restrict__ out, int64_t n, int shift,
+    uint32_t dmask, int tiles, const uint32_t* __restrict__ start)
+{
+    __shared__ uint32_t run[256];                 // next output position per digit
+    __shared__ uint32_t wcnt[kSortWaves][256];    // this round's count per wave and digit
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    run[tid] = start[static_cast<int64_t>(tid) * tiles + blockIdx.x];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    for (int k = 0; k < kSortRounds; ++k) {
+        const int64_t i = base + k * kSortThreads + tid;
+        const bool valid = i < n;
+        const uint64_t key = valid ? in[i] : 0ull;
+        const uint32_t d = static_cast<uint32_t>(key >> shift) & dmask;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t v = __ballot(bit);
+            m &= bit ? v : ~v;
+        }
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0u;
+        __syncthreads();
+        const uint32_t rank = static_cast<uint32_t>(__popcll(m & lt));
+        if (valid && rank == 0u) wcnt[wave][d] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            out[pos] = key;
+        }
+        __syncthreads();
+        uint32_t add = 0u;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+// Ascending stable radix sort of n 64-bit keys on their bits [0, low_end)
+// and [high_begin, high_end): in -> out (in is left as it was).  Keys whose
+// other bits differ may come out in either order of those bits; callers pass
+// the ranges their keys use.  tmp holds the second key buffer and the
+// histograms.
+void sort_keys(const uint64_t* in, uint64_t* out, int64_t n, int low_end, int high_begin,
+               int high_end, DevBuf& tmp, DevBuf& scan_tmp, hipStream_t s)
 {
     if (n <= 0) return;
     if (n > static_cast<int64_t>(INT32_MAX)) throw value_error("too many keys to sort");
-    size_t bytes = 0;
-    DL_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, in, out, static_cast<int>(n), 0,
-                                             end_bit, s));
-    tmp.reserve(std::max<size_t>(bytes, 16));
-    DL_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.ptr, bytes, in, out, static_cast<int>(n), 0,
-                                             end_bit, s));
+    low_end = std::max(0, std::min(low_end, 64));
+    high_begin = std::max(low_end, std::min(high_begin, 64));
+    high_end = std::max(high_begin, std::min(high_end, 64));
+    // passes: (shift, bits), up to 8 bits each, low range first
+    int shifts[16], widths[16], passes = 0;
+    for (int b = 0; b < low_end; b += 8) {
+        shifts[passes] = b;
+        widths[passes++] = std::min(8, low_end - b);
+    }
+    for (int b = high_begin; b < high_end; b += 8) {
+        shifts[passes] = b;
+        widths[passes++] = std::min(8, high_end - b);
+    }
+    if (passes == 0) {
+        DL_HIP(hipMemcpyAsync(out, in, sizeof(uint64_t) * static_cast<size_t>(n),
+                              hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    const int tiles = static_cast<int>((n + kSortTile - 1) / kSortTile);
+    const size_t key_bytes = (sizeof(uint64_t) * static_cast<size_t>(n) + 255) / 256 * 256;
+    const size_t hist_words = 256 * static_cast<size_t>(tiles) + 1;
+    tmp.reserve(key_bytes + 2 * sizeof(uint32_t) * hist_words);
+    uint64_t* alt = tmp.as<uint64_t>();
+    uint32_t* hist = reinterpret_cast<uint32_t*>(static_cast<char*>(tmp.ptr) + key_bytes);
+    uint32_t* start = hist + hist_words;
+    // the last pass writes `out`
+    const uint64_t* src = in;
+    for (int p = 0; p < passes; ++p) {
+        uint64_t* dst = ((passes - 1 - p) % 2 == 0) ? out : alt;
+        const int shift = shifts[p];
+        const uint32_t dmask = (1u << widths[p]) - 1u;
+        radix_hist_kernel<<<tiles, kSortThreads, 0, s>>>(src, n, shift, dmask, tiles, hist);
+        DL_LAUNCH_CHECK();
+        exclusive_scan_u32(hist, 256 * static_cast<int64_t>(tiles), start, scan_tmp, s);
+        radix_scatter_kernel<<<tiles, kSortThreads, 0, s>>>(src, dst, n, shift, dmask, tiles,
+                                                            start);
+        DL_LAUNCH_CHECK();
+        src = dst;
+    }
 }
 
 uint32_t read_u32(const uint32_t* d, QueryWork& w, hipStream_t s)
@@ -446,7 +555,7 @@ int64_t build_edges(const SessionView& v, DevBuf& edges, QueryWork& w, hipStream
     uint64_t* k1 = w.b.as<uint64_t>();
     edge_keys_kernel<<<blocks_of(T), 256, 0, s>>>(v.rows, T, ins, b, k0);
     DL_LAUNCH_CHECK();
-    sort_keys(k0, k1, nk, 2 * b, w.sort_tmp, s);
+    sort_keys(k0, k1, nk, 2 * b, 64, 64, w.sort_tmp, w.scan_tmp, s);
     w.c.reserve(sizeof(uint32_t) * static_cast<size_t>(nk));
     w.d.reserve(sizeof(uint32_t) * static_cast<size_t>(nk + 1));
     uint32_t* flag = w.c.as<uint32_t>();
@@ -485,12 +594,18 @@ int64_t select_midpoints(const SessionView& v, const int32_t* d_edges, int64_t n
     uint64_t* k1 = w.b.as<uint64_t>();
     score_key_kernel<<<blocks_of(n_edges), 256, 0, s>>>(score, n_edges, threshold, k0);
     DL_LAUNCH_CHECK();
-    sort_keys(k0, k1, n_edges, 64, w.sort_tmp, s);
+    // (key: score word above, edge index below)
+    sort_keys(k0, k1, n_edges, bits_for(static_cast<uint64_t>(n_edges)), 32, 64, w.sort_tmp,
+              w.scan_tmp, s);
     // midpoints of the first `take`, keyed by (pixel, rank), sorted
     midpoint_kernel<<<blocks_of(cap), 256, 0, s>>>(k1, cap, d_edges, v.list, v.W, v.H, v.P, v.bitmap,
                                                    k0);
     DL_LAUNCH_CHECK();
-    sort_keys(k0, k1, cap, 64, w.sort_tmp, s);
+    // (key: pixel above, rank below; a dead key has every bit set, so it stays
+    // above every live key on these bits too)
+    sort_keys(k0, k1, cap, bits_for(static_cast<uint64_t>(cap)), 32,
+              32 + bits_for(static_cast<uint64_t>(v.W) * static_cast<uint64_t>(v.H)), w.sort_tmp,
+              w.scan_tmp, s);
     w.c.reserve(sizeof(uint32_t) * static_cast<size_t>(cap));
     w.d.reserve(sizeof(uint32_t) * static_cast<size_t>(cap + 1));
     uint32_t* flag = w.c.as<uint32_t>();

@@ -72,6 +72,20 @@ class TestEdges:
         want[length < MIN_LEN] = 0.0
         np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
 
+    def test_large_mesh_equals_the_oracle(self):
+        """120 k edge keys (about 120 tiles of the device radix sort, 2 x 17
+        bits in 5 passes) and the selection's two sorts over 64-bit keys, at
+        1024 x 768 with 20,000 seeds."""
+        w, h = 1024, 768
+        mesh, pts, vals = make_mesh(20_000, seed=21, w=w, h=h, max_seeds=20_000)
+        rows, _, _ = oracle_state(pts, w, h)
+        e = mesh.get_edges()
+        np.testing.assert_array_equal(e, Q.edges(rows, pts))
+        for take in (5_000, 100_000):
+            got = mesh.select_midpoints(MIN_LEN, take)
+            np.testing.assert_array_equal(
+                got, Q.select_midpoints(w, h, P, pts, vals, e, MIN_LEN, take))
+
     def test_edges_are_the_triangles_edges(self):
         mesh, pts, _ = make_mesh(300, seed=3)
         tris = mesh.get_triangles()
