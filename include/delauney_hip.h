@@ -406,6 +406,16 @@ int dl_set_fill_list_threshold(dl_context* ctx, int min_tiles);
  * same either way.  No reference counterpart. */
 int dl_set_fill_min_encoding(dl_context* ctx, int enabled);
 
+/* Test hook: enabled = 0 keeps every flood fill's padding watch on its
+ * per-row form; 1 (the default) lets fills whose watched interiors begin and
+ * end on the tile kernel's wave-row boundaries (the default tile shape, the
+ * min-based update, P and H - P both multiples of 8) test one row per y pass
+ * instead.  Labels are the same either way, and so is the watch's flag.
+ * dl_get_fill_aligned_watch reports which form the context's last fill used.
+ * No reference counterpart. */
+int dl_set_fill_aligned_watch(dl_context* ctx, int enabled);
+int dl_get_fill_aligned_watch(dl_context* ctx, int* aligned);
+
 /* Cross-checking only: makes speculative sizes too small on purpose, so that
  * every call takes the exact-size rerun.  enabled: 0 off; 1 every speculative
  * size of detection and assignment; 2 only the capacity of the registration

@@ -122,6 +122,8 @@ _SIGNATURES = {
     "dl_set_fill_list_threshold": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_set_tiny_capacities": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_set_fill_min_encoding": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_set_fill_aligned_watch": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_get_fill_aligned_watch": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
     "dl_debug_fill_watch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                            ctypes.c_int, _I32P, ctypes.POINTER(ctypes.c_int)]),
 }

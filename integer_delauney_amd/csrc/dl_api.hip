@@ -1544,6 +1544,22 @@ int dl_set_fill_min_encoding(dl_context* ctx, int enabled)
     });
 }
 
+int dl_set_fill_aligned_watch(dl_context* ctx, int enabled)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->bfs.allow_aligned_watch = enabled != 0;
+    });
+}
+
+int dl_get_fill_aligned_watch(dl_context* ctx, int* aligned)
+{
+    return guarded([&] {
+        if (!ctx || !aligned) throw value_error("null argument");
+        *aligned = ctx->bfs.aligned ? 1 : 0;
+    });
+}
+
 int dl_set_fill_prediction(dl_context* ctx, int enabled)
 {
     return guarded([&] {

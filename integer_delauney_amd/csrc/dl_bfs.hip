@@ -183,7 +183,13 @@ struct Watch {
 //    change (q = chy of the previous iteration or chx of this one, all false).
 // A tile where no wave has q set is a fixed point: the remaining iterations
 // are skipped.
-template <int NW, int R, int K, bool F>
+//
+// AW (aligned watch): every wave's rows lie wholly inside or wholly outside
+// the watched interior's rows (bfs_watch_aligned), so the watch of passes 1-2
+// needs no row condition (the lane masks of waves outside it are zero), and
+// the interior's first / last row is a wave's first / last row: passes 3-4
+// test that one row.  The general form keeps a word of row bits per pass.
+template <int NW, int R, int K, bool F, bool AW = false>
 __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], int c0, int y0,
                                         const Watch& wt,
                                         uint32_t (*ldsA)[64][2], uint32_t (*ldsB)[64][2],
@@ -205,7 +211,8 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
         bool cx = false;
         if (chx || chy) {   // wave-uniform
             bool ch = false;
-            uint32_t hr = 0u;   // rows with a watched update in pass 1 or 2
+            uint32_t hr = 0u;    // rows with a watched update in pass 1 or 2
+            uint64_t wm = 0ull;  // (AW) watched lanes updated in pass 1 or 2
             // pass 1: source x-1
             uint32_t left = far_inv<F>();
 #pragma unroll
@@ -215,7 +222,10 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t1 = relax_key<F>(K1[r], own0, DL_P1(r));
                 const bool t0 = relax_key<F>(K0[r], left, DL_P0(r));
                 ch |= t0 | t1;
-                hr |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) ? (2u << r) : 0u;
+                if constexpr (AW)
+                    wm |= (__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1);
+                else
+                    hr |= ((__ballot(t0) & wt.l0) | (__ballot(t1) & wt.l1)) ? (2u << r) : 0u;
             }
             // pass 2: source x+1
             uint32_t right = far_inv<F>();
@@ -226,9 +236,15 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t0 = relax_key<F>(K0[r], own1, DL_P0(r));
                 const bool t1 = relax_key<F>(K1[r], right, DL_P1(r));
                 ch |= t0 | t1;
-                hr |= ((__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1)) ? (2u << r) : 0u;
+                if constexpr (AW)
+                    wm |= (__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1);
+                else
+                    hr |= ((__ballot(t0) & wt.r0) | (__ballot(t1) & wt.r1)) ? (2u << r) : 0u;
             }
-            esc |= (hr & (wt.rows_in << 1)) != 0u;
+            if constexpr (AW)
+                esc |= wm != 0ull;
+            else
+                esc |= (hr & (wt.rows_in << 1)) != 0u;
             cx = __ballot(ch) != 0ull;
         }
         // pass 3: source y-1.  Publish this wave's last row and its q flag.
@@ -244,6 +260,7 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                            (wave < NW - 1 && lds_q[k & 1][wave + 1]);
         bool ch = false;
         uint32_t h3 = 0u, h4 = 0u;   // rows with an update of a watched column, pass 3 / 4
+        uint64_t w3 = 0ull, w4 = 0ull;   // (AW) watched lanes of row 0 (pass 3) / R - 1 (pass 4)
         if (run_y) {
             const uint32_t top0 = wave > 0 ? ldsA[wave - 1][lane][0] : far_inv<F>();
             const uint32_t top1 = wave > 0 ? ldsA[wave - 1][lane][1] : far_inv<F>();
@@ -254,7 +271,11 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t0 = relax_key<F>(K0[r], a0, DL_P0(r));
                 const bool t1 = relax_key<F>(K1[r], a1, DL_P1(r));
                 ch |= t0 | t1;
-                h3 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
+                if constexpr (AW) {
+                    if (r == 0) w3 = (__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1);
+                } else {
+                    h3 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
+                }
             }
         }
         // pass 4: source y+1
@@ -271,12 +292,19 @@ __device__ __forceinline__ void iterate(uint64_t (&K0)[R], uint64_t (&K1)[R], in
                 const bool t0 = relax_key<F>(K0[r], b0, DL_P0(r));
                 const bool t1 = relax_key<F>(K1[r], b1, DL_P1(r));
                 ch |= t0 | t1;
-                h4 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
+                if constexpr (AW) {
+                    if (r == R - 1) w4 = (__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1);
+                } else {
+                    h4 |= ((__ballot(t0) & wt.i0) | (__ballot(t1) & wt.i1)) ? (2u << r) : 0u;
+                }
             }
         }
         // only the interior's first row (pass 3) and last row (pass 4) are watched
-        esc |= (wt.r_top >= 0 && wt.r_top < R && ((h3 >> (wt.r_top + 1)) & 1u)) ||
-               (wt.r_bot >= 0 && wt.r_bot < R && ((h4 >> (wt.r_bot + 1)) & 1u));
+        if constexpr (AW)
+            esc |= (wt.r_top == 0 && w3 != 0ull) || (wt.r_bot == R - 1 && w4 != 0ull);
+        else
+            esc |= (wt.r_top >= 0 && wt.r_top < R && ((h3 >> (wt.r_top + 1)) & 1u)) ||
+                   (wt.r_bot >= 0 && wt.r_bot < R && ((h4 >> (wt.r_bot + 1)) & 1u));
         chx = cx;
         chy = __ballot(ch) != 0ull;
     }
@@ -324,7 +352,7 @@ __device__ __forceinline__ int rec_slot_at(const TileExtra& ex, int u)
 // plane where they differ from the record and from the record's pool
 // elsewhere, the result is compared with the record's next state, and the
 // tile is recorded as in MODE 1.
-template <int NW, int R, int K, bool F, int MODE = 0>
+template <int NW, int R, int K, bool F, int MODE = 0, bool AW = false>
 __device__ __forceinline__ void tile_body(
     const BfsCanvas* __restrict__ cvs, int n_canvas, int tile, int src_plane,
     const int* __restrict__ flags_prev, int* __restrict__ flags_cur, int first_launch,
@@ -470,10 +498,12 @@ __device__ __forceinline__ void tile_body(
     Watch wt;
     const bool on = tile_watched(P, W, H, tx, ty, IW, IH, K);   // block-uniform
     const uint64_t onm = on ? ~0ull : 0ull;
-    wt.l0 = __ballot(c0 == P) & onm;
-    wt.l1 = __ballot(c0 + 1 == P) & onm;
-    wt.r0 = __ballot(c0 == W - P - 1) & onm;
-    wt.r1 = __ballot(c0 + 1 == W - P - 1) & onm;
+    // (AW: the column masks of passes 1-2 only in waves whose rows are inside)
+    const uint64_t xm = (!AW || (y0 >= P && y0 < H - P)) ? onm : 0ull;
+    wt.l0 = __ballot(c0 == P) & xm;
+    wt.l1 = __ballot(c0 + 1 == P) & xm;
+    wt.r0 = __ballot(c0 == W - P - 1) & xm;
+    wt.r1 = __ballot(c0 + 1 == W - P - 1) & xm;
     wt.i0 = __ballot(c0 >= P && c0 < W - P) & onm;
     wt.i1 = __ballot(c0 + 1 >= P && c0 + 1 < W - P) & onm;
     wt.r_top = on ? P - y0 : -1;
@@ -482,7 +512,7 @@ __device__ __forceinline__ void tile_body(
 #pragma unroll
     for (int r = 0; r < R; ++r)
         if (on && y0 + r >= P && y0 + r < H - P) wt.rows_in |= 1u << r;
-    iterate<NW, R, K, F>(K0, K1, c0, y0, wt, ldsA, ldsB, lds_any, wave, lane, esc);
+    iterate<NW, R, K, F, AW>(K0, K1, c0, y0, wt, ldsA, ldsB, lds_any, wave, lane, esc);
 
     // Write the interior back and note whether it changed.  The row origin is
     // passed through an empty asm statement so that the compiler recomputes
@@ -661,7 +691,7 @@ __device__ __forceinline__ void tile_body(
 template <int NW, int R>
 constexpr int kFillWavesPerEu = (NW <= 16 && R <= 8) ? 8 : 1;
 
-template <int NW, int R, int K, bool F, int MODE = 0>
+template <int NW, int R, int K, bool F, int MODE = 0, bool AW = false>
 //
 // prev_changed (nullable): the previous launch's change counter.  When it is
 // 0 the fill has converged (no cell changed anywhere, so both planes hold the
@@ -679,7 +709,7 @@ __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_tile_ke
     const int b = blockIdx.x;
     const int tile = (b % 8) * per_xcd + b / 8;
     if (tile >= total_tiles) return;
-    tile_body<NW, R, K, F, MODE>(cvs, n_canvas, tile, src_plane, flags_prev, flags_cur,
+    tile_body<NW, R, K, F, MODE, AW>(cvs, n_canvas, tile, src_plane, flags_prev, flags_cur,
                                  first_launch, changed_any, visits, ex);
 }
 
@@ -688,7 +718,7 @@ __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_tile_ke
 // On large canvases late launches have few active tiles, and a block per tile
 // would spend most of the launch starting and retiring idle blocks.
 // MODE 2 (replay): the list is the plan kernel's and ex describes the record.
-template <int NW, int R, int K, bool F, int MODE = 0>
+template <int NW, int R, int K, bool F, int MODE = 0, bool AW = false>
 __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_list_kernel(
     const BfsCanvas* __restrict__ cvs, int n_canvas, const int* __restrict__ list,
     const int* __restrict__ count, int src_plane, const int* __restrict__ flags_prev,
@@ -700,8 +730,8 @@ __global__ __launch_bounds__(NW * 64, (kFillWavesPerEu<NW, R>)) void bfs_list_ke
     const int lo = static_cast<int>(static_cast<int64_t>(n) * xcd / 8);
     const int hi = static_cast<int>(static_cast<int64_t>(n) * (xcd + 1) / 8);
     for (int i = lo + rank; i < hi; i += per) {
-        tile_body<NW, R, K, F, MODE>(cvs, n_canvas, list[i], src_plane, flags_prev, flags_cur, 0,
-                                     changed_any, visits, ex);
+        tile_body<NW, R, K, F, MODE, AW>(cvs, n_canvas, list[i], src_plane, flags_prev, flags_cur,
+                                         0, changed_any, visits, ex);
         __syncthreads();   // the next tile reuses the LDS exchange buffers
     }
 }
@@ -863,23 +893,31 @@ using ListKernel = void (*)(const BfsCanvas*, int, const int*, const int*, int, 
                             int*, int*, TileExtra);
 
 // fn / list_fn[e]: e = 1 for the min encoding (canvases within
-// bfs_min_encoding_ok), 0 for the compare-and-select one.
+// bfs_min_encoding_ok), 0 for the compare-and-select one.  fn_aw / list_fn_aw:
+// the min encoding with the aligned watch (the default shape only; null
+// elsewhere).
 struct Variant {
     int waves, rows, iters;
     TileKernel fn[2];
     ListKernel list_fn[2];
+    TileKernel fn_aw;
+    ListKernel list_fn_aw;
 };
 
 #define DL_VARIANT(W_, R_, K_)                                                       \
     {W_, R_, K_, {bfs_tile_kernel<W_, R_, K_, false>, bfs_tile_kernel<W_, R_, K_, true>}, \
-     {bfs_list_kernel<W_, R_, K_, false>, bfs_list_kernel<W_, R_, K_, true>}}
+     {bfs_list_kernel<W_, R_, K_, false>, bfs_list_kernel<W_, R_, K_, true>}, nullptr, nullptr}
+#define DL_VARIANT_AW(W_, R_, K_)                                                    \
+    {W_, R_, K_, {bfs_tile_kernel<W_, R_, K_, false>, bfs_tile_kernel<W_, R_, K_, true>}, \
+     {bfs_list_kernel<W_, R_, K_, false>, bfs_list_kernel<W_, R_, K_, true>},              \
+     bfs_tile_kernel<W_, R_, K_, true, 0, true>, bfs_list_kernel<W_, R_, K_, true, 0, true>}
 
 // Instantiated configurations.  The tile is 128 wide; height waves*rows.
 const Variant kVariants[] = {
     DL_VARIANT(16, 8, 8), DL_VARIANT(16, 8, 7),  DL_VARIANT(16, 8, 4),  DL_VARIANT(16, 8, 12),
     DL_VARIANT(16, 8, 16),
     DL_VARIANT(8, 8, 4),  DL_VARIANT(8, 8, 5),   DL_VARIANT(8, 8, 6),   DL_VARIANT(8, 8, 7),
-    DL_VARIANT(8, 8, 8),  DL_VARIANT(8, 16, 8),  DL_VARIANT(16, 4, 4),  DL_VARIANT(16, 4, 8),
+    DL_VARIANT_AW(8, 8, 8), DL_VARIANT(8, 16, 8), DL_VARIANT(16, 4, 4), DL_VARIANT(16, 4, 8),
 };
 
 const Variant* find_variant(const BfsConfig& c)
@@ -907,7 +945,9 @@ struct RecKernels {
     TileKernel fn;
     ListKernel list_fn;
 };
-const RecKernels kRecKernels = {bfs_tile_kernel<8, 8, 8, true, 1>, bfs_list_kernel<8, 8, 8, true, 1>};
+const RecKernels kRecKernels[2] = {
+    {bfs_tile_kernel<8, 8, 8, true, 1>, bfs_list_kernel<8, 8, 8, true, 1>},
+    {bfs_tile_kernel<8, 8, 8, true, 1, true>, bfs_list_kernel<8, 8, 8, true, 1, true>}};
 
 // rec (nullable): record this launch into the record ex describes.
 void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_canvas, int total,
@@ -931,11 +971,16 @@ void launch_tiles(const Variant* v, const BfsConfig& cfg, BfsWork& work, int n_c
         bfs_active_list_kernel<<<(total + 255) / 256, 256, 0, s>>>(cv, n_canvas, total, fprev, fcur,
                                                                    list, listcnt, prev_changed);
         DL_LAUNCH_CHECK();
-        (rec ? kRecKernels.list_fn : v->list_fn[work.enc])<<<kListGrid, v->waves * 64, 0, s>>>(
+        const ListKernel lk = rec ? kRecKernels[work.aligned ? 1 : 0].list_fn
+                                  : (work.aligned && v->list_fn_aw ? v->list_fn_aw
+                                                                   : v->list_fn[work.enc]);
+        lk<<<kListGrid, v->waves * 64, 0, s>>>(
             cv, n_canvas, list, listcnt, src_plane, fprev, fcur, changed, visits, ex);
     } else {
         const int grid_blocks = (total + 7) / 8 * 8;
-        (rec ? kRecKernels.fn : v->fn[work.enc])<<<grid_blocks, v->waves * 64, 0, s>>>(
+        const TileKernel tk = rec ? kRecKernels[work.aligned ? 1 : 0].fn
+                                  : (work.aligned && v->fn_aw ? v->fn_aw : v->fn[work.enc]);
+        tk<<<grid_blocks, v->waves * 64, 0, s>>>(
             cv, n_canvas, total, src_plane, fprev, fcur, launch == 0 ? 1 : 0, changed, visits,
             launch > 0 ? prev_changed : nullptr, ex);
     }
@@ -996,8 +1041,10 @@ static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     // The planes hold labels, not keys, so the key encoding may differ from
     // one fill (or launch) to the next.
     bool min_ok = work.allow_min_encoding;
+    bool aligned = work.allow_aligned_watch;
     for (auto& c : canvases) {
         min_ok = min_ok && bfs_min_encoding_ok(c.W, c.H);
+        aligned = aligned && bfs_watch_aligned(c.escape ? c.P : 0, c.H, cfg);
         c.tiles_x = (c.W + iw - 1) / iw;
         c.tiles_y = (c.H + ih - 1) / ih;
         c.tile_base = total;
@@ -1006,6 +1053,7 @@ static int bfs_setup(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
     }
     if (total == 0) return 0;
     work.enc = min_ok ? 1 : 0;
+    work.aligned = aligned && min_ok;   // (aligned kernels use the min encoding)
     const int n_canvas = static_cast<int>(canvases.size());
     work.canvases.reserve(sizeof(BfsCanvas) * n_canvas);
     work.host_canvases.reserve(sizeof(BfsCanvas) * n_canvas);
@@ -1493,7 +1541,8 @@ __global__ __launch_bounds__(256) void incr_materialise_kernel(
     }
 }
 
-const ListKernel kIncrKernel = bfs_list_kernel<8, 8, 8, true, 2>;
+const ListKernel kIncrKernel[2] = {bfs_list_kernel<8, 8, 8, true, 2>,
+                                   bfs_list_kernel<8, 8, 8, true, 2, true>};
 
 int tiles_of(const BfsCanvas& cv, const BfsConfig& cfg, int* tx, int* ty)
 {
@@ -1710,9 +1759,8 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         ex.dirty_cur = ts + (kTsDirty + p) * tiles;
         ex.dirty_next = ts + (kTsDirty + q) * tiles;
         ex.mask_next = ts + (kTsMask + q) * tiles;
-        kIncrKernel<<<kListGrid, v->waves * 64, 0, s>>>(dcv, 1, list, counts + L, p, nullptr,
-                                                        nullptr, changed + L,
-                                                        work.visits.as<int>(), ex);
+        kIncrKernel[work.aligned ? 1 : 0]<<<kListGrid, v->waves * 64, 0, s>>>(
+            dcv, 1, list, counts + L, p, nullptr, nullptr, changed + L, work.visits.as<int>(), ex);
         DL_LAUNCH_CHECK();
     };
     // Launches are issued a window ahead of the flags read back: the host

@@ -202,12 +202,32 @@ struct BfsWork {
     // false (a test hook) keeps every fill on the compare-and-select one.
     int enc = 0;
     bool allow_min_encoding = true;
+    // Padding watch of the fill (set per fill): true when every watched
+    // canvas's interior rows begin and end on wave-row boundaries
+    // (bfs_watch_aligned), so the tile kernels test one row per y pass and no
+    // row condition in the x passes.  allow_aligned_watch false (a test hook)
+    // keeps every fill on the per-row form.
+    bool aligned = false;
+    bool allow_aligned_watch = true;
     DevBuf scan_tmp;  // scratch of the fill record's prefix scan
 };
 
 // Interior tile size for a configuration.
 inline int bfs_tile_iw(const BfsConfig& c) { return 128 - 2 * c.iters; }
 inline int bfs_tile_ih(const BfsConfig& c) { return c.waves * c.rows - 2 * c.iters; }
+
+// A wave of a tile holds rows [y0, y0 + rows) with y0 = ty * IH - iters +
+// w * rows.  When IH is a multiple of rows every y0 is -iters modulo rows, and
+// the watched interior [P, H - P) of a canvas of height H starts and ends on
+// wave boundaries when P and H - P are such rows.  (P <= 0: nothing watched.)
+inline bool bfs_watch_aligned(int P, int H, const BfsConfig& c)
+{
+    if (P <= 0) return true;
+    const int R = c.rows, K = c.iters, IH = bfs_tile_ih(c);
+    if (R <= 0 || IH % R != 0) return false;
+    auto boundary = [&](int y) { return ((y + K) % R + R) % R == 0; };
+    return boundary(P) && boundary(H - P);
+}
 
 bool bfs_config_supported(const BfsConfig& c);
 

@@ -448,6 +448,89 @@ def test_active_tile_list_fill_equals_oracle(case):
         _lib.check(lib.dl_set_fill_list_threshold(ctx.handle, -1))
 
 
+def _aligned_used():
+    import ctypes
+    v = ctypes.c_int(-1)
+    _lib.check(_lib.load().dl_get_fill_aligned_watch(_lib.context().handle, ctypes.byref(v)))
+    return v.value
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_aligned_watch_equals_the_per_row_watch(seed):
+    """Canvases whose watched box starts and ends on the tile kernel's
+    wave-row boundaries (margin and height multiples of 8) take the aligned
+    padding watch: one row per y pass, no row condition in the x passes.  On
+    boxes with seeds on both sides its flag must equal the per-row watch's
+    (dl_set_fill_aligned_watch(0)) on the same input, catch every crossing
+    the oracle counts, and the labels must equal the oracle's.  A box that
+    is not aligned keeps the per-row form."""
+    lib = _lib.load()
+    ctx = _lib.context()
+    rng = np.random.default_rng(900 + seed)
+    seen_cross = seen_clear = 0
+    for k in range(10):
+        margin = int(rng.choice([8, 16, 24, 40]))
+        W = int(rng.integers(2 * margin + 8, 700))
+        H = 8 * int(rng.integers((2 * margin) // 8 + 2, 64))
+        n = int(rng.integers(2, max(3, W * H // int(rng.choice([20, 300, 5000])))))
+        idx = rng.choice(W * H, size=n, replace=False)
+        s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+        cross = O.voronoi_crossings(W, H, O.sort_seeds(W, H, s), margin)
+        ref = O.voronoi(W, H, s)
+        got, esc = _fill_watch(W, H, s, margin)
+        assert _aligned_used() == 1, f"W={W} H={H} margin={margin}"
+        _lib.check(lib.dl_set_fill_aligned_watch(ctx.handle, 0))
+        try:
+            got_r, esc_r = _fill_watch(W, H, s, margin)
+            assert _aligned_used() == 0
+        finally:
+            _lib.check(lib.dl_set_fill_aligned_watch(ctx.handle, 1))
+        np.testing.assert_array_equal(got, ref)
+        np.testing.assert_array_equal(got_r, ref)
+        assert esc == esc_r, f"W={W} H={H} n={n} margin={margin}: {esc} vs {esc_r}"
+        assert esc or cross == 0, f"W={W} H={H} n={n} margin={margin} crossings={cross}"
+        seen_cross += cross > 0
+        seen_clear += not esc
+    assert seen_cross > 0
+    # not aligned: the height leaves the box's end off a wave boundary
+    W, H, margin = 200, 8 * 20 + 3, 16
+    idx = rng.choice(W * H, size=300, replace=False)
+    s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    got, _ = _fill_watch(W, H, s, margin)
+    assert _aligned_used() == 0
+    np.testing.assert_array_equal(got, O.voronoi(W, H, s))
+
+
+def test_pipeline_takes_the_aligned_watch_exactly():
+    """The pipeline's padded canvas (P = 16, image height a multiple of 8, as
+    at 4096 x 4096) takes the aligned watch; goldens and oracle cases agree
+    with the per-row watch bit for bit."""
+    lib = _lib.load()
+    ctx = _lib.context()
+    rng = np.random.default_rng(77)
+    W, H = 520, 384
+    idx = rng.choice(W * H, size=3000, replace=False)
+    s = np.stack([idx % W, idx // W], 1).astype(np.int32)
+    rows, ref = O.triangulate(W, H, s, 16, return_rows=True)
+    m, got = dl.triangulate(W, H, s, 16)
+    assert _aligned_used() == 1
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(_rows(m), rows)
+    _lib.check(lib.dl_set_fill_aligned_watch(ctx.handle, 0))
+    try:
+        m2, got2 = dl.triangulate(W, H, s, 16)
+        assert _aligned_used() == 0
+        for name in FULL:
+            g = goldens.load(name)
+            mg, tg = dl.triangulate(g["W"], g["H"], g["seeds"], _pad_arg(g["P"]))
+            np.testing.assert_array_equal(tg[:, :, :2], g["vgrid"])
+            np.testing.assert_array_equal(_rows(mg), g["rows"])
+    finally:
+        _lib.check(lib.dl_set_fill_aligned_watch(ctx.handle, 1))
+    np.testing.assert_array_equal(got2, got)
+    np.testing.assert_array_equal(_rows(m2), _rows(m))
+
+
 @pytest.mark.parametrize("case", range(10))
 def test_wide_key_encoding_equals_oracle(case):
     """Canvases with sides beyond about 23,170 use the compare-and-select key
