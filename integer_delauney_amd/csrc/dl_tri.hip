@@ -2033,10 +2033,24 @@ constexpr int kBinSlots = 1024;   // LDS table: at most 256 triangles x 4 tiles
 constexpr int kBinDirect = 4;
 constexpr uint32_t kBinEmpty = 0xFFFFFFFFu;
 
+constexpr int kBinCount = 0, kBinFill = 1, kBinFixed = 2;   // bin_kernel modes (below)
+
+// A triangle whose box covers more than kBinWide tiles (the long edges
+// between clusters of a clustered seed set reach thousands) is queued, and the
+// whole block then shares out its tiles: one thread walking 2,000 tiles with
+// an atomic whose result it waits for took most of a 0.8 ms fill pass.
+constexpr int kBinWide = 64;
+constexpr int kBinQueue = 64;
+
 struct BinLds {
     uint32_t key[kBinSlots];     // global tile index, kBinEmpty = free
     uint32_t count[kBinSlots];   // pairs of this chunk in the tile
     uint32_t base[kBinSlots];    // (fill) first list position of the chunk's pairs
+    uint32_t wide_t[kBinQueue];  // wide triangles of the chunk
+    int4 wide_box[kBinQueue];    // their tile boxes (tx0, ty0, tx1, ty1)
+    uint32_t wide_off[kBinQueue];   // their image's first tile
+    int wide_tx[kBinQueue];         // ... and tiles per row
+    int wide_n;
 };
 
 // Slot of `tile` in the chunk table (inserting it); its count is incremented
@@ -2061,6 +2075,27 @@ __device__ __forceinline__ void bin_clear(BinLds& L)
         L.key[i] = kBinEmpty;
         L.count[i] = 0u;
     }
+    if (threadIdx.x == 0) L.wide_n = 0;
+}
+
+// One (triangle, tile) pair written straight to the global bins.
+template <int MODE>
+__device__ __forceinline__ void bin_direct(uint32_t tile, uint32_t t, uint32_t* __restrict__ counts,
+                                           const uint32_t* __restrict__ offsets,
+                                           uint32_t* __restrict__ list, uint32_t cap,
+                                           uint32_t* __restrict__ flags)
+{
+    if (MODE == kBinCount) {
+        atomicAdd(&counts[tile], 1u);
+    } else if (MODE == kBinFill) {
+        const uint32_t k = offsets[tile] + atomicAdd(&counts[tile], 1u);
+        if (k < cap) list[k] = t;
+        else atomicOr(flags, kFlagOverflow);
+    } else {
+        const uint32_t k = atomicAdd(&counts[tile], 1u);
+        if (k < cap) list[static_cast<size_t>(tile) * cap + k] = t;
+        else atomicOr(flags, kFlagBinOverflow);
+    }
 }
 
 // Triangles t < min(device count, rows_cap) (the rows buffer's capacity).
@@ -2070,7 +2105,6 @@ __device__ __forceinline__ void bin_clear(BinLds& L)
 // entries [t * cap, (t + 1) * cap) and counts[t] is its cursor; entries past
 // a tile's capacity are dropped and flagged with kFlagBinOverflow (only the
 // assignment needs to run again).
-constexpr int kBinCount = 0, kBinFill = 1, kBinFixed = 2;
 
 template <int MODE>
 __global__ __launch_bounds__(256) void bin_kernel(
@@ -2096,30 +2130,46 @@ __global__ __launch_bounds__(256) void bin_kernel(
             int x0, y0, x1, y1;
             if (tri_box(v, d.W, d.H, x0, y0, x1, y1)) {
                 const int tx0 = x0 / kTS, tx1 = x1 / kTS, ty0 = y0 / kTS, ty1 = y1 / kTS;
-                const bool direct = (tx1 - tx0 + 1) * (ty1 - ty0 + 1) > kBinDirect;
-                for (int ty = ty0; ty <= ty1; ++ty)
-                    for (int tx = tx0; tx <= tx1; ++tx) {
-                        const uint32_t tile = static_cast<uint32_t>(d.tile_off + ty * d.tiles_x + tx);
-                        if (direct) {
-                            if (MODE == kBinCount) {
-                                atomicAdd(&counts[tile], 1u);
-                            } else if (MODE == kBinFill) {
-                                const uint32_t k = offsets[tile] + atomicAdd(&counts[tile], 1u);
-                                if (k < cap) list[k] = static_cast<uint32_t>(t);
-                                else atomicOr(flags, kFlagOverflow);
-                            } else {
-                                const uint32_t k = atomicAdd(&counts[tile], 1u);
-                                if (k < cap) list[static_cast<size_t>(tile) * cap + k] = static_cast<uint32_t>(t);
-                                else atomicOr(flags, kFlagBinOverflow);
-                            }
-                        } else {
-                            slot[n_local] = bin_slot(L, tile, rank[n_local]);
-                            ++n_local;
-                        }
+                const int nbox = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+                int q = -1;
+                if (nbox > kBinWide) {
+                    q = atomicAdd(&L.wide_n, 1);
+                    if (q < kBinQueue) {
+                        L.wide_t[q] = static_cast<uint32_t>(t);
+                        L.wide_box[q] = make_int4(tx0, ty0, tx1, ty1);
+                        L.wide_off[q] = static_cast<uint32_t>(d.tile_off);
+                        L.wide_tx[q] = d.tiles_x;
                     }
+                }
+                const bool direct = nbox > kBinDirect;
+                if (q < 0 || q >= kBinQueue)   // (a full queue: this thread walks them)
+                    for (int ty = ty0; ty <= ty1; ++ty)
+                        for (int tx = tx0; tx <= tx1; ++tx) {
+                            const uint32_t tile =
+                                static_cast<uint32_t>(d.tile_off + ty * d.tiles_x + tx);
+                            if (direct) {
+                                bin_direct<MODE>(tile, static_cast<uint32_t>(t), counts, offsets,
+                                                 list, cap, flags);
+                            } else {
+                                slot[n_local] = bin_slot(L, tile, rank[n_local]);
+                                ++n_local;
+                            }
+                        }
             }
         }
         __syncthreads();
+        // the chunk's wide triangles, their tiles shared out over the block
+        const int nw = min(L.wide_n, kBinQueue);
+        for (int q = 0; q < nw; ++q) {
+            const int4 bx = L.wide_box[q];
+            const int bw = bx.z - bx.x + 1;
+            const int nbox = bw * (bx.w - bx.y + 1);
+            for (int k = threadIdx.x; k < nbox; k += blockDim.x) {
+                const uint32_t tile = L.wide_off[q] +
+                                      static_cast<uint32_t>((bx.y + k / bw) * L.wide_tx[q] + bx.x + k % bw);
+                bin_direct<MODE>(tile, L.wide_t[q], counts, offsets, list, cap, flags);
+            }
+        }
         // one global atomic per distinct tile of the chunk
         for (int i = threadIdx.x; i < kBinSlots; i += blockDim.x) {
             const uint32_t tile = L.key[i];
