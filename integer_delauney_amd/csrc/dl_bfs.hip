@@ -329,22 +329,14 @@ __host__ __device__ inline bool tile_watched(int P, int W, int H, int tx, int ty
 struct TileExtra {
     RecArgs rec;
     int launch;
-    // MODE 2: the record being replayed and the per-tile replay state
-    const int4* csr;              // entries by tile: (state, mask, slot, 0)
-    const int* tile_off;
-    const int* next_ent;          // per tile: first entry of state >= launch + 1
+    // MODE 2: the per-tile replay state (see ReplayTs)
+    const int* slot_cur;          // per tile: the record's pool slot at state launch
+    const int* slot_next;         // ... at state launch + 1 (its latest entry of a
+                                  // state <= launch + 1; every tile has one of state 1)
     const int* dirty_cur;         // per tile: differs from the record at state launch
     int* dirty_next;              // ... at state launch + 1 (written for this tile)
     int* mask_next;               // this tile's change mask of this launch
 };
-
-// Pool slot of tile u at state `launch` in the record (its latest entry of a
-// state <= launch; every tile has an entry of state 1).
-__device__ __forceinline__ int rec_slot_at(const TileExtra& ex, int u)
-{
-    const int e = ex.next_ent[u];
-    return e > ex.tile_off[u] ? ex.csr[e - 1].z : 0;
-}
 
 // One launch of one tile.  MODE 0: the plain fill.  MODE 1: also records the
 // tile's interior after the launch (when it changed, and always in launch 0).
@@ -360,6 +352,7 @@ __device__ __forceinline__ void tile_body(
 {
     constexpr int IW = 128 - 2 * K;
     constexpr int IH = NW * R - 2 * K;
+    static_assert(MODE != 2 || NW >= 4, "the replay's set-up uses one lane of waves 0-3");
     __shared__ uint32_t ldsA[NW][64][2];
     __shared__ uint32_t ldsB[NW][64][2];
     __shared__ int lds_any[2][NW];   // per-wave flags (see iterate)
@@ -368,6 +361,10 @@ __device__ __forceinline__ void tile_body(
     __shared__ const uint32_t* lds_src[9];
     __shared__ int lds_pitch[9], lds_col[9], lds_row[9];
     __shared__ int lds_slot;
+    // MODE 2: the record's slot at state launch + 1, this tile's entry count,
+    // and the slot counter value taken for this tile's result
+    __shared__ int lds_cmp, lds_nent;
+    __shared__ unsigned long long lds_take;
 
     int ci = 0;
     if (n_canvas > 1) {
@@ -418,15 +415,21 @@ __device__ __forceinline__ void tile_body(
     if constexpr (MODE == 2) {
         // Sources: the plane for tiles that differ from the record (and for
         // every tile in launch 0, whose input is the seeds), the record's
-        // pool slot of state `launch` otherwise.
+        // pool slot of state `launch` otherwise.  The same round trip fetches
+        // what the end of the launch needs: the record's slot to compare
+        // with, the tile's entry count, and a pool slot for the result (taken
+        // whether or not it will be used: a slot that no entry references is
+        // free again when the record is finished).
         if (tid < 9) {
             const int nx = tx + tid % 3 - 1, ny = ty + tid / 3 - 1;
             const uint32_t* base = src_plane ? cv.lab1 : cv.lab0;
             int pitch = W, col = 0, row = 0;
             if (nx >= 0 && ny >= 0 && nx < tiles_x && ny < tiles_y) {
                 const int u = tile_base + ny * tiles_x + nx;
-                if (ex.launch > 0 && !ex.dirty_cur[u]) {
-                    base = ex.rec.pool + static_cast<size_t>(rec_slot_at(ex, u)) * (IW * IH);
+                const int dirty = ex.dirty_cur[u];
+                const int slot = ex.slot_cur[u];
+                if (ex.launch > 0 && !dirty) {
+                    base = ex.rec.pool + static_cast<size_t>(slot) * (IW * IH);
                     pitch = IW;
                     col = nx * IW;
                     row = ny * IH;
@@ -436,6 +439,12 @@ __device__ __forceinline__ void tile_body(
             lds_pitch[tid] = pitch;
             lds_col[tid] = col;
             lds_row[tid] = row;
+        } else if (tid == 64) {   // (one lane of three other waves: the loads overlap)
+            lds_cmp = ex.slot_next[tile];
+        } else if (tid == 128) {
+            lds_nent = ex.rec.nent[tile];
+        } else if (tid == 192) {
+            lds_take = atomicAdd(ex.rec.used, 1ull);
         }
         __syncthreads();
     }
@@ -530,20 +539,40 @@ __device__ __forceinline__ void tile_body(
     bool top0 = false, top1 = false;    // ... in a row of the top band
     bool bot0 = false, bot1 = false;    // ... in a row of the bottom band
     // The rows' launch-start labels, all loaded before the first store.
+    // MODE 2: with them, the record's state launch + 1 to compare with (its
+    // entry of that state, or the state before when the recorded launch
+    // changed nothing) and the pool slot this tile's result would take.
     uint32_t old0[R], old1[R];
+    uint32_t cmp0[MODE == 2 ? R : 1], cmp1[MODE == 2 ? R : 1];
+    if constexpr (MODE == 2) {
+        // (kept in LDS, not in a register that could be spilled and then
+        // reloaded after the stores)
+        if (tid == 0) {
+            const unsigned long long j = lds_take;
+            const unsigned long long fn = static_cast<unsigned long long>(ex.rec.free_n);
+            const unsigned long long sl =
+                j < fn ? static_cast<unsigned long long>(ex.rec.freelist[j])
+                       : static_cast<unsigned long long>(ex.rec.fresh) + (j - fn);
+            lds_take = sl < static_cast<unsigned long long>(ex.rec.pool_cap) ? sl : ~0ull;
+        }
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ly = wave * R + r - K;
         const int y = yb + r;
         old0[r] = old1[r] = 0u;
+        if constexpr (MODE == 2) cmp0[r] = cmp1[r] = 0u;
         if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
         const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
         if constexpr (MODE == 2) {
             // the tile's own source (the plane or its record slot)
             const uint32_t* own = lds_src[4] + static_cast<size_t>(y - lds_row[4]) * lds_pitch[4] -
                                   lds_col[4];
+            const uint32_t* cmp = ex.rec.pool + static_cast<size_t>(lds_cmp) * (IW * IH);
             if (w0) old0[r] = own[c0];
             if (w1) old1[r] = own[c0 + 1];
+            if (w0) cmp0[r] = cmp[ly * IW + lx0];
+            if (w1) cmp1[r] = cmp[ly * IW + lx0 + 1];
         } else {
             if (w0) old0[r] = src[rowoff + c0];
             if (w1) old1[r] = src[rowoff + c0 + 1];
@@ -576,15 +605,19 @@ __device__ __forceinline__ void tile_body(
             bot1 |= c1r;
         }
     }
+    // (MODE 2 stores at the end of the tile, after its record bookkeeping.)
+    auto store_rows = [&]() {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int ly = wave * R + r - K;
-        const int y = yb + r;
-        if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
-        const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
-        if (w0) dst[rowoff + c0] = nv0[r];
-        if (w1) dst[rowoff + c0 + 1] = nv1[r];
-    }
+        for (int r = 0; r < R; ++r) {
+            const int ly = wave * R + r - K;
+            const int y = yb + r;
+            if (ly < 0 || ly >= IH || y >= H) continue;   // wave-uniform
+            const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
+            if (w0) dst[rowoff + c0] = nv0[r];
+            if (w1) dst[rowoff + c0 + 1] = nv1[r];
+        }
+    };
+    if constexpr (MODE != 2) store_rows();
     if (esc && lane == 0 && cv.escape) atomicOr(cv.escape, 1);   // esc is wave-uniform
     const bool l0 = lx0 < K, l1 = lx0 + 1 < K;                  // left band columns
     const bool r0 = lx0 >= IW - K, r1 = lx0 + 1 >= IW - K;      // right band columns
@@ -599,19 +632,14 @@ __device__ __forceinline__ void tile_body(
     if (__ballot((bot0 && l0) || (bot1 && l1))) m |= kChBL;
     if (__ballot((bot0 && r0) || (bot1 && r1))) m |= kChBR;
     if constexpr (MODE == 2) {
-        // Compare with the record's state launch + 1 (its entry of that state,
-        // or the state before when the recorded launch changed nothing).
-        const int e = ex.next_ent[tile];
-        const bool has1 = e < ex.tile_off[tile + 1] && ex.csr[e].x == ex.launch + 1;
-        const int slot = has1 ? ex.csr[e].z : (e > ex.tile_off[tile] ? ex.csr[e - 1].z : 0);
-        const uint32_t* cmp = ex.rec.pool + static_cast<size_t>(slot) * (IW * IH);
+        // Compare with the record's state launch + 1.
         bool diff = false;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int ly = wave * R + r - K;
             if (ly < 0 || ly >= IH || yb + r >= H) continue;   // wave-uniform
-            if (w0) diff |= cmp[ly * IW + lx0] != nv0[r];
-            if (w1) diff |= cmp[ly * IW + lx0 + 1] != nv1[r];
+            if (w0) diff |= cmp0[r] != nv0[r];
+            if (w1) diff |= cmp1[r] != nv1[r];
         }
         if (__ballot(diff)) m |= kDiffers;
     }
@@ -638,14 +666,30 @@ __device__ __forceinline__ void tile_body(
         for (int w = 0; w < NW; ++w) any |= lds_any[0][w];
         const int mask = any & kChMask;
         if (tid == 0) {
+            int slot = -1;
             if constexpr (MODE == 2) {
+                // The entry's atomic is waited for before this lane's first
+                // store: on this GPU a wait for a returning memory operation
+                // also waits for every store issued before it.
+                const bool keep = (mask & kChAny) || ex.launch == 0;
+                const unsigned long long id = keep ? atomicAdd(ex.rec.used + 1, 1ull) : 0ull;
+                asm volatile("" ::"v"(id));
+                const unsigned long long take = lds_take;   // ~0: beyond the pool
                 ex.dirty_next[tile] = (any & kDiffers) ? 1 : 0;
                 ex.mask_next[tile] = mask;
-                visits[tile] += 1;
+                atomicAdd(visits + tile, 1);   // (statistics; no value returned)
                 if (mask & kChAny) *changed_any = 1;
-            }
-            int slot = -1;
-            if ((mask & kChAny) || ex.launch == 0) {
+                if (keep) {
+                    if (take != ~0ull && id < static_cast<unsigned long long>(ex.rec.ent_cap)) {
+                        slot = static_cast<int>(take);
+                        ex.rec.app[id] = make_int4(tile, ex.launch + 1, mask, slot);
+                        ex.rec.rank[id] = lds_nent;
+                        ex.rec.nent[tile] = lds_nent + 1;
+                    } else {
+                        *ex.rec.overflow = 1;
+                    }
+                }
+            } else if ((mask & kChAny) || ex.launch == 0) {
                 const unsigned long long j = atomicAdd(ex.rec.used, 1ull);
                 const unsigned long long id = atomicAdd(ex.rec.used + 1, 1ull);
                 const unsigned long long fn = static_cast<unsigned long long>(ex.rec.free_n);
@@ -665,6 +709,7 @@ __device__ __forceinline__ void tile_body(
         }
         __syncthreads();
         const int slot = lds_slot;
+        if constexpr (MODE == 2) store_rows();
         if (slot >= 0) {
             uint32_t* out = ex.rec.pool + static_cast<size_t>(slot) * (IW * IH);
 #pragma unroll
@@ -1280,7 +1325,12 @@ constexpr int kTsNext = 0;    // [2][tiles] first record entry of state >= L (pa
 constexpr int kTsDirty = 2;   // [2][tiles]
 constexpr int kTsMask = 4;    // [2][tiles] the new fill's change mask of the last launch
 constexpr int kTsNent = 6;    // [tiles] entries of the fill being recorded
-constexpr int kTsWords = 7;
+// The record at the tile itself, carried from launch to launch so that no
+// kernel of the replay walks from a cursor to an entry before it can load:
+constexpr int kTsSlot = 7;    // [2][tiles] the record's pool slot at state L
+constexpr int kTsRecm = 9;    // [2][tiles] the record's change mask of launch L - 1 (0: none)
+constexpr int kTsLast = 11;   // [tiles] state of the tile's last record entry
+constexpr int kTsWords = 12;
 
 // Control words (FillRecord::ctl): 64-bit slot / entry counters, overflow,
 // two scratch words, then one change flag per launch and one list count per
@@ -1348,15 +1398,19 @@ __global__ __launch_bounds__(256) void rec_free_kernel(const uint32_t* __restric
 }
 
 // Replay set-up: cursors at the first entries, nothing dirty or changed yet.
-__global__ void incr_init_kernel(const int* __restrict__ tile_off, int* ts, int tiles,
-                                 int* ctl, int n_flags)
+__global__ void incr_init_kernel(const int4* __restrict__ csr, const int* __restrict__ tile_off,
+                                 int* ts, int tiles, int* ctl, int n_flags)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < tiles) {
-        ts[kTsNext * tiles + i] = tile_off[i];
+        const int beg = tile_off[i], end = tile_off[i + 1];
+        ts[kTsNext * tiles + i] = beg;
         ts[kTsDirty * tiles + i] = 0;
         ts[kTsMask * tiles + i] = 0;
         ts[kTsNent * tiles + i] = 0;
+        ts[kTsSlot * tiles + i] = 0;   // (state 0 is read from the plane)
+        ts[kTsRecm * tiles + i] = 0;
+        ts[kTsLast * tiles + i] = end > beg ? csr[end - 1].x : 0;
     }
     if (i < 3 * n_flags) ctl[kCtlChanged + i] = 0;   // change flags, list counts, busy flags
 }
@@ -1380,100 +1434,138 @@ __global__ void incr_seed_diff_kernel(const uint32_t* __restrict__ prev,
     }
 }
 
+// Per-tile replay state of one launch: parity L (cur, state L) and L + 1 (nxt).
+struct ReplayTs {
+    const int* next_cur;   // first record entry of state >= L
+    int* next_nxt;
+    const int* dirty_cur;
+    int* dirty_nxt;
+    const int* mask_cur;   // the new fill's change mask of launch L - 1
+    int* mask_nxt;
+    const int* slot_cur;   // the record's pool slot at state L
+    int* slot_nxt;
+    const int* recm_cur;   // the record's change mask of launch L - 1
+    int* recm_nxt;
+    const int* last;       // state of the tile's last record entry
+};
+
 // Launch L's plan: which tiles to compute; the others' next dirty flag and
-// change mask; record entries of the tiles that follow the record.
+// change mask; record entries of the tiles that follow the record; every
+// tile's record slot and change mask at state L + 1.
 // busy[L] is set unless the fill is quiet: nothing to compute in this launch
 // and no record tile next to a tile near a dirty one changes in a later
 // launch, so that no later launch computes anything either.  Nothing after
 // convergence (the previous launch changed nothing) or a quiet launch.
+//
+// The kernel is a chain of dependent memory round trips on a few thousand
+// tiles, so it is built for latency: every load that depends only on the tile
+// index is issued before the uniform exit test, an entry and its successor are
+// loaded together, neighbours' record masks come from the carried per-tile
+// arrays, and the block's two atomics are issued back to back.
 __global__ __launch_bounds__(256) void incr_plan_kernel(
-    int tiles_x, int tiles_y, int L, const int4* __restrict__ csr, const int* __restrict__ tile_off,
-    const int* __restrict__ next_cur, int* __restrict__ next_nxt, const int* __restrict__ dirty_cur,
-    int* __restrict__ dirty_nxt, const int* __restrict__ mask_cur, int* __restrict__ mask_nxt,
-    int* __restrict__ list, int* count, int* ctl, int* busy, RecArgs rec)
+    int tiles_x, int tiles_y, int L, const int4* __restrict__ csr,
+    const int* __restrict__ tile_off, ReplayTs ts, int* __restrict__ list, int* count, int* ctl,
+    int* busy, RecArgs rec)
 {
-    __shared__ int wave_n[4], base;
+    __shared__ int wave_c[4], wave_f[4], base_c;
+    __shared__ unsigned long long base_f;
     int* changed = ctl + kCtlChanged;
-    if (L > 0 && (changed[L - 1] == 0 || busy[L - 1] == 0)) return;   // uniform
     const int tiles = tiles_x * tiles_y;
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    bool comp = false;
-    bool follow = false;   // a record entry of state L + 1 to reuse
-    int4 follow_e{};
-    if (t < tiles) {
-        const int e = next_cur[t], end = tile_off[t + 1];
-        const int e1 = e + ((e < end && csr[e].x == L) ? 1 : 0);
-        next_nxt[t] = e1;
-        const bool has1 = e1 < end && csr[e1].x == L + 1;
-        const int tx = t % tiles_x, ty = t / tiles_x;
-        // bit of neighbour (dx, dy) = (k % 3 - 1, k / 3 - 1) that faces this tile
-        constexpr int kFacing[9] = {kChBR, kChBottom, kChBL, kChRight, kChAny,
-                                    kChLeft, kChTR, kChTop, kChTL};
-        int nd = 0, act = L == 0 ? 1 : 0;
+    const bool own = t < tiles;
+    const int tx = t % tiles_x, ty = t / tiles_x;
+    // bit of neighbour (dx, dy) = (k % 3 - 1, k / 3 - 1) that faces this tile
+    constexpr int kFacing[9] = {kChBR, kChBottom, kChBL, kChRight, kChAny,
+                                kChLeft, kChTR, kChTop, kChTL};
+    const int prev_changed = L > 0 ? changed[L - 1] : 1;
+    const int prev_busy = L > 0 ? busy[L - 1] : 1;
+    int e = 0, end = 0, dc = 0, slot_l = 0, nent = 0, nd = 0;
+    if (own) {
+        e = ts.next_cur[t];
+        end = tile_off[t + 1];
+        dc = ts.dirty_cur[t];
+        slot_l = ts.slot_cur[t];
+        nent = rec.nent[t];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
             if (nx < 0 || ny < 0 || nx >= tiles_x || ny >= tiles_y) continue;
-            nd |= dirty_cur[ny * tiles_x + nx];
+            nd |= ts.dirty_cur[ny * tiles_x + nx];
         }
-        // (activity only matters next to a tile that differs from the record)
-        bool later = false;   // a neighbour's record changes in a later launch
-        if (nd && L > 0) {
+    }
+    if (prev_changed == 0 || prev_busy == 0) return;   // uniform
+    // The entries at the cursor and after it (state -1: none).
+    int4 c0 = make_int4(-1, 0, 0, 0), c1 = make_int4(-1, 0, 0, 0);
+    if (own && e < end) c0 = csr[e];
+    if (own && e + 1 < end) c1 = csr[e + 1];
+    // (activity only matters next to a tile that differs from the record)
+    int act = L == 0 ? 1 : 0;
+    bool later = false;   // a neighbour's record changes in a later launch
+    if (own && nd && L > 0) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
-                if (nx < 0 || ny < 0 || nx >= tiles_x || ny >= tiles_y) continue;
-                const int u = ny * tiles_x + nx;
-                const int eu = next_cur[u], end = tile_off[u + 1];
-                const int old_m = (eu < end && csr[eu].x == L) ? csr[eu].y : 0;
-                act |= ((mask_cur[u] | old_m) & kFacing[k]) ? 1 : 0;
-                later |= end > tile_off[u] && csr[end - 1].x > L;
-            }
+        for (int k = 0; k < 9; ++k) {
+            const int nx = tx + k % 3 - 1, ny = ty + k / 3 - 1;
+            if (nx < 0 || ny < 0 || nx >= tiles_x || ny >= tiles_y) continue;
+            const int u = ny * tiles_x + nx;
+            act |= ((ts.mask_cur[u] | ts.recm_cur[u]) & kFacing[k]) ? 1 : 0;
+            later |= ts.last[u] > L;
         }
-        comp = nd && act;
+    }
+    const bool at_l = c0.x == L;          // an entry of state L at the cursor
+    const int4 f1 = at_l ? c1 : c0;       // the entry after it
+    const bool has1 = f1.x == L + 1;      // ... is of state L + 1
+    const bool comp = own && nd && act;
+    bool follow = false;                  // a record entry of state L + 1 to reuse
+    if (own) {
+        ts.next_nxt[t] = e + (at_l ? 1 : 0);
+        ts.slot_nxt[t] = has1 ? f1.z : slot_l;
+        ts.recm_nxt[t] = has1 ? f1.y : 0;
         if (comp || later) busy[L] = 1;
         if (!comp) {
-            const int dc = dirty_cur[t];
-            dirty_nxt[t] = dc;
+            ts.dirty_nxt[t] = dc;
             int m = 0;
             if (!dc && has1) {
-                follow_e = csr[e1];
-                m = follow_e.y;
+                m = f1.y;
                 follow = true;
             }
-            mask_nxt[t] = m;
+            ts.mask_nxt[t] = m;
             if (m & kChAny) changed[L] = 1;
         }
     }
-    // list of tiles to compute (one atomic per block)
-    const uint64_t mc = __ballot(comp);
-    if (lane == 0) wave_n[w] = __popcll(mc);
+    // The list of tiles to compute and the followed entries: one atomic each
+    // per block.
+    const uint64_t mc = __ballot(comp), mf = __ballot(follow);
+    if (lane == 0) {
+        wave_c[w] = __popcll(mc);
+        wave_f[w] = __popcll(mf);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int n = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
-        base = n ? atomicAdd(count, n) : 0;
+        const int nc = wave_c[0] + wave_c[1] + wave_c[2] + wave_c[3];
+        const int nf = wave_f[0] + wave_f[1] + wave_f[2] + wave_f[3];
+        const int bc = atomicAdd(count, nc);
+        const unsigned long long bf = atomicAdd(rec.used + 1, static_cast<unsigned long long>(nf));
+        base_c = bc;
+        base_f = bf;
     }
     __syncthreads();
-    if (comp) {
-        int off = base;
-        for (int k = 0; k < w; ++k) off += wave_n[k];
-        list[off + __popcll(mc & ((1ull << lane) - 1ull))] = t;
+    int oc = base_c;
+    unsigned long long of = base_f;
+    for (int k = 0; k < w; ++k) {
+        oc += wave_c[k];
+        of += static_cast<unsigned long long>(wave_f[k]);
     }
-    // record entries reused from the record (one atomic per wave)
-    const uint64_t mf = __ballot(follow);
-    if (mf) {
-        unsigned long long b0 = 0ull;
-        if (lane == 0) b0 = atomicAdd(rec.used + 1, static_cast<unsigned long long>(__popcll(mf)));
-        b0 = __shfl(b0, 0, 64);
-        if (follow) {
-            const unsigned long long id = b0 + __popcll(mf & ((1ull << lane) - 1ull));
-            if (id < static_cast<unsigned long long>(rec.ent_cap)) {
-                rec.app[id] = make_int4(t, L + 1, follow_e.y, follow_e.z);
-                rec.rank[id] = rec.nent[t]++;
-            } else {
-                *rec.overflow = 1;
-            }
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (comp) list[oc + __popcll(mc & below)] = t;
+    if (follow) {
+        const unsigned long long id = of + static_cast<unsigned long long>(__popcll(mf & below));
+        if (id < static_cast<unsigned long long>(rec.ent_cap)) {
+            rec.app[id] = make_int4(t, L + 1, f1.y, f1.z);
+            rec.rank[id] = nent;
+            rec.nent[t] = nent + 1;
+        } else {
+            *rec.overflow = 1;
         }
     }
 }
@@ -1622,8 +1714,10 @@ bool rec_finish(FillRecord& rec, const RecArgs& a, int tiles, DevBuf& tmp, hipSt
     // Slots this fill took beyond the free list extend the pool; every slot
     // below the new high-water mark that the new record does not use is
     // free for the next fill.
+    // (A replay takes a slot for every tile it computes, used or not; slots
+    // at or past the capacity were never written.)
     const int64_t taken = static_cast<int64_t>(used[0]);
-    rec.hw = a.fresh + std::max<int64_t>(0, taken - a.free_n);
+    rec.hw = std::min<int64_t>(a.fresh + std::max<int64_t>(0, taken - a.free_n), rec.pool_cap);
     const size_t words = static_cast<size_t>((rec.hw + 31) / 32);
     rec.mark.reserve(sizeof(uint32_t) * words);
     rec.freelist.reserve(sizeof(int) * static_cast<size_t>(rec.pool_cap));
@@ -1729,7 +1823,7 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
     work.visits.reserve(sizeof(int) * static_cast<size_t>(tiles));
     fill_u32(work.visits.as<uint32_t>(), 0u, static_cast<size_t>(tiles), s);
     incr_init_kernel<<<(std::max(tiles, 3 * max_launches) + 255) / 256, 256, 0, s>>>(
-        toff, ts, tiles, ctl, max_launches);
+        csr, toff, ts, tiles, ctl, max_launches);
     DL_LAUNCH_CHECK();
     {
         const int64_t words = (static_cast<int64_t>(img_w) * img_h + 31) / 32;
@@ -1745,17 +1839,26 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
     auto issue = [&](int L) {
         const int p = L & 1, q = p ^ 1;
         ProfScope ps(kKFill, s);
-        incr_plan_kernel<<<(tiles + 255) / 256, 256, 0, s>>>(
-            tiles_x, tiles_y, L, csr, toff, ts + (kTsNext + p) * tiles, ts + (kTsNext + q) * tiles,
-            ts + (kTsDirty + p) * tiles, ts + (kTsDirty + q) * tiles, ts + (kTsMask + p) * tiles,
-            ts + (kTsMask + q) * tiles, list, counts + L, ctl, busy, ra);
+        ReplayTs rt{};
+        rt.next_cur = ts + (kTsNext + p) * tiles;
+        rt.next_nxt = ts + (kTsNext + q) * tiles;
+        rt.dirty_cur = ts + (kTsDirty + p) * tiles;
+        rt.dirty_nxt = ts + (kTsDirty + q) * tiles;
+        rt.mask_cur = ts + (kTsMask + p) * tiles;
+        rt.mask_nxt = ts + (kTsMask + q) * tiles;
+        rt.slot_cur = ts + (kTsSlot + p) * tiles;
+        rt.slot_nxt = ts + (kTsSlot + q) * tiles;
+        rt.recm_cur = ts + (kTsRecm + p) * tiles;
+        rt.recm_nxt = ts + (kTsRecm + q) * tiles;
+        rt.last = ts + kTsLast * tiles;
+        incr_plan_kernel<<<(tiles + 255) / 256, 256, 0, s>>>(tiles_x, tiles_y, L, csr, toff, rt,
+                                                            list, counts + L, ctl, busy, ra);
         DL_LAUNCH_CHECK();
         TileExtra ex{};
         ex.rec = ra;
         ex.launch = L;
-        ex.csr = csr;
-        ex.tile_off = toff;
-        ex.next_ent = ts + (kTsNext + q) * tiles;
+        ex.slot_cur = rt.slot_cur;
+        ex.slot_next = rt.slot_nxt;
         ex.dirty_cur = ts + (kTsDirty + p) * tiles;
         ex.dirty_next = ts + (kTsDirty + q) * tiles;
         ex.mask_next = ts + (kTsMask + q) * tiles;
