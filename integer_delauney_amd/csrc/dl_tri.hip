@@ -1218,11 +1218,20 @@ constexpr uint32_t kOwnCap = 1536;              // entries per bucket: 6 per see
 constexpr uint32_t kBlockedKey = 0xFFFFFFFFu;   // key of a blocked-triplet entry
 constexpr int kRouteSpan = 512;                 // buckets a routing block counts in LDS
 
+// What route_position produces for listed position i besides its entries:
+// the candidate records of slots 2i and 2i+1 and the scans they came from.
+// route_kernel stores them after its atomics: on this GPU a wait for a load or
+// a returning atomic also waits for every store issued before it.
+struct RouteOut {
+    Cand o0, o1;   // (two fields, not an array: a runtime index would put it in scratch)
+    uint32_t code;
+};
+
 // The entries of listed position i: bit k of the result marks entry ev[k]
 // valid; entries 0 and 1 are the candidates in slots 2i and 2i+1, entries 2
-// and 3 the triplets a four-region block blocks.  Also writes the candidate
-// records, clears the slots' first flags and records their scans.
-__device__ uint32_t route_position(const DetectArgs& A, uint32_t i, uint4 ev[4])
+// and 3 the triplets a four-region block blocks.  The candidate records and
+// the scan code go to ro.
+__device__ uint32_t route_position(const DetectArgs& A, uint32_t i, uint4 ev[4], RouteOut& ro)
 {
     const uint2 ip = A.positions[i];
     const uint32_t img = ip.x, pos = ip.y;
@@ -1231,8 +1240,10 @@ __device__ uint32_t route_position(const DetectArgs& A, uint32_t i, uint4 ev[4])
     const int ry = static_cast<int>(pos / bw);
     const int rx = static_cast<int>(pos - static_cast<uint32_t>(ry) * bw);
     const BlockCorners c = block_corners(A.grid + d.cv_off, d.Wc, rx, ry);
-    Cand* out = A.cand + 2 * static_cast<size_t>(i);
-    *reinterpret_cast<uint16_t*>(A.first + 2 * static_cast<size_t>(i)) = 0;
+    auto put = [&ro](int slot, const Cand& v) {
+        if (slot == 0) ro.o0 = v;
+        else ro.o1 = v;
+    };
     const int32_t tl = to_id(A, d, c.tl), tr = to_id(A, d, c.tr);
     const int32_t bl = to_id(A, d, c.bl), br = to_id(A, d, c.br);
     Cand cd;
@@ -1240,9 +1251,10 @@ __device__ uint32_t route_position(const DetectArgs& A, uint32_t i, uint4 ev[4])
     cd.img = img;
     if (tl < 0 || tr < 0 || bl < 0 || br < 0) {   // flagged: the host raises
         cd.key = 0xFFFFFFFFu;
-        out[0] = cd;
-        out[1] = cd;
-        A.scode[i] = 0;
+        cd.a = cd.b = cd.c = 0;
+        ro.o0 = cd;
+        ro.o1 = cd;
+        ro.code = 0;
         return 0u;
     }
     const uint32_t off = static_cast<uint32_t>(d.seed_off);
@@ -1260,11 +1272,11 @@ __device__ uint32_t route_position(const DetectArgs& A, uint32_t i, uint4 ev[4])
         cd.a = tri[k][0];
         cd.b = tri[k][1];
         cd.c = tri[k][2];
-        out[0] = cd;
+        ro.o0 = cd;
         ev[0] = entry(cd.a, cd.b, cd.c, cd.key);
         cd.key = 0xFFFFFFFFu;
-        out[1] = cd;
-        A.scode[i] = static_cast<uint8_t>(k);
+        ro.o1 = cd;
+        ro.code = static_cast<uint32_t>(k);
         return 1u;
     }
     const int32_t ids[4] = {tl, tr, bl, br};
@@ -1291,16 +1303,16 @@ __device__ uint32_t route_position(const DetectArgs& A, uint32_t i, uint4 ev[4])
         cd.a = tri[k][0];
         cd.b = tri[k][1];
         cd.c = tri[k][2];
-        out[slot] = cd;
+        put(slot, cd);
         code |= static_cast<uint32_t>(k) << (2 * slot);
         if (slot == 0) ev[0] = make_uint4(e.x, e.y, e.z, cd.key);
         else ev[1] = make_uint4(e.x, e.y, e.z, cd.key);
         ++slot;
     }
-    A.scode[i] = static_cast<uint8_t>(code);
+    ro.code = code;
     // (a four-region block leaves exactly two; the rest is defensive)
     const uint32_t m = slot == 2 ? 0xFu : (slot == 1 ? 0xDu : 0xCu);
-    for (cd.key = 0xFFFFFFFFu; slot < 2; ++slot) out[slot] = cd;
+    for (cd.key = 0xFFFFFFFFu; slot < 2; ++slot) put(slot, cd);
     return m;
 }
 
@@ -1330,7 +1342,8 @@ __global__ __launch_bounds__(kDetThreads) void route_kernel(DetectArgs A)
     if (s_skip) return;       // block-uniform (read once above)
     const uint32_t i = p0 + threadIdx.x;
     uint4 ev[4];
-    const uint32_t m = i < npos ? route_position(A, i, ev) : 0u;
+    RouteOut ro;
+    const uint32_t m = i < npos ? route_position(A, i, ev, ro) : 0u;
     uint32_t bmin = 0xFFFFFFFFu;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -1356,19 +1369,34 @@ __global__ __launch_bounds__(kDetThreads) void route_kernel(DetectArgs A)
         if (n) s_base[t] = atomicAdd(&A.fill[b0 + t], n);
     }
     __syncthreads();
+    // Every slot first (an entry outside the block's span takes its slot
+    // with an atomic), then every store.
+    uint32_t sl[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        sl[k] = 0xFFFFFFFFu;
+        if (!(m & (1u << k))) continue;
+        const uint32_t b = ev[k].x >> kOwnShift;
+        sl[k] = rank[k] != 0xFFFFFFFFu ? s_base[b - b0] + rank[k] : atomicAdd(&A.fill[b], 1u);
+    }
+    asm volatile("" ::"v"(sl[0]), "v"(sl[1]), "v"(sl[2]), "v"(sl[3]));   // (waited for here)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (!(m & (1u << k))) continue;
-        const uint32_t b = ev[k].x >> kOwnShift;
-        const uint32_t sl = rank[k] != 0xFFFFFFFFu ? s_base[b - b0] + rank[k]
-                                                   : atomicAdd(&A.fill[b], 1u);
-        if (sl >= A.own_cap) {
+        if (sl[k] >= A.own_cap) {
             atomicOr(A.flags, kFlagOverflow);
             continue;
         }
-        const size_t e = static_cast<size_t>(b) * A.own_cap + sl;
+        const size_t e = static_cast<size_t>(ev[k].x >> kOwnShift) * A.own_cap + sl[k];
         A.ent[e] = ev[k];
         A.eidx[e] = k < 2 ? 2 * i + static_cast<uint32_t>(k) : 0xFFFFFFFFu;
+    }
+    if (i < npos) {   // the candidate records, their scans, their first flags cleared
+        Cand* out = A.cand + 2 * static_cast<size_t>(i);
+        out[0] = ro.o0;
+        out[1] = ro.o1;
+        A.scode[i] = static_cast<uint8_t>(ro.code);
+        *reinterpret_cast<uint16_t*>(A.first + 2 * static_cast<size_t>(i)) = 0;
     }
 }
 
