@@ -311,6 +311,8 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
         _lib.check(lib.dl_session_insert(ctx.handle, h, _lib.i32p(seeds0), None, seeds0.shape[0]))
         _lib.check(lib.dl_session_finalise(ctx.handle, h, ctypes.byref(nt)))
         t_build = time.perf_counter() - t0
+        rs0 = [ctypes.c_int64(0) for _ in range(3)]
+        _lib.check(lib.dl_get_record_stats(ctx.handle, *[ctypes.byref(v) for v in rs0]))
         per_round = []
         fill_rounds = []   # per round: fill launches, launches to convergence, tile visits
         for ins, dele in plan:
@@ -327,6 +329,9 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
                                                         ctypes.byref(rc)))
             fill_rounds.append((la.value, cv.value, vi.value, rc.value, rv.value))
         log("update rounds done")
+        rs1 = [ctypes.c_int64(0) for _ in range(3)]
+        _lib.check(lib.dl_get_record_stats(ctx.handle, *[ctypes.byref(v) for v in rs1]))
+        retained_rounds = int(rs1[2].value - rs0[2].value)
         # bit-exact check against a fresh batch run on the final seed set
         n_live = ctypes.c_int64(0)
         _lib.check(lib.dl_session_seed_count(h, ctypes.byref(n_live)))
@@ -367,6 +372,7 @@ def bench_incremental(lib, ctx, side: int, n0: int, rounds: int, n_ins: int, n_d
             "fill_tile_computations_per_round": float(np.mean([f[3] for f in fill_rounds]))
             if record else None,
             "rounds_with_a_record": int(sum(f[4] for f in fill_rounds)) if record else None,
+            "rounds_replayed_from_the_retained_plane": retained_rounds if record else None,
             "fill_tile_visits_final_batch": batch_fill[2],
             "fill_launches_final_batch": batch_fill[0],
             "final_equals_batch_bitwise": same}

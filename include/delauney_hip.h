@@ -384,6 +384,19 @@ int dl_reset_kernel_stats(dl_context* ctx);
  * padding (cross-checking only). */
 int dl_set_force_unpadded(dl_context* ctx, int enabled);
 
+/* Replays of a session's fill record (dl_session_set_fill_record) start from
+ * the context's label plane as the previous finalise left it when nothing
+ * has written it since, seeding and writing back only the tiles the replay
+ * touches.  enabled = 0 makes every replay re-initialise the whole canvas
+ * (cross-checking only; the default is 1).  No counterpart in the reference. */
+int dl_set_replay_plane_reuse(dl_context* ctx, int enabled);
+
+/* Fills of this context, over its life: recorded (dl_session_set_fill_record,
+ * from scratch), replayed from a record, and replayed from the retained
+ * plane (a subset of the replayed).  Any pointer may be NULL.  No counterpart
+ * in the reference. */
+int dl_get_record_stats(dl_context* ctx, int64_t* recorded, int64_t* replayed, int64_t* retained);
+
 /* Flood fill of the width x height canvas (seeds as given, no padding) that
  * also reports in *escaped whether any cell of the box
  * [margin, width-margin) x [margin, height-margin) ever received an update

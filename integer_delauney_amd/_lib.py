@@ -71,6 +71,8 @@ _SIGNATURES = {
     "dl_voronoi_simple": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _I32P, ctypes.c_int64,
                                          _I32P, _I64P]),
     "dl_set_force_unpadded": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_set_replay_plane_reuse": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dl_get_record_stats": (ctypes.c_int, [_VP, _I64P, _I64P, _I64P]),
     "dl_set_kernel_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dl_get_kernel_stats": (ctypes.c_int, [_VP, _I64P, ctypes.POINTER(ctypes.c_double)]),
     "dl_reset_kernel_stats": (ctypes.c_int, [_VP]),

@@ -321,6 +321,13 @@ struct dl_context {
     int rec_w = 0, rec_h = 0;
     bool last_escaped = false;
     int64_t record_fills = 0, replay_fills = 0;
+    // padA holds the final labels of the fill recorded as generation
+    // plane_gen (FillRecord::gen; 0: no claim) at address plane_ptr.  Every
+    // fill and graph replay on the context drops the claim before it writes
+    // the planes; a recorded or replayed fill makes it again.
+    uint64_t plane_gen = 0;
+    const void* plane_ptr = nullptr;
+    int64_t retained_fills = 0;
     TriWork tri;
     BatchPlan plan;   // images of the last call
     DevBuf xy, sorted;
@@ -351,6 +358,7 @@ struct dl_context {
     std::vector<BfsCanvas> fill_cvs;   // canvases of the last padded fill
     BfsConfig last_cfg{0, 0, 0};
     bool force_unpadded = false;
+    bool force_full_replay = false;   // (test hook) replays re-initialise the planes
     std::vector<hipEvent_t> events;
     bool profiling = false;
     KernelProfiler prof;
@@ -620,6 +628,22 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
     int plane = 0;
     const auto pkey = pred_key(ctx->device, plan.n_cells, plan.n_seeds, cfg);
     const int pred = pend ? fill_prediction(ctx, cfg, tune) : 0;
+    // A session's replay may start from the first plane as its last fill left
+    // it (same record generation, same buffer); anything else re-initialises.
+    BfsCanvas probe{};
+    if (n_img == 1) {
+        probe.W = plan.img[0].Wc;
+        probe.H = plan.img[0].Hc;
+        probe.P = watch[0] >= 0 ? watch[0] : 0;
+    }
+    const bool recorded = ctx->rec && !tune && n_img == 1 && bfs_record_supported(cfg, probe);
+    const bool replay = recorded && bfs_record_usable(*ctx->rec, probe, cfg) &&
+                        ctx->rec->prev_bitmap.ptr;
+    const bool retained = replay && !ctx->force_full_replay && ctx->plane_gen != 0 &&
+                          ctx->plane_gen == ctx->rec->gen && ctx->plane_ptr == ctx->padA.ptr &&
+                          plan.img[0].cv_off == 0;
+    ctx->plane_gen = 0;
+    ctx->plane_ptr = nullptr;
     // A predicted fill's launch counters are cleared in the canvases' clear
     // launch.
     FillSeg cnt_clear{};
@@ -627,9 +651,12 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         ctx->bfs.counters.reserve(sizeof(int) * bfs_fixed_counter_words(pred));
         cnt_clear = FillSeg{ctx->bfs.counters.as<uint32_t>(), bfs_fixed_counter_words(pred), 0u};
     }
-    init_canvases(plan, ctx->sorted.as<uint32_t>(), ctx->padA.as<uint32_t>(),
-                  ctx->padB.as<uint32_t>(), ctx->tri, s, &esc_clear,
-                  pred > 0 ? &cnt_clear : nullptr);
+    if (retained)
+        fill_u32(esc_clear.p, 0u, esc_clear.n, s);
+    else
+        init_canvases(plan, ctx->sorted.as<uint32_t>(), ctx->padA.as<uint32_t>(),
+                      ctx->padB.as<uint32_t>(), ctx->tri, s, &esc_clear,
+                      pred > 0 ? &cnt_clear : nullptr);
     std::vector<BfsCanvas> cvs(n_img);
     for (int i = 0; i < n_img; ++i) {
         const ImageDesc& d = plan.img[i];
@@ -642,19 +669,25 @@ int fill_detection(dl_context* ctx, const std::vector<int>& watch, hipStream_t s
         cv.P = watch[i] >= 0 ? watch[i] : 0;
         cv.escape = watch[i] >= 0 ? ctx->esc.as<int>() + i : nullptr;
     }
-    const bool recorded = ctx->rec && !tune && n_img == 1 && bfs_record_supported(cfg, cvs[0]);
     if (recorded) {
         // A session keeping its fill record: replay from it when it describes
         // this canvas, otherwise record this fill (host waits, no prediction).
         FillRecord& rec = *ctx->rec;
-        if (bfs_record_usable(rec, cvs[0], cfg) && rec.prev_bitmap.ptr) {
+        if (replay) {
             plane = bfs_run_incremental(cvs, cfg, ctx->bfs, s, &st, rec,
                                         rec.prev_bitmap.as<uint32_t>(), ctx->rec_bitmap,
-                                        ctx->rec_w, ctx->rec_h, plan.img[0].P);
+                                        ctx->rec_w, ctx->rec_h, plan.img[0].P, retained);
             ++ctx->replay_fills;
+            if (retained) ++ctx->retained_fills;
         } else {
             plane = bfs_run_recorded(cvs, cfg, ctx->bfs, s, &st, rec);
             ++ctx->record_fills;
+        }
+        // After convergence both planes hold the final labels; the next
+        // replay of this record may start from the first one.
+        if (rec.valid && plan.img[0].cv_off == 0) {
+            ctx->plane_gen = rec.gen;
+            ctx->plane_ptr = ctx->padA.ptr;
         }
         ctx->fill_converged += st.converged + 1;
         if (visits_async) *visits_async = st.tile_visits;
@@ -1068,6 +1101,8 @@ void replay_pipeline(dl_context* ctx, hipStream_t s)
     ctx->pad_plane = g.iss.plane;
     ctx->host_small.as<int>()[64] = 0;   // visit total (see issue_pipeline)
     ctx->timings[7] += g.iss.pend.n;
+    ctx->plane_gen = 0;   // the graph's fill writes the planes
+    ctx->plane_ptr = nullptr;
     DL_HIP(hipGraphLaunch(g.exec, s));
     for (const CopyBack& c : g.copy_back)
         DL_HIP(hipMemcpyAsync(c.host, c.dev, c.n, hipMemcpyDeviceToHost, s));
@@ -1475,6 +1510,8 @@ int dl_context_trim(dl_context* ctx)
         DL_HIP(hipStreamSynchronize(ctx->stream));
         ctx->graph.reset();
         for_each_devbuf(ctx, [](DevBuf& d) { d.release(); });
+        ctx->plane_gen = 0;
+        ctx->plane_ptr = nullptr;
         ctx->n_tri = 0;
         ctx->have_padded = false;
         ctx->tri.tri_base.clear();
@@ -1618,6 +1655,24 @@ int dl_set_force_unpadded(dl_context* ctx, int enabled)
     return guarded([&] {
         if (!ctx) throw value_error("null context");
         ctx->force_unpadded = enabled != 0;
+    });
+}
+
+int dl_set_replay_plane_reuse(dl_context* ctx, int enabled)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        ctx->force_full_replay = enabled == 0;
+    });
+}
+
+int dl_get_record_stats(dl_context* ctx, int64_t* recorded, int64_t* replayed, int64_t* retained)
+{
+    return guarded([&] {
+        if (!ctx) throw value_error("null context");
+        if (recorded) *recorded = ctx->record_fills;
+        if (replayed) *replayed = ctx->replay_fills;
+        if (retained) *retained = ctx->retained_fills;
     });
 }
 

@@ -25,6 +25,7 @@
 #include "dl_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstring>
 
@@ -1330,7 +1331,8 @@ constexpr int kTsNent = 6;    // [tiles] entries of the fill being recorded
 constexpr int kTsSlot = 7;    // [2][tiles] the record's pool slot at state L
 constexpr int kTsRecm = 9;    // [2][tiles] the record's change mask of launch L - 1 (0: none)
 constexpr int kTsLast = 11;   // [tiles] state of the tile's last record entry
-constexpr int kTsWords = 12;
+constexpr int kTsSeeded = 12; // [tiles] (retained plane) seeded for launch 0 by this replay
+constexpr int kTsWords = 13;
 
 // Control words (FillRecord::ctl): 64-bit slot / entry counters, overflow,
 // two scratch words, then one change flag per launch and one list count per
@@ -1411,6 +1413,7 @@ __global__ void incr_init_kernel(const int4* __restrict__ csr, const int* __rest
         ts[kTsSlot * tiles + i] = 0;   // (state 0 is read from the plane)
         ts[kTsRecm * tiles + i] = 0;
         ts[kTsLast * tiles + i] = end > beg ? csr[end - 1].x : 0;
+        ts[kTsSeeded * tiles + i] = 0;
     }
     if (i < 3 * n_flags) ctl[kCtlChanged + i] = 0;   // change flags, list counts, busy flags
 }
@@ -1431,6 +1434,43 @@ __global__ void incr_seed_diff_kernel(const uint32_t* __restrict__ prev,
         const int x = static_cast<int>(bit / img_h), y = static_cast<int>(bit % img_h);
         if (x >= img_w) continue;         // (bits past the image are never set)
         dirty[((y + P) / IH) * tiles_x + (x + P) / IW] = 1;
+    }
+}
+
+// Replay onto a retained plane (the plane already holds the recorded fill's
+// final labels): launch 0 reads the plane only in the 3 x 3 neighbourhoods
+// of the tiles it computes, which lie next to tiles differing at state 0, so
+// only tiles within two tiles of those need the seeded canvas (undefined
+// cells and this fill's seeds, from the seed bitmap: bit x * img_h + y of the
+// image, which sits at (P, P) in the canvas).  Every other tile keeps its
+// final labels, which are this fill's too unless the replay touches it.
+__global__ __launch_bounds__(256) void incr_seed_region_kernel(
+    int W, int H, int tiles_x, int tiles_y, int IW, int IH, const int* __restrict__ dirty0,
+    int* __restrict__ seeded, const uint32_t* __restrict__ bitmap, int img_w, int img_h, int P,
+    uint32_t* __restrict__ plane)
+{
+    const int t = blockIdx.x;
+    const int tx = t % tiles_x, ty = t / tiles_x;
+    int near = 0;
+    if (threadIdx.x < 25) {
+        const int nx = tx + static_cast<int>(threadIdx.x) % 5 - 2;
+        const int ny = ty + static_cast<int>(threadIdx.x) / 5 - 2;
+        if (nx >= 0 && ny >= 0 && nx < tiles_x && ny < tiles_y) near = dirty0[ny * tiles_x + nx];
+    }
+    if (!__syncthreads_or(near)) return;   // block-uniform
+    if (threadIdx.x == 0) seeded[t] = 1;
+    const int ox = tx * IW, oy = ty * IH;
+    for (int i = threadIdx.x; i < IW * IH; i += 256) {
+        const int cx = ox + i % IW, cy = oy + i / IW;
+        if (cx >= W || cy >= H) continue;
+        uint32_t v = kInvalid;
+        const int x = cx - P, y = cy - P;
+        if (x >= 0 && y >= 0 && x < img_w && y < img_h) {
+            const int64_t bit = static_cast<int64_t>(x) * img_h + y;
+            if ((bitmap[bit >> 5] >> (bit & 31)) & 1u)
+                v = (static_cast<uint32_t>(cx) << 16) | static_cast<uint32_t>(cy);
+        }
+        plane[static_cast<size_t>(cy) * W + cx] = v;
     }
 }
 
@@ -1616,13 +1656,17 @@ __global__ __launch_bounds__(256) void incr_tail_kernel(int tiles, int st,
 
 // Clean tiles take the record's state at state `st` (into `plane`).  With
 // next_ent = tile_off + 1 they take the record's last (final) state.
+// seeded (retained plane, else null): only tiles the replay seeded or computed
+// (visits) are written; the others hold that state already.
 __global__ __launch_bounds__(256) void incr_materialise_kernel(
     int W, int H, int tiles_x, int IW, int IH, int st, const int4* __restrict__ csr,
     const int* __restrict__ tile_off, const int* __restrict__ next_ent,
-    const int* __restrict__ dirty, const uint32_t* __restrict__ pool, uint32_t* __restrict__ plane)
+    const int* __restrict__ dirty, const int* __restrict__ seeded, const int* __restrict__ visits,
+    const uint32_t* __restrict__ pool, uint32_t* __restrict__ plane)
 {
     const int t = blockIdx.x;
     if (dirty[t]) return;
+    if (seeded && !seeded[t] && !visits[t]) return;
     const int e = next_ent[t], beg = tile_off[t], end = tile_off[t + 1];
     const int slot = (e < end && csr[e].x == st) ? csr[e].z : (e > beg ? csr[e - 1].z : 0);
     const uint32_t* src = pool + static_cast<size_t>(slot) * IW * IH;
@@ -1711,6 +1755,8 @@ bool rec_finish(FillRecord& rec, const RecArgs& a, int tiles, DevBuf& tmp, hipSt
     rec.cur = nx;
     rec.entries = n;
     rec.valid = true;
+    static std::atomic<uint64_t> g_rec_gen{0};
+    rec.gen = ++g_rec_gen;
     // Slots this fill took beyond the free list extend the pool; every slot
     // below the new high-water mark that the new record does not use is
     // free for the next fill.
@@ -1796,7 +1842,7 @@ int bfs_run_recorded(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
 int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
                         hipStream_t s, BfsStats* stats, FillRecord& rec,
                         const uint32_t* prev_bitmap, const uint32_t* bitmap, int img_w, int img_h,
-                        int img_pad)
+                        int img_pad, bool retained)
 {
     if (canvases.size() != 1 || !bfs_record_usable(rec, canvases[0], cfg))
         throw runtime_error("no usable fill record for this canvas");
@@ -1830,6 +1876,13 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         incr_seed_diff_kernel<<<static_cast<unsigned>((words + 255) / 256), 256, 0, s>>>(
             prev_bitmap, bitmap, words, img_w, img_h, img_pad, IW, IH, tiles_x,
             ts + kTsDirty * tiles);
+        DL_LAUNCH_CHECK();
+    }
+    const int* seeded = retained ? ts + kTsSeeded * tiles : nullptr;
+    if (retained) {
+        incr_seed_region_kernel<<<tiles, 256, 0, s>>>(cv.W, cv.H, tiles_x, tiles_y, IW, IH,
+                                                      ts + kTsDirty * tiles, ts + kTsSeeded * tiles,
+                                                      bitmap, img_w, img_h, img_pad, cv.lab0);
         DL_LAUNCH_CHECK();
     }
     const BfsCanvas* dcv = reinterpret_cast<const BfsCanvas*>(work.canvases.ptr);
@@ -1906,7 +1959,7 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
     if (converged) {
         incr_materialise_kernel<<<tiles, 256, 0, s>>>(
             cv.W, cv.H, tiles_x, IW, IH, conv + 1, csr, toff, ts + (kTsNext + q) * tiles,
-            ts + (kTsDirty + q) * tiles, ra.pool, out_plane);
+            ts + (kTsDirty + q) * tiles, seeded, work.visits.as<int>(), ra.pool, out_plane);
         DL_LAUNCH_CHECK();
     } else {
         // No tile is computed after the quiet launch `stop`: a dirty tile's
@@ -1919,7 +1972,8 @@ int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, 
         DL_LAUNCH_CHECK();
         incr_materialise_kernel<<<tiles, 256, 0, s>>>(cv.W, cv.H, tiles_x, IW, IH, conv + 1, csr,
                                                        toff, toff + 1, ts + (kTsDirty + q) * tiles,
-                                                       ra.pool, out_plane);
+                                                       seeded, work.visits.as<int>(), ra.pool,
+                                                       out_plane);
         DL_LAUNCH_CHECK();
     }
     // visit total (statistics)

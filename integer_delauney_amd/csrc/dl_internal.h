@@ -323,6 +323,7 @@ struct FillRecord {
     PinnedBuf host;          // read-back of the counters
     HipEvent polled;         // replay: the change flags read back so far have landed
     int64_t last_computed = 0;   // tile runs of the last fill (statistics)
+    uint64_t gen = 0;            // a process-wide number given to each record when it is made
 };
 
 // Device view of a record's buffers for the fill kernels.
@@ -353,11 +354,14 @@ int bfs_run_recorded(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, Bfs
 // x * img_h + y of the img_w x img_h image, which sits at offset img_pad in
 // the canvas) of the recorded fill and of this one, and only tiles whose
 // neighbourhood differs from the record are computed.  Records this fill in
-// turn.  Returns the plane holding the labels.
+// turn.  Returns the plane holding the labels.  retained: the canvas's first
+// plane still holds the recorded fill's final labels (nothing has written it
+// since), so only the tiles near the changed seeds are seeded and only the
+// tiles the replay touches are written back.
 int bfs_run_incremental(std::vector<BfsCanvas>& canvases, const BfsConfig& cfg, BfsWork& work,
                         hipStream_t s, BfsStats* stats, FillRecord& rec,
                         const uint32_t* prev_bitmap, const uint32_t* bitmap, int img_w,
-                        int img_h, int img_pad);
+                        int img_h, int img_pad, bool retained = false);
 // Can rec serve the next fill of this canvas with this shape?
 bool bfs_record_usable(const FillRecord& rec, const BfsCanvas& cv, const BfsConfig& cfg);
 // Shapes a record can be made with.

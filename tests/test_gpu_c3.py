@@ -309,3 +309,61 @@ def test_fill_record_replay_equals_oracle():
     live = np.ascontiguousarray(live[:1])
     check("single seed")
     s.close()
+
+
+def _record_stats():
+    lib = _lib.load()
+    rec, rep, ret = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    _lib.check(lib.dl_get_record_stats(_lib.context().handle, ctypes.byref(rec), ctypes.byref(rep),
+                                       ctypes.byref(ret)))
+    return rec.value, rep.value, ret.value
+
+
+def test_replay_from_the_retained_plane():
+    """A replay starts from the label plane as the previous finalise left it
+    when nothing has written it since: only tiles near the changed seeds are
+    seeded, only tiles the replay touches are written back.  Every round
+    equals the oracle; a round after another call on the same context (which
+    rewrites the plane) re-initialises instead, and so does a round with the
+    reuse switched off; both still equal the oracle."""
+    W = H = 1536
+    comps, sigma = 16, 20.0
+    O.set_threads(0)
+    lib = _lib.load()
+    seeds = clustered_seeds(W, H, 20_000, 9, components=comps, sigma=sigma)
+    rng = np.random.default_rng(41)
+    s = dl.Delaunay(W, H, 60_000, 16)
+    s.set_fill_record(True)
+    s.insert_deferred(seeds)
+    live = seeds
+
+    def round_(tag, r, expect_retained):
+        nonlocal live
+        ins, dele, live = _round_plan(W, H, live, 1_000, 500, rng, 300 + r, comps, sigma, 9)
+        s.insert_deferred(ins)
+        s.remove(dele)
+        before = _record_stats()
+        m, grid = s.finalise()
+        after = _record_stats()
+        ref_rows, ref_grid = O.triangulate(W, H, live, 16, return_rows=True)
+        _say(f"{tag}: stats {before} -> {after}")
+        np.testing.assert_array_equal(grid, ref_grid, err_msg=tag)
+        np.testing.assert_array_equal(_rows(m), ref_rows, err_msg=tag)
+        assert after[1] == before[1] + 1, f"{tag}: not replayed"
+        assert after[2] == before[2] + (1 if expect_retained else 0), tag
+
+    s.finalise()   # the recorded fill
+    round_("first replay", 0, True)
+    round_("second replay", 1, True)
+    # another image on the same context rewrites the planes
+    other = clustered_seeds(700, 500, 3000, 4)
+    dl.triangulate(700, 500, other)
+    round_("after another call", 2, False)
+    round_("retained again", 3, True)
+    _lib.check(lib.dl_set_replay_plane_reuse(_lib.context().handle, 0))
+    try:
+        round_("reuse off", 4, False)
+    finally:
+        _lib.check(lib.dl_set_replay_plane_reuse(_lib.context().handle, 1))
+    round_("reuse back on", 5, True)   # (a full-init replay leaves the plane claimed too)
+    s.close()
