@@ -1449,6 +1449,10 @@ __global__ __launch_bounds__(256) void incr_seed_region_kernel(
     int* __restrict__ seeded, const uint32_t* __restrict__ bitmap, int img_w, int img_h, int P,
     uint32_t* __restrict__ plane)
 {
+    // The tile's bitmap words, per interior column: its rows [oy - P, oy - P + IH)
+    // of the image are at most three consecutive words of the column's bits.
+    constexpr int kMaxIW = 128, kWords = 3;
+    __shared__ uint32_t words[kMaxIW][kWords];
     const int t = blockIdx.x;
     const int tx = t % tiles_x, ty = t / tiles_x;
     int near = 0;
@@ -1460,14 +1464,29 @@ __global__ __launch_bounds__(256) void incr_seed_region_kernel(
     if (!__syncthreads_or(near)) return;   // block-uniform
     if (threadIdx.x == 0) seeded[t] = 1;
     const int ox = tx * IW, oy = ty * IH;
+    const int y_lo = max(oy - P, 0);                          // first image row of the tile
+    const int64_t last_word = (static_cast<int64_t>(img_w) * img_h - 1) >> 5;
+    for (int j = threadIdx.x; j < IW * kWords; j += 256) {
+        const int c = j / kWords, k = j % kWords;
+        const int x = ox + c - P;
+        uint32_t w = 0u;
+        if (x >= 0 && x < img_w && y_lo < img_h) {
+            const int64_t wi = ((static_cast<int64_t>(x) * img_h + y_lo) >> 5) + k;
+            if (wi <= last_word) w = bitmap[wi];
+        }
+        words[c][k] = w;
+    }
+    __syncthreads();
     for (int i = threadIdx.x; i < IW * IH; i += 256) {
-        const int cx = ox + i % IW, cy = oy + i / IW;
+        const int c = i % IW;
+        const int cx = ox + c, cy = oy + i / IW;
         if (cx >= W || cy >= H) continue;
         uint32_t v = kInvalid;
         const int x = cx - P, y = cy - P;
         if (x >= 0 && y >= 0 && x < img_w && y < img_h) {
             const int64_t bit = static_cast<int64_t>(x) * img_h + y;
-            if ((bitmap[bit >> 5] >> (bit & 31)) & 1u)
+            const int64_t w0 = (static_cast<int64_t>(x) * img_h + y_lo) >> 5;
+            if ((words[c][(bit >> 5) - w0] >> (bit & 31)) & 1u)
                 v = (static_cast<uint32_t>(cx) << 16) | static_cast<uint32_t>(cy);
         }
         plane[static_cast<size_t>(cy) * W + cx] = v;
