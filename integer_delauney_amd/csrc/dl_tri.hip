@@ -2362,11 +2362,22 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
     // as chunks of four by lanes 0..47 (chunk L = ints 4L .. 4L + 3), so every
     // store instruction covers one contiguous run of the row.  The buffer
     // reuses the raster entries, which are dead by now.
+    // The label source and the output are named as global memory and the row
+    // buffer as LDS: through generic pointers (the descriptor's, or a cast of
+    // the shared array) every access would be a FLAT one, which counts as a
+    // memory operation too, so each row's wait for its LDS buffer would also
+    // wait for every output store issued before it.
+    using GI32 = __attribute__((address_space(1))) int32_t;
+    using GCU32 = const __attribute__((address_space(1))) uint32_t;
+    using LI32 = __attribute__((address_space(3))) int32_t;
+    typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int off = d.out_off, shift = d.out_shift, Wl = d.out_W;
     const int n_rows = Y1 - Y0 + 1;
     const int n_out = min(4, 3 * Xw - 4 * lane);            // ints of the chunk inside the row
-    int32_t* rowbuf = reinterpret_cast<int32_t*>(ent) + wv * (3 * kTS);   // 3 ints per pixel
+    GCU32* out_lab = (GCU32*)d.out_lab;
+    GI32* out_g = (GI32*)d.out;
+    LI32* rowbuf = (LI32*)reinterpret_cast<int32_t*>(ent) + wv * (3 * kTS);   // 3 ints per pixel
     constexpr int kOutRows = kTS / 4;   // four waves cover the tile in one batch each
     for (int r0 = kOutRows * wv; r0 < n_rows; r0 += 4 * kOutRows) {
         uint32_t lab[kOutRows];
@@ -2374,7 +2385,7 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
         for (int i = 0; i < kOutRows; ++i) {
             const int r = r0 + i;
             lab[i] = (r < n_rows && lane < Xw)
-                         ? d.out_lab[static_cast<size_t>(Y0 + r + off) * Wl + off + X0 + lane]
+                         ? out_lab[static_cast<size_t>(Y0 + r + off) * Wl + off + X0 + lane]
                          : kInvalid;
         }
         int32_t id[kOutRows], dd[kOutRows];
@@ -2402,10 +2413,10 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
             rowbuf[3 * lane + 2] = tid;
             __builtin_amdgcn_wave_barrier();
             if (n_out <= 0) continue;
-            const int4 o = *reinterpret_cast<const int4*>(rowbuf + 4 * lane);
-            int32_t* dst = d.out + 3 * (static_cast<int64_t>(Y0 + r) * W + X0) + 4 * lane;
+            const i32x4 o = *reinterpret_cast<const __attribute__((address_space(3))) i32x4*>(rowbuf + 4 * lane);
+            GI32* dst = out_g + 3 * (static_cast<int64_t>(Y0 + r) * W + X0) + 4 * lane;
             if (n_out == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
-                *reinterpret_cast<int4*>(dst) = o;
+                *reinterpret_cast<__attribute__((address_space(1))) i32x4*>(dst) = o;
             } else {
                 dst[0] = o.x;
                 if (n_out > 1) dst[1] = o.y;
