@@ -470,10 +470,11 @@ __device__ __forceinline__ void tile_body(
             if constexpr (MODE == 2) {
                 const int sy = y < oy ? 0 : y >= oy + IH ? 6 : 3;   // row of sources, wave-uniform
                 const int k0 = sy + sx0, k1 = sy + sx1;
-                la[r] = lds_src[k0][static_cast<size_t>(y - lds_row[k0]) * lds_pitch[k0] +
-                                    (cc0 - lds_col[k0])];
-                lc[r] = lds_src[k1][static_cast<size_t>(y - lds_row[k1]) * lds_pitch[k1] +
-                                    (cc1 - lds_col[k1])];
+                // (plane or pool: global memory, named so to avoid FLAT loads)
+                la[r] = ((const GU32*)lds_src[k0])[static_cast<size_t>(y - lds_row[k0]) *
+                                                       lds_pitch[k0] + (cc0 - lds_col[k0])];
+                lc[r] = ((const GU32*)lds_src[k1])[static_cast<size_t>(y - lds_row[k1]) *
+                                                       lds_pitch[k1] + (cc1 - lds_col[k1])];
             } else {
                 const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(W);
                 la[r] = src[rowoff + cc0];
@@ -567,9 +568,10 @@ __device__ __forceinline__ void tile_body(
         const size_t rowoff = static_cast<size_t>(y) * static_cast<size_t>(Wb);
         if constexpr (MODE == 2) {
             // the tile's own source (the plane or its record slot)
-            const uint32_t* own = lds_src[4] + static_cast<size_t>(y - lds_row[4]) * lds_pitch[4] -
-                                  lds_col[4];
-            const uint32_t* cmp = ex.rec.pool + static_cast<size_t>(lds_cmp) * (IW * IH);
+            const GU32* own = ((const GU32*)lds_src[4]) +
+                              static_cast<size_t>(y - lds_row[4]) * lds_pitch[4] - lds_col[4];
+            const GU32* cmp =
+                ((const GU32*)ex.rec.pool) + static_cast<size_t>(lds_cmp) * (IW * IH);
             if (w0) old0[r] = own[c0];
             if (w1) old1[r] = own[c0 + 1];
             if (w0) cmp0[r] = cmp[ly * IW + lx0];
