@@ -2343,9 +2343,14 @@ __global__ __launch_bounds__(256) void tile_assign_kernel(
             const int4 t4 = *reinterpret_cast<const int4*>(&tid_tile[r * kTS + 4 * q]);
             const int32_t tv[4] = {t4.x, t4.y, t4.z, t4.w};
             const int n = min(4, X1 - x + 1);
-            int32_t* dst = d.out + p;
+            // (named as global memory: a FLAT store would make the next row's
+            // LDS read wait for it)
+            __attribute__((address_space(1))) int32_t* dst =
+                (__attribute__((address_space(1))) int32_t*)d.out + p;
             if (n == 4 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
-                *reinterpret_cast<int4*>(dst) = t4;
+                typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+                *reinterpret_cast<__attribute__((address_space(1))) i32x4*>(dst) =
+                    i32x4{t4.x, t4.y, t4.z, t4.w};
             } else {
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
